@@ -1,0 +1,139 @@
+#!/usr/bin/env python3
+"""Benchmark: static PageRank on one MI355X, printed as one JSON line.
+
+A step is one complete static solve (tolerance 1e-10 on the L-inf change, damping 0.85, FP64 ranks, at most 500
+iterations) of a synthetic graph that is already resident in HBM. Default workload: directed RMAT scale 24
+(16,777,216 vertices, 268,435,456 samples, a/b/c/d = 0.57/0.19/0.19/0.05, seed 42, duplicates removed, one
+self-loop per vertex). `value` is GEdges/s per iteration over the whole timed region
+(|E| x iterations / wall time of the K solves, including partition and initialisation).
+
+roofline: the dominant kernel is the rank update (one launch per iteration). Its algorithmic bytes per launch are
+12|E| + 24|V| (per edge a 4-byte source id and an 8-byte gathered FP64 value; per vertex a 4-byte offset, 4-byte
+out-degree, 8-byte old-rank read and 8-byte new-rank write; the L-inf reduction is fused). Its duration is measured
+with HIP events recorded on the engine's stream around every launch that did work inside the timed region.
+
+cpu_baseline: the oracle's OpenMP restatement of the reference's pagerankStaticOmp (bit-identical results to the
+compiled reference on the test graphs), run on this host's cores on the same graph for a bounded number of
+iterations (about 10 s).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "pagerank-cuda-dynamic_amd"))
+sys.path.insert(0, str(REPO / "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "GEdges/s/iter + %HBM peak at τ=1e-10; batched & adjoint vs plain solve"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_graph(args):
+    from pagerank_amd import HostGraph
+    if args.workload == "rmat":
+        g = HostGraph.rmat(args.scale, 16.0, 0.57, 0.19, 0.19, seed=42)
+        desc = (f"static PageRank, directed RMAT scale {args.scale} (edge factor 16, a/b/c/d=.57/.19/.19/.05, "
+                f"seed 42, dedup, self-loops)")
+    else:
+        g = HostGraph.grid(args.rows, args.cols, 0.62, seed=42)
+        desc = f"static PageRank, road-like grid {args.rows}x{args.cols} (edges kept p=0.62, symmetric, self-loops)"
+    return g, desc
+
+
+def cpu_baseline(g, budget_s: float):
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    _, it, ms = oracle.static_omp(g, max_iterations=2, threads=threads)
+    per_iter = ms / max(it, 1) / 1e3
+    k = int(max(2, min(500, budget_s / max(per_iter, 1e-6))))
+    _, it, ms = oracle.static_omp(g, max_iterations=k, threads=threads)
+    value = g.M * it / (ms / 1e3) / 1e9
+    return {"value": round(value, 4), "unit": "GEdges/s/iter", "cores": threads, "kind": "port",
+            "sample": f"{it} synchronous iterations of the OpenMP static solver on the same graph ({ms / 1e3:.1f} s)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["rmat", "grid"], default="rmat")
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--rows", type=int, default=6000)
+    ap.add_argument("--cols", type=int, default=4000)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported (the multi-GPU path is a design "
+              "only, see DESIGN.md).", file=sys.stderr)
+        return 2
+
+    from pagerank_amd import Context, PagerankOptions, pagerankStaticHip
+
+    t0 = time.time()
+    g, desc = make_graph(args)
+    log(f"graph N={g.N} M={g.M} built in {time.time() - t0:.1f}s")
+    ctx = Context(0)
+    yg = ctx.upload(g, with_out=False)
+    o = PagerankOptions()
+    for i in range(args.warmup):
+        r = pagerankStaticHip(yg, o)
+        log(f"warmup {i}: {r.iterations} iterations, {r.time:.2f} ms")
+    ctx.kernel_timing(True)
+    ctx.kernel_timing_read()
+    iters = 0
+    tcomp = 0.0
+    t_start = time.perf_counter()  # every solve returns after its stream has drained
+    for _ in range(args.steps):
+        r = pagerankStaticHip(yg, o)
+        iters += r.iterations
+        tcomp += r.computationTime
+    t_total = time.perf_counter() - t_start
+    launches, kms = ctx.kernel_timing_read()
+    ctx.kernel_timing(False)
+    log(f"timed {args.steps} solves: {iters} iterations, {t_total * 1e3:.1f} ms, rank-update launches {launches}")
+
+    value = g.M * iters / t_total / 1e9
+    kavg_s = kms / max(launches, 1) / 1e3
+    bytes_alg = 12 * g.M + 24 * g.N
+    achieved = bytes_alg / kavg_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "GEdges/s/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_total * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "vertices": g.N, "edges": g.M, "tolerance": 1e-10, "damping": 0.85,
+                   "iterations_per_solve": iters / args.steps,
+                   "gedges_per_s_iter_compute_only": round(g.M * iters / (tcomp / 1e3) / 1e9, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "rank_update_kernel", "kernel_avg_us": round(kavg_s * 1e6, 2),
+                     "bytes_per_launch": bytes_alg},
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,156 @@
+/*
+ * pagerank_hip.h -- C ABI of the MI355X (gfx950) PageRank engine: libpagerank_hip.so.
+ *
+ * This is the drop-in boundary for the GPU PageRank path of puzzlef/pagerank-cuda-dynamic
+ * (inc/pagerankCuda.hxx). The reference's entry points are C++ templates that marshal a host DiGraph into
+ * uint32 CSR arrays, allocate device memory, copy, run, copy back and free, all in one call
+ * (pagerankInvokeCuda, pagerankCuda.hxx:713-849). Here the same work is split at the host->device boundary:
+ *   - prh_graph_create uploads the CSR arrays the reference builds at pagerankCuda.hxx:756-775 and copies at
+ *     :798-805 (plain pointers and sizes, no C++ types);
+ *   - prh_solve runs one of the five solvers with the reference's options and returns its result fields.
+ * The C++ header pagerankHip.hxx (package host/ directory) rebuilds the reference's template signatures on top.
+ *
+ * Layout (the reference's, pagerankCuda.hxx:715-761): vertices are dense ids 0..N-1; offsets are uint32
+ * (so M < 2^32); the graph handed to a solver is the updated graph y, given as its transpose's CSR
+ * (in_offsets[N+1], in_sources[M], in-neighbours ascending), out_degrees[N] (the transpose's vertex value,
+ * transpose.hxx:109-134) and, for the dynamic solvers, y's own CSR (out_offsets[N+1], out_targets[M]).
+ * Every vertex needs out-degree >= 1 (the reference adds a self-loop per vertex, main.cu:168).
+ *
+ * Errors: every call returns a prh_status; nothing aborts the process (the reference exits on any CUDA
+ * error, TRY_CUDA, inc/_cuda.hxx:136-150). prh_context_last_error() describes the last failure.
+ * Threading: a context owns one HIP stream; calls on one context are serialised by the caller, separate
+ * contexts may be used from separate threads.
+ */
+#ifndef PAGERANK_HIP_H
+#define PAGERANK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum prh_status {
+  PRH_SUCCESS = 0,
+  PRH_ERROR_INVALID_ARGUMENT = 1, /* null pointer, bad method, bad option, id out of range, NaN rank */
+  PRH_ERROR_INVALID_GRAPH = 2,    /* offsets not 0..M or not monotone, id >= N, out-degree 0 (dead end) */
+  PRH_ERROR_OUT_OF_MEMORY = 3,    /* device allocation failed or exceeded the context's memory limit */
+  PRH_ERROR_DEVICE = 4,           /* HIP runtime error */
+  PRH_ERROR_UNSUPPORTED = 5       /* e.g. a dynamic solver on a graph uploaded without its out-CSR */
+} prh_status;
+
+/* The five solvers of pagerankCuda.hxx. */
+typedef enum prh_method {
+  PRH_STATIC = 0,                 /* pagerankStaticCuda, pagerankCuda.hxx:864-875 */
+  PRH_NAIVE_DYNAMIC = 1,          /* pagerankNaiveDynamicCuda, pagerankCuda.hxx:891-901 */
+  PRH_DYNAMIC_TRAVERSAL = 2,      /* pagerankDynamicTraversalCuda, pagerankCuda.hxx:921-930 */
+  PRH_DYNAMIC_FRONTIER = 3,       /* pagerankDynamicFrontierCuda, pagerankCuda.hxx:950-959 */
+  PRH_PRUNE_DYNAMIC_FRONTIER = 4  /* pagerankPruneDynamicFrontierCuda, pagerankCuda.hxx:979-988 */
+} prh_method;
+
+/* PagerankOptions<double> (pagerank.hxx:24-55); prh_default_options gives the reference's defaults
+ * (repeat 1, tolerance 1e-10, frontier and prune tolerance 1e-6, damping 0.85, 500 iterations). */
+typedef struct prh_options {
+  int repeat;
+  double tolerance;
+  double frontier_tolerance;
+  double prune_tolerance;
+  double damping;
+  int max_iterations;
+} prh_options;
+
+/* PagerankResult<double> (pagerank.hxx:64-113) minus the rank vector: iterations of the last repeat and
+ * phase times in milliseconds averaged over repeat (pagerankCuda.hxx:807-824, :848). */
+typedef struct prh_stats {
+  int iterations;
+  float time;
+  float initialization_time;
+  float marking_time;
+  float computation_time;
+} prh_stats;
+
+/* Launch tuning (no counterpart in the reference, whose values are fixed: switch degree 64, 256-thread blocks).
+ * Results do not depend on these beyond floating-point summation order. */
+typedef struct prh_tuning {
+  uint32_t switch_degree; /* in-degree <= this: several vertices per wave [64] */
+  uint32_t huge_degree;   /* in-degree > this: one workgroup per vertex [2048] */
+  int low_width;          /* lanes per low-degree vertex: 1, 2, 4, 8, 16 or 32; 0 = pick from |E|/|V| [0] */
+  int check_interval;     /* iterations launched between host reads of the device convergence flag [8] */
+} prh_tuning;
+
+typedef struct prh_context prh_context;
+typedef struct prh_graph prh_graph;
+
+/* Replaces the PagerankOptions constructor defaults (pagerank.hxx:52). */
+void prh_default_options(prh_options *options);
+/* Replaces the fixed launch constants of pagerankCuda.hxx / _cuda.hxx:62-120. */
+void prh_default_tuning(prh_tuning *tuning);
+
+/* Context: device, stream and reusable device workspace. Replaces the per-call cudaSetDeviceFlags and
+ * cudaMalloc/cudaFree of pagerankInvokeCuda (pagerankCuda.hxx:777-796, :829-847). */
+int prh_context_create(int device, prh_context **context);
+void prh_context_destroy(prh_context *context);
+const char *prh_context_last_error(const prh_context *context);
+/* Upper bound on device bytes this context may allocate (graphs + workspace); 0 = unlimited. */
+int prh_context_set_memory_limit(prh_context *context, size_t bytes);
+size_t prh_context_memory_used(const prh_context *context);
+int prh_context_set_tuning(prh_context *context, const prh_tuning *tuning);
+int prh_context_get_tuning(const prh_context *context, prh_tuning *tuning);
+/* The hipStream_t the context's work runs on. */
+void *prh_context_stream(prh_context *context);
+
+/* Upload y's CSR arrays (host pointers; out_offsets/out_targets may be NULL for static-only use).
+ * Replaces the host->device copies of pagerankInvokeCuda (pagerankCuda.hxx:798-802). The arrays are checked
+ * on the device: offsets 0..M monotone, ids < N, out-degree >= 1 and equal to the out-CSR degree. */
+int prh_graph_create(prh_context *context, uint32_t n, uint64_t m, const uint32_t *in_offsets,
+                     const uint32_t *in_sources, const uint32_t *out_degrees, const uint32_t *out_offsets,
+                     const uint32_t *out_targets, prh_graph **graph);
+void prh_graph_destroy(prh_graph *graph);
+uint32_t prh_graph_order(const prh_graph *graph);
+uint64_t prh_graph_size(const prh_graph *graph);
+
+/* Run a solver on y. Replaces pagerankInvokeCuda (pagerankCuda.hxx:713-849) as called by the five entry
+ * points (:864-988). Host pointers. The batch is (deletion_sources[i], deletion_targets[i]) and
+ * insertion_sources[i] in dense ids (the reference's delu/delv/insu, :765-775); it is used by the dynamic
+ * traversal / frontier methods only. initial_ranks (N values, may be NULL for PRH_STATIC) is the reference's q.
+ * ranks receives N values. partition is the reference's PARTITION switch; results do not depend on it.
+ * With an empty batch the dynamic methods mark nothing (the reference leaves its flags uninitialised). */
+int prh_solve(prh_context *context, const prh_graph *graph, prh_method method, int partition,
+              const prh_options *options, const uint32_t *deletion_sources, const uint32_t *deletion_targets,
+              size_t n_deletions, const uint32_t *insertion_sources, size_t n_insertions,
+              const double *initial_ranks, double *ranks, prh_stats *stats);
+
+/* Same as prh_solve with every array (batch, initial_ranks, ranks) in device memory of the context's device;
+ * no host copies happen and inputs stay resident. */
+int prh_solve_device(prh_context *context, const prh_graph *graph, prh_method method, int partition,
+                     const prh_options *options, const uint32_t *deletion_sources,
+                     const uint32_t *deletion_targets, size_t n_deletions, const uint32_t *insertion_sources,
+                     size_t n_insertions, const double *initial_ranks, double *ranks, prh_stats *stats);
+
+/* The degree partition of a CSR given by its offsets (host pointers): order[N] receives the vertices with
+ * degree <= switch_degree in ascending id order followed by the rest in ascending id order, and
+ * *partition_point the size of the first part. Replaces pagerankPartitionVerticesCuW (pagerankCuda.hxx:625-638),
+ * which the reference calls with switch degree 64. */
+int prh_partition(prh_context *context, uint32_t n, const uint32_t *offsets, uint32_t switch_degree,
+                  uint32_t *order, uint32_t *partition_point);
+
+/* Affected flags right after the marking phase of a dynamic method (host pointers; affected receives N bytes,
+ * 0 or 1). PRH_DYNAMIC_TRAVERSAL gives the traversal closure (pagerankAffectedTraversalCuW,
+ * pagerankCuda.hxx:438-456); PRH_DYNAMIC_FRONTIER and PRH_PRUNE_DYNAMIC_FRONTIER give the initial frontier
+ * (pagerankAffectedFrontierCuW, pagerankCuda.hxx:405-412). */
+int prh_mark_affected(prh_context *context, const prh_graph *graph, prh_method method,
+                      const uint32_t *deletion_sources, const uint32_t *deletion_targets, size_t n_deletions,
+                      const uint32_t *insertion_sources, size_t n_insertions, uint8_t *affected);
+
+/* Per-launch timing of the rank-update kernel with HIP events on the context's stream (no counterpart in the
+ * reference). While enabled, every solve accumulates the summed duration and count of the rank-update
+ * launches that did work; reading resets the accumulators. */
+int prh_kernel_timing_enable(prh_context *context, int enable);
+int prh_kernel_timing_read(prh_context *context, uint64_t *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PAGERANK_HIP_H */

@@ -1,0 +1,628 @@
+// capi.hip -- the C ABI (include/pagerank_hip.h): contexts, graph upload and the solver driver.
+//
+// The driver restates pagerankInvokeCuda + pagerankLoopCuU (pagerankCuda.hxx:675-692, :713-849) for one HIP
+// stream: per repeat, partition (and initial ranks) = initialization time, affected marking = marking time,
+// iterations = computation time, each measured with HIP events. Iterations run without a host round trip per
+// iteration: convergence is detected on the device (LoopCtl), and the host polls the device flag only every
+// check_interval iterations, so the iteration count and final ranks equal those of the per-iteration check.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pagerank_hip.h"
+#include "prh_internal.hpp"
+
+namespace {
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t cap = 0;  // elements
+};
+
+}  // namespace
+
+struct prh_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  size_t limit = 0, used = 0;
+  prh_tuning tuning{};
+  DevBuf<double> rank[2], contrib[2];
+  DevBuf<uint8_t> vaff, naff, buff;
+  DevBuf<uint32_t> sched_in, sched_out, tiles, counts, batch, scratch_u32;
+  DevBuf<unsigned long long> loop;  // [L] error bits, then one slot holding `done`
+  DevBuf<int> flags;                // [0] validation bits, [1] traversal "changed"
+  uint64_t *pinned = nullptr;       // small host-pinned readback area
+  hipEvent_t ev[4] = {};
+  bool ktiming = false;
+  std::vector<hipEvent_t> kev;  // start/stop pairs per iteration
+  uint64_t k_launches = 0;
+  double k_ms = 0.0;
+};
+
+struct prh_graph {
+  prh_context *ctx = nullptr;
+  uint32_t N = 0;
+  uint64_t M = 0;
+  uint32_t *in_off = nullptr, *in_src = nullptr, *outdeg = nullptr, *out_off = nullptr, *out_dst = nullptr;
+  size_t bytes = 0;
+};
+
+namespace {
+
+int fail(prh_context *c, int status, const char *fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->error = buf;
+  }
+  return status;
+}
+
+#define PRH_HIP(ctx, expr)                                                                              \
+  do {                                                                                                  \
+    hipError_t e_ = (expr);                                                                             \
+    if (e_ != hipSuccess) return fail(ctx, PRH_ERROR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define PRH_CHECK(expr)      \
+  do {                       \
+    int s_ = (expr);         \
+    if (s_ != PRH_SUCCESS) return s_; \
+  } while (0)
+
+int dev_alloc(prh_context *c, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (c->limit && c->used + bytes > c->limit)
+    return fail(c, PRH_ERROR_OUT_OF_MEMORY, "device memory limit: %zu bytes in use, %zu more requested, limit %zu",
+                c->used, bytes, c->limit);
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, PRH_ERROR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  c->used += bytes;
+  return PRH_SUCCESS;
+}
+
+void dev_free(prh_context *c, void *p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  c->used -= std::min(c->used, bytes ? bytes : 16);
+}
+
+template <class T>
+int ensure(prh_context *c, DevBuf<T> &b, size_t n) {
+  if (b.cap >= n && b.p) return PRH_SUCCESS;
+  if (b.p) {
+    PRH_HIP(c, hipStreamSynchronize(c->stream));
+    dev_free(c, b.p, b.cap * sizeof(T));
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  void *p = nullptr;
+  PRH_CHECK(dev_alloc(c, &p, n * sizeof(T)));
+  b.p = static_cast<T *>(p);
+  b.cap = n;
+  return PRH_SUCCESS;
+}
+
+template <class T>
+void release(prh_context *c, DevBuf<T> &b) {
+  dev_free(c, b.p, b.cap * sizeof(T));
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+bool valid_method(int m) { return m >= PRH_STATIC && m <= PRH_PRUNE_DYNAMIC_FRONTIER; }
+
+int low_width_for(const prh_tuning &t, uint32_t N, uint64_t M) {
+  if (t.low_width) return t.low_width;
+  const double avg = N ? double(M) / N : 1.0;
+  return avg < 3 ? 1 : avg < 6 ? 2 : avg < 12 ? 4 : avg < 24 ? 8 : 16;
+}
+
+uint32_t cap_blocks(uint64_t n, uint64_t per, uint32_t cap) {
+  const uint64_t g = (n + per - 1) / per;
+  return uint32_t(std::min<uint64_t>(g, cap));
+}
+
+prh::RankShape rank_shape(const prh::Schedule &s, int width) {
+  prh::RankShape sh;
+  sh.low_width = width;
+  sh.nb_huge = cap_blocks(s.n_huge, 1, 2048);
+  sh.nb_high = cap_blocks(s.n_high, prh::kWavesPerBlock, 16384);
+  sh.nb_low = cap_blocks(s.n_low, uint64_t(prh::kWavesPerBlock) * (prh::kWave / width), 32768);
+  return sh;
+}
+
+// Batch arrays in device memory: either the caller's (device API) or staged through the context.
+struct DevBatch {
+  const uint32_t *delu = nullptr, *delv = nullptr, *insu = nullptr;
+  size_t ND = 0, NI = 0;
+};
+
+int stage_batch(prh_context *c, const uint32_t *du, const uint32_t *dv, size_t nd, const uint32_t *iu, size_t ni,
+                bool host, DevBatch &b) {
+  b.ND = nd;
+  b.NI = ni;
+  if (nd + ni == 0) return PRH_SUCCESS;
+  if ((nd && (!du || !dv)) || (ni && !iu)) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null batch array");
+  if (!host) {
+    b.delu = du;
+    b.delv = dv;
+    b.insu = iu;
+    return PRH_SUCCESS;
+  }
+  PRH_CHECK(ensure(c, c->batch, 2 * nd + ni));
+  uint32_t *d = c->batch.p;
+  if (nd) {
+    PRH_HIP(c, hipMemcpyAsync(d, du, nd * 4, hipMemcpyHostToDevice, c->stream));
+    PRH_HIP(c, hipMemcpyAsync(d + nd, dv, nd * 4, hipMemcpyHostToDevice, c->stream));
+  }
+  if (ni) PRH_HIP(c, hipMemcpyAsync(d + 2 * nd, iu, ni * 4, hipMemcpyHostToDevice, c->stream));
+  b.delu = d;
+  b.delv = d + nd;
+  b.insu = d + 2 * nd;
+  return PRH_SUCCESS;
+}
+
+int check_batch(prh_context *c, const DevBatch &b, uint32_t N) {
+  if (b.ND + b.NI == 0) return PRH_SUCCESS;
+  PRH_CHECK(ensure(c, c->flags, 4));
+  PRH_HIP(c, prh::launch_fill_i32(c->flags.p, 0, 1, c->stream));
+  PRH_HIP(c, prh::launch_validate_ids(b.delu, b.ND, N, c->flags.p, 1, c->stream));
+  PRH_HIP(c, prh::launch_validate_ids(b.delv, b.ND, N, c->flags.p, 1, c->stream));
+  PRH_HIP(c, prh::launch_validate_ids(b.insu, b.NI, N, c->flags.p, 1, c->stream));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PRH_HIP(c, hipStreamSynchronize(c->stream));
+  if (*reinterpret_cast<int *>(c->pinned)) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "batch vertex id >= N (%u)", N);
+  return PRH_SUCCESS;
+}
+
+// Partition the in-CSR (rank kernel schedule) and, for the dynamic methods, the out-CSR (extension schedule);
+// one readback of the part sizes, as the reference reads its partition point back (pagerankCuda.hxx:633).
+int partition_graph(prh_context *c, const prh_graph *g, bool dynamic, prh::Schedule &in_s, prh::Schedule &out_s) {
+  const uint32_t N = g->N;
+  const uint32_t nt = prh::partition_tiles(N);
+  PRH_CHECK(ensure(c, c->tiles, size_t(3) * nt * 2));
+  PRH_CHECK(ensure(c, c->counts, 8));
+  PRH_HIP(c, prh::launch_partition(g->in_off, N, c->tuning.switch_degree, c->tuning.huge_degree, c->sched_in.p,
+                                   c->tiles.p, c->counts.p, c->stream));
+  if (dynamic)
+    PRH_HIP(c, prh::launch_partition(g->out_off, N, 64u, 0xFFFFFFFFu, c->sched_out.p, c->tiles.p + size_t(3) * nt,
+                                     c->counts.p + 4, c->stream));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->counts.p, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  PRH_HIP(c, hipStreamSynchronize(c->stream));
+  const uint32_t *cnt = reinterpret_cast<const uint32_t *>(c->pinned);
+  in_s = prh::Schedule{c->sched_in.p, cnt[0], cnt[1], cnt[2]};
+  if (dynamic) out_s = prh::Schedule{c->sched_out.p, cnt[4], cnt[5] + cnt[6], 0};
+  return PRH_SUCCESS;
+}
+
+// Marking phase of the dynamic methods (pagerankCuda.hxx:821 with the fm lambdas :926-928, :955-957, :984-986).
+int mark_affected(prh_context *c, const prh_graph *g, int method, const DevBatch &b, const prh::Schedule &out_s) {
+  const uint32_t N = g->N;
+  hipStream_t s = c->stream;
+  PRH_HIP(c, prh::launch_fill_u8(c->vaff.p, 0, N, s));
+  PRH_HIP(c, prh::launch_fill_u8(c->naff.p, 0, N, s));
+  if (b.ND + b.NI == 0) return PRH_SUCCESS;
+  PRH_HIP(c, prh::launch_mark_partial(c->naff.p, c->vaff.p, b.delu, b.delv, b.ND, b.insu, b.NI, s));
+  prh::ExtendArgs E{c->vaff.p, c->naff.p, g->out_off, g->out_dst, out_s, nullptr, nullptr};
+  PRH_HIP(c, prh::launch_extend(E, nullptr, nullptr, 0, 0.0, s));
+  if (method != PRH_DYNAMIC_TRAVERSAL) return PRH_SUCCESS;
+  // Traversal closure: breadth-first levels from every marked vertex until no vertex is newly marked.
+  PRH_CHECK(ensure(c, c->flags, 4));
+  uint8_t *front = c->naff.p, *fresh = c->buff.p;
+  PRH_HIP(c, hipMemcpyAsync(front, c->vaff.p, N, hipMemcpyDeviceToDevice, s));
+  for (;;) {
+    PRH_HIP(c, prh::launch_fill_u8(fresh, 0, N, s));
+    PRH_HIP(c, prh::launch_fill_i32(c->flags.p + 1, 0, 1, s));
+    prh::ExtendArgs T{c->vaff.p, front, g->out_off, g->out_dst, out_s, fresh, c->flags.p + 1};
+    PRH_HIP(c, prh::launch_extend(T, nullptr, nullptr, 0, 0.0, s));
+    PRH_HIP(c, hipMemcpyAsync(c->pinned, c->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipStreamSynchronize(s));
+    if (*reinterpret_cast<int *>(c->pinned) == 0) break;
+    std::swap(front, fresh);
+  }
+  return PRH_SUCCESS;
+}
+
+int ensure_workspace(prh_context *c, uint32_t N, bool dynamic, int L) {
+  for (int i = 0; i < 2; ++i) {
+    PRH_CHECK(ensure(c, c->rank[i], N));
+    PRH_CHECK(ensure(c, c->contrib[i], N));
+  }
+  PRH_CHECK(ensure(c, c->sched_in, N));
+  if (dynamic) {
+    PRH_CHECK(ensure(c, c->sched_out, N));
+    PRH_CHECK(ensure(c, c->vaff, N));
+    PRH_CHECK(ensure(c, c->naff, N));
+    PRH_CHECK(ensure(c, c->buff, N));
+  }
+  PRH_CHECK(ensure(c, c->loop, size_t(L) + 1));
+  PRH_CHECK(ensure(c, c->flags, 4));
+  return PRH_SUCCESS;
+}
+
+int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options *o, const uint32_t *du,
+               const uint32_t *dv, size_t nd, const uint32_t *iu, size_t ni, const double *q, double *ranks,
+               prh_stats *st, bool host) {
+  if (!c) return PRH_ERROR_INVALID_ARGUMENT;
+  if (!g || !o || !valid_method(method)) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null graph/options or bad method");
+  if (g->ctx != c) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "graph belongs to another context");
+  if (o->repeat < 1 || o->max_iterations < 0 || !(o->tolerance >= 0) || !(o->damping > 0 && o->damping < 1))
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad options (repeat >= 1, max_iterations >= 0, tolerance >= 0, 0 < damping < 1)");
+  if (method != PRH_STATIC && !q) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "initial ranks required for this method");
+  const uint32_t N = g->N;
+  prh_stats stats{};
+  if (N == 0) {  // the reference returns an empty result for an empty graph (pagerankCuda.hxx:869)
+    if (st) *st = stats;
+    return PRH_SUCCESS;
+  }
+  if (!ranks) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null ranks");
+  const bool dynamic = method >= PRH_DYNAMIC_TRAVERSAL;
+  const bool frontier = method >= PRH_DYNAMIC_FRONTIER;
+  if (dynamic && !g->out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "dynamic method needs the graph's out-CSR");
+  const int L = o->max_iterations;
+  const int variant = method == PRH_PRUNE_DYNAMIC_FRONTIER ? prh::kPrune
+                      : method == PRH_DYNAMIC_FRONTIER    ? prh::kFrontier
+                      : method == PRH_DYNAMIC_TRAVERSAL   ? prh::kTraversal
+                                                          : prh::kPlain;
+  hipStream_t s = c->stream;
+  PRH_HIP(c, hipSetDevice(c->device));
+  PRH_CHECK(ensure_workspace(c, N, dynamic, L));
+  DevBatch b;
+  if (dynamic) {
+    PRH_CHECK(stage_batch(c, du, dv, nd, iu, ni, host, b));
+    PRH_CHECK(check_batch(c, b, N));
+  }
+  if (q && !host) {  // device initial ranks: reject NaN/Inf (the reference would propagate them)
+    PRH_HIP(c, prh::launch_fill_i32(c->flags.p, 0, 1, s));
+    PRH_HIP(c, prh::launch_check_finite(q, N, c->flags.p, 1, s));
+    PRH_HIP(c, hipMemcpyAsync(c->pinned, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipStreamSynchronize(s));
+    if (*reinterpret_cast<int *>(c->pinned)) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "initial ranks not finite");
+  }
+  if (q && host)
+    for (uint32_t v = 0; v < N; ++v)
+      if (!std::isfinite(q[v])) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "initial ranks not finite (vertex %u)", v);
+  const int width = low_width_for(c->tuning, N, g->M);
+  if (c->ktiming && c->kev.size() < size_t(2) * L) {
+    size_t old = c->kev.size();
+    c->kev.resize(size_t(2) * L);
+    for (size_t i = old; i < c->kev.size(); ++i) PRH_HIP(c, hipEventCreate(&c->kev[i]));
+  }
+  double ti = 0, tm = 0, tc = 0;
+  int iterations = 0;
+  unsigned long long *err = c->loop.p;
+  int *done = reinterpret_cast<int *>(c->loop.p + L);
+  const int check = std::max(1, c->tuning.check_interval);
+  for (int rep = 0; rep < o->repeat; ++rep) {
+    if (q)  // untimed, as the reference's q copy (pagerankCuda.hxx:810)
+      PRH_HIP(c, hipMemcpyAsync(c->rank[0].p, q, size_t(N) * 8, host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+    PRH_HIP(c, hipEventRecord(c->ev[0], s));
+    prh::Schedule in_s{}, out_s{};
+    PRH_CHECK(partition_graph(c, g, dynamic, in_s, out_s));
+    PRH_HIP(c, prh::launch_init_ranks(c->rank[0].p, c->rank[1].p, c->contrib[0].p, c->contrib[1].p, g->outdeg, N,
+                                      q != nullptr, s));
+    PRH_HIP(c, hipEventRecord(c->ev[1], s));
+    if (dynamic) PRH_CHECK(mark_affected(c, g, method, b, out_s));
+    PRH_HIP(c, hipEventRecord(c->ev[2], s));
+    // Iterations.
+    PRH_HIP(c, prh::launch_fill_u64(err, 0ull, size_t(L) + 1, s));
+    const prh::RankShape shape = rank_shape(in_s, width);
+    const double P = o->damping;
+    const double C0 = (1 - P) / N;
+    int n = -1, launched = 0;
+    for (int l = 0; l < L; ++l) {
+      prh::RankArgs A{c->rank[(l + 1) & 1].p, c->rank[l & 1].p, c->contrib[(l + 1) & 1].p, c->contrib[l & 1].p,
+                      c->naff.p, c->vaff.p, g->in_off, g->in_src, g->outdeg, in_s, C0, P, o->frontier_tolerance,
+                      o->prune_tolerance};
+      if (c->ktiming) PRH_HIP(c, hipEventRecord(c->kev[2 * l], s));
+      PRH_HIP(c, prh::launch_rank_update(variant, A, shape, err, done, l, o->tolerance, s));
+      if (c->ktiming) PRH_HIP(c, hipEventRecord(c->kev[2 * l + 1], s));
+      if (frontier) {
+        prh::ExtendArgs E{c->vaff.p, c->naff.p, g->out_off, g->out_dst, out_s, nullptr, nullptr};
+        PRH_HIP(c, prh::launch_extend(E, err, done, l, o->tolerance, s));
+      }
+      launched = l + 1;
+      if (launched % check == 0 || launched == L) {
+        PRH_HIP(c, hipMemcpyAsync(c->pinned, err + l, 8, hipMemcpyDeviceToHost, s));
+        PRH_HIP(c, hipMemcpyAsync(c->pinned + 1, done, sizeof(int), hipMemcpyDeviceToHost, s));
+        PRH_HIP(c, hipStreamSynchronize(s));
+        double el;
+        std::memcpy(&el, c->pinned, 8);
+        const int d = *reinterpret_cast<const int *>(c->pinned + 1);
+        if (d > 0) { n = d; break; }
+        if (el < o->tolerance) { n = launched; break; }
+      }
+    }
+    if (n < 0) n = launched;
+    PRH_HIP(c, hipEventRecord(c->ev[3], s));
+    PRH_HIP(c, hipEventSynchronize(c->ev[3]));
+    float t01 = 0, t12 = 0, t23 = 0;
+    PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
+    PRH_HIP(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+    PRH_HIP(c, hipEventElapsedTime(&t23, c->ev[2], c->ev[3]));
+    ti += t01;
+    tm += t12;
+    tc += t23;
+    iterations = n;
+    if (c->ktiming) {
+      for (int l = 0; l < n; ++l) {
+        float ms = 0;
+        PRH_HIP(c, hipEventElapsedTime(&ms, c->kev[2 * l], c->kev[2 * l + 1]));
+        c->k_ms += ms;
+        ++c->k_launches;
+      }
+    }
+  }
+  // Final ranks are in the buffer written by the last iteration (pagerankCuda.hxx:688, :826).
+  const double *res = c->rank[iterations & 1].p;
+  PRH_HIP(c, hipMemcpyAsync(ranks, res, size_t(N) * 8, host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+  PRH_HIP(c, hipStreamSynchronize(s));
+  stats.iterations = iterations;
+  stats.initialization_time = float(ti / o->repeat);
+  stats.marking_time = float(tm / o->repeat);
+  stats.computation_time = float(tc / o->repeat);
+  stats.time = float((ti + tm + tc) / o->repeat);
+  if (st) *st = stats;
+  return PRH_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+void prh_default_options(prh_options *o) {
+  if (!o) return;
+  o->repeat = 1;
+  o->tolerance = 1e-10;
+  o->frontier_tolerance = 1e-6;
+  o->prune_tolerance = 1e-6;
+  o->damping = 0.85;
+  o->max_iterations = 500;
+}
+
+void prh_default_tuning(prh_tuning *t) {
+  if (!t) return;
+  t->switch_degree = 64;
+  t->huge_degree = 2048;
+  t->low_width = 0;
+  t->check_interval = 8;
+}
+
+int prh_context_create(int device, prh_context **out) {
+  if (!out) return PRH_ERROR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    (void)hipGetLastError();
+    return PRH_ERROR_DEVICE;
+  }
+  prh_context *c = new prh_context();
+  c->device = device;
+  prh_default_tuning(&c->tuning);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void **>(&c->pinned), 256, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    prh_context_destroy(c);
+    return PRH_ERROR_DEVICE;
+  }
+  for (auto &e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      prh_context_destroy(c);
+      return PRH_ERROR_DEVICE;
+    }
+  *out = c;
+  return PRH_SUCCESS;
+}
+
+void prh_context_destroy(prh_context *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int i = 0; i < 2; ++i) {
+    release(c, c->rank[i]);
+    release(c, c->contrib[i]);
+  }
+  release(c, c->vaff);
+  release(c, c->naff);
+  release(c, c->buff);
+  release(c, c->sched_in);
+  release(c, c->sched_out);
+  release(c, c->tiles);
+  release(c, c->counts);
+  release(c, c->batch);
+  release(c, c->scratch_u32);
+  release(c, c->loop);
+  release(c, c->flags);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : c->kev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *prh_context_last_error(const prh_context *c) { return c ? c->error.c_str() : "null context"; }
+
+int prh_context_set_memory_limit(prh_context *c, size_t bytes) {
+  if (!c) return PRH_ERROR_INVALID_ARGUMENT;
+  c->limit = bytes;
+  return PRH_SUCCESS;
+}
+
+size_t prh_context_memory_used(const prh_context *c) { return c ? c->used : 0; }
+
+int prh_context_set_tuning(prh_context *c, const prh_tuning *t) {
+  if (!c || !t) return PRH_ERROR_INVALID_ARGUMENT;
+  const int w = t->low_width;
+  if (!(w == 0 || w == 1 || w == 2 || w == 4 || w == 8 || w == 16 || w == 32) || t->check_interval < 1 ||
+      t->huge_degree < t->switch_degree)
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad tuning");
+  c->tuning = *t;
+  return PRH_SUCCESS;
+}
+
+int prh_context_get_tuning(const prh_context *c, prh_tuning *t) {
+  if (!c || !t) return PRH_ERROR_INVALID_ARGUMENT;
+  *t = c->tuning;
+  return PRH_SUCCESS;
+}
+
+void *prh_context_stream(prh_context *c) { return c ? reinterpret_cast<void *>(c->stream) : nullptr; }
+
+int prh_graph_create(prh_context *c, uint32_t N, uint64_t M, const uint32_t *in_off, const uint32_t *in_src,
+                     const uint32_t *outdeg, const uint32_t *out_off, const uint32_t *out_dst, prh_graph **out) {
+  if (!c || !out) return PRH_ERROR_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (M >= (uint64_t(1) << 32)) return fail(c, PRH_ERROR_INVALID_GRAPH, "M = %llu does not fit uint32 offsets", (unsigned long long)M);
+  if (N > 0 && (!in_off || (M && !in_src) || !outdeg))
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null in-CSR array");
+  if ((out_off == nullptr) != (out_dst == nullptr) || (out_off && M && !out_dst))
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "out-CSR needs both offsets and targets");
+  PRH_HIP(c, hipSetDevice(c->device));
+  prh_graph *g = new prh_graph();
+  g->ctx = c;
+  g->N = N;
+  g->M = M;
+  if (N == 0) {
+    *out = g;
+    return PRH_SUCCESS;
+  }
+  auto up = [&](uint32_t **dst, const uint32_t *src, size_t n) -> int {
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(dst), n * 4));
+    g->bytes += (n ? n : 4) * 4;
+    PRH_HIP(c, hipMemcpyAsync(*dst, src, n * 4, hipMemcpyHostToDevice, c->stream));
+    return PRH_SUCCESS;
+  };
+  int st = up(&g->in_off, in_off, size_t(N) + 1);
+  if (st == PRH_SUCCESS) st = up(&g->in_src, in_src, M);
+  if (st == PRH_SUCCESS) st = up(&g->outdeg, outdeg, N);
+  if (st == PRH_SUCCESS && out_off) st = up(&g->out_off, out_off, size_t(N) + 1);
+  if (st == PRH_SUCCESS && out_off) st = up(&g->out_dst, out_dst, M);
+  if (st == PRH_SUCCESS) st = ensure(c, c->flags, 4);
+  if (st == PRH_SUCCESS) {
+    hipError_t e = prh::launch_fill_i32(c->flags.p, 0, 1, c->stream);
+    if (e == hipSuccess)
+      e = prh::launch_validate_graph(g->in_off, g->in_src, g->outdeg, g->out_off, g->out_dst, N, M, c->flags.p,
+                                     c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->pinned, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) st = fail(c, PRH_ERROR_DEVICE, "graph validation: %s", hipGetErrorString(e));
+    else {
+      const int bits = *reinterpret_cast<int *>(c->pinned);
+      if (bits)
+        st = fail(c, PRH_ERROR_INVALID_GRAPH, "invalid graph (bits 0x%x:%s%s%s%s%s%s)", bits,
+                  bits & 1 ? " in-offsets" : "", bits & 2 ? " dead-end" : "", bits & 4 ? " out-degree-mismatch" : "",
+                  bits & 8 ? " in-id-range" : "", bits & 16 ? " out-id-range" : "", bits & 32 ? " out-offsets" : "");
+    }
+  }
+  if (st != PRH_SUCCESS) {
+    prh_graph_destroy(g);
+    return st;
+  }
+  *out = g;
+  return PRH_SUCCESS;
+}
+
+void prh_graph_destroy(prh_graph *g) {
+  if (!g) return;
+  prh_context *c = g->ctx;
+  if (c && c->stream) (void)hipStreamSynchronize(c->stream);
+  const size_t N = g->N, M = g->M;
+  dev_free(c, g->in_off, (N + 1) * 4);
+  dev_free(c, g->in_src, (M ? M : 4) * 4);
+  dev_free(c, g->outdeg, N * 4);
+  dev_free(c, g->out_off, (N + 1) * 4);
+  dev_free(c, g->out_dst, (M ? M : 4) * 4);
+  delete g;
+}
+
+uint32_t prh_graph_order(const prh_graph *g) { return g ? g->N : 0; }
+uint64_t prh_graph_size(const prh_graph *g) { return g ? g->M : 0; }
+
+int prh_solve(prh_context *c, const prh_graph *g, prh_method method, int partition, const prh_options *o,
+              const uint32_t *du, const uint32_t *dv, size_t nd, const uint32_t *iu, size_t ni, const double *q,
+              double *ranks, prh_stats *st) {
+  (void)partition;
+  return solve_impl(c, g, method, o, du, dv, nd, iu, ni, q, ranks, st, true);
+}
+
+int prh_solve_device(prh_context *c, const prh_graph *g, prh_method method, int partition, const prh_options *o,
+                     const uint32_t *du, const uint32_t *dv, size_t nd, const uint32_t *iu, size_t ni,
+                     const double *q, double *ranks, prh_stats *st) {
+  (void)partition;
+  return solve_impl(c, g, method, o, du, dv, nd, iu, ni, q, ranks, st, false);
+}
+
+int prh_partition(prh_context *c, uint32_t N, const uint32_t *off, uint32_t d, uint32_t *order, uint32_t *np) {
+  if (!c) return PRH_ERROR_INVALID_ARGUMENT;
+  if (N == 0) {
+    if (np) *np = 0;
+    return PRH_SUCCESS;
+  }
+  if (!off || !order || !np) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null argument");
+  PRH_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  PRH_CHECK(ensure(c, c->scratch_u32, size_t(N) + 1));
+  PRH_CHECK(ensure(c, c->sched_in, N));
+  PRH_CHECK(ensure(c, c->tiles, size_t(3) * prh::partition_tiles(N)));
+  PRH_CHECK(ensure(c, c->counts, 8));
+  PRH_HIP(c, hipMemcpyAsync(c->scratch_u32.p, off, (size_t(N) + 1) * 4, hipMemcpyHostToDevice, s));
+  PRH_HIP(c, prh::launch_partition(c->scratch_u32.p, N, d, 0xFFFFFFFFu, c->sched_in.p, c->tiles.p, c->counts.p, s));
+  PRH_HIP(c, hipMemcpyAsync(order, c->sched_in.p, size_t(N) * 4, hipMemcpyDeviceToHost, s));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->counts.p, 4, hipMemcpyDeviceToHost, s));
+  PRH_HIP(c, hipStreamSynchronize(s));
+  *np = *reinterpret_cast<uint32_t *>(c->pinned);
+  return PRH_SUCCESS;
+}
+
+int prh_mark_affected(prh_context *c, const prh_graph *g, prh_method method, const uint32_t *du, const uint32_t *dv,
+                      size_t nd, const uint32_t *iu, size_t ni, uint8_t *affected) {
+  if (!c) return PRH_ERROR_INVALID_ARGUMENT;
+  if (!g || !affected || method < PRH_DYNAMIC_TRAVERSAL || method > PRH_PRUNE_DYNAMIC_FRONTIER)
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null argument or non-dynamic method");
+  if (!g->out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "marking needs the graph's out-CSR");
+  if (g->N == 0) return PRH_SUCCESS;
+  PRH_HIP(c, hipSetDevice(c->device));
+  PRH_CHECK(ensure_workspace(c, g->N, true, 1));
+  DevBatch b;
+  PRH_CHECK(stage_batch(c, du, dv, nd, iu, ni, true, b));
+  PRH_CHECK(check_batch(c, b, g->N));
+  prh::Schedule in_s{}, out_s{};
+  PRH_CHECK(partition_graph(c, g, true, in_s, out_s));
+  PRH_CHECK(mark_affected(c, g, method, b, out_s));
+  PRH_HIP(c, hipMemcpyAsync(affected, c->vaff.p, g->N, hipMemcpyDeviceToHost, c->stream));
+  PRH_HIP(c, hipStreamSynchronize(c->stream));
+  return PRH_SUCCESS;
+}
+
+int prh_kernel_timing_enable(prh_context *c, int enable) {
+  if (!c) return PRH_ERROR_INVALID_ARGUMENT;
+  c->ktiming = enable != 0;
+  return PRH_SUCCESS;
+}
+
+int prh_kernel_timing_read(prh_context *c, uint64_t *launches, double *total_ms) {
+  if (!c) return PRH_ERROR_INVALID_ARGUMENT;
+  if (launches) *launches = c->k_launches;
+  if (total_ms) *total_ms = c->k_ms;
+  c->k_launches = 0;
+  c->k_ms = 0.0;
+  return PRH_SUCCESS;
+}
+
+}  // extern "C"

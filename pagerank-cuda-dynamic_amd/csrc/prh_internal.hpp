@@ -1,0 +1,94 @@
+// prh_internal.hpp -- host-side declarations shared by the kernel translation units and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace prh {
+
+constexpr int kWave = 64;    // CDNA wavefront width
+constexpr int kBlock = 256;  // every kernel here runs 4 waves per workgroup
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+// Device view of a loop's convergence state (see LoopCtl in prh_device.hpp).
+struct LoopState {
+  unsigned long long *err;  // [capacity] per-iteration L-inf bits
+  int *done;                // single int
+};
+
+// Solver variant bits, as the reference's template switches <DYNAMIC, FRONTIER, PRUNE>
+// (pagerankCuda.hxx:87, :162, :675).
+enum Variant : int {
+  kPlain = 0,       // Static / Naive-dynamic: every vertex updated
+  kTraversal = 1,   // DYNAMIC: unaffected vertices copied through (Dynamic Traversal)
+  kFrontier = 2,    // DYNAMIC + FRONTIER (Dynamic Frontier)
+  kPrune = 3,       // DYNAMIC + FRONTIER + PRUNE (Dynamic Frontier with Pruning)
+};
+
+// Schedule of one CSR: vertex ids stably partitioned by degree into [low | high | huge] (see partition kernels).
+struct Schedule {
+  const uint32_t *order;  // [N]
+  uint32_t n_low, n_high, n_huge;
+};
+
+// Launch shape of the rank-update kernel: lanes per low-degree vertex and block counts of the three sections.
+struct RankShape {
+  int low_width;  // 1, 2, 4, 8, 16, 32 or 64 lanes per low-degree vertex
+  uint32_t nb_huge, nb_high, nb_low;
+};
+
+struct RankArgs {
+  double *a;          // new ranks
+  const double *r;    // old ranks
+  double *ca;         // new contributions a[v]/deg[v]
+  const double *cr;   // old contributions r[v]/deg[v]
+  uint8_t *naff;      // neighbour-affected flags (FRONTIER)
+  uint8_t *vaff;      // vertex-affected flags (DYNAMIC)
+  const uint32_t *off;  // in-CSR offsets [N+1]
+  const uint32_t *src;  // in-CSR sources [M]
+  const uint32_t *deg;  // out-degree [N]
+  Schedule sched;
+  double C0, P, D, C;  // teleport share (1-P)/N, damping, frontier and prune tolerances
+};
+
+struct ExtendArgs {
+  uint8_t *vaff;         // flags to set
+  const uint8_t *naff;   // vertices whose out-neighbours get marked
+  const uint32_t *off;   // out-CSR offsets [N+1]
+  const uint32_t *dst;   // out-CSR targets [M]
+  Schedule sched;        // out-degree partition
+  uint8_t *fresh;        // optional: also set for newly marked vertices (traversal levels)
+  int *changed;          // optional: set to 1 when a vertex is newly marked
+};
+
+// ---- partition.hip ----
+// Stable 3-way partition of [0,N) by degree d = off[v+1]-off[v]: d <= t1, t1 < d <= t2, d > t2.
+// tiles: scratch of 3 * partition_tiles(N) uint32; counts_dev: 3 uint32 outputs (device).
+uint32_t partition_tiles(uint32_t N);
+hipError_t launch_partition(const uint32_t *off, uint32_t N, uint32_t t1, uint32_t t2, uint32_t *order,
+                            uint32_t *tiles, uint32_t *counts_dev, hipStream_t s);
+
+// ---- rank.hip ----
+hipError_t launch_init_ranks(double *r, double *a, double *cr, double *ca, const uint32_t *deg, uint32_t N,
+                             bool from_r, hipStream_t s);
+hipError_t launch_rank_update(int variant, const RankArgs &A, const RankShape &shape, unsigned long long *err,
+                              int *done, int l, double tol, hipStream_t s);
+hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s);
+hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s);
+hipError_t launch_copy_f64(double *dst, const double *src, size_t n, hipStream_t s);
+hipError_t launch_validate_graph(const uint32_t *in_off, const uint32_t *in_src, const uint32_t *outdeg,
+                                 const uint32_t *out_off, const uint32_t *out_dst, uint32_t N, uint64_t M,
+                                 int *flags, hipStream_t s);
+hipError_t launch_validate_ids(const uint32_t *ids, size_t n, uint32_t N, int *flags, int bit, hipStream_t s);
+hipError_t launch_check_finite(const double *x, size_t n, int *flags, int bit, hipStream_t s);
+
+// ---- frontier.hip ----
+hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *delu, const uint32_t *delv,
+                               size_t ND, const uint32_t *insu, size_t NI, hipStream_t s);
+// Extension; when err/done are non-null the kernel honours the iteration control of iteration l.
+hipError_t launch_extend(const ExtendArgs &A, unsigned long long *err, int *done, int l, double tol,
+                         hipStream_t s);
+
+}  // namespace prh

@@ -1,0 +1,309 @@
+// rank.hip -- the pull-based, atomic-free rank update with the fused L-inf reduction, plus small map kernels.
+//
+// One launch per iteration covers all vertices, in three sections of workgroups over the degree schedule:
+//   huge  (in-degree > t2): one workgroup (4 waves) per vertex, cross-wave sum through LDS;
+//   high  (t1 < in-degree <= t2): one wave per vertex, DPP/swizzle wave sum;
+//   low   (in-degree <= t1): W lanes per vertex (W = 1..32), DPP group sum.
+// The huge section comes first so its long rows start early instead of forming the tail.
+//
+// Per vertex v (pagerankUpdateRanks{Thread,Block}CukU, pagerankCuda.hxx:87-109, :162-188):
+//   cv = sum_{u in in(v)} r[u]/deg[u]     -- gathered from the precomputed contributions cr[u] = r[u]/deg[u],
+//                                            the same FP64 value the reference divides per edge
+//   a[v] = C0 + P*cv                      -- or the DF-P closed form (C0 + P*(cv - r[v]/d))/(1 - P/d)
+//   prune / frontier flags                 -- vaff[v] = 0 if |a-r|/max(r,a) <= C; naff[v] = (|a-r|/max > D)
+// Each vertex has exactly one writer, so the update needs no atomics; the only atomic is one 64-bit max per
+// workgroup for the L-inf change of the iteration.
+#include "prh_device.hpp"
+#include "prh_internal.hpp"
+
+namespace prh {
+namespace {
+
+template <int VARIANT>
+struct Traits {
+  static constexpr bool dynamic = VARIANT >= kTraversal;
+  static constexpr bool frontier = VARIANT >= kFrontier;
+  static constexpr bool prune = VARIANT == kPrune;
+};
+
+template <int VARIANT>
+__device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv) {
+  using T = Traits<VARIANT>;
+  const uint32_t d = A.deg[v];
+  const double rv = A.r[v];
+  double av;
+  if constexpr (T::prune) av = fma(A.P, cv - A.cr[v], A.C0) * (1.0 / (1.0 - A.P / d));
+  else av = fma(A.P, cv, A.C0);
+  A.a[v] = av;
+  A.ca[v] = av / d;
+  const double ev = fabs(av - rv);
+  if constexpr (T::frontier) {
+    const double rel = ev / fmax(rv, av);
+    if (T::prune && rel <= A.C) A.vaff[v] = 0;
+    A.naff[v] = rel > A.D ? 1 : 0;
+  }
+  return ev;
+}
+
+template <int VARIANT>
+__device__ __forceinline__ void copy_vertex(const RankArgs &A, uint32_t v) {
+  A.a[v] = A.r[v];
+  A.ca[v] = A.cr[v];
+  if constexpr (Traits<VARIANT>::frontier) A.naff[v] = 0;
+}
+
+template <int VARIANT>
+__device__ __forceinline__ bool affected(const RankArgs &A, uint32_t v) {
+  if constexpr (Traits<VARIANT>::dynamic) return A.vaff[v] != 0;
+  else return true;
+}
+
+// Sum of cr[src[e]] for e = b + first, b + first + step, ... < end, four gathers in flight per lane.
+template <int STEP>
+__device__ __forceinline__ double gather_sum(const RankArgs &A, uint32_t e, uint32_t end) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (; e + 3 * STEP < end; e += 4 * STEP) {
+    const uint32_t u0 = A.src[e], u1 = A.src[e + STEP], u2 = A.src[e + 2 * STEP], u3 = A.src[e + 3 * STEP];
+    s0 += A.cr[u0];
+    s1 += A.cr[u1];
+    s2 += A.cr[u2];
+    s3 += A.cr[u3];
+  }
+  for (; e < end; e += STEP) s0 += A.cr[A.src[e]];
+  return (s0 + s1) + (s2 + s3);
+}
+
+template <int VARIANT>
+__device__ __forceinline__ void huge_section(const RankArgs &A, uint32_t blk, uint32_t nblk, double &emax,
+                                             double *lds) {
+  const uint32_t n = A.sched.n_huge;
+  const uint32_t *order = A.sched.order + A.sched.n_low + A.sched.n_high;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  for (uint32_t i = blk; i < n; i += nblk) {
+    const uint32_t v = order[i];
+    if (!affected<VARIANT>(A, v)) {
+      if (threadIdx.x == 0) copy_vertex<VARIANT>(A, v);
+      continue;
+    }
+    const uint32_t beg = A.off[v], end = A.off[v + 1];
+    double acc = gather_sum<kBlock>(A, beg + threadIdx.x, end);
+    acc = group_sum<kWave>(acc);
+    if (lane == 0) lds[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) emax = fmax(emax, finish_vertex<VARIANT>(A, v, (lds[0] + lds[1]) + (lds[2] + lds[3])));
+    __syncthreads();
+  }
+}
+
+template <int VARIANT>
+__device__ __forceinline__ void high_section(const RankArgs &A, uint32_t blk, uint32_t nblk, double &emax) {
+  const uint32_t n = A.sched.n_high;
+  const uint32_t *order = A.sched.order + A.sched.n_low;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  for (uint32_t i = blk * kWavesPerBlock + w; i < n; i += nblk * kWavesPerBlock) {
+    const uint32_t v = order[i];
+    if (!affected<VARIANT>(A, v)) {
+      if (lane == 0) copy_vertex<VARIANT>(A, v);
+      continue;
+    }
+    const uint32_t beg = A.off[v], end = A.off[v + 1];
+    double acc = gather_sum<kWave>(A, beg + lane, end);
+    acc = group_sum<kWave>(acc);
+    if (lane == 0) emax = fmax(emax, finish_vertex<VARIANT>(A, v, acc));
+  }
+}
+
+template <int VARIANT, int W>
+__device__ __forceinline__ void low_section(const RankArgs &A, uint32_t blk, uint32_t nblk, double &emax) {
+  constexpr int G = kWave / W;  // vertices per wave per round
+  const uint32_t n = A.sched.n_low;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int g = lane / W, j = lane % W;
+  const uint32_t stride = nblk * kWavesPerBlock * G;
+  for (uint32_t i0 = (blk * kWavesPerBlock + w) * G; i0 < n; i0 += stride) {  // wave-uniform loop
+    const uint32_t i = i0 + g;
+    const bool valid = i < n;
+    uint32_t v = 0, beg = 0, end = 0;
+    bool upd = false;
+    if (valid) {
+      v = A.sched.order[i];
+      beg = A.off[v];
+      end = A.off[v + 1];
+      upd = affected<VARIANT>(A, v);
+    }
+    double acc = 0.0;
+    if (upd) acc = gather_sum<W>(A, beg + j, end);
+    acc = group_sum<W>(acc);  // all lanes active again here
+    if (valid && j == 0) {
+      if (upd) emax = fmax(emax, finish_vertex<VARIANT>(A, v, acc));
+      else copy_vertex<VARIANT>(A, v);
+    }
+  }
+}
+
+template <int VARIANT, int W>
+__global__ __launch_bounds__(kBlock) void rank_update_kernel(RankArgs A, RankShape S, LoopCtl ctl) {
+  if (!iteration_begin(ctl)) return;
+  __shared__ double lds_sum[kWavesPerBlock];
+  __shared__ double lds_max[kWavesPerBlock];
+  double emax = 0.0;
+  const uint32_t b = blockIdx.x;
+  if (b < S.nb_huge) huge_section<VARIANT>(A, b, S.nb_huge, emax, lds_sum);
+  else if (b < S.nb_huge + S.nb_high) high_section<VARIANT>(A, b - S.nb_huge, S.nb_high, emax);
+  else low_section<VARIANT, W>(A, b - S.nb_huge - S.nb_high, S.nb_low, emax);
+  emax = block_max(emax, lds_max);
+  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[ctl.l], emax);
+}
+
+template <int VARIANT>
+hipError_t launch_rank_variant(const RankArgs &A, const RankShape &S, const LoopCtl &ctl, hipStream_t s) {
+  const uint32_t grid = S.nb_huge + S.nb_high + S.nb_low;
+  if (grid == 0) return hipSuccess;
+  switch (S.low_width) {
+    case 1: rank_update_kernel<VARIANT, 1><<<grid, kBlock, 0, s>>>(A, S, ctl); break;
+    case 2: rank_update_kernel<VARIANT, 2><<<grid, kBlock, 0, s>>>(A, S, ctl); break;
+    case 4: rank_update_kernel<VARIANT, 4><<<grid, kBlock, 0, s>>>(A, S, ctl); break;
+    case 8: rank_update_kernel<VARIANT, 8><<<grid, kBlock, 0, s>>>(A, S, ctl); break;
+    case 16: rank_update_kernel<VARIANT, 16><<<grid, kBlock, 0, s>>>(A, S, ctl); break;
+    case 32: rank_update_kernel<VARIANT, 32><<<grid, kBlock, 0, s>>>(A, S, ctl); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// r = a = 1/N (or keep r as loaded and copy it to a); contributions cr = ca = r/deg.
+// pagerankInitializeRanksCukW (pagerankCuda.hxx:232-239) and the warm-start copy (:817).
+__global__ __launch_bounds__(kBlock) void init_ranks_kernel(double *r, double *a, double *cr, double *ca,
+                                                            const uint32_t *deg, uint32_t N, int from_r) {
+  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < N; v += gridDim.x * kBlock) {
+    const double rv = from_r ? r[v] : 1.0 / N;
+    const double c = rv / deg[v];
+    if (!from_r) r[v] = rv;
+    a[v] = rv;
+    cr[v] = c;
+    ca[v] = c;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void fill_kernel(T *p, T v, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void copy_f64_kernel(double *dst, const double *src, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock) dst[i] = src[i];
+}
+
+// Graph checks; flag bits: 1 offsets not monotone / not 0..M, 2 out-degree 0 (dead end), 4 out-degree disagrees
+// with the out-CSR, 8 in-CSR id out of range, 16 out-CSR id out of range, 32 out-CSR offsets not 0..M.
+__global__ __launch_bounds__(kBlock) void validate_graph_kernel(const uint32_t *in_off, const uint32_t *in_src,
+                                                                const uint32_t *outdeg, const uint32_t *out_off,
+                                                                const uint32_t *out_dst, uint32_t N, uint64_t M,
+                                                                int *flags) {
+  int bits = 0;
+  const size_t tid = size_t(blockIdx.x) * kBlock + threadIdx.x, stride = size_t(gridDim.x) * kBlock;
+  if (tid == 0) {
+    if (in_off[0] != 0 || in_off[N] != M) bits |= 1;
+    if (out_off && (out_off[0] != 0 || out_off[N] != M)) bits |= 32;
+  }
+  for (size_t v = tid; v < N; v += stride) {
+    if (in_off[v + 1] < in_off[v]) bits |= 1;
+    if (outdeg[v] == 0) bits |= 2;
+    if (out_off) {
+      if (out_off[v + 1] < out_off[v]) bits |= 32;
+      else if (out_off[v + 1] - out_off[v] != outdeg[v]) bits |= 4;
+    }
+  }
+  for (size_t e = tid; e < M; e += stride) {
+    if (in_src[e] >= N) bits |= 8;
+    if (out_dst && out_dst[e] >= N) bits |= 16;
+  }
+  if (bits) atomicOr(flags, bits);
+}
+
+__global__ __launch_bounds__(kBlock) void validate_ids_kernel(const uint32_t *ids, size_t n, uint32_t N, int *flags,
+                                                              int bit) {
+  bool bad = false;
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock)
+    bad |= ids[i] >= N;
+  if (bad) atomicOr(flags, bit);
+}
+
+__global__ __launch_bounds__(kBlock) void check_finite_kernel(const double *x, size_t n, int *flags, int bit) {
+  bool bad = false;
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock)
+    bad |= !isfinite(x[i]);
+  if (bad) atomicOr(flags, bit);
+}
+
+inline uint32_t map_grid(size_t n) {
+  const size_t g = (n + kBlock - 1) / kBlock;
+  return uint32_t(g < 1 ? 1 : (g > 8192 ? 8192 : g));
+}
+
+}  // namespace
+
+hipError_t launch_rank_update(int variant, const RankArgs &A, const RankShape &S, unsigned long long *err, int *done,
+                              int l, double tol, hipStream_t s) {
+  const LoopCtl ctl{err, done, l, tol};
+  switch (variant) {
+    case kPlain: return launch_rank_variant<kPlain>(A, S, ctl, s);
+    case kTraversal: return launch_rank_variant<kTraversal>(A, S, ctl, s);
+    case kFrontier: return launch_rank_variant<kFrontier>(A, S, ctl, s);
+    case kPrune: return launch_rank_variant<kPrune>(A, S, ctl, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_init_ranks(double *r, double *a, double *cr, double *ca, const uint32_t *deg, uint32_t N,
+                             bool from_r, hipStream_t s) {
+  if (N == 0) return hipSuccess;
+  init_ranks_kernel<<<map_grid(N), kBlock, 0, s>>>(r, a, cr, ca, deg, N, from_r ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  fill_kernel<uint8_t><<<map_grid(n), kBlock, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  fill_kernel<unsigned long long><<<map_grid(n), kBlock, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  fill_kernel<int><<<map_grid(n), kBlock, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_f64(double *dst, const double *src, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  copy_f64_kernel<<<map_grid(n), kBlock, 0, s>>>(dst, src, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_validate_graph(const uint32_t *in_off, const uint32_t *in_src, const uint32_t *outdeg,
+                                 const uint32_t *out_off, const uint32_t *out_dst, uint32_t N, uint64_t M,
+                                 int *flags, hipStream_t s) {
+  validate_graph_kernel<<<map_grid(M > N ? M : N), kBlock, 0, s>>>(in_off, in_src, outdeg, out_off, out_dst, N, M,
+                                                                   flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_validate_ids(const uint32_t *ids, size_t n, uint32_t N, int *flags, int bit, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  validate_ids_kernel<<<map_grid(n), kBlock, 0, s>>>(ids, n, N, flags, bit);
+  return hipGetLastError();
+}
+
+hipError_t launch_check_finite(const double *x, size_t n, int *flags, int bit, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  check_finite_kernel<<<map_grid(n), kBlock, 0, s>>>(x, n, flags, bit);
+  return hipGetLastError();
+}
+
+}  // namespace prh

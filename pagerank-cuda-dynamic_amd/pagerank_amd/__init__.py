@@ -1,0 +1,17 @@
+"""MI355X-native PageRank engine (static, naive-dynamic, dynamic traversal, dynamic frontier, DF with pruning).
+
+The compute path is libpagerank_hip.so (hand-written HIP for gfx950 behind the C ABI of include/pagerank_hip.h);
+this package is its Python mirror plus host-side graph preparation.
+"""
+from .engine import (DYNAMIC_FRONTIER, DYNAMIC_TRAVERSAL, METHOD_NAMES, NAIVE_DYNAMIC, PRUNE_DYNAMIC_FRONTIER,
+                     STATIC, Context, DeviceGraph, PagerankError, PagerankOptions, PagerankResult, Tuning,
+                     pagerankDynamicFrontierHip, pagerankDynamicTraversalHip, pagerankNaiveDynamicHip,
+                     pagerankPruneDynamicFrontierHip, pagerankStaticHip, solve, solve_device)
+from .graphs import Batch, HostGraph, apply_batch
+
+__all__ = [
+    "STATIC", "NAIVE_DYNAMIC", "DYNAMIC_TRAVERSAL", "DYNAMIC_FRONTIER", "PRUNE_DYNAMIC_FRONTIER", "METHOD_NAMES",
+    "Context", "DeviceGraph", "PagerankError", "PagerankOptions", "PagerankResult", "Tuning", "solve",
+    "solve_device", "pagerankStaticHip", "pagerankNaiveDynamicHip", "pagerankDynamicTraversalHip",
+    "pagerankDynamicFrontierHip", "pagerankPruneDynamicFrontierHip", "Batch", "HostGraph", "apply_batch",
+]

@@ -1,0 +1,171 @@
+"""GPU parity of the five solvers, the degree partition and the affected marking, through the C ABI.
+
+Bars (stated per assertion):
+  * ranks vs the oracle's restatement of the reference GPU path: L-inf <= 1e-9 (FP64; only summation order differs);
+  * iterations vs the oracle: equal or +-1 (the L-inf change can land within rounding of the tolerance);
+  * Static / ND / DT / DF vs the compensated FP64 power iteration (the exact fixed point): L-inf <= 1e-9;
+  * DF-P vs the power iteration: the pruning method is approximate by design (vertices whose relative change
+    is <= 1e-6 stop updating), so its own bound is 1e-5 relative to the largest rank;
+  * degree partition and affected flags: bit-exact.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import complete_graph, cycle_graph, star_graph, torus_graph
+
+pytestmark = pytest.mark.gpu
+
+RANK_TOL = 1e-9
+
+
+def _graphs():
+    from pagerank_amd import HostGraph
+    return {
+        "rmat12": lambda: HostGraph.rmat(12, 16, seed=7),
+        "rmat14": lambda: HostGraph.rmat(14, 16, seed=11),
+        "grid60x50": lambda: HostGraph.grid(60, 50, 0.62, seed=5),
+        "star5000": lambda: star_graph(5000),
+    }
+
+
+GRAPH_NAMES = ["rmat12", "rmat14", "grid60x50", "star5000"]
+
+
+@pytest.fixture(scope="module")
+def graphs():
+    return {k: f() for k, f in _graphs().items()}
+
+
+def _close_iters(a: int, b: int) -> bool:
+    return abs(a - b) <= 1
+
+
+@pytest.mark.parametrize("name", GRAPH_NAMES)
+def test_static_matches_oracle_and_fixed_point(ctx, oracle_mod, graphs, name):
+    from pagerank_amd import pagerankStaticHip
+    y = graphs[name]
+    print(f"[static] graph={name} N={y.N} M={y.M}", flush=True)
+    yg = ctx.upload(y)
+    res = pagerankStaticHip(yg)
+    ref, it, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, y)
+    exact, _ = oracle_mod.power_iteration(y)
+    assert _close_iters(res.iterations, it), (res.iterations, it)
+    assert np.abs(res.ranks - ref).max() <= RANK_TOL
+    assert np.abs(res.ranks - exact).max() <= RANK_TOL
+    assert abs(res.ranks.sum() - 1.0) < 1e-9  # no dead ends: ranks sum to 1
+    assert res.ranks.min() >= 0.15 / y.N * (1 - 1e-9)  # r_v >= (1-alpha)/N
+    assert res.computationTime > 0 and res.time >= res.computationTime
+
+
+@pytest.mark.parametrize("make", [lambda: cycle_graph(1000), lambda: complete_graph(70), lambda: torus_graph(40, 30)],
+                         ids=["cycle", "complete", "torus"])
+def test_vertex_transitive_closed_form(ctx, make):
+    """Vertex-transitive graphs with self-loops have PageRank exactly 1/N; from r0 = 1/N one iteration suffices."""
+    from pagerank_amd import pagerankStaticHip
+    y = make()
+    res = pagerankStaticHip(ctx.upload(y))
+    assert res.iterations == 1
+    assert np.abs(res.ranks - 1.0 / y.N).max() <= 1e-15
+
+
+def test_dense_solve_small(ctx):
+    """(I - alpha A) r = (1-alpha)/N 1, solved densely (numpy.linalg.solve), pins the fixed point."""
+    from pagerank_amd import HostGraph, PagerankOptions, pagerankStaticHip
+    y = HostGraph.rmat(8, 8, seed=3)
+    A = y.dense_transition()
+    exact = np.linalg.solve(np.eye(y.N) - 0.85 * A, np.full(y.N, 0.15 / y.N))
+    res = pagerankStaticHip(ctx.upload(y), PagerankOptions(tolerance=1e-14))
+    assert np.abs(res.ranks - exact).max() <= 1e-12
+
+
+@pytest.mark.parametrize("frac", [1e-3, 1e-2])
+@pytest.mark.parametrize("method", [1, 2, 3, 4], ids=["ND", "DT", "DF", "DFP"])
+def test_dynamic_matches_oracle(ctx, oracle_mod, method, frac):
+    from pagerank_amd import Batch, HostGraph, apply_batch, solve
+    x = HostGraph.rmat(13, 16, seed=21)
+    q, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, x)
+    b = Batch.random(x, max(1, int(frac * x.M)), 0.8, seed=9)
+    y = apply_batch(x, b)
+    print(f"[dynamic] method={method} frac={frac} N={y.N} M={y.M} del={b.deletions} ins={b.insertions}", flush=True)
+    res = solve(ctx.upload(y), method, batch=b, q=q)
+    ref, it, _ = oracle_mod.pagerank_cuda(method, y, b, q)
+    exact, _ = oracle_mod.power_iteration(y)
+    assert _close_iters(res.iterations, it), (res.iterations, it)
+    assert np.abs(res.ranks - ref).max() <= RANK_TOL
+    if method == 4:
+        # Pruned vertices stop updating while their neighbours still move, so DF-P stops short of the exact
+        # fixed point (about 3.5e-6 of the largest rank on this graph, in the oracle and on the GPU alike).
+        assert np.abs(res.ranks - exact).max() <= 1e-5 * exact.max()
+        assert np.abs(ref - exact).max() <= 1e-5 * exact.max()
+    else:
+        assert np.abs(res.ranks - exact).max() <= RANK_TOL
+
+
+@pytest.mark.parametrize("method", [2, 3, 4], ids=["DT", "DF", "DFP"])
+def test_empty_batch_returns_q(ctx, oracle_mod, method):
+    """With nothing affected, one iteration copies q through (L-inf change 0 < tolerance)."""
+    from pagerank_amd import Batch, HostGraph, solve
+    y = HostGraph.rmat(10, 16, seed=2)
+    q, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, y)
+    res = solve(ctx.upload(y), method, batch=Batch.empty(), q=q)
+    assert res.iterations == 1
+    assert np.array_equal(res.ranks, q)
+
+
+def test_warm_start_from_solution_converges_in_one(ctx, oracle_mod):
+    from pagerank_amd import HostGraph, PagerankOptions, pagerankNaiveDynamicHip
+    y = HostGraph.rmat(11, 16, seed=4)
+    exact, _ = oracle_mod.power_iteration(y)
+    res = pagerankNaiveDynamicHip(ctx.upload(y), exact, PagerankOptions())
+    assert res.iterations == 1
+
+
+def test_resume_is_bit_exact(ctx):
+    """Static with k iterations, then ND from that result for the rest, equals the uninterrupted run bit for bit
+    (a Jacobi step is a pure function of the rank vector)."""
+    from pagerank_amd import HostGraph, PagerankOptions, pagerankNaiveDynamicHip, pagerankStaticHip
+    y = HostGraph.rmat(12, 16, seed=8)
+    yg = ctx.upload(y)
+    full = pagerankStaticHip(yg, PagerankOptions(tolerance=0.0, maxIterations=40))
+    part = pagerankStaticHip(yg, PagerankOptions(tolerance=0.0, maxIterations=15))
+    rest = pagerankNaiveDynamicHip(yg, part.ranks, PagerankOptions(tolerance=0.0, maxIterations=25))
+    assert full.iterations == 40 and part.iterations == 15 and rest.iterations == 25
+    assert np.array_equal(full.ranks, rest.ranks)
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 2047, 2048, 2049, 100_003])
+def test_partition_bit_exact(ctx, oracle_mod, n):
+    rng = np.random.default_rng(n)
+    deg = rng.choice([0, 1, 63, 64, 65, 200, 5000], size=n, p=[.1, .3, .1, .1, .1, .2, .1]).astype(np.int64)
+    off = np.zeros(n + 1, dtype=np.uint32)
+    np.cumsum(deg, out=off[1:])
+    order, npart = ctx.partition(off, 64)
+    ref_order, ref_np = oracle_mod.partition(off, 64)
+    assert npart == ref_np
+    assert np.array_equal(order, ref_order)
+
+
+@pytest.mark.parametrize("method", [2, 3], ids=["DT", "DF"])
+@pytest.mark.parametrize("frac", [1e-4, 1e-3, 5e-2])
+def test_affected_marking_bit_exact(ctx, oracle_mod, method, frac):
+    from pagerank_amd import Batch, HostGraph, apply_batch
+    x = HostGraph.rmat(13, 8, seed=33)
+    b = Batch.random(x, max(1, int(frac * x.M)), 0.8, seed=17)
+    y = apply_batch(x, b)
+    got = ctx.mark_affected(ctx.upload(y), method, b)
+    want = oracle_mod.mark(method, y, b)
+    assert np.array_equal(got, want)
+    if oracle_mod.ref_available():  # the reference's CPU marking gives the same set
+        assert np.array_equal(got, oracle_mod.ref_affected(0 if method == 3 else 1, x, b))
+
+
+def test_grid_traversal_long_chain(ctx, oracle_mod):
+    """A grid has a large diameter: the traversal closure needs many breadth-first levels."""
+    from pagerank_amd import Batch, HostGraph, apply_batch
+    x = HostGraph.grid(40, 40, 0.9, seed=1)
+    b = Batch.tidy(x, [[0, 1]], [[5, 700]])
+    y = apply_batch(x, b)
+    got = ctx.mark_affected(ctx.upload(y), 2, b)
+    assert np.array_equal(got, oracle_mod.mark(2, y, b))

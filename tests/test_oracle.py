@@ -1,0 +1,203 @@
+"""Pins of the oracle (CPU, no GPU needed).
+
+The oracle (oracle/prh_oracle.c) is checked against
+  * the reference's own CPU path, through committed fixtures (tests/golden/*.npz, written by make_golden.py from
+    oracle/_ref) and, where oracle/_ref is built, live on fresh graphs;
+  * closed forms: vertex-transitive graphs with self-loops have PageRank exactly 1/N;
+  * a library routine: the dense solve (I - alpha A) r = (1-alpha)/N 1 (numpy.linalg.solve);
+  * invariants: ranks sum to 1 with no dead ends, r_v >= (1-alpha)/N;
+  * brute force: the stable degree partition against a direct Python statement, the affected sets against a
+    Python breadth-first search.
+The last test plants plausible mistakes in the oracle's inputs (a transposed operand, a wrong damping term, an
+off-by-one source index) and shows the pins reject them.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import complete_graph, cycle_graph, star_graph, torus_graph
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+FIXTURES = sorted(p.stem for p in GOLDEN.glob("*.npz"))
+
+
+def _load(name):
+    from pagerank_amd import Batch, HostGraph
+    z = np.load(GOLDEN / f"{name}.npz")  # data only (allow_pickle stays False)
+    x = HostGraph.from_csr(z["x_off"], z["x_edg"])
+    b = Batch(z["del_u"], z["del_v"], z["ins_u"], z["ins_v"])
+    return z, x, b
+
+
+def test_fixtures_present():
+    assert set(FIXTURES) >= {"rmat8", "rmat10", "grid24"}
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_batch_application_matches_reference(name):
+    from pagerank_amd import apply_batch
+    z, x, b = _load(name)
+    y = apply_batch(x, b)
+    assert np.array_equal(y.out_off, z["y_off"]) and np.array_equal(y.out_edg, z["y_edg"])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_static_port_bit_exact_with_reference(oracle_mod, name):
+    """The OpenMP port (the CPU baseline) reproduces pagerankStaticOmp bit for bit."""
+    from pagerank_amd import apply_batch
+    z, x, b = _load(name)
+    y = apply_batch(x, b)
+    r, it, _ = oracle_mod.static_omp(y)
+    assert it == int(z["static_iterations"])
+    assert np.array_equal(r, z["static_ranks"])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_cuda_restatement_static_nd(oracle_mod, name):
+    """The GPU-path restatement differs from the CPU path only by fused multiply-adds and the 256-way block sum."""
+    from pagerank_amd import apply_batch
+    z, x, b = _load(name)
+    y = apply_batch(x, b)
+    r, it, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, y)
+    assert abs(it - int(z["static_iterations"])) <= 1
+    assert np.abs(r - z["static_ranks"]).max() <= 1e-15
+    r, it, _ = oracle_mod.pagerank_cuda(oracle_mod.NAIVE_DYNAMIC, y, b, z["q"])
+    assert abs(it - int(z["ND_iterations"])) <= 1
+    assert np.abs(r - z["ND_ranks"]).max() <= 1e-15
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_cuda_restatement_dynamic_traversal(oracle_mod, name):
+    """DT updates a fixed affected set synchronously, as the CPU path does: same iterations, same ranks."""
+    from pagerank_amd import apply_batch
+    z, x, b = _load(name)
+    y = apply_batch(x, b)
+    r, it, vaff = oracle_mod.pagerank_cuda(oracle_mod.DYNAMIC_TRAVERSAL, y, b, z["q"])
+    assert np.array_equal(vaff, z["affected_dt"])
+    assert abs(it - int(z["DT_iterations"])) <= 1
+    assert np.abs(r - z["DT_ranks"]).max() <= 1e-15
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_affected_sets_match_reference(oracle_mod, name):
+    from pagerank_amd import apply_batch
+    z, x, b = _load(name)
+    y = apply_batch(x, b)
+    assert np.array_equal(oracle_mod.mark(oracle_mod.DYNAMIC_FRONTIER, y, b), z["affected_df"])
+    assert np.array_equal(oracle_mod.mark(oracle_mod.DYNAMIC_TRAVERSAL, y, b), z["affected_dt"])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_frontier_methods_reach_reference_solution(oracle_mod, name):
+    """The CPU DF / DF-P mark neighbours during the sweep and the GPU path after it, so their iterates differ; both
+    must land within the tolerance-level distance of each other and of the exact fixed point."""
+    from pagerank_amd import apply_batch
+    z, x, b = _load(name)
+    y = apply_batch(x, b)
+    exact, _ = oracle_mod.power_iteration(y)
+    r, _, _ = oracle_mod.pagerank_cuda(oracle_mod.DYNAMIC_FRONTIER, y, b, z["q"])
+    assert np.abs(r - exact).max() <= 1e-9
+    assert np.abs(z["DF_ranks"] - exact).max() <= 1e-9
+    r, _, _ = oracle_mod.pagerank_cuda(oracle_mod.PRUNE_DYNAMIC_FRONTIER, y, b, z["q"])
+    assert np.abs(r - exact).max() <= 1e-5 * exact.max()
+    assert np.abs(z["DFP_ranks"] - exact).max() <= 1e-5 * exact.max()
+
+
+@pytest.mark.parametrize("make", [lambda: cycle_graph(257), lambda: complete_graph(33), lambda: torus_graph(9, 7),
+                                  lambda: star_graph(300)], ids=["cycle", "complete", "torus", "star"])
+def test_closed_forms_and_invariants(oracle_mod, make):
+    y = make()
+    r, it, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, y)
+    assert abs(r.sum() - 1) < 1e-12 and r.min() >= 0.15 / y.N * (1 - 1e-12)
+    if y.N != 300:  # vertex-transitive: uniform up to the rounding of the N-term sum, one iteration
+        assert it == 1 and np.abs(r * y.N - 1.0).max() <= 1e-14
+    else:  # star: hub rank from the 2x2 reduced system of the symmetric star
+        n = y.N
+        # leaves: l = (1-a)/n + a*(l/2 + h/n) with hub out-degree n; hub: h = (1-a)/n + a*(h/n + (n-1) l/2)
+        a = 0.85
+        A = np.array([[1 - a / 2, -a / n], [-a * (n - 1) / 2, 1 - a / n]])
+        l, h = np.linalg.solve(A, np.full(2, (1 - a) / n))
+        assert abs(r[0] - h) <= 1e-9 and np.abs(r[1:] - l).max() <= 1e-9
+
+
+def test_dense_solve(oracle_mod):
+    from pagerank_amd import HostGraph
+    y = HostGraph.rmat(9, 8, seed=5)
+    exact = np.linalg.solve(np.eye(y.N) - 0.85 * y.dense_transition(), np.full(y.N, 0.15 / y.N))
+    p, _ = oracle_mod.power_iteration(y)
+    assert np.abs(p - exact).max() <= 1e-15
+    r, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, y, tolerance=1e-15)
+    assert np.abs(r - exact).max() <= 1e-14
+
+
+@pytest.mark.parametrize("n", [1, 5, 64, 3000])
+def test_partition_brute_force(oracle_mod, n):
+    rng = np.random.default_rng(n)
+    deg = rng.integers(0, 130, size=n)
+    off = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint32)
+    order, npart = oracle_mod.partition(off, 64)
+    want = [v for v in range(n) if deg[v] <= 64] + [v for v in range(n) if deg[v] > 64]
+    assert npart == int((deg <= 64).sum()) and order.tolist() == want
+
+
+def test_traversal_brute_force(oracle_mod):
+    from pagerank_amd import Batch, HostGraph, apply_batch
+    x = HostGraph.rmat(9, 2, seed=13)
+    b = Batch.random(x, 6, 0.5, seed=2)
+    y = apply_batch(x, b)
+    got = oracle_mod.mark(oracle_mod.DYNAMIC_TRAVERSAL, y, b)
+    seen = np.zeros(y.N, bool)
+    stack = [int(v) for v in b.del_v] + [int(w) for u in np.concatenate([b.del_u, b.ins_u])
+                                         for w in y.out_edg[y.out_off[u]:y.out_off[u + 1]]]
+    while stack:
+        v = stack.pop()
+        if seen[v]:
+            continue
+        seen[v] = True
+        stack.extend(int(w) for w in y.out_edg[y.out_off[v]:y.out_off[v + 1]])
+    assert np.array_equal(got.astype(bool), seen)
+
+
+def test_live_reference_agrees(oracle_mod):
+    """Where oracle/_ref is built (this container), also check a fresh graph against it directly."""
+    if not oracle_mod.ref_available():
+        pytest.skip("oracle/_ref not built here (the reference sources are only present in the build container)")
+    from pagerank_amd import Batch, HostGraph, apply_batch
+    x = HostGraph.rmat(11, 16, seed=99)
+    b = Batch.random(x, 300, 0.8, seed=4)
+    y = apply_batch(x, b)
+    rr, itr, _ = oracle_mod.ref_pagerank(0, x, b)
+    r, it, _ = oracle_mod.static_omp(y)
+    assert it == itr and np.array_equal(r, rr)
+    assert np.array_equal(oracle_mod.mark(2, y, b), oracle_mod.ref_affected(1, x, b))
+
+
+def test_pins_reject_planted_mistakes(oracle_mod):
+    """Each planted mistake in the oracle's inputs must fail at least one pin."""
+    from pagerank_amd import HostGraph
+    y = HostGraph.rmat(8, 8, seed=21)
+    exact = np.linalg.solve(np.eye(y.N) - 0.85 * y.dense_transition(), np.full(y.N, 0.15 / y.N))
+
+    def pin(g):
+        r, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, g, tolerance=1e-14)
+        return np.abs(r - exact).max() <= 1e-12
+
+    assert pin(y)
+    # transposed operand: pull over out-neighbours instead of in-neighbours
+    t = HostGraph(arrays=dict(out_off=y.out_off, out_edg=y.out_edg, in_off=y.out_off, in_edg=y.out_edg,
+                              in_eid=y.in_eid, outdeg=y.outdeg))
+    assert not pin(t)
+    # off-by-one source index
+    s = HostGraph(arrays=dict(out_off=y.out_off, out_edg=y.out_edg, in_off=y.in_off,
+                              in_edg=(y.in_edg.astype(np.int64) + 1) % y.N, in_eid=y.in_eid, outdeg=y.outdeg))
+    assert not pin(s)
+    # wrong per-source divisor (in-degree instead of out-degree)
+    d = HostGraph(arrays=dict(out_off=y.out_off, out_edg=y.out_edg, in_off=y.in_off, in_edg=y.in_edg,
+                              in_eid=y.in_eid, outdeg=np.diff(y.in_off.astype(np.int64))))
+    assert not pin(d)
+    # dropped teleport term / wrong damping
+    r, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, y, damping=0.86, tolerance=1e-14)
+    assert np.abs(r - exact).max() > 1e-9
