@@ -143,6 +143,17 @@ int prh_mark_affected(prh_context *context, const prh_graph *graph, prh_method m
                       const uint32_t *deletion_sources, const uint32_t *deletion_targets, size_t n_deletions,
                       const uint32_t *insertion_sources, size_t n_insertions, uint8_t *affected);
 
+/* Batched personalized PageRank (no counterpart in the reference): k <= 64 teleport columns solved in one
+ * multi-column pull, each column j with teleport s_j uniform on its seed set
+ * seeds[seed_offsets[j] .. seed_offsets[j+1]) (host arrays; a repeated seed adds its share again). Column j
+ * iterates r_j <- (1-damping) s_j + damping * P^T r_j from r_j = s_j and stops at the first iteration whose L-inf
+ * change is below options->tolerance, as a single solve would (pagerankLoopCuU's rule, pagerankCuda.hxx:680-690).
+ * ranks receives N*k values, row-major by vertex (ranks[v*k + j]); iterations receives k counts. Iterations run as
+ * a HIP graph of check_interval iterations replayed until every column has converged. */
+int prh_ppr_batched(prh_context *context, const prh_graph *graph, int k, const uint32_t *seed_offsets,
+                    const uint32_t *seeds, const prh_options *options, double *ranks, int *iterations,
+                    prh_stats *stats);
+
 /* Per-launch timing of the rank-update kernel with HIP events on the context's stream (no counterpart in the
  * reference). While enabled, every solve accumulates the summed duration and count of the rank-update
  * launches that did work; reading resets the accumulators. */

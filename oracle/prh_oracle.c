@@ -161,7 +161,8 @@ int orc_pagerank_cuda(int mode, uint32_t N, const uint32_t *xtoff, const uint32_
   if (dynamic && (!xoff || !xedg)) return -1;
   double *a = (double *)malloc(sizeof(double) * N), *r = (double *)malloc(sizeof(double) * N);
   uint8_t *vaff = (uint8_t *)calloc(N, 1), *naff = (uint8_t *)calloc(N, 1);
-  /* Initial ranks (pagerankCuda.hxx:816-819): a = r = q, or 1/N. */
+  /* Initial ranks (pagerankCuda.hxx:816-819): a = r = q, or 1/N; the static solver has no q (:872). */
+  if (mode == ORC_STATIC) q = NULL;
   for (uint32_t v = 0; v < N; ++v) r[v] = a[v] = q ? q[v] : 1.0 / N;
   /* Marking (pagerankCuda.hxx:821, the fm lambdas :926-928, :955-957, :984-986). With an empty batch the
    * reference leaves vaff uninitialised; here it is all-zero (nothing affected). */

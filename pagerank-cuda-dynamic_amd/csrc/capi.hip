@@ -20,6 +20,8 @@
 
 namespace {
 
+constexpr size_t kPinnedBytes = 4096;
+
 template <class T>
 struct DevBuf {
   T *p = nullptr;
@@ -39,7 +41,13 @@ struct prh_context {
   DevBuf<uint32_t> sched_in, sched_out, tiles, counts, batch, scratch_u32;
   DevBuf<unsigned long long> loop;  // [L] error bits, then one slot holding `done`
   DevBuf<int> flags;                // [0] validation bits, [1] traversal "changed"
-  uint64_t *pinned = nullptr;       // small host-pinned readback area
+  // batched personalized PageRank
+  DevBuf<double> ppr_rank[2], ppr_contrib[2], ppr_partial, ppr_mass, ppr_out;
+  DevBuf<int32_t> ppr_row;
+  DevBuf<uint32_t> ppr_tables;
+  DevBuf<unsigned long long> ppr_err;
+  DevBuf<prh::PprState> ppr_state;
+  uint64_t *pinned = nullptr;       // small host-pinned readback area (kPinnedBytes)
   hipEvent_t ev[4] = {};
   bool ktiming = false;
   std::vector<hipEvent_t> kev;  // start/stop pairs per iteration
@@ -381,9 +389,192 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   return PRH_SUCCESS;
 }
 
+// Capture `body` (launches on the context stream only) into an executable graph.
+template <class F>
+int capture_graph(prh_context *c, F body, hipGraphExec_t *exec) {
+  hipGraph_t graph = nullptr;
+  PRH_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int st = body();
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (st != PRH_SUCCESS || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st != PRH_SUCCESS ? st : fail(c, PRH_ERROR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  }
+  const hipError_t ie = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) return fail(c, PRH_ERROR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+  return PRH_SUCCESS;
+}
+
+// Edge chunk of a heavy vertex for the batched solver.
+constexpr uint32_t kPprChunk = 1024;
+
+int ppr_impl(prh_context *c, const prh_graph *g, int k, const uint32_t *so, const uint32_t *seeds,
+             const prh_options *o, double *ranks, int *iterations, prh_stats *st) {
+  if (!c) return PRH_ERROR_INVALID_ARGUMENT;
+  if (!g || !o || !so || !ranks || !iterations || k < 1 || k > prh::kPprCols)
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null argument or k outside 1..64");
+  if (g->ctx != c) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "graph belongs to another context");
+  if (o->max_iterations < 0 || !(o->tolerance >= 0) || !(o->damping > 0 && o->damping < 1))
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad options");
+  const uint32_t N = g->N;
+  const int L = o->max_iterations;
+  // Seed table: one row per distinct seed vertex, s_j[v] = (occurrences of v in seed set j) / |seed set j|.
+  std::vector<int32_t> row(N, -1);
+  std::vector<double> mass;
+  int32_t rows = 0;
+  for (int j = 0; j < k; ++j) {
+    if (so[j + 1] <= so[j]) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "seed set %d is empty", j);
+    const double share = 1.0 / double(so[j + 1] - so[j]);
+    for (uint32_t i = so[j]; i < so[j + 1]; ++i) {
+      const uint32_t v = seeds[i];
+      if (v >= N) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "seed %u >= N", v);
+      if (row[v] < 0) {
+        row[v] = rows++;
+        mass.resize(size_t(rows) * prh::kPprCols, 0.0);
+      }
+      mass[size_t(row[v]) * prh::kPprCols + j] += share;
+    }
+  }
+  hipStream_t s = c->stream;
+  PRH_HIP(c, hipSetDevice(c->device));
+  // Light / heavy split of the in-degree at the chunk size, and the heavy chunks (built on the host: few).
+  const uint32_t nt = prh::partition_tiles(N);
+  PRH_CHECK(ensure(c, c->sched_in, N));
+  PRH_CHECK(ensure(c, c->tiles, size_t(3) * nt));
+  PRH_CHECK(ensure(c, c->counts, 8));
+  PRH_HIP(c, prh::launch_partition(g->in_off, N, kPprChunk, 0xFFFFFFFFu, c->sched_in.p, c->tiles.p, c->counts.p, s));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->counts.p, 8, hipMemcpyDeviceToHost, s));
+  PRH_HIP(c, hipStreamSynchronize(s));
+  const uint32_t n_light = reinterpret_cast<uint32_t *>(c->pinned)[0];
+  const uint32_t n_heavy = reinterpret_cast<uint32_t *>(c->pinned)[1];
+  std::vector<uint32_t> hv(n_heavy), hb(n_heavy), he(n_heavy);
+  PRH_CHECK(ensure(c, c->scratch_u32, size_t(2) * n_heavy + 1));
+  if (n_heavy) {
+    PRH_HIP(c, prh::launch_gather_ranges(g->in_off, c->sched_in.p + n_light, n_heavy, c->scratch_u32.p,
+                                         c->scratch_u32.p + n_heavy, s));
+    PRH_HIP(c, hipMemcpyAsync(hv.data(), c->sched_in.p + n_light, size_t(n_heavy) * 4, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipMemcpyAsync(hb.data(), c->scratch_u32.p, size_t(n_heavy) * 4, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipMemcpyAsync(he.data(), c->scratch_u32.p + n_heavy, size_t(n_heavy) * 4, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipStreamSynchronize(s));
+  }
+  std::vector<uint32_t> cb, ce, hf(size_t(n_heavy) + 1);
+  for (uint32_t h = 0; h < n_heavy; ++h) {
+    hf[h] = uint32_t(cb.size());
+    for (uint32_t e = hb[h]; e < he[h]; e += kPprChunk) {
+      cb.push_back(e);
+      ce.push_back(std::min(he[h], e + kPprChunk));
+    }
+  }
+  hf[n_heavy] = uint32_t(cb.size());
+  const uint32_t n_chunks = uint32_t(cb.size());
+  // Device buffers.
+  const size_t NC = size_t(N) * prh::kPprCols;
+  for (int i = 0; i < 2; ++i) {
+    PRH_CHECK(ensure(c, c->ppr_rank[i], NC));
+    PRH_CHECK(ensure(c, c->ppr_contrib[i], NC));
+  }
+  PRH_CHECK(ensure(c, c->ppr_out, size_t(N) * k));
+  PRH_CHECK(ensure(c, c->ppr_partial, std::max<size_t>(1, size_t(n_chunks) * prh::kPprCols)));
+  PRH_CHECK(ensure(c, c->ppr_mass, std::max<size_t>(1, mass.size())));
+  PRH_CHECK(ensure(c, c->ppr_row, N));
+  PRH_CHECK(ensure(c, c->ppr_tables, size_t(2) * n_chunks + size_t(2) * n_heavy + 1));
+  PRH_CHECK(ensure(c, c->ppr_err, std::max<size_t>(1, size_t(L) * prh::kPprCols)));
+  PRH_CHECK(ensure(c, c->ppr_state, 1));
+  uint32_t *t = c->ppr_tables.p;
+  PRH_HIP(c, hipMemcpyAsync(c->ppr_row.p, row.data(), size_t(N) * 4, hipMemcpyHostToDevice, s));
+  if (!mass.empty()) PRH_HIP(c, hipMemcpyAsync(c->ppr_mass.p, mass.data(), mass.size() * 8, hipMemcpyHostToDevice, s));
+  if (n_chunks) {
+    PRH_HIP(c, hipMemcpyAsync(t, cb.data(), size_t(n_chunks) * 4, hipMemcpyHostToDevice, s));
+    PRH_HIP(c, hipMemcpyAsync(t + n_chunks, ce.data(), size_t(n_chunks) * 4, hipMemcpyHostToDevice, s));
+  }
+  if (n_heavy) PRH_HIP(c, hipMemcpyAsync(t + 2 * n_chunks, hv.data(), size_t(n_heavy) * 4, hipMemcpyHostToDevice, s));
+  PRH_HIP(c, hipMemcpyAsync(t + 2 * n_chunks + n_heavy, hf.data(), (size_t(n_heavy) + 1) * 4, hipMemcpyHostToDevice, s));
+  prh::PprArgs A{};
+  for (int i = 0; i < 2; ++i) {
+    A.rank[i] = c->ppr_rank[i].p;
+    A.contrib[i] = c->ppr_contrib[i].p;
+  }
+  A.off = g->in_off;
+  A.src = g->in_src;
+  A.deg = g->outdeg;
+  A.seed_row = c->ppr_row.p;
+  A.seed_mass = c->ppr_mass.p;
+  A.light = c->sched_in.p;
+  A.n_light = n_light;
+  A.chunk_beg = t;
+  A.chunk_end = t + n_chunks;
+  A.n_chunks = n_chunks;
+  A.heavy_v = t + 2 * n_chunks;
+  A.heavy_first = t + 2 * n_chunks + n_heavy;
+  A.n_heavy = n_heavy;
+  A.partial = c->ppr_partial.p;
+  A.state = c->ppr_state.p;
+  A.err = c->ppr_err.p;
+  A.P = o->damping;
+  A.tol = o->tolerance;
+  A.k = k;
+  A.max_iter = L;
+  A.N = N;
+  // Initialization (timed), then iterations as replays of a captured run of check_interval iterations.
+  PRH_HIP(c, hipEventRecord(c->ev[0], s));
+  PRH_HIP(c, prh::launch_ppr_reset(A, s));
+  PRH_HIP(c, prh::launch_ppr_init(A, s));
+  PRH_HIP(c, hipEventRecord(c->ev[1], s));
+  const int chunk = std::max(1, c->tuning.check_interval);
+  hipGraphExec_t exec = nullptr;
+  int status = L > 0 ? capture_graph(c, [&]() -> int {
+    for (int i = 0; i < chunk; ++i) PRH_HIP(c, prh::launch_ppr_iteration(A, s));
+    return PRH_SUCCESS;
+  }, &exec) : PRH_SUCCESS;
+  prh::PprState hs{};
+  for (int launched = 0; status == PRH_SUCCESS && launched < L; launched += chunk) {
+    hipError_t e = hipGraphLaunch(exec, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->pinned, c->ppr_state.p, sizeof(prh::PprState), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      status = fail(c, PRH_ERROR_DEVICE, "batched iterations: %s", hipGetErrorString(e));
+      break;
+    }
+    std::memcpy(&hs, c->pinned, sizeof(hs));
+    bool all = true;
+    for (int j = 0; j < k; ++j) all = all && hs.done[j] != 0;
+    if (all) break;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  PRH_CHECK(status);
+  PRH_HIP(c, hipEventRecord(c->ev[2], s));
+  PRH_HIP(c, prh::launch_ppr_collect(A, c->ppr_out.p, s));
+  PRH_HIP(c, hipMemcpyAsync(ranks, c->ppr_out.p, size_t(N) * k * 8, hipMemcpyDeviceToHost, s));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->ppr_state.p, sizeof(prh::PprState), hipMemcpyDeviceToHost, s));
+  PRH_HIP(c, hipStreamSynchronize(s));
+  std::memcpy(&hs, c->pinned, sizeof(hs));
+  int most = 0;
+  for (int j = 0; j < k; ++j) {
+    iterations[j] = hs.done[j] ? hs.done[j] : L;
+    most = std::max(most, iterations[j]);
+  }
+  float t01 = 0, t12 = 0;
+  PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
+  PRH_HIP(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+  if (st) *st = prh_stats{most, t01 + t12, t01, 0.0f, t12};
+  return PRH_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
+
+int prh_ppr_batched(prh_context *c, const prh_graph *g, int k, const uint32_t *so, const uint32_t *seeds,
+                    const prh_options *o, double *ranks, int *iterations, prh_stats *st) {
+  if (c && g && g->N == 0) {
+    if (st) *st = prh_stats{};
+    if (iterations)
+      for (int j = 0; j < k && j < prh::kPprCols; ++j) iterations[j] = 0;
+    return PRH_SUCCESS;
+  }
+  return ppr_impl(c, g, k, so, seeds, o, ranks, iterations, st);
+}
 
 void prh_default_options(prh_options *o) {
   if (!o) return;
@@ -415,7 +606,7 @@ int prh_context_create(int device, prh_context **out) {
   c->device = device;
   prh_default_tuning(&c->tuning);
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void **>(&c->pinned), 256, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc(reinterpret_cast<void **>(&c->pinned), kPinnedBytes, hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
     prh_context_destroy(c);
     return PRH_ERROR_DEVICE;
@@ -447,6 +638,17 @@ void prh_context_destroy(prh_context *c) {
   release(c, c->scratch_u32);
   release(c, c->loop);
   release(c, c->flags);
+  for (int i = 0; i < 2; ++i) {
+    release(c, c->ppr_rank[i]);
+    release(c, c->ppr_contrib[i]);
+  }
+  release(c, c->ppr_partial);
+  release(c, c->ppr_mass);
+  release(c, c->ppr_out);
+  release(c, c->ppr_row);
+  release(c, c->ppr_tables);
+  release(c, c->ppr_err);
+  release(c, c->ppr_state);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c->kev)

@@ -83,6 +83,43 @@ hipError_t launch_validate_graph(const uint32_t *in_off, const uint32_t *in_src,
                                  int *flags, hipStream_t s);
 hipError_t launch_validate_ids(const uint32_t *ids, size_t n, uint32_t N, int *flags, int bit, hipStream_t s);
 hipError_t launch_check_finite(const double *x, size_t n, int *flags, int bit, hipStream_t s);
+// beg[i] = off[ids[i]], end[i] = off[ids[i] + 1]
+hipError_t launch_gather_ranges(const uint32_t *off, const uint32_t *ids, size_t n, uint32_t *beg, uint32_t *end,
+                                hipStream_t s);
+
+// ---- ppr.hip ----
+constexpr int kPprCols = 64;  // teleport columns per batch (one per lane)
+
+struct PprState {
+  int iter;              // index of the iteration the next launch performs
+  int done[kPprCols];    // 0 while column j runs, else its number of iterations
+};
+
+struct PprArgs {
+  double *rank[2];                 // [N][64] ping-pong ranks
+  double *contrib[2];              // [N][64] ping-pong rank / out-degree
+  const uint32_t *off, *src, *deg;  // in-CSR and out-degrees
+  const int32_t *seed_row;          // [N]: row of v in seed_mass, or -1
+  const double *seed_mass;          // [rows][64]: s_j[v]
+  const uint32_t *light;            // vertices with in-degree <= chunk
+  uint32_t n_light;
+  const uint32_t *chunk_beg, *chunk_end;  // edge ranges of the heavy vertices' chunks, heavy vertex by vertex
+  uint32_t n_chunks;
+  const uint32_t *heavy_v, *heavy_first;  // heavy vertices; chunks of heavy_v[h] are [heavy_first[h], heavy_first[h+1])
+  uint32_t n_heavy;
+  double *partial;                  // [n_chunks][64]
+  PprState *state;
+  unsigned long long *err;          // [max_iter][64] per-column L-inf change bits
+  double P, tol;
+  int k;                            // columns in use (1..64)
+  int max_iter;
+  uint32_t N;
+};
+
+hipError_t launch_ppr_reset(const PprArgs &a, hipStream_t s);
+hipError_t launch_ppr_init(const PprArgs &a, hipStream_t s);
+hipError_t launch_ppr_iteration(const PprArgs &a, hipStream_t s);  // identical launches every iteration
+hipError_t launch_ppr_collect(const PprArgs &a, double *out, hipStream_t s);
 
 // ---- frontier.hip ----
 hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *delu, const uint32_t *delv,

@@ -236,6 +236,15 @@ __global__ __launch_bounds__(kBlock) void check_finite_kernel(const double *x, s
   if (bad) atomicOr(flags, bit);
 }
 
+__global__ __launch_bounds__(kBlock) void gather_ranges_kernel(const uint32_t *off, const uint32_t *ids, size_t n,
+                                                               uint32_t *beg, uint32_t *end) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock) {
+    const uint32_t v = ids[i];
+    beg[i] = off[v];
+    end[i] = off[v + 1];
+  }
+}
+
 inline uint32_t map_grid(size_t n) {
   const size_t g = (n + kBlock - 1) / kBlock;
   return uint32_t(g < 1 ? 1 : (g > 8192 ? 8192 : g));
@@ -297,6 +306,13 @@ hipError_t launch_validate_graph(const uint32_t *in_off, const uint32_t *in_src,
 hipError_t launch_validate_ids(const uint32_t *ids, size_t n, uint32_t N, int *flags, int bit, hipStream_t s) {
   if (n == 0) return hipSuccess;
   validate_ids_kernel<<<map_grid(n), kBlock, 0, s>>>(ids, n, N, flags, bit);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_ranges(const uint32_t *off, const uint32_t *ids, size_t n, uint32_t *beg, uint32_t *end,
+                                hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  gather_ranges_kernel<<<map_grid(n), kBlock, 0, s>>>(off, ids, n, beg, end);
   return hipGetLastError();
 }
 

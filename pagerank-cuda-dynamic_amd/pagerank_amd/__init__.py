@@ -4,7 +4,8 @@ The compute path is libpagerank_hip.so (hand-written HIP for gfx950 behind the C
 this package is its Python mirror plus host-side graph preparation.
 """
 from .engine import (DYNAMIC_FRONTIER, DYNAMIC_TRAVERSAL, METHOD_NAMES, NAIVE_DYNAMIC, PRUNE_DYNAMIC_FRONTIER,
-                     STATIC, Context, DeviceGraph, PagerankError, PagerankOptions, PagerankResult, Tuning,
+                     STATIC, BatchedResult, Context, DeviceGraph, PagerankError, PagerankOptions, PagerankResult,
+                     Tuning, ppr_batched,
                      pagerankDynamicFrontierHip, pagerankDynamicTraversalHip, pagerankNaiveDynamicHip,
                      pagerankPruneDynamicFrontierHip, pagerankStaticHip, solve, solve_device)
 from .graphs import Batch, HostGraph, apply_batch
@@ -12,6 +13,6 @@ from .graphs import Batch, HostGraph, apply_batch
 __all__ = [
     "STATIC", "NAIVE_DYNAMIC", "DYNAMIC_TRAVERSAL", "DYNAMIC_FRONTIER", "PRUNE_DYNAMIC_FRONTIER", "METHOD_NAMES",
     "Context", "DeviceGraph", "PagerankError", "PagerankOptions", "PagerankResult", "Tuning", "solve",
-    "solve_device", "pagerankStaticHip", "pagerankNaiveDynamicHip", "pagerankDynamicTraversalHip",
+    "solve_device", "BatchedResult", "ppr_batched", "pagerankStaticHip", "pagerankNaiveDynamicHip", "pagerankDynamicTraversalHip",
     "pagerankDynamicFrontierHip", "pagerankPruneDynamicFrontierHip", "Batch", "HostGraph", "apply_batch",
 ]

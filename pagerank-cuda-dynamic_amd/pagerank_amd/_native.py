@@ -69,6 +69,8 @@ ENGINE_SIGNATURES = {
                                    C.c_size_t, vp, vp, C.POINTER(prh_stats)]),
     "prh_partition": (C.c_int, [vp, C.c_uint32, u32p, C.c_uint32, u32p, u32p]),
     "prh_mark_affected": (C.c_int, [vp, vp, C.c_int, u32p, u32p, C.c_size_t, u32p, C.c_size_t, u8p]),
+    "prh_ppr_batched": (C.c_int, [vp, vp, C.c_int, u32p, u32p, C.POINTER(prh_options), f64p, C.POINTER(C.c_int),
+                                  C.POINTER(prh_stats)]),
     "prh_kernel_timing_enable": (C.c_int, [vp, C.c_int]),
     "prh_kernel_timing_read": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
 }

@@ -216,6 +216,36 @@ def solve_device(yg: DeviceGraph, method: int, o: PagerankOptions, q_ptr: int | 
     return st
 
 
+@dataclass
+class BatchedResult:
+    """Batched personalized PageRank: ranks[v, j] for column j, and each column's iteration count."""
+    ranks: np.ndarray
+    iterations: np.ndarray
+    time: float = 0.0
+    initializationTime: float = 0.0
+    computationTime: float = 0.0
+
+
+def ppr_batched(yg: DeviceGraph, seed_sets, o: PagerankOptions | None = None) -> BatchedResult:
+    """Personalized PageRank for up to 64 teleport columns at once (prh_ppr_batched): column j teleports uniformly
+    to seed_sets[j]."""
+    o = o or PagerankOptions()
+    k = len(seed_sets)
+    ctx = yg.ctx
+    sizes = [len(s) for s in seed_sets]
+    offsets = np.zeros(k + 1, dtype=np.uint32)
+    np.cumsum(sizes, out=offsets[1:])
+    seeds = np.ascontiguousarray(np.concatenate([np.asarray(s, dtype=np.uint32) for s in seed_sets])
+                                 if k else np.zeros(0, np.uint32))
+    ranks = np.zeros((yg.N, max(k, 1)), dtype=np.float64)
+    its = (C.c_int * max(k, 1))()
+    st = prh_stats()
+    ctx.check(ctx._lib.prh_ppr_batched(ctx.handle, yg.handle, k, offsets.ctypes.data_as(u32p),
+                                       seeds.ctypes.data_as(u32p), C.byref(o.native()), ranks.ctypes.data_as(f64p),
+                                       its, C.byref(st)))
+    return BatchedResult(ranks[:, :k], np.array(its[:k]), st.time, st.initialization_time, st.computation_time)
+
+
 # Reference-named wrappers (pagerankCuda.hxx:864-988). x is accepted for signature parity and, like the
 # reference, only its emptiness matters.
 def pagerankStaticHip(yg: DeviceGraph, o: PagerankOptions | None = None, partition: bool = True) -> PagerankResult:
