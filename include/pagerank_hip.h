@@ -154,6 +154,25 @@ int prh_ppr_batched(prh_context *context, const prh_graph *graph, int k, const u
                     const uint32_t *seeds, const prh_options *options, double *ranks, int *iterations,
                     prh_stats *stats);
 
+/* Weighted PageRank and its adjoint (no counterpart in the reference). Edge weights are FP32, one per edge of the
+ * graph's out-CSR (weights[e] belongs to out_targets[e]), and are row-normalised to FP64 transition probabilities
+ * P[u,v] = w_uv / W_u, W_u = sum of u's out-weights. The graph needs its out-CSR with ascending rows.
+ *
+ * prh_weighted_pagerank: solves r = (1-damping) s + damping P^T r by the pull iteration over the in-CSR from
+ * r = s, stopping at the first iteration whose L-inf change is below options->tolerance (max_iterations caps it).
+ *
+ * prh_weighted_adjoint: given the forward solution r and g = dL/dr, solves lambda = g + damping P lambda by the
+ * same pull over the out-CSR from lambda = g, then writes dL/dw (grad_weights, FP64, out-CSR edge order) and
+ * dL/ds = (1-damping) lambda (grad_teleport); lambda may be NULL.
+ *
+ * device_pointers = 0: every array is host memory; 1: every array is device memory of the context's device (the
+ * call synchronises the context stream before returning, callers order their own streams before the call). */
+int prh_weighted_pagerank(prh_context *context, const prh_graph *graph, const float *weights, const double *teleport,
+                          const prh_options *options, double *ranks, prh_stats *stats, int device_pointers);
+int prh_weighted_adjoint(prh_context *context, const prh_graph *graph, const float *weights, const double *ranks,
+                         const double *grad_ranks, const prh_options *options, double *lambda,
+                         double *grad_weights, double *grad_teleport, prh_stats *stats, int device_pointers);
+
 /* Per-launch timing of the rank-update kernel with HIP events on the context's stream (no counterpart in the
  * reference). While enabled, every solve accumulates the summed duration and count of the rank-update
  * launches that did work; reading resets the accumulators. */

@@ -47,6 +47,9 @@ struct prh_context {
   DevBuf<uint32_t> ppr_tables;
   DevBuf<unsigned long long> ppr_err;
   DevBuf<prh::PprState> ppr_state;
+  // weighted forward / adjoint
+  DevBuf<double> wt_x[2], wt_z[2], wt_W, wt_term, wt_r, wt_gw, wt_gs;
+  DevBuf<float> wt_wout, wt_win;
   uint64_t *pinned = nullptr;       // small host-pinned readback area (kPinnedBytes)
   hipEvent_t ev[4] = {};
   bool ktiming = false;
@@ -60,6 +63,7 @@ struct prh_graph {
   uint32_t N = 0;
   uint64_t M = 0;
   uint32_t *in_off = nullptr, *in_src = nullptr, *outdeg = nullptr, *out_off = nullptr, *out_dst = nullptr;
+  uint32_t *in_eid = nullptr;  // out-CSR position of every in-CSR slot, built on first weighted use
   size_t bytes = 0;
 };
 
@@ -561,9 +565,202 @@ int ppr_impl(prh_context *c, const prh_graph *g, int k, const uint32_t *so, cons
   return PRH_SUCCESS;
 }
 
+// Up to L iterations of step(l, err, done) -- launches honouring the device stop control of iteration l -- with
+// the device flag polled every check_interval iterations. *n receives the number of iterations performed.
+template <class F>
+int run_iterations(prh_context *c, int L, double tol, F step, int *n) {
+  hipStream_t s = c->stream;
+  PRH_CHECK(ensure(c, c->loop, size_t(L) + 1));
+  unsigned long long *err = c->loop.p;
+  int *done = reinterpret_cast<int *>(c->loop.p + L);
+  PRH_HIP(c, prh::launch_fill_u64(err, 0ull, size_t(L) + 1, s));
+  const int check = std::max(1, c->tuning.check_interval);
+  int result = -1, launched = 0;
+  for (int l = 0; l < L; ++l) {
+    PRH_CHECK(step(l, err, done));
+    launched = l + 1;
+    if (launched % check == 0 || launched == L) {
+      PRH_HIP(c, hipMemcpyAsync(c->pinned, err + l, 8, hipMemcpyDeviceToHost, s));
+      PRH_HIP(c, hipMemcpyAsync(c->pinned + 1, done, sizeof(int), hipMemcpyDeviceToHost, s));
+      PRH_HIP(c, hipStreamSynchronize(s));
+      double el;
+      std::memcpy(&el, c->pinned, 8);
+      const int d = *reinterpret_cast<const int *>(c->pinned + 1);
+      if (d > 0) { result = d; break; }
+      if (el < tol) { result = launched; break; }
+    }
+  }
+  *n = result < 0 ? launched : result;
+  return PRH_SUCCESS;
+}
+
+// Device copy of a caller array: the pointer itself for device memory, else staged into `buf`.
+template <class T>
+int stage(prh_context *c, const T *src, size_t n, bool device, DevBuf<T> &buf, const T **out) {
+  if (device) {
+    *out = src;
+    return PRH_SUCCESS;
+  }
+  PRH_CHECK(ensure(c, buf, std::max<size_t>(1, n)));
+  if (n) PRH_HIP(c, hipMemcpyAsync(buf.p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  *out = buf.p;
+  return PRH_SUCCESS;
+}
+
+int weighted_common(prh_context *c, const prh_graph *g, const float *w, const prh_options *o, bool device,
+                    const float **w_out) {
+  if (!g || !o || !w) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null argument");
+  if (g->ctx != c) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "graph belongs to another context");
+  if (!g->out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "weighted solves need the graph's out-CSR");
+  if (o->max_iterations < 0 || !(o->tolerance >= 0) || !(o->damping > 0 && o->damping < 1))
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad options");
+  PRH_HIP(c, hipSetDevice(c->device));
+  PRH_CHECK(stage(c, w, size_t(g->M), device, c->wt_wout, w_out));
+  PRH_CHECK(ensure(c, c->wt_W, g->N));
+  PRH_HIP(c, prh::launch_out_weight_sum(g->out_off, *w_out, g->N, c->wt_W.p, c->stream));
+  for (int i = 0; i < 2; ++i) {
+    PRH_CHECK(ensure(c, c->wt_x[i], g->N));
+    PRH_CHECK(ensure(c, c->wt_z[i], g->N));
+  }
+  return PRH_SUCCESS;
+}
+
+// Two-way degree schedule (<= 64 | rest) of a CSR into `order`.
+int two_way_schedule(prh_context *c, const uint32_t *off, uint32_t N, DevBuf<uint32_t> &order, prh::Schedule &sc) {
+  PRH_CHECK(ensure(c, order, N));
+  PRH_CHECK(ensure(c, c->tiles, size_t(3) * prh::partition_tiles(N)));
+  PRH_CHECK(ensure(c, c->counts, 8));
+  PRH_HIP(c, prh::launch_partition(off, N, 64u, 0xFFFFFFFFu, order.p, c->tiles.p, c->counts.p, c->stream));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->counts.p, 8, hipMemcpyDeviceToHost, c->stream));
+  PRH_HIP(c, hipStreamSynchronize(c->stream));
+  const uint32_t *cnt = reinterpret_cast<const uint32_t *>(c->pinned);
+  sc = prh::Schedule{order.p, cnt[0], cnt[1], 0};
+  return PRH_SUCCESS;
+}
+
+int weighted_forward_impl(prh_context *c, const prh_graph *g, const float *w, const double *s, const prh_options *o,
+                          double *ranks, prh_stats *st, bool device) {
+  if (!c) return PRH_ERROR_INVALID_ARGUMENT;
+  if (!s || !ranks) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null argument");
+  const float *w_out = nullptr;
+  PRH_CHECK(weighted_common(c, g, w, o, device, &w_out));
+  const uint32_t N = g->N;
+  const uint64_t M = g->M;
+  hipStream_t hs = c->stream;
+  const double *term = nullptr;
+  PRH_CHECK(stage(c, s, N, device, c->wt_term, &term));
+  if (!g->in_eid) {  // the out-CSR slot of every in-CSR slot, once per graph
+    prh_graph *gm = const_cast<prh_graph *>(g);
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&gm->in_eid), M * 4));
+    PRH_CHECK(ensure(c, c->flags, 4));
+    PRH_HIP(c, prh::launch_fill_i32(c->flags.p, 0, 1, hs));
+    PRH_HIP(c, prh::launch_edge_map(g->in_off, g->in_src, g->out_off, g->out_dst, N, gm->in_eid, c->flags.p, hs));
+    PRH_HIP(c, hipMemcpyAsync(c->pinned, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, hs));
+    PRH_HIP(c, hipStreamSynchronize(hs));
+    if (*reinterpret_cast<int *>(c->pinned)) {
+      dev_free(c, gm->in_eid, M * 4);
+      gm->in_eid = nullptr;
+      return fail(c, PRH_ERROR_INVALID_GRAPH, "in-CSR and out-CSR disagree or out-rows are not ascending");
+    }
+  }
+  PRH_CHECK(ensure(c, c->wt_win, std::max<uint64_t>(1, M)));
+  PRH_HIP(c, prh::launch_permute_weights(w_out, g->in_eid, M, c->wt_win.p, hs));
+  prh::Schedule sc{};
+  PRH_CHECK(two_way_schedule(c, g->in_off, N, c->sched_in, sc));
+  PRH_HIP(c, hipEventRecord(c->ev[0], hs));
+  PRH_HIP(c, prh::launch_weighted_init(term, c->wt_W.p, N, false, c->wt_x[0].p, c->wt_z[0].p, hs));
+  PRH_HIP(c, hipEventRecord(c->ev[1], hs));
+  const int width = low_width_for(c->tuning, N, M);
+  int n = 0;
+  PRH_CHECK(run_iterations(c, o->max_iterations, o->tolerance, [&](int l, unsigned long long *err, int *done) -> int {
+    const prh::WeightedArgs A{g->in_off, g->in_src, c->wt_win.p, c->wt_z[l & 1].p, c->wt_x[(l + 1) & 1].p,
+                              c->wt_z[(l + 1) & 1].p, c->wt_x[l & 1].p, term, c->wt_W.p, sc, o->damping};
+    PRH_HIP(c, prh::launch_weighted_step(false, A, width, err, done, l, o->tolerance, hs));
+    return PRH_SUCCESS;
+  }, &n));
+  PRH_HIP(c, hipEventRecord(c->ev[2], hs));
+  PRH_HIP(c, hipMemcpyAsync(ranks, c->wt_x[n & 1].p, size_t(N) * 8,
+                            device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, hs));
+  PRH_HIP(c, hipStreamSynchronize(hs));
+  float t01 = 0, t12 = 0;
+  PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
+  PRH_HIP(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+  if (st) *st = prh_stats{n, t01 + t12, t01, 0.0f, t12};
+  return PRH_SUCCESS;
+}
+
+int weighted_adjoint_impl(prh_context *c, const prh_graph *g, const float *w, const double *r, const double *gr,
+                          const prh_options *o, double *lam, double *grad_w, double *grad_s, prh_stats *st,
+                          bool device) {
+  if (!c) return PRH_ERROR_INVALID_ARGUMENT;
+  if (!r || !gr || !grad_w || !grad_s) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null argument");
+  const float *w_out = nullptr;
+  PRH_CHECK(weighted_common(c, g, w, o, device, &w_out));
+  const uint32_t N = g->N;
+  const uint64_t M = g->M;
+  hipStream_t hs = c->stream;
+  const double *term = nullptr, *rr = nullptr;
+  PRH_CHECK(stage(c, gr, N, device, c->wt_term, &term));
+  PRH_CHECK(stage(c, r, N, device, c->wt_r, &rr));
+  prh::Schedule sc{};
+  PRH_CHECK(two_way_schedule(c, g->out_off, N, c->sched_out, sc));
+  PRH_HIP(c, hipEventRecord(c->ev[0], hs));
+  PRH_HIP(c, prh::launch_weighted_init(term, c->wt_W.p, N, true, c->wt_x[0].p, c->wt_z[0].p, hs));
+  PRH_HIP(c, hipEventRecord(c->ev[1], hs));
+  const int width = low_width_for(c->tuning, N, M);
+  int n = 0;
+  PRH_CHECK(run_iterations(c, o->max_iterations, o->tolerance, [&](int l, unsigned long long *err, int *done) -> int {
+    const prh::WeightedArgs A{g->out_off, g->out_dst, w_out, c->wt_x[l & 1].p, c->wt_x[(l + 1) & 1].p,
+                              c->wt_x[(l + 1) & 1].p, c->wt_x[l & 1].p, term, c->wt_W.p, sc, o->damping};
+    PRH_HIP(c, prh::launch_weighted_step(true, A, width, err, done, l, o->tolerance, hs));
+    return PRH_SUCCESS;
+  }, &n));
+  const double *lam_d = c->wt_x[n & 1].p;
+  double *gw = grad_w, *gs = grad_s;
+  if (!device) {
+    PRH_CHECK(ensure(c, c->wt_gw, std::max<uint64_t>(1, M)));
+    PRH_CHECK(ensure(c, c->wt_gs, N));
+    gw = c->wt_gw.p;
+    gs = c->wt_gs.p;
+  }
+  PRH_HIP(c, prh::launch_weighted_grad(g->out_off, g->out_dst, c->wt_W.p, rr, lam_d, term, o->damping, N, gw, gs, hs));
+  PRH_HIP(c, hipEventRecord(c->ev[2], hs));
+  const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (lam) PRH_HIP(c, hipMemcpyAsync(lam, lam_d, size_t(N) * 8, kind, hs));
+  if (!device) {
+    PRH_HIP(c, hipMemcpyAsync(grad_w, gw, size_t(M) * 8, kind, hs));
+    PRH_HIP(c, hipMemcpyAsync(grad_s, gs, size_t(N) * 8, kind, hs));
+  }
+  PRH_HIP(c, hipStreamSynchronize(hs));
+  float t01 = 0, t12 = 0;
+  PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
+  PRH_HIP(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+  if (st) *st = prh_stats{n, t01 + t12, t01, 0.0f, t12};
+  return PRH_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
+
+int prh_weighted_pagerank(prh_context *c, const prh_graph *g, const float *w, const double *s, const prh_options *o,
+                          double *ranks, prh_stats *st, int device_pointers) {
+  if (c && g && g->N == 0) {
+    if (st) *st = prh_stats{};
+    return PRH_SUCCESS;
+  }
+  return weighted_forward_impl(c, g, w, s, o, ranks, st, device_pointers != 0);
+}
+
+int prh_weighted_adjoint(prh_context *c, const prh_graph *g, const float *w, const double *r, const double *gr,
+                         const prh_options *o, double *lam, double *grad_w, double *grad_s, prh_stats *st,
+                         int device_pointers) {
+  if (c && g && g->N == 0) {
+    if (st) *st = prh_stats{};
+    return PRH_SUCCESS;
+  }
+  return weighted_adjoint_impl(c, g, w, r, gr, o, lam, grad_w, grad_s, st, device_pointers != 0);
+}
 
 int prh_ppr_batched(prh_context *c, const prh_graph *g, int k, const uint32_t *so, const uint32_t *seeds,
                     const prh_options *o, double *ranks, int *iterations, prh_stats *st) {
@@ -649,6 +846,17 @@ void prh_context_destroy(prh_context *c) {
   release(c, c->ppr_tables);
   release(c, c->ppr_err);
   release(c, c->ppr_state);
+  for (int i = 0; i < 2; ++i) {
+    release(c, c->wt_x[i]);
+    release(c, c->wt_z[i]);
+  }
+  release(c, c->wt_W);
+  release(c, c->wt_term);
+  release(c, c->wt_r);
+  release(c, c->wt_gw);
+  release(c, c->wt_gs);
+  release(c, c->wt_wout);
+  release(c, c->wt_win);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c->kev)
@@ -750,6 +958,7 @@ void prh_graph_destroy(prh_graph *g) {
   dev_free(c, g->outdeg, N * 4);
   dev_free(c, g->out_off, (N + 1) * 4);
   dev_free(c, g->out_dst, (M ? M : 4) * 4);
+  dev_free(c, g->in_eid, (M ? M : 4) * 4);
   delete g;
 }
 

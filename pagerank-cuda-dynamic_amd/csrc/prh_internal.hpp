@@ -121,6 +121,31 @@ hipError_t launch_ppr_init(const PprArgs &a, hipStream_t s);
 hipError_t launch_ppr_iteration(const PprArgs &a, hipStream_t s);  // identical launches every iteration
 hipError_t launch_ppr_collect(const PprArgs &a, double *out, hipStream_t s);
 
+// ---- weighted.hip ----
+struct WeightedArgs {
+  const uint32_t *off, *idx;  // pull CSR: in-CSR (forward) or out-CSR (adjoint)
+  const float *w;             // FP32 weights aligned with idx
+  const double *z;            // gathered values: r/W (forward) or lambda (adjoint)
+  double *x_new, *z_new;      // new values and new gathered values
+  const double *x_old;        // previous values (for the L-inf change)
+  const double *term;         // teleport s (forward, scaled by 1-P in the kernel) or g (adjoint)
+  const double *W;            // out-weight sums
+  Schedule sched;             // degree schedule of the pull CSR (two-way)
+  double P;
+};
+
+hipError_t launch_weighted_step(bool adjoint, const WeightedArgs &A, int width, unsigned long long *err, int *done,
+                                int l, double tol, hipStream_t s);
+hipError_t launch_out_weight_sum(const uint32_t *out_off, const float *w, uint32_t N, double *W, hipStream_t s);
+hipError_t launch_edge_map(const uint32_t *in_off, const uint32_t *in_src, const uint32_t *out_off,
+                           const uint32_t *out_dst, uint32_t N, uint32_t *in_eid, int *missing, hipStream_t s);
+hipError_t launch_permute_weights(const float *w_out, const uint32_t *in_eid, size_t M, float *w_in, hipStream_t s);
+hipError_t launch_weighted_init(const double *start, const double *W, uint32_t N, bool adjoint, double *x, double *z,
+                                hipStream_t s);
+hipError_t launch_weighted_grad(const uint32_t *out_off, const uint32_t *out_dst, const double *W, const double *r,
+                                const double *lam, const double *g, double P, uint32_t N, double *grad_w,
+                                double *grad_s, hipStream_t s);
+
 // ---- frontier.hip ----
 hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *delu, const uint32_t *delv,
                                size_t ND, const uint32_t *insu, size_t NI, hipStream_t s);

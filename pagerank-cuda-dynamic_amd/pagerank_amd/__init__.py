@@ -5,7 +5,7 @@ this package is its Python mirror plus host-side graph preparation.
 """
 from .engine import (DYNAMIC_FRONTIER, DYNAMIC_TRAVERSAL, METHOD_NAMES, NAIVE_DYNAMIC, PRUNE_DYNAMIC_FRONTIER,
                      STATIC, BatchedResult, Context, DeviceGraph, PagerankError, PagerankOptions, PagerankResult,
-                     Tuning, ppr_batched,
+                     Tuning, ppr_batched, weighted_adjoint, weighted_pagerank,
                      pagerankDynamicFrontierHip, pagerankDynamicTraversalHip, pagerankNaiveDynamicHip,
                      pagerankPruneDynamicFrontierHip, pagerankStaticHip, solve, solve_device)
 from .graphs import Batch, HostGraph, apply_batch
@@ -13,6 +13,6 @@ from .graphs import Batch, HostGraph, apply_batch
 __all__ = [
     "STATIC", "NAIVE_DYNAMIC", "DYNAMIC_TRAVERSAL", "DYNAMIC_FRONTIER", "PRUNE_DYNAMIC_FRONTIER", "METHOD_NAMES",
     "Context", "DeviceGraph", "PagerankError", "PagerankOptions", "PagerankResult", "Tuning", "solve",
-    "solve_device", "BatchedResult", "ppr_batched", "pagerankStaticHip", "pagerankNaiveDynamicHip", "pagerankDynamicTraversalHip",
+    "solve_device", "BatchedResult", "ppr_batched", "weighted_pagerank", "weighted_adjoint", "pagerankStaticHip", "pagerankNaiveDynamicHip", "pagerankDynamicTraversalHip",
     "pagerankDynamicFrontierHip", "pagerankPruneDynamicFrontierHip", "Batch", "HostGraph", "apply_batch",
 ]

@@ -71,6 +71,9 @@ ENGINE_SIGNATURES = {
     "prh_mark_affected": (C.c_int, [vp, vp, C.c_int, u32p, u32p, C.c_size_t, u32p, C.c_size_t, u8p]),
     "prh_ppr_batched": (C.c_int, [vp, vp, C.c_int, u32p, u32p, C.POINTER(prh_options), f64p, C.POINTER(C.c_int),
                                   C.POINTER(prh_stats)]),
+    "prh_weighted_pagerank": (C.c_int, [vp, vp, vp, vp, C.POINTER(prh_options), vp, C.POINTER(prh_stats), C.c_int]),
+    "prh_weighted_adjoint": (C.c_int, [vp, vp, vp, vp, vp, C.POINTER(prh_options), vp, vp, vp,
+                                       C.POINTER(prh_stats), C.c_int]),
     "prh_kernel_timing_enable": (C.c_int, [vp, C.c_int]),
     "prh_kernel_timing_read": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
 }

@@ -246,6 +246,42 @@ def ppr_batched(yg: DeviceGraph, seed_sets, o: PagerankOptions | None = None) ->
     return BatchedResult(ranks[:, :k], np.array(its[:k]), st.time, st.initialization_time, st.computation_time)
 
 
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def weighted_pagerank(yg: DeviceGraph, weights: np.ndarray, teleport: np.ndarray,
+                      o: PagerankOptions | None = None) -> PagerankResult:
+    """Weighted PageRank (prh_weighted_pagerank) on host arrays: weights are FP32 in out-CSR edge order."""
+    o = o or PagerankOptions()
+    ctx = yg.ctx
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    s = np.ascontiguousarray(teleport, dtype=np.float64)
+    if len(w) != yg.M or len(s) != yg.N:
+        raise ValueError("weights need one value per edge and teleport one per vertex")
+    r = np.zeros(yg.N)
+    st = prh_stats()
+    ctx.check(ctx._lib.prh_weighted_pagerank(ctx.handle, yg.handle, _ptr(w), _ptr(s), C.byref(o.native()), _ptr(r),
+                                             C.byref(st), 0))
+    return PagerankResult(r, st.iterations, st.time, st.initialization_time, st.marking_time, st.computation_time)
+
+
+def weighted_adjoint(yg: DeviceGraph, weights: np.ndarray, ranks: np.ndarray, grad_ranks: np.ndarray,
+                     o: PagerankOptions | None = None):
+    """Adjoint solve and gradient assembly (prh_weighted_adjoint) on host arrays.
+    Returns (lambda, dL/dweights in out-CSR order, dL/dteleport, iterations)."""
+    o = o or PagerankOptions()
+    ctx = yg.ctx
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    r = np.ascontiguousarray(ranks, dtype=np.float64)
+    g = np.ascontiguousarray(grad_ranks, dtype=np.float64)
+    lam, gw, gs = np.zeros(yg.N), np.zeros(yg.M), np.zeros(yg.N)
+    st = prh_stats()
+    ctx.check(ctx._lib.prh_weighted_adjoint(ctx.handle, yg.handle, _ptr(w), _ptr(r), _ptr(g), C.byref(o.native()),
+                                            _ptr(lam), _ptr(gw), _ptr(gs), C.byref(st), 0))
+    return lam, gw, gs, st.iterations
+
+
 # Reference-named wrappers (pagerankCuda.hxx:864-988). x is accepted for signature parity and, like the
 # reference, only its emptiness matters.
 def pagerankStaticHip(yg: DeviceGraph, o: PagerankOptions | None = None, partition: bool = True) -> PagerankResult:

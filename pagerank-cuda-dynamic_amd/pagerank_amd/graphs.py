@@ -49,7 +49,10 @@ class HostGraph:
 
     def __del__(self):
         if getattr(self, "_handle", None) is not None:
-            _native.host().prhh_free(self._handle)
+            try:
+                _native.host().prhh_free(self._handle)
+            except Exception:  # interpreter teardown: the library may already be unloaded
+                pass
             self._handle = None
 
     # --- construction -------------------------------------------------------------------------------------

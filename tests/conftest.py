@@ -11,6 +11,13 @@ REPO = Path(__file__).resolve().parent.parent
 sys.path.insert(0, str(REPO / "pagerank-cuda-dynamic_amd"))
 sys.path.insert(0, str(REPO / "oracle"))
 
+# torch bundles its own HIP runtime under the same soname as the system one the engine links; whichever loads first
+# serves the process. Load torch's first so the autograd-harness tests and the engine share one runtime.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP engine library")
