@@ -73,9 +73,11 @@ typedef struct prh_stats {
 /* Launch tuning (no counterpart in the reference, whose values are fixed: switch degree 64, 256-thread blocks).
  * Results do not depend on these beyond floating-point summation order. */
 typedef struct prh_tuning {
-  uint32_t switch_degree; /* in-degree <= this: several vertices per wave [64] */
-  uint32_t huge_degree;   /* in-degree > this: one workgroup per vertex [2048] */
-  int low_width;          /* lanes per low-degree vertex: 1, 2, 4, 8, 16 or 32; 0 = pick from |E|/|V| [0] */
+  uint32_t switch_degree; /* in-degree <= this: several vertices per wave; above: one wave per vertex [128] */
+  uint32_t huge_degree;   /* in-degree > this: split into chunks of this many edges, one workgroup per chunk,
+                             summed by a second launch [16384] */
+  int low_width;          /* lanes per vertex of in-degree <= switch_degree: 1, 2, 4, 8, 16 or 32; 0 = by degree
+                             band, W lanes for in-degree in (W, 2W] (weighted kernels: from |E|/|V|) [0] */
   int check_interval;     /* iterations launched between host reads of the device convergence flag [8] */
 } prh_tuning;
 

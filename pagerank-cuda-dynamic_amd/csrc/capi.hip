@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -62,9 +63,24 @@ struct prh_graph {
   prh_context *ctx = nullptr;
   uint32_t N = 0;
   uint64_t M = 0;
+  // As uploaded (caller's ids): used by the batched and weighted solvers.
   uint32_t *in_off = nullptr, *in_src = nullptr, *outdeg = nullptr, *out_off = nullptr, *out_dst = nullptr;
   uint32_t *in_eid = nullptr;  // out-CSR position of every in-CSR slot, built on first weighted use
-  size_t bytes = 0;
+  // Degree-ordered layout used by the five solvers: layout row i is vertex perm[i] (in-degree descending, stable),
+  // inv is the inverse; in-rows sorted by new source id.
+  uint32_t *perm = nullptr, *inv = nullptr;
+  uint32_t *l_in_off = nullptr, *l_in_src = nullptr, *l_outdeg = nullptr, *l_out_off = nullptr, *l_out_dst = nullptr;
+  std::vector<uint32_t> sorted_deg;  // host: in-degree of layout row i
+  float prep_ms = 0;
+  // Rank-kernel plan for the current tuning: section table and the chunk tables of the heaviest rows.
+  prh_tuning plan_tuning{};
+  bool plan_valid = false;
+  prh::RankPlan plan{};
+  uint32_t *chunk_tables = nullptr;  // item_row, item_beg, item_end, row_first
+  size_t chunk_tables_bytes = 0;
+  double *chunk_partial = nullptr;
+  size_t chunk_partial_bytes = 0;
+  uint32_t n_chunked = 0;
 };
 
 namespace {
@@ -149,80 +165,162 @@ uint32_t cap_blocks(uint64_t n, uint64_t per, uint32_t cap) {
   return uint32_t(std::min<uint64_t>(g, cap));
 }
 
-prh::RankShape rank_shape(const prh::Schedule &s, int width) {
-  prh::RankShape sh;
-  sh.low_width = width;
-  sh.nb_huge = cap_blocks(s.n_huge, 1, 2048);
-  sh.nb_high = cap_blocks(s.n_high, prh::kWavesPerBlock, 16384);
-  sh.nb_low = cap_blocks(s.n_low, uint64_t(prh::kWavesPerBlock) * (prh::kWave / width), 32768);
-  return sh;
+// Rank-kernel plan over the degree-ordered layout for the context's tuning (cached in the graph):
+//   rows with in-degree > huge_degree: chunks of huge_degree edges, one workgroup each;
+//   switch_degree < in-degree <= huge_degree: one wave per row;
+//   in-degree <= switch_degree: bands of equal lanes-per-row W, W = pow2ceil(d)/2 in 1..32 (or the fixed
+//   low_width), each band a contiguous row range.
+// Blocks of the row sections get about kEdgesPerBlock edges each.
+constexpr uint64_t kEdgesPerBlock = 4096;
+
+int build_rank_plan(prh_context *c, prh_graph *g) {
+  const prh_tuning &t = c->tuning;
+  if (g->plan_valid && std::memcmp(&g->plan_tuning, &t, sizeof(t)) == 0) return PRH_SUCCESS;
+  const uint32_t N = g->N;
+  const std::vector<uint32_t> &deg = g->sorted_deg;
+  PRH_HIP(c, hipStreamSynchronize(c->stream));  // the old tables may still be in use
+  // number of rows with degree > x (deg is descending: the first row with degree <= x)
+  auto above = [&](uint32_t x) -> uint32_t {
+    return uint32_t(std::lower_bound(deg.begin(), deg.end(), x, [](uint32_t a, uint32_t b) { return a > b; }) -
+                    deg.begin());
+  };
+  auto edges = [&](uint32_t b, uint32_t e) {
+    uint64_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += deg[i];
+    return s;
+  };
+  prh::RankPlan p{};
+  uint32_t blocks = 0;
+  auto add = [&](int kind, int width, uint32_t b, uint32_t e, uint32_t per_block, uint32_t nblocks) {
+    if (e <= b || p.nsec >= prh::kMaxSections) return;
+    p.sec[p.nsec++] = prh::RankSection{kind, width, b, e, blocks, per_block};
+    blocks += nblocks;
+  };
+  // Chunked rows and their tables.
+  const uint32_t chunk = std::max<uint32_t>(t.huge_degree, 64);
+  const uint32_t n_chunked = above(t.huge_degree);
+  std::vector<uint32_t> item_row, item_beg, item_end, row_first(size_t(n_chunked) + 1);
+  uint64_t off = 0;
+  for (uint32_t r = 0; r < n_chunked; ++r) {
+    row_first[r] = uint32_t(item_row.size());
+    for (uint64_t e = 0; e < deg[r]; e += chunk) {
+      item_row.push_back(r);
+      item_beg.push_back(uint32_t(off + e));
+      item_end.push_back(uint32_t(off + std::min<uint64_t>(deg[r], e + chunk)));
+    }
+    off += deg[r];
+  }
+  row_first[n_chunked] = uint32_t(item_row.size());
+  const uint32_t n_items = uint32_t(item_row.size());
+  // (Re)allocate and upload the chunk tables.
+  const size_t tb = (3 * size_t(n_items) + n_chunked + 1) * 4;
+  dev_free(c, g->chunk_tables, g->chunk_tables_bytes);
+  dev_free(c, g->chunk_partial, g->chunk_partial_bytes);
+  g->chunk_tables = nullptr;
+  g->chunk_partial = nullptr;
+  g->plan_valid = false;
+  PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&g->chunk_tables), tb));
+  g->chunk_tables_bytes = tb;
+  PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&g->chunk_partial), std::max<size_t>(16, size_t(n_items) * 8)));
+  g->chunk_partial_bytes = std::max<size_t>(16, size_t(n_items) * 8);
+  uint32_t *tp = g->chunk_tables;
+  if (n_items) {
+    PRH_HIP(c, hipMemcpy(tp, item_row.data(), size_t(n_items) * 4, hipMemcpyHostToDevice));
+    PRH_HIP(c, hipMemcpy(tp + n_items, item_beg.data(), size_t(n_items) * 4, hipMemcpyHostToDevice));
+    PRH_HIP(c, hipMemcpy(tp + 2 * size_t(n_items), item_end.data(), size_t(n_items) * 4, hipMemcpyHostToDevice));
+  }
+  PRH_HIP(c, hipMemcpy(tp + 3 * size_t(n_items), row_first.data(), (size_t(n_chunked) + 1) * 4,
+                       hipMemcpyHostToDevice));
+  p.item_row = tp;
+  p.item_beg = tp + n_items;
+  p.item_end = tp + 2 * size_t(n_items);
+  p.row_first = tp + 3 * size_t(n_items);
+  p.partial = g->chunk_partial;
+  p.n_chunked = n_chunked;
+  add(prh::kChunkedRows, 0, 0, n_items, 1, n_items);
+  // Wave rows.
+  const uint32_t w_end = above(t.switch_degree);
+  auto row_section = [&](int kind, int width, uint32_t b, uint32_t e) {
+    if (e <= b) return;
+    const uint32_t unit = kind == prh::kWaveRows ? prh::kWavesPerBlock : prh::kWavesPerBlock * (prh::kWave / width);
+    const uint64_t nb = std::max<uint64_t>(1, (edges(b, e) + kEdgesPerBlock - 1) / kEdgesPerBlock);
+    uint64_t per = (uint64_t(e - b) + nb - 1) / nb;
+    per = std::max<uint64_t>(unit, (per + unit - 1) / unit * unit);
+    add(kind, width, b, e, uint32_t(per), uint32_t((uint64_t(e - b) + per - 1) / per));
+  };
+  row_section(prh::kWaveRows, 64, n_chunked, std::max(n_chunked, w_end));
+  // Group-row bands.
+  const uint32_t lb = std::max(n_chunked, w_end);
+  if (t.low_width) {
+    row_section(prh::kGroupRows, t.low_width, lb, N);
+  } else {
+    uint32_t b = lb;
+    for (int width = 32; width >= 1; width /= 2) {
+      // rows with degree in (width, 2*width] get `width` lanes; width 1 also takes degrees 1 and 2
+      const uint32_t e = width == 1 ? N : std::max(b, above(uint32_t(width)));
+      row_section(prh::kGroupRows, width, b, e);
+      b = e;
+    }
+  }
+  p.blocks = blocks;
+  g->plan = p;
+  g->plan_tuning = t;
+  g->plan_valid = true;
+  g->n_chunked = n_chunked;
+  return PRH_SUCCESS;
 }
 
-// Batch arrays in device memory: either the caller's (device API) or staged through the context.
+// Batch arrays staged in the context (host or device source), checked against N and mapped to layout ids.
 struct DevBatch {
   const uint32_t *delu = nullptr, *delv = nullptr, *insu = nullptr;
   size_t ND = 0, NI = 0;
 };
 
-int stage_batch(prh_context *c, const uint32_t *du, const uint32_t *dv, size_t nd, const uint32_t *iu, size_t ni,
-                bool host, DevBatch &b) {
+int stage_batch(prh_context *c, const prh_graph *g, const uint32_t *du, const uint32_t *dv, size_t nd,
+                const uint32_t *iu, size_t ni, bool host, DevBatch &b) {
   b.ND = nd;
   b.NI = ni;
   if (nd + ni == 0) return PRH_SUCCESS;
   if ((nd && (!du || !dv)) || (ni && !iu)) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null batch array");
-  if (!host) {
-    b.delu = du;
-    b.delv = dv;
-    b.insu = iu;
-    return PRH_SUCCESS;
-  }
   PRH_CHECK(ensure(c, c->batch, 2 * nd + ni));
   uint32_t *d = c->batch.p;
+  const hipMemcpyKind kind = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
   if (nd) {
-    PRH_HIP(c, hipMemcpyAsync(d, du, nd * 4, hipMemcpyHostToDevice, c->stream));
-    PRH_HIP(c, hipMemcpyAsync(d + nd, dv, nd * 4, hipMemcpyHostToDevice, c->stream));
+    PRH_HIP(c, hipMemcpyAsync(d, du, nd * 4, kind, c->stream));
+    PRH_HIP(c, hipMemcpyAsync(d + nd, dv, nd * 4, kind, c->stream));
   }
-  if (ni) PRH_HIP(c, hipMemcpyAsync(d + 2 * nd, iu, ni * 4, hipMemcpyHostToDevice, c->stream));
+  if (ni) PRH_HIP(c, hipMemcpyAsync(d + 2 * nd, iu, ni * 4, kind, c->stream));
+  const uint32_t N = g->N;
+  PRH_CHECK(ensure(c, c->flags, 4));
+  PRH_HIP(c, prh::launch_fill_i32(c->flags.p, 0, 1, c->stream));
+  PRH_HIP(c, prh::launch_validate_ids(d, 2 * nd + ni, N, c->flags.p, 1, c->stream));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PRH_HIP(c, hipStreamSynchronize(c->stream));
+  if (*reinterpret_cast<int *>(c->pinned)) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "batch vertex id >= N (%u)", N);
+  PRH_HIP(c, prh::launch_map_ids(d, 2 * nd + ni, g->inv, c->stream));
   b.delu = d;
   b.delv = d + nd;
   b.insu = d + 2 * nd;
   return PRH_SUCCESS;
 }
 
-int check_batch(prh_context *c, const DevBatch &b, uint32_t N) {
-  if (b.ND + b.NI == 0) return PRH_SUCCESS;
-  PRH_CHECK(ensure(c, c->flags, 4));
-  PRH_HIP(c, prh::launch_fill_i32(c->flags.p, 0, 1, c->stream));
-  PRH_HIP(c, prh::launch_validate_ids(b.delu, b.ND, N, c->flags.p, 1, c->stream));
-  PRH_HIP(c, prh::launch_validate_ids(b.delv, b.ND, N, c->flags.p, 1, c->stream));
-  PRH_HIP(c, prh::launch_validate_ids(b.insu, b.NI, N, c->flags.p, 1, c->stream));
-  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  PRH_HIP(c, hipStreamSynchronize(c->stream));
-  if (*reinterpret_cast<int *>(c->pinned)) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "batch vertex id >= N (%u)", N);
-  return PRH_SUCCESS;
-}
-
-// Partition the in-CSR (rank kernel schedule) and, for the dynamic methods, the out-CSR (extension schedule);
-// one readback of the part sizes, as the reference reads its partition point back (pagerankCuda.hxx:633).
-int partition_graph(prh_context *c, const prh_graph *g, bool dynamic, prh::Schedule &in_s, prh::Schedule &out_s) {
+// Out-degree schedule of the layout's out-CSR for the extension kernels; one readback of the part sizes, as the
+// reference reads its partition point back (pagerankCuda.hxx:633).
+int out_schedule(prh_context *c, const prh_graph *g, prh::Schedule &out_s) {
   const uint32_t N = g->N;
-  const uint32_t nt = prh::partition_tiles(N);
-  PRH_CHECK(ensure(c, c->tiles, size_t(3) * nt * 2));
+  PRH_CHECK(ensure(c, c->tiles, size_t(3) * prh::partition_tiles(N)));
   PRH_CHECK(ensure(c, c->counts, 8));
-  PRH_HIP(c, prh::launch_partition(g->in_off, N, c->tuning.switch_degree, c->tuning.huge_degree, c->sched_in.p,
-                                   c->tiles.p, c->counts.p, c->stream));
-  if (dynamic)
-    PRH_HIP(c, prh::launch_partition(g->out_off, N, 64u, 0xFFFFFFFFu, c->sched_out.p, c->tiles.p + size_t(3) * nt,
-                                     c->counts.p + 4, c->stream));
-  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->counts.p, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  PRH_HIP(c, prh::launch_partition(g->l_out_off, N, 64u, 0xFFFFFFFFu, c->sched_out.p, c->tiles.p, c->counts.p,
+                                   c->stream));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->counts.p, 8, hipMemcpyDeviceToHost, c->stream));
   PRH_HIP(c, hipStreamSynchronize(c->stream));
   const uint32_t *cnt = reinterpret_cast<const uint32_t *>(c->pinned);
-  in_s = prh::Schedule{c->sched_in.p, cnt[0], cnt[1], cnt[2]};
-  if (dynamic) out_s = prh::Schedule{c->sched_out.p, cnt[4], cnt[5] + cnt[6], 0};
+  out_s = prh::Schedule{c->sched_out.p, cnt[0], cnt[1], 0};
   return PRH_SUCCESS;
 }
 
-// Marking phase of the dynamic methods (pagerankCuda.hxx:821 with the fm lambdas :926-928, :955-957, :984-986).
+// Marking phase of the dynamic methods (pagerankCuda.hxx:821 with the fm lambdas :926-928, :955-957, :984-986),
+// on the layout's out-CSR with layout ids.
 int mark_affected(prh_context *c, const prh_graph *g, int method, const DevBatch &b, const prh::Schedule &out_s) {
   const uint32_t N = g->N;
   hipStream_t s = c->stream;
@@ -230,7 +328,7 @@ int mark_affected(prh_context *c, const prh_graph *g, int method, const DevBatch
   PRH_HIP(c, prh::launch_fill_u8(c->naff.p, 0, N, s));
   if (b.ND + b.NI == 0) return PRH_SUCCESS;
   PRH_HIP(c, prh::launch_mark_partial(c->naff.p, c->vaff.p, b.delu, b.delv, b.ND, b.insu, b.NI, s));
-  prh::ExtendArgs E{c->vaff.p, c->naff.p, g->out_off, g->out_dst, out_s, nullptr, nullptr};
+  prh::ExtendArgs E{c->vaff.p, c->naff.p, g->l_out_off, g->l_out_dst, out_s, nullptr, nullptr};
   PRH_HIP(c, prh::launch_extend(E, nullptr, nullptr, 0, 0.0, s));
   if (method != PRH_DYNAMIC_TRAVERSAL) return PRH_SUCCESS;
   // Traversal closure: breadth-first levels from every marked vertex until no vertex is newly marked.
@@ -240,7 +338,7 @@ int mark_affected(prh_context *c, const prh_graph *g, int method, const DevBatch
   for (;;) {
     PRH_HIP(c, prh::launch_fill_u8(fresh, 0, N, s));
     PRH_HIP(c, prh::launch_fill_i32(c->flags.p + 1, 0, 1, s));
-    prh::ExtendArgs T{c->vaff.p, front, g->out_off, g->out_dst, out_s, fresh, c->flags.p + 1};
+    prh::ExtendArgs T{c->vaff.p, front, g->l_out_off, g->l_out_dst, out_s, fresh, c->flags.p + 1};
     PRH_HIP(c, prh::launch_extend(T, nullptr, nullptr, 0, 0.0, s));
     PRH_HIP(c, hipMemcpyAsync(c->pinned, c->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     PRH_HIP(c, hipStreamSynchronize(s));
@@ -255,7 +353,6 @@ int ensure_workspace(prh_context *c, uint32_t N, bool dynamic, int L) {
     PRH_CHECK(ensure(c, c->rank[i], N));
     PRH_CHECK(ensure(c, c->contrib[i], N));
   }
-  PRH_CHECK(ensure(c, c->sched_in, N));
   if (dynamic) {
     PRH_CHECK(ensure(c, c->sched_out, N));
     PRH_CHECK(ensure(c, c->vaff, N));
@@ -264,6 +361,18 @@ int ensure_workspace(prh_context *c, uint32_t N, bool dynamic, int L) {
   }
   PRH_CHECK(ensure(c, c->loop, size_t(L) + 1));
   PRH_CHECK(ensure(c, c->flags, 4));
+  return PRH_SUCCESS;
+}
+
+// Start ranks in layout order: rank[0][i] = q[perm[i]] (q staged through contrib[1], which the initialisation
+// overwrites).
+int load_initial_ranks(prh_context *c, const prh_graph *g, const double *q, bool host) {
+  const double *src = q;
+  if (host) {
+    PRH_HIP(c, hipMemcpyAsync(c->contrib[1].p, q, size_t(g->N) * 8, hipMemcpyHostToDevice, c->stream));
+    src = c->contrib[1].p;
+  }
+  PRH_HIP(c, prh::launch_gather_f64(src, g->perm, g->N, c->rank[0].p, c->stream));
   return PRH_SUCCESS;
 }
 
@@ -294,11 +403,10 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   hipStream_t s = c->stream;
   PRH_HIP(c, hipSetDevice(c->device));
   PRH_CHECK(ensure_workspace(c, N, dynamic, L));
+  prh_graph *gm = const_cast<prh_graph *>(g);
+  PRH_CHECK(build_rank_plan(c, gm));
   DevBatch b;
-  if (dynamic) {
-    PRH_CHECK(stage_batch(c, du, dv, nd, iu, ni, host, b));
-    PRH_CHECK(check_batch(c, b, N));
-  }
+  if (dynamic) PRH_CHECK(stage_batch(c, g, du, dv, nd, iu, ni, host, b));
   if (q && !host) {  // device initial ranks: reject NaN/Inf (the reference would propagate them)
     PRH_HIP(c, prh::launch_fill_i32(c->flags.p, 0, 1, s));
     PRH_HIP(c, prh::launch_check_finite(q, N, c->flags.p, 1, s));
@@ -309,7 +417,6 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   if (q && host)
     for (uint32_t v = 0; v < N; ++v)
       if (!std::isfinite(q[v])) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "initial ranks not finite (vertex %u)", v);
-  const int width = low_width_for(c->tuning, N, g->M);
   if (c->ktiming && c->kev.size() < size_t(2) * L) {
     size_t old = c->kev.size();
     c->kev.resize(size_t(2) * L);
@@ -320,32 +427,33 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   unsigned long long *err = c->loop.p;
   int *done = reinterpret_cast<int *>(c->loop.p + L);
   const int check = std::max(1, c->tuning.check_interval);
+  const prh::RankPlan &plan = g->plan;
   for (int rep = 0; rep < o->repeat; ++rep) {
-    if (q)  // untimed, as the reference's q copy (pagerankCuda.hxx:810)
-      PRH_HIP(c, hipMemcpyAsync(c->rank[0].p, q, size_t(N) * 8, host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+    if (q) PRH_CHECK(load_initial_ranks(c, g, q, host));  // untimed, as the reference's q copy (pagerankCuda.hxx:810)
     PRH_HIP(c, hipEventRecord(c->ev[0], s));
-    prh::Schedule in_s{}, out_s{};
-    PRH_CHECK(partition_graph(c, g, dynamic, in_s, out_s));
-    PRH_HIP(c, prh::launch_init_ranks(c->rank[0].p, c->rank[1].p, c->contrib[0].p, c->contrib[1].p, g->outdeg, N,
+    // Initialisation. The degree classes the reference partitions here (pagerankCuda.hxx:811-814) are the row
+    // ranges of the layout, fixed when the graph was prepared; the extension kernels' out-degree split is per solve.
+    prh::Schedule out_s{};
+    if (dynamic) PRH_CHECK(out_schedule(c, g, out_s));
+    PRH_HIP(c, prh::launch_init_ranks(c->rank[0].p, c->rank[1].p, c->contrib[0].p, c->contrib[1].p, g->l_outdeg, N,
                                       q != nullptr, s));
     PRH_HIP(c, hipEventRecord(c->ev[1], s));
     if (dynamic) PRH_CHECK(mark_affected(c, g, method, b, out_s));
     PRH_HIP(c, hipEventRecord(c->ev[2], s));
     // Iterations.
     PRH_HIP(c, prh::launch_fill_u64(err, 0ull, size_t(L) + 1, s));
-    const prh::RankShape shape = rank_shape(in_s, width);
     const double P = o->damping;
     const double C0 = (1 - P) / N;
     int n = -1, launched = 0;
     for (int l = 0; l < L; ++l) {
       prh::RankArgs A{c->rank[(l + 1) & 1].p, c->rank[l & 1].p, c->contrib[(l + 1) & 1].p, c->contrib[l & 1].p,
-                      c->naff.p, c->vaff.p, g->in_off, g->in_src, g->outdeg, in_s, C0, P, o->frontier_tolerance,
+                      c->naff.p, c->vaff.p, g->l_in_off, g->l_in_src, g->l_outdeg, C0, P, o->frontier_tolerance,
                       o->prune_tolerance};
       if (c->ktiming) PRH_HIP(c, hipEventRecord(c->kev[2 * l], s));
-      PRH_HIP(c, prh::launch_rank_update(variant, A, shape, err, done, l, o->tolerance, s));
+      PRH_HIP(c, prh::launch_rank_update(variant, A, plan, err, done, l, o->tolerance, s));
       if (c->ktiming) PRH_HIP(c, hipEventRecord(c->kev[2 * l + 1], s));
       if (frontier) {
-        prh::ExtendArgs E{c->vaff.p, c->naff.p, g->out_off, g->out_dst, out_s, nullptr, nullptr};
+        prh::ExtendArgs E{c->vaff.p, c->naff.p, g->l_out_off, g->l_out_dst, out_s, nullptr, nullptr};
         PRH_HIP(c, prh::launch_extend(E, err, done, l, o->tolerance, s));
       }
       launched = l + 1;
@@ -380,9 +488,16 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
       }
     }
   }
-  // Final ranks are in the buffer written by the last iteration (pagerankCuda.hxx:688, :826).
+  // Final ranks are in the buffer written by the last iteration (pagerankCuda.hxx:688, :826); back to the
+  // caller's vertex order (scatter through perm), as the reference scatters to span (:827).
   const double *res = c->rank[iterations & 1].p;
-  PRH_HIP(c, hipMemcpyAsync(ranks, res, size_t(N) * 8, host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+  if (host) {
+    double *tmp = c->contrib[(iterations + 1) & 1].p;
+    PRH_HIP(c, prh::launch_scatter_f64(res, g->perm, N, tmp, s));
+    PRH_HIP(c, hipMemcpyAsync(ranks, tmp, size_t(N) * 8, hipMemcpyDeviceToHost, s));
+  } else {
+    PRH_HIP(c, prh::launch_scatter_f64(res, g->perm, N, ranks, s));
+  }
   PRH_HIP(c, hipStreamSynchronize(s));
   stats.iterations = iterations;
   stats.initialization_time = float(ti / o->repeat);
@@ -562,6 +677,97 @@ int ppr_impl(prh_context *c, const prh_graph *g, int k, const uint32_t *so, cons
   PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
   PRH_HIP(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
   if (st) *st = prh_stats{most, t01 + t12, t01, 0.0f, t12};
+  return PRH_SUCCESS;
+}
+
+// The degree-ordered layout of a freshly uploaded graph (see layout.hip): perm = vertices by in-degree, descending,
+// stable in id (counting sort on the host; degrees above kDegreeBuckets are few and sorted directly), then the
+// relabelled in-CSR (rows sorted by new source id), out-degrees and, if present, out-CSR on the device.
+constexpr uint32_t kDegreeBuckets = 1u << 20;
+
+int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uint32_t *outdeg, bool with_out) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint32_t N = g->N;
+  const uint64_t M = g->M;
+  hipStream_t s = c->stream;
+  std::vector<uint32_t> perm(N), big;
+  std::vector<uint64_t> cnt(size_t(kDegreeBuckets) + 1, 0);
+  for (uint32_t v = 0; v < N; ++v) {
+    const uint32_t d = in_off[v + 1] - in_off[v];
+    if (d > kDegreeBuckets) big.push_back(v);
+    else ++cnt[d];
+  }
+  std::stable_sort(big.begin(), big.end(), [&](uint32_t a, uint32_t b) {
+    return in_off[a + 1] - in_off[a] > in_off[b + 1] - in_off[b];
+  });
+  uint64_t pos = big.size();
+  for (int64_t d = kDegreeBuckets; d >= 0; --d) {
+    const uint64_t n = cnt[d];
+    cnt[d] = pos;
+    pos += n;
+  }
+  std::copy(big.begin(), big.end(), perm.begin());
+  for (uint32_t v = 0; v < N; ++v) {
+    const uint32_t d = in_off[v + 1] - in_off[v];
+    if (d <= kDegreeBuckets) perm[cnt[d]++] = v;
+  }
+  std::vector<uint32_t>().swap(big);
+  std::vector<uint64_t>().swap(cnt);
+  g->sorted_deg.resize(N);
+  std::vector<uint32_t> loff(size_t(N) + 1), oout(with_out ? size_t(N) + 1 : 0);
+  uint64_t a = 0, b = 0;
+  for (uint32_t i = 0; i < N; ++i) {
+    const uint32_t v = perm[i];
+    const uint32_t d = in_off[v + 1] - in_off[v];
+    g->sorted_deg[i] = d;
+    loff[i] = uint32_t(a);
+    a += d;
+    if (with_out) {
+      oout[i] = uint32_t(b);
+      b += outdeg[v];
+    }
+  }
+  loff[N] = uint32_t(a);
+  if (with_out) oout[N] = uint32_t(b);
+  // Device side.
+  auto alloc = [&](uint32_t **p, size_t n) { return dev_alloc(c, reinterpret_cast<void **>(p), n * 4); };
+  PRH_CHECK(alloc(&g->perm, N));
+  PRH_CHECK(alloc(&g->inv, N));
+  PRH_CHECK(alloc(&g->l_in_off, size_t(N) + 1));
+  PRH_CHECK(alloc(&g->l_in_src, M));
+  PRH_CHECK(alloc(&g->l_outdeg, N));
+  if (with_out) {
+    PRH_CHECK(alloc(&g->l_out_off, size_t(N) + 1));
+    PRH_CHECK(alloc(&g->l_out_dst, M));
+  }
+  PRH_HIP(c, hipMemcpyAsync(g->perm, perm.data(), size_t(N) * 4, hipMemcpyHostToDevice, s));
+  PRH_HIP(c, hipMemcpyAsync(g->l_in_off, loff.data(), (size_t(N) + 1) * 4, hipMemcpyHostToDevice, s));
+  if (with_out) PRH_HIP(c, hipMemcpyAsync(g->l_out_off, oout.data(), (size_t(N) + 1) * 4, hipMemcpyHostToDevice, s));
+  PRH_HIP(c, prh::launch_invert(g->perm, N, g->inv, s));
+  PRH_HIP(c, prh::launch_gather_u32(g->outdeg, g->perm, N, g->l_outdeg, s));
+  if (with_out) PRH_HIP(c, prh::launch_relabel_rows(g->out_off, g->out_dst, g->perm, g->inv, g->l_out_off, N,
+                                                    g->l_out_dst, s));
+  // Relabelled in-rows into a scratch array, then sorted per row into place.
+  uint32_t *tmp = nullptr;
+  void *sort_tmp = nullptr;
+  size_t sort_bytes = 0;
+  int st = alloc(&tmp, M);
+  if (st == PRH_SUCCESS) {
+    hipError_t e = prh::launch_relabel_rows(g->in_off, g->in_src, g->perm, g->inv, g->l_in_off, N, tmp, s);
+    if (e == hipSuccess) e = prh::sort_rows(nullptr, sort_bytes, tmp, g->l_in_src, M, N, g->l_in_off, s);
+    if (e != hipSuccess) st = fail(c, PRH_ERROR_DEVICE, "layout: %s", hipGetErrorString(e));
+  }
+  if (st == PRH_SUCCESS) st = dev_alloc(c, &sort_tmp, sort_bytes);
+  if (st == PRH_SUCCESS) {
+    hipError_t e = prh::sort_rows(sort_tmp, sort_bytes, tmp, g->l_in_src, M, N, g->l_in_off, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) st = fail(c, PRH_ERROR_DEVICE, "layout row sort: %s", hipGetErrorString(e));
+  }
+  (void)hipStreamSynchronize(s);
+  dev_free(c, sort_tmp, sort_bytes);
+  dev_free(c, tmp, (M ? M : 4) * 4);
+  PRH_CHECK(st);
+  g->prep_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return PRH_SUCCESS;
 }
 
@@ -785,8 +991,8 @@ void prh_default_options(prh_options *o) {
 
 void prh_default_tuning(prh_tuning *t) {
   if (!t) return;
-  t->switch_degree = 64;
-  t->huge_degree = 2048;
+  t->switch_degree = 128;
+  t->huge_degree = 16384;
   t->low_width = 0;
   t->check_interval = 8;
 }
@@ -914,7 +1120,6 @@ int prh_graph_create(prh_context *c, uint32_t N, uint64_t M, const uint32_t *in_
   }
   auto up = [&](uint32_t **dst, const uint32_t *src, size_t n) -> int {
     PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(dst), n * 4));
-    g->bytes += (n ? n : 4) * 4;
     PRH_HIP(c, hipMemcpyAsync(*dst, src, n * 4, hipMemcpyHostToDevice, c->stream));
     return PRH_SUCCESS;
   };
@@ -940,6 +1145,7 @@ int prh_graph_create(prh_context *c, uint32_t N, uint64_t M, const uint32_t *in_
                   bits & 8 ? " in-id-range" : "", bits & 16 ? " out-id-range" : "", bits & 32 ? " out-offsets" : "");
     }
   }
+  if (st == PRH_SUCCESS) st = build_layout(c, g, in_off, outdeg, out_off != nullptr);
   if (st != PRH_SUCCESS) {
     prh_graph_destroy(g);
     return st;
@@ -959,6 +1165,15 @@ void prh_graph_destroy(prh_graph *g) {
   dev_free(c, g->out_off, (N + 1) * 4);
   dev_free(c, g->out_dst, (M ? M : 4) * 4);
   dev_free(c, g->in_eid, (M ? M : 4) * 4);
+  dev_free(c, g->perm, N * 4);
+  dev_free(c, g->inv, N * 4);
+  dev_free(c, g->l_in_off, (N + 1) * 4);
+  dev_free(c, g->l_in_src, (M ? M : 4) * 4);
+  dev_free(c, g->l_outdeg, N * 4);
+  dev_free(c, g->l_out_off, (N + 1) * 4);
+  dev_free(c, g->l_out_dst, (M ? M : 4) * 4);
+  dev_free(c, g->chunk_tables, g->chunk_tables_bytes);
+  dev_free(c, g->chunk_partial, g->chunk_partial_bytes);
   delete g;
 }
 
@@ -1011,12 +1226,13 @@ int prh_mark_affected(prh_context *c, const prh_graph *g, prh_method method, con
   PRH_HIP(c, hipSetDevice(c->device));
   PRH_CHECK(ensure_workspace(c, g->N, true, 1));
   DevBatch b;
-  PRH_CHECK(stage_batch(c, du, dv, nd, iu, ni, true, b));
-  PRH_CHECK(check_batch(c, b, g->N));
-  prh::Schedule in_s{}, out_s{};
-  PRH_CHECK(partition_graph(c, g, true, in_s, out_s));
+  PRH_CHECK(stage_batch(c, g, du, dv, nd, iu, ni, true, b));
+  prh::Schedule out_s{};
+  PRH_CHECK(out_schedule(c, g, out_s));
   PRH_CHECK(mark_affected(c, g, method, b, out_s));
-  PRH_HIP(c, hipMemcpyAsync(affected, c->vaff.p, g->N, hipMemcpyDeviceToHost, c->stream));
+  // back to the caller's vertex order (naff is free after marking)
+  PRH_HIP(c, prh::launch_scatter_u8(c->vaff.p, g->perm, g->N, c->naff.p, c->stream));
+  PRH_HIP(c, hipMemcpyAsync(affected, c->naff.p, g->N, hipMemcpyDeviceToHost, c->stream));
   PRH_HIP(c, hipStreamSynchronize(c->stream));
   return PRH_SUCCESS;
 }

@@ -32,10 +32,34 @@ struct Schedule {
   uint32_t n_low, n_high, n_huge;
 };
 
-// Launch shape of the rank-update kernel: lanes per low-degree vertex and block counts of the three sections.
-struct RankShape {
-  int low_width;  // 1, 2, 4, 8, 16, 32 or 64 lanes per low-degree vertex
-  uint32_t nb_huge, nb_high, nb_low;
+// The rank kernel works on the degree-ordered layout (layout.hip): rows sorted by in-degree, descending, so each
+// degree class is a contiguous row range. A launch is a table of sections, each a block range doing one kind of
+// work over a row (or item) range.
+enum SectionKind : int {
+  kGroupRows = 0,    // `width` lanes per row (1..32), rows split into contiguous per-block ranges
+  kWaveRows = 1,     // one wave per row
+  kChunkedRows = 2,  // one workgroup per fixed-size edge chunk of a row; a second launch finishes the rows
+};
+
+struct RankSection {
+  int kind;
+  int width;                 // lanes per row for kGroupRows
+  uint32_t begin, end;       // rows (kGroupRows, kWaveRows) or items (kChunkedRows)
+  uint32_t block_begin;      // first block of the section
+  uint32_t per_block;        // rows or items per block
+};
+
+constexpr int kMaxSections = 8;
+
+struct RankPlan {
+  RankSection sec[kMaxSections];
+  int nsec;
+  uint32_t blocks;
+  // Chunked rows (the heaviest in-degrees, rows 0..n_chunked-1): item i covers edges [item_beg[i], item_end[i]) of
+  // row item_row[i] and leaves its sum in partial[i]; row r owns items [row_first[r], row_first[r+1]).
+  const uint32_t *item_row, *item_beg, *item_end, *row_first;
+  double *partial;
+  uint32_t n_chunked;
 };
 
 struct RankArgs {
@@ -45,10 +69,9 @@ struct RankArgs {
   const double *cr;   // old contributions r[v]/deg[v]
   uint8_t *naff;      // neighbour-affected flags (FRONTIER)
   uint8_t *vaff;      // vertex-affected flags (DYNAMIC)
-  const uint32_t *off;  // in-CSR offsets [N+1]
+  const uint32_t *off;  // in-CSR offsets [N+1] (degree-ordered layout)
   const uint32_t *src;  // in-CSR sources [M]
   const uint32_t *deg;  // out-degree [N]
-  Schedule sched;
   double C0, P, D, C;  // teleport share (1-P)/N, damping, frontier and prune tolerances
 };
 
@@ -72,8 +95,9 @@ hipError_t launch_partition(const uint32_t *off, uint32_t N, uint32_t t1, uint32
 // ---- rank.hip ----
 hipError_t launch_init_ranks(double *r, double *a, double *cr, double *ca, const uint32_t *deg, uint32_t N,
                              bool from_r, hipStream_t s);
-hipError_t launch_rank_update(int variant, const RankArgs &A, const RankShape &shape, unsigned long long *err,
+hipError_t launch_rank_update(int variant, const RankArgs &A, const RankPlan &plan, unsigned long long *err,
                               int *done, int l, double tol, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s);
 hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s);
@@ -145,6 +169,19 @@ hipError_t launch_weighted_init(const double *start, const double *W, uint32_t N
 hipError_t launch_weighted_grad(const uint32_t *out_off, const uint32_t *out_dst, const double *W, const double *r,
                                 const double *lam, const double *g, double P, uint32_t N, double *grad_w,
                                 double *grad_s, hipStream_t s);
+
+// ---- layout.hip ----
+hipError_t launch_invert(const uint32_t *perm, uint32_t N, uint32_t *inv, hipStream_t s);
+hipError_t launch_relabel_rows(const uint32_t *off, const uint32_t *idx, const uint32_t *perm, const uint32_t *inv,
+                               const uint32_t *new_off, uint32_t N, uint32_t *out, hipStream_t s);
+// Segmented ascending sort of each row (rocPRIM); call with tmp = nullptr to size tmp_bytes.
+hipError_t sort_rows(void *tmp, size_t &tmp_bytes, const uint32_t *in, uint32_t *out, size_t M, uint32_t N,
+                     const uint32_t *off, hipStream_t s);
+hipError_t launch_gather_u32(const uint32_t *src, const uint32_t *idx, size_t n, uint32_t *dst, hipStream_t s);
+hipError_t launch_gather_f64(const double *src, const uint32_t *idx, size_t n, double *dst, hipStream_t s);
+hipError_t launch_scatter_f64(const double *src, const uint32_t *idx, size_t n, double *dst, hipStream_t s);
+hipError_t launch_scatter_u8(const uint8_t *src, const uint32_t *idx, size_t n, uint8_t *dst, hipStream_t s);
+hipError_t launch_map_ids(uint32_t *ids, size_t n, const uint32_t *inv, hipStream_t s);
 
 // ---- frontier.hip ----
 hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *delu, const uint32_t *delv,

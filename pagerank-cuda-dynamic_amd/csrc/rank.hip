@@ -1,18 +1,20 @@
 // rank.hip -- the pull-based, atomic-free rank update with the fused L-inf reduction, plus small map kernels.
 //
-// One launch per iteration covers all vertices, in three sections of workgroups over the degree schedule:
-//   huge  (in-degree > t2): one workgroup (4 waves) per vertex, cross-wave sum through LDS;
-//   high  (t1 < in-degree <= t2): one wave per vertex, DPP/swizzle wave sum;
-//   low   (in-degree <= t1): W lanes per vertex (W = 1..32), DPP group sum.
-// The huge section comes first so its long rows start early instead of forming the tail.
+// One launch per iteration covers all vertices of the degree-ordered layout (rows sorted by in-degree,
+// descending; layout.hip) through a table of sections, heaviest first so that long rows start early:
+//   chunked rows  (in-degree above the chunk threshold): one workgroup per fixed-size edge chunk; each chunk's
+//                 sum goes to a partial slot and the workgroup that arrives last for the row adds the row's
+//                 partials in chunk order and finishes it (deterministic; the arrival count only picks who adds);
+//   wave rows     one wave per row, DPP/swizzle wave sum;
+//   group rows    W lanes per row (W = 1..32), one band per W over a contiguous row range, DPP group sum.
 //
 // Per vertex v (pagerankUpdateRanks{Thread,Block}CukU, pagerankCuda.hxx:87-109, :162-188):
 //   cv = sum_{u in in(v)} r[u]/deg[u]     -- gathered from the precomputed contributions cr[u] = r[u]/deg[u],
 //                                            the same FP64 value the reference divides per edge
 //   a[v] = C0 + P*cv                      -- or the DF-P closed form (C0 + P*(cv - r[v]/d))/(1 - P/d)
 //   prune / frontier flags                 -- vaff[v] = 0 if |a-r|/max(r,a) <= C; naff[v] = (|a-r|/max > D)
-// Each vertex has exactly one writer, so the update needs no atomics; the only atomic is one 64-bit max per
-// workgroup for the L-inf change of the iteration.
+// Each vertex has exactly one writer, so the update needs no atomics on ranks; the atomics are one 64-bit max
+// per workgroup for the L-inf change and one arrival count per chunk of a chunked row.
 #include "prh_device.hpp"
 #include "prh_internal.hpp"
 
@@ -58,81 +60,75 @@ __device__ __forceinline__ bool affected(const RankArgs &A, uint32_t v) {
   else return true;
 }
 
-// Sum of cr[src[e]] for e = b + first, b + first + step, ... < end, four gathers in flight per lane.
+// Streamed arrays (source ids, offsets) are read with the non-temporal hint so that they do not displace the
+// gathered contributions from L2; the gathers themselves use the default policy.
+__device__ __forceinline__ uint32_t stream_u32(const uint32_t *p) { return __builtin_nontemporal_load(p); }
+
+// Sum of cr[src[e]] for e = first, first + STEP, ... < end, four gathers in flight per lane.
 template <int STEP>
 __device__ __forceinline__ double gather_sum(const RankArgs &A, uint32_t e, uint32_t end) {
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   for (; e + 3 * STEP < end; e += 4 * STEP) {
-    const uint32_t u0 = A.src[e], u1 = A.src[e + STEP], u2 = A.src[e + 2 * STEP], u3 = A.src[e + 3 * STEP];
+    const uint32_t u0 = stream_u32(A.src + e), u1 = stream_u32(A.src + e + STEP);
+    const uint32_t u2 = stream_u32(A.src + e + 2 * STEP), u3 = stream_u32(A.src + e + 3 * STEP);
     s0 += A.cr[u0];
     s1 += A.cr[u1];
     s2 += A.cr[u2];
     s3 += A.cr[u3];
   }
-  for (; e < end; e += STEP) s0 += A.cr[A.src[e]];
+  for (; e < end; e += STEP) s0 += A.cr[stream_u32(A.src + e)];
   return (s0 + s1) + (s2 + s3);
 }
 
+// One workgroup per chunk item: its partial sum goes to the item's slot; rows are finished by
+// finish_chunked_kernel after this launch (the kernel boundary makes every chunk's partial visible).
 template <int VARIANT>
-__device__ __forceinline__ void huge_section(const RankArgs &A, uint32_t blk, uint32_t nblk, double &emax,
+__device__ __forceinline__ void chunked_rows(const RankArgs &A, const RankPlan &P, const RankSection &S, uint32_t blk,
                                              double *lds) {
-  const uint32_t n = A.sched.n_huge;
-  const uint32_t *order = A.sched.order + A.sched.n_low + A.sched.n_high;
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  for (uint32_t i = blk; i < n; i += nblk) {
-    const uint32_t v = order[i];
-    if (!affected<VARIANT>(A, v)) {
-      if (threadIdx.x == 0) copy_vertex<VARIANT>(A, v);
-      continue;
-    }
-    const uint32_t beg = A.off[v], end = A.off[v + 1];
-    double acc = gather_sum<kBlock>(A, beg + threadIdx.x, end);
+  const uint32_t ib = S.begin + blk * S.per_block, ie = min(S.end, ib + S.per_block);
+  for (uint32_t it = ib; it < ie; ++it) {  // block-uniform
+    const uint32_t v = P.item_row[it];
+    if (!affected<VARIANT>(A, v)) continue;
+    double acc = gather_sum<kBlock>(A, P.item_beg[it] + threadIdx.x, P.item_end[it]);
     acc = group_sum<kWave>(acc);
     if (lane == 0) lds[w] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) emax = fmax(emax, finish_vertex<VARIANT>(A, v, (lds[0] + lds[1]) + (lds[2] + lds[3])));
+    if (threadIdx.x == 0) P.partial[it] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
     __syncthreads();
   }
 }
 
 template <int VARIANT>
-__device__ __forceinline__ void high_section(const RankArgs &A, uint32_t blk, uint32_t nblk, double &emax) {
-  const uint32_t n = A.sched.n_high;
-  const uint32_t *order = A.sched.order + A.sched.n_low;
+__device__ __forceinline__ void wave_rows(const RankArgs &A, const RankSection &S, uint32_t blk, double &emax) {
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  for (uint32_t i = blk * kWavesPerBlock + w; i < n; i += nblk * kWavesPerBlock) {
-    const uint32_t v = order[i];
+  const uint32_t rb = S.begin + blk * S.per_block, re = min(S.end, rb + S.per_block);
+  for (uint32_t v = rb + w; v < re; v += kWavesPerBlock) {  // wave-uniform
     if (!affected<VARIANT>(A, v)) {
       if (lane == 0) copy_vertex<VARIANT>(A, v);
       continue;
     }
-    const uint32_t beg = A.off[v], end = A.off[v + 1];
-    double acc = gather_sum<kWave>(A, beg + lane, end);
+    double acc = gather_sum<kWave>(A, stream_u32(A.off + v) + lane, stream_u32(A.off + v + 1));
     acc = group_sum<kWave>(acc);
     if (lane == 0) emax = fmax(emax, finish_vertex<VARIANT>(A, v, acc));
   }
 }
 
 template <int VARIANT, int W>
-__device__ __forceinline__ void low_section(const RankArgs &A, uint32_t blk, uint32_t nblk, double &emax) {
-  constexpr int G = kWave / W;  // vertices per wave per round
-  const uint32_t n = A.sched.n_low;
+__device__ __forceinline__ void group_rows(const RankArgs &A, const RankSection &S, uint32_t blk, double &emax) {
+  constexpr int G = kWave / W;  // rows per wave per round
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   const int g = lane / W, j = lane % W;
-  const uint32_t stride = nblk * kWavesPerBlock * G;
-  for (uint32_t i0 = (blk * kWavesPerBlock + w) * G; i0 < n; i0 += stride) {  // wave-uniform loop
-    const uint32_t i = i0 + g;
-    const bool valid = i < n;
-    uint32_t v = 0, beg = 0, end = 0;
+  const uint32_t rb = S.begin + blk * S.per_block, re = min(S.end, rb + S.per_block);
+  for (uint32_t r0 = rb + w * G; r0 < re; r0 += kWavesPerBlock * G) {  // wave-uniform
+    const uint32_t v = r0 + g;
+    const bool valid = v < re;
     bool upd = false;
-    if (valid) {
-      v = A.sched.order[i];
-      beg = A.off[v];
-      end = A.off[v + 1];
-      upd = affected<VARIANT>(A, v);
-    }
     double acc = 0.0;
-    if (upd) acc = gather_sum<W>(A, beg + j, end);
+    if (valid) {
+      upd = affected<VARIANT>(A, v);
+      if (upd) acc = gather_sum<W>(A, stream_u32(A.off + v) + j, stream_u32(A.off + v + 1));
+    }
     acc = group_sum<W>(acc);  // all lanes active again here
     if (valid && j == 0) {
       if (upd) emax = fmax(emax, finish_vertex<VARIANT>(A, v, acc));
@@ -141,32 +137,62 @@ __device__ __forceinline__ void low_section(const RankArgs &A, uint32_t blk, uin
   }
 }
 
-template <int VARIANT, int W>
-__global__ __launch_bounds__(kBlock) void rank_update_kernel(RankArgs A, RankShape S, LoopCtl ctl) {
+template <int VARIANT>
+__global__ __launch_bounds__(kBlock) void rank_update_kernel(RankArgs A, RankPlan P, LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;
   __shared__ double lds_sum[kWavesPerBlock];
   __shared__ double lds_max[kWavesPerBlock];
+  int s = 0;
+  while (s + 1 < P.nsec && blockIdx.x >= P.sec[s + 1].block_begin) ++s;
+  const RankSection &S = P.sec[s];
+  const uint32_t blk = blockIdx.x - S.block_begin;
   double emax = 0.0;
-  const uint32_t b = blockIdx.x;
-  if (b < S.nb_huge) huge_section<VARIANT>(A, b, S.nb_huge, emax, lds_sum);
-  else if (b < S.nb_huge + S.nb_high) high_section<VARIANT>(A, b - S.nb_huge, S.nb_high, emax);
-  else low_section<VARIANT, W>(A, b - S.nb_huge - S.nb_high, S.nb_low, emax);
+  if (S.kind == kChunkedRows) {
+    chunked_rows<VARIANT>(A, P, S, blk, lds_sum);
+  } else if (S.kind == kWaveRows) {
+    wave_rows<VARIANT>(A, S, blk, emax);
+  } else {
+    switch (S.width) {
+      case 1: group_rows<VARIANT, 1>(A, S, blk, emax); break;
+      case 2: group_rows<VARIANT, 2>(A, S, blk, emax); break;
+      case 4: group_rows<VARIANT, 4>(A, S, blk, emax); break;
+      case 8: group_rows<VARIANT, 8>(A, S, blk, emax); break;
+      case 16: group_rows<VARIANT, 16>(A, S, blk, emax); break;
+      default: group_rows<VARIANT, 32>(A, S, blk, emax); break;
+    }
+  }
+  emax = block_max(emax, lds_max);
+  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[ctl.l], emax);
+}
+
+// Chunked rows, after the rank kernel of the same iteration: one wave per row adds the row's chunk partials (in a
+// fixed order) and finishes the row, or copies it through when unaffected. Runs only if that kernel ran.
+template <int VARIANT>
+__global__ __launch_bounds__(kBlock) void finish_chunked_kernel(RankArgs A, RankPlan P, uint32_t n_rows, LoopCtl ctl) {
+  if (!iteration_begin(ctl)) return;  // same outcome as the rank kernel's entry test of this iteration
+  __shared__ double lds_max[kWavesPerBlock];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  double emax = 0.0;
+  for (uint32_t v = blockIdx.x * kWavesPerBlock + w; v < n_rows; v += gridDim.x * kWavesPerBlock) {
+    if (!affected<VARIANT>(A, v)) {
+      if (lane == 0) copy_vertex<VARIANT>(A, v);
+      continue;
+    }
+    double s = 0.0;
+    for (uint32_t k = P.row_first[v] + lane; k < P.row_first[v + 1]; k += kWave) s += P.partial[k];
+    s = group_sum<kWave>(s);
+    if (lane == 0) emax = fmax(emax, finish_vertex<VARIANT>(A, v, s));
+  }
   emax = block_max(emax, lds_max);
   if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[ctl.l], emax);
 }
 
 template <int VARIANT>
-hipError_t launch_rank_variant(const RankArgs &A, const RankShape &S, const LoopCtl &ctl, hipStream_t s) {
-  const uint32_t grid = S.nb_huge + S.nb_high + S.nb_low;
-  if (grid == 0) return hipSuccess;
-  switch (S.low_width) {
-    case 1: rank_update_kernel<VARIANT, 1><<<grid, kBlock, 0, s>>>(A, S, ctl); break;
-    case 2: rank_update_kernel<VARIANT, 2><<<grid, kBlock, 0, s>>>(A, S, ctl); break;
-    case 4: rank_update_kernel<VARIANT, 4><<<grid, kBlock, 0, s>>>(A, S, ctl); break;
-    case 8: rank_update_kernel<VARIANT, 8><<<grid, kBlock, 0, s>>>(A, S, ctl); break;
-    case 16: rank_update_kernel<VARIANT, 16><<<grid, kBlock, 0, s>>>(A, S, ctl); break;
-    case 32: rank_update_kernel<VARIANT, 32><<<grid, kBlock, 0, s>>>(A, S, ctl); break;
-    default: return hipErrorInvalidValue;
+hipError_t launch_variant(const RankArgs &A, const RankPlan &P, const LoopCtl &ctl, hipStream_t s) {
+  rank_update_kernel<VARIANT><<<P.blocks, kBlock, 0, s>>>(A, P, ctl);
+  if (P.n_chunked) {
+    const uint32_t g = (P.n_chunked + kWavesPerBlock - 1) / kWavesPerBlock;
+    finish_chunked_kernel<VARIANT><<<g > 4096 ? 4096 : g, kBlock, 0, s>>>(A, P, P.n_chunked, ctl);
   }
   return hipGetLastError();
 }
@@ -252,14 +278,15 @@ inline uint32_t map_grid(size_t n) {
 
 }  // namespace
 
-hipError_t launch_rank_update(int variant, const RankArgs &A, const RankShape &S, unsigned long long *err, int *done,
+hipError_t launch_rank_update(int variant, const RankArgs &A, const RankPlan &P, unsigned long long *err, int *done,
                               int l, double tol, hipStream_t s) {
+  if (P.blocks == 0) return hipSuccess;
   const LoopCtl ctl{err, done, l, tol};
   switch (variant) {
-    case kPlain: return launch_rank_variant<kPlain>(A, S, ctl, s);
-    case kTraversal: return launch_rank_variant<kTraversal>(A, S, ctl, s);
-    case kFrontier: return launch_rank_variant<kFrontier>(A, S, ctl, s);
-    case kPrune: return launch_rank_variant<kPrune>(A, S, ctl, s);
+    case kPlain: return launch_variant<kPlain>(A, P, ctl, s);
+    case kTraversal: return launch_variant<kTraversal>(A, P, ctl, s);
+    case kFrontier: return launch_variant<kFrontier>(A, P, ctl, s);
+    case kPrune: return launch_variant<kPrune>(A, P, ctl, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -274,6 +301,12 @@ hipError_t launch_init_ranks(double *r, double *a, double *cr, double *ca, const
 hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   fill_kernel<uint8_t><<<map_grid(n), kBlock, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  fill_kernel<uint32_t><<<map_grid(n), kBlock, 0, s>>>(p, v, n);
   return hipGetLastError();
 }
 

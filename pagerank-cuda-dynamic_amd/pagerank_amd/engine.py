@@ -62,8 +62,8 @@ class PagerankResult:
 
 @dataclass
 class Tuning:
-    switch_degree: int = 64
-    huge_degree: int = 2048
+    switch_degree: int = 128
+    huge_degree: int = 16384
     low_width: int = 0
     check_interval: int = 8
 

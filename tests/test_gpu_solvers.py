@@ -103,6 +103,31 @@ def test_dynamic_matches_oracle(ctx, oracle_mod, method, frac):
         assert np.abs(res.ranks - exact).max() <= RANK_TOL
 
 
+PLANS = [(128, 16384, 0), (8, 64, 0), (16, 100, 4), (2, 2, 1), (32, 33, 32), (64, 1000, 2)]
+
+
+@pytest.mark.parametrize("plan", PLANS, ids=[f"sw{a}_huge{b}_w{c}" for a, b, c in PLANS])
+@pytest.mark.parametrize("method", [0, 4], ids=["static", "DFP"])
+def test_launch_plans_agree(ctx, oracle_mod, plan, method):
+    """Every launch plan (chunked rows with their second combining launch, wave rows, fixed or banded lanes per
+    row) gives the oracle's ranks to 1e-9: the plan only changes the summation order."""
+    from pagerank_amd import Batch, HostGraph, Tuning, apply_batch, solve
+    x = HostGraph.rmat(12, 24, seed=4)
+    q, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, x)
+    b = Batch.random(x, 200, 0.8, seed=3)
+    y = apply_batch(x, b)
+    old = ctx.get_tuning()
+    print(f"[plan] {plan} method={method} N={y.N} M={y.M} max_in={int(np.diff(y.in_off).max())}", flush=True)
+    try:
+        ctx.set_tuning(Tuning(plan[0], plan[1], plan[2], old.check_interval))
+        res = solve(ctx.upload(y), method, batch=b if method else None, q=q if method else None)
+    finally:
+        ctx.set_tuning(old)
+    ref, it, _ = oracle_mod.pagerank_cuda(method, y, b if method else None, q if method else None)
+    assert _close_iters(res.iterations, it), (res.iterations, it)
+    assert np.abs(res.ranks - ref).max() <= RANK_TOL
+
+
 @pytest.mark.parametrize("method", [2, 3, 4], ids=["DT", "DF", "DFP"])
 def test_empty_batch_returns_q(ctx, oracle_mod, method):
     """With nothing affected, one iteration copies q through (L-inf change 0 < tolerance)."""
