@@ -12,6 +12,11 @@ roofline: the dominant kernel is the rank update (one launch per iteration). Its
 out-degree, 8-byte old-rank read and 8-byte new-rank write; the L-inf reduction is fused). Its duration is measured
 with HIP events recorded on the engine's stream around every launch that did work inside the timed region.
 
+roofline.traffic: HBM bytes per rank-kernel launch, from rocprofv3 counter runs of this script on the same workload
+(FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor that the same counter needs to give the known byte
+count of a plain device copy, measured with --copy-calibration). The record, with the commands that produced it, is
+profiles/rank_update_traffic.json; it is reported only when its workload string equals this run's.
+
 cpu_baseline: the oracle's OpenMP restatement of the reference's pagerankStaticOmp (bit-identical results to the
 compiled reference on the test graphs), run on this host's cores on the same graph for a bounded number of
 iterations (about 10 s).
@@ -61,6 +66,31 @@ def cpu_baseline(g, budget_s: float):
             "sample": f"{it} synchronous iterations of the OpenMP static solver on the same graph ({ms / 1e3:.1f} s)"}
 
 
+def copy_calibration(gib: float) -> int:
+    """A plain device-to-device copy of a known size, for scaling the HBM byte counters."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy calibration: 3 copies of {n * 8} bytes (read {n * 8}, write {n * 8} per copy)")
+    return 0
+
+
+def measured_traffic(desc: str):
+    path = REPO / "profiles" / "rank_update_traffic.json"
+    try:
+        rec = json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None
+    if rec.get("workload") != desc:
+        return None
+    return rec.get("hbm_bytes_per_launch")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,11 +102,15 @@ def main() -> int:
     ap.add_argument("--cols", type=int, default=4000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=float, metavar="GIB", default=0.0,
+                    help="only run a plain device copy of GIB GiB (for scaling the HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported (the multi-GPU path is a design "
-              "only, see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported (the engine has no multi-GPU "
+              "path: a solve runs on one device).", file=sys.stderr)
         return 2
+    if args.copy_calibration > 0:
+        return copy_calibration(args.copy_calibration)
 
     from pagerank_amd import Context, PagerankOptions, pagerankStaticHip
 
@@ -124,7 +158,7 @@ def main() -> int:
                    "iterations_per_solve": iters / args.steps,
                    "gedges_per_s_iter_compute_only": round(g.M * iters / (tcomp / 1e3) / 1e9, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(desc),
                      "kernel": "rank_update_kernel", "kernel_avg_us": round(kavg_s * 1e6, 2),
                      "bytes_per_launch": bytes_alg},
     }

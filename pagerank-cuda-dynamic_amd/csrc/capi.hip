@@ -450,8 +450,8 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
                       c->naff.p, c->vaff.p, g->l_in_off, g->l_in_src, g->l_outdeg, C0, P, o->frontier_tolerance,
                       o->prune_tolerance};
       if (c->ktiming) PRH_HIP(c, hipEventRecord(c->kev[2 * l], s));
-      PRH_HIP(c, prh::launch_rank_update(variant, A, plan, err, done, l, o->tolerance, s));
-      if (c->ktiming) PRH_HIP(c, hipEventRecord(c->kev[2 * l + 1], s));
+      PRH_HIP(c, prh::launch_rank_update(variant, A, plan, err, done, l, o->tolerance, s,
+                                         c->ktiming ? c->kev[2 * l + 1] : nullptr));
       if (frontier) {
         prh::ExtendArgs E{c->vaff.p, c->naff.p, g->l_out_off, g->l_out_dst, out_s, nullptr, nullptr};
         PRH_HIP(c, prh::launch_extend(E, err, done, l, o->tolerance, s));

@@ -95,8 +95,10 @@ hipError_t launch_partition(const uint32_t *off, uint32_t N, uint32_t t1, uint32
 // ---- rank.hip ----
 hipError_t launch_init_ranks(double *r, double *a, double *cr, double *ca, const uint32_t *deg, uint32_t N,
                              bool from_r, hipStream_t s);
+// Rank update of iteration l: the rank kernel, then (if the plan has chunked rows) their combining kernel. `mark`,
+// when non-null, is recorded between the two, so that it times the rank kernel alone.
 hipError_t launch_rank_update(int variant, const RankArgs &A, const RankPlan &plan, unsigned long long *err,
-                              int *done, int l, double tol, hipStream_t s);
+                              int *done, int l, double tol, hipStream_t s, hipEvent_t mark = nullptr);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s);

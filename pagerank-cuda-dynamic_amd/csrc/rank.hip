@@ -188,8 +188,12 @@ __global__ __launch_bounds__(kBlock) void finish_chunked_kernel(RankArgs A, Rank
 }
 
 template <int VARIANT>
-hipError_t launch_variant(const RankArgs &A, const RankPlan &P, const LoopCtl &ctl, hipStream_t s) {
+hipError_t launch_variant(const RankArgs &A, const RankPlan &P, const LoopCtl &ctl, hipStream_t s, hipEvent_t mark) {
   rank_update_kernel<VARIANT><<<P.blocks, kBlock, 0, s>>>(A, P, ctl);
+  if (mark) {
+    const hipError_t e = hipEventRecord(mark, s);
+    if (e != hipSuccess) return e;
+  }
   if (P.n_chunked) {
     const uint32_t g = (P.n_chunked + kWavesPerBlock - 1) / kWavesPerBlock;
     finish_chunked_kernel<VARIANT><<<g > 4096 ? 4096 : g, kBlock, 0, s>>>(A, P, P.n_chunked, ctl);
@@ -279,14 +283,14 @@ inline uint32_t map_grid(size_t n) {
 }  // namespace
 
 hipError_t launch_rank_update(int variant, const RankArgs &A, const RankPlan &P, unsigned long long *err, int *done,
-                              int l, double tol, hipStream_t s) {
-  if (P.blocks == 0) return hipSuccess;
+                              int l, double tol, hipStream_t s, hipEvent_t mark) {
+  if (P.blocks == 0) return mark ? hipEventRecord(mark, s) : hipSuccess;
   const LoopCtl ctl{err, done, l, tol};
   switch (variant) {
-    case kPlain: return launch_variant<kPlain>(A, P, ctl, s);
-    case kTraversal: return launch_variant<kTraversal>(A, P, ctl, s);
-    case kFrontier: return launch_variant<kFrontier>(A, P, ctl, s);
-    case kPrune: return launch_variant<kPrune>(A, P, ctl, s);
+    case kPlain: return launch_variant<kPlain>(A, P, ctl, s, mark);
+    case kTraversal: return launch_variant<kTraversal>(A, P, ctl, s, mark);
+    case kFrontier: return launch_variant<kFrontier>(A, P, ctl, s, mark);
+    case kPrune: return launch_variant<kPrune>(A, P, ctl, s, mark);
     default: return hipErrorInvalidValue;
   }
 }

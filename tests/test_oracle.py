@@ -237,3 +237,42 @@ def test_weighted_oracle_matches_dense_autograd(oracle_mod):
     # a planted mistake (dropping the normalisation term of dP/dw) must fail the pin
     wrong = gw + np.repeat(((lam - g) / 0.85) * 0.85 * r / W, np.diff(y.out_off.astype(np.int64)))
     assert np.abs(wrong - gwd).max() > 1e-9
+
+
+def test_ppr_oracle_matches_dense_solve(oracle_mod):
+    """Personalized PageRank r = (1-alpha) s + alpha A r, solved densely (numpy.linalg.solve), for a seed-set teleport
+    and for a point mass."""
+    from pagerank_amd import HostGraph
+    y = HostGraph.rmat(8, 8, seed=13)
+    A = y.dense_transition()
+    rng = np.random.default_rng(5)
+    for seeds in (rng.choice(y.N, 17, replace=False), np.array([3])):
+        s = np.zeros(y.N)
+        s[seeds] = 1.0 / len(seeds)
+        exact = np.linalg.solve(np.eye(y.N) - 0.85 * A, 0.15 * s)
+        r, _ = oracle_mod.ppr(y, s, tolerance=1e-14, max_iterations=2000)
+        assert np.abs(r - exact).max() <= 1e-12
+        assert abs(r.sum() - 1.0) <= 1e-12
+
+
+def test_dfp_update_matches_compiled_reference(oracle_mod):
+    """One DF-P iteration from q: on the initially affected vertices, the oracle's update equals the compiled
+    reference's pagerankPruneUpdateRank (pagerankPrune.hxx:39) up to FMA rounding. The CPU path marks neighbours
+    during its sweep, so the comparison is restricted to the vertices both mark before the first update."""
+    if not oracle_mod.ref_available():
+        pytest.skip("oracle/_ref not built here (the reference sources are only present in the build container)")
+    from pagerank_amd import Batch, HostGraph, apply_batch
+    x = HostGraph.rmat(10, 8, seed=17)
+    q, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, x)
+    b = Batch.random(x, 40, 0.8, seed=2)
+    y = apply_batch(x, b)
+    rr, itr, _ = oracle_mod.ref_pagerank(oracle_mod.PRUNE_DYNAMIC_FRONTIER, x, b, q, max_iterations=1)
+    r, it, vaff = oracle_mod.pagerank_cuda(oracle_mod.PRUNE_DYNAMIC_FRONTIER, y, b, q, max_iterations=1)
+    init = vaff.astype(bool)
+    assert it == itr == 1 and init.sum() > 10
+    moved = np.abs(r - q) > 0
+    assert moved[init].mean() > 0.5  # the update did change the affected ranks
+    assert np.abs(r[init] - rr[init]).max() <= 1e-15 * np.abs(rr[init]).max()
+    # a plain Jacobi update (the self-loop term not solved in closed form) would fail the same comparison
+    j, _, _ = oracle_mod.pagerank_cuda(oracle_mod.DYNAMIC_FRONTIER, y, b, q, max_iterations=1)
+    assert np.abs(j[init] - rr[init]).max() > 1e-6 * np.abs(rr[init]).max()
