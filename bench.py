@@ -102,6 +102,8 @@ def main() -> int:
     ap.add_argument("--cols", type=int, default=4000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--source-slices", type=int, default=None, help="override the tuning's source_slices")
+    ap.add_argument("--huge-degree", type=int, default=None, help="override the tuning's huge_degree")
     ap.add_argument("--copy-calibration", type=float, metavar="GIB", default=0.0,
                     help="only run a plain device copy of GIB GiB (for scaling the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -118,6 +120,14 @@ def main() -> int:
     g, desc = make_graph(args)
     log(f"graph N={g.N} M={g.M} built in {time.time() - t0:.1f}s")
     ctx = Context(0)
+    if args.source_slices is not None or args.huge_degree is not None:
+        t = ctx.get_tuning()
+        if args.source_slices is not None:
+            t.source_slices = args.source_slices
+        if args.huge_degree is not None:
+            t.huge_degree = args.huge_degree
+        ctx.set_tuning(t)
+    tuning = ctx.get_tuning()
     yg = ctx.upload(g, with_out=False)
     o = PagerankOptions()
     for i in range(args.warmup):
@@ -155,6 +165,8 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": desc, "vertices": g.N, "edges": g.M, "tolerance": 1e-10, "damping": 0.85,
+                   "tuning": {"switch_degree": tuning.switch_degree, "huge_degree": tuning.huge_degree,
+                              "source_slices": tuning.source_slices},
                    "iterations_per_solve": iters / args.steps,
                    "gedges_per_s_iter_compute_only": round(g.M * iters / (tcomp / 1e3) / 1e9, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -79,6 +79,10 @@ typedef struct prh_tuning {
   int low_width;          /* lanes per vertex of in-degree <= switch_degree: 1, 2, 4, 8, 16 or 32; 0 = by degree
                              band, W lanes for in-degree in (W, 2W] (weighted kernels: from |E|/|V|) [0] */
   int check_interval;     /* iterations launched between host reads of the device convergence flag [8] */
+  int source_slices;      /* Static / Naive-dynamic: 0 = row kernel; 1, 2, 4 or 8 = sum each row slice by slice,
+                             source u in slice (u / 16) mod source_slices, slice x on the workgroups b with
+                             b mod source_slices = x (8: one slice per XCD and its L2) [0] */
+  uint32_t block_edges;   /* edges per workgroup of the rank kernels (>= 256) [4096] */
 } prh_tuning;
 
 typedef struct prh_context prh_context;

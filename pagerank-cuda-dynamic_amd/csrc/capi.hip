@@ -81,6 +81,13 @@ struct prh_graph {
   double *chunk_partial = nullptr;
   size_t chunk_partial_bytes = 0;
   uint32_t n_chunked = 0;
+  // Source-sliced plan of the plain update, built on first use for the current slices / chunk threshold.
+  int sliced_slices = 0;
+  uint32_t sliced_huge = 0;
+  uint32_t sliced_block_edges = 0;
+  bool sliced_valid = false;
+  prh::SlicedPlan sliced{};
+  std::vector<std::pair<void *, size_t>> sliced_bufs;  // its device allocations
 };
 
 namespace {
@@ -165,25 +172,226 @@ uint32_t cap_blocks(uint64_t n, uint64_t per, uint32_t cap) {
   return uint32_t(std::min<uint64_t>(g, cap));
 }
 
+
+// Number of rows with in-degree > x in the descending degree list.
+uint32_t rows_above(const std::vector<uint32_t> &deg, uint32_t x) {
+  return uint32_t(std::lower_bound(deg.begin(), deg.end(), x, [](uint32_t a, uint32_t b) { return a > b; }) -
+                  deg.begin());
+}
+
+void free_sliced(prh_context *c, prh_graph *g) {
+  for (auto &b : g->sliced_bufs) dev_free(c, b.first, b.second);
+  g->sliced_bufs.clear();
+  g->sliced_valid = false;
+}
+
+// Source-sliced plan (prh::SlicedPlan) for the context's tuning, from the degree-ordered layout:
+//   1. per row and slice, the number of sources (slice_count);
+//   2. per slice, exclusive scans over rows of "has an item" and of the item's edge count (rows >= n_huge);
+//   3. the slice-major source array, item offsets, row masks and per-64-row item bases (slice_scatter, wbase);
+//   4. chunk tables for the rows above the chunk threshold, whose sources stay in their own edge range grouped
+//      by slice, in chunks of huge_degree edges;
+//   5. per slice, a section table: its chunks, then bands of items by row in-degree d: a wave per item for
+//      d > 64 S, W lanes for d in (W S, 2 W S], W = 32 .. 1 (items hold about d / S sources).
+int build_sliced_plan(prh_context *c, prh_graph *g) {
+  const prh_tuning &t = c->tuning;
+  const int S = t.source_slices;
+  if (g->sliced_valid && g->sliced_slices == S && g->sliced_huge == t.huge_degree &&
+      g->sliced_block_edges == t.block_edges)
+    return PRH_SUCCESS;
+  hipStream_t s = c->stream;
+  PRH_HIP(c, hipStreamSynchronize(s));
+  free_sliced(c, g);
+  const uint32_t N = g->N;
+  const uint64_t M = g->M;
+  const std::vector<uint32_t> &deg = g->sorted_deg;
+  const uint32_t n_huge = rows_above(deg, t.huge_degree);
+  const uint32_t chunk = std::max<uint32_t>(t.huge_degree, 64);
+  const uint64_t kEdgesPerBlock = t.block_edges;
+  auto keep = [&](void **p, size_t bytes) -> int {
+    PRH_CHECK(dev_alloc(c, p, bytes));
+    g->sliced_bufs.emplace_back(*p, bytes ? bytes : 16);
+    return PRH_SUCCESS;
+  };
+  // Temporaries, freed at the end (also on error, by the guard).
+  struct Temps {
+    prh_context *c;
+    std::vector<std::pair<void *, size_t>> v;
+    ~Temps() {
+      for (auto &b : v) dev_free(c, b.first, b.second);
+    }
+  } tmp{c, {}};
+  auto temp = [&](void **p, size_t bytes) -> int {
+    PRH_CHECK(dev_alloc(c, p, bytes));
+    tmp.v.emplace_back(*p, bytes ? bytes : 16);
+    return PRH_SUCCESS;
+  };
+  const size_t N1 = size_t(N) + 1;
+  uint32_t *cnt, *item, *eoff, *has, *ecnt;
+  PRH_CHECK(temp(reinterpret_cast<void **>(&cnt), size_t(S) * N * 4));
+  PRH_CHECK(temp(reinterpret_cast<void **>(&item), size_t(S) * N1 * 4));
+  PRH_CHECK(temp(reinterpret_cast<void **>(&eoff), size_t(S) * N1 * 4));
+  PRH_CHECK(temp(reinterpret_cast<void **>(&has), N1 * 4));
+  PRH_CHECK(temp(reinterpret_cast<void **>(&ecnt), N1 * 4));
+  size_t scan_bytes = 0;
+  PRH_HIP(c, prh::scan_u32(nullptr, scan_bytes, has, item, N1, s));
+  void *scan_tmp;
+  PRH_CHECK(temp(&scan_tmp, scan_bytes));
+  PRH_HIP(c, prh::launch_slice_count(g->l_in_off, g->l_in_src, N, S, cnt, s));
+  for (int x = 0; x < S; ++x) {
+    PRH_HIP(c, prh::launch_slice_flags(cnt + size_t(x) * N, N, n_huge, has, ecnt, s));
+    PRH_HIP(c, prh::scan_u32(scan_tmp, scan_bytes, has, item + size_t(x) * N1, N1, s));
+    PRH_HIP(c, prh::scan_u32(scan_tmp, scan_bytes, ecnt, eoff + size_t(x) * N1, N1, s));
+  }
+  // Values of the scans at given rows, read back: at(a, x, r) = a[x][r].
+  auto at = [&](const uint32_t *a, int x, uint32_t r, uint32_t &out) -> int {
+    PRH_HIP(c, hipMemcpyAsync(c->pinned, a + size_t(x) * N1 + r, 4, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipStreamSynchronize(s));
+    out = *reinterpret_cast<uint32_t *>(c->pinned);
+    return PRH_SUCCESS;
+  };
+  uint64_t huge_edges = 0;
+  for (uint32_t r = 0; r < n_huge; ++r) huge_edges += deg[r];
+  prh::SliceBases B{};
+  uint64_t n_items = 0, edge = huge_edges;
+  for (int x = 0; x < S; ++x) {
+    uint32_t k, e;
+    PRH_CHECK(at(item, x, N, k));
+    PRH_CHECK(at(eoff, x, N, e));
+    B.item[x] = uint32_t(n_items);
+    B.edge[x] = uint32_t(edge);
+    n_items += k;
+    edge += e;
+  }
+  if (edge != M) return fail(c, PRH_ERROR_DEVICE, "sliced layout: %llu edges placed of %llu",
+                             (unsigned long long)edge, (unsigned long long)M);
+  prh::SlicedPlan P{};
+  P.slices = S;
+  P.N = N;
+  P.n_huge = n_huge;
+  const uint32_t groups = (N - n_huge + prh::kWave - 1) / prh::kWave;
+  void *p;
+  PRH_CHECK(keep(&p, M * 4));
+  uint32_t *s_src = static_cast<uint32_t *>(p);
+  PRH_CHECK(keep(&p, (n_items + 1) * 4));
+  uint32_t *item_off = static_cast<uint32_t *>(p);
+  PRH_CHECK(keep(&p, n_items * 8));
+  P.partial = static_cast<double *>(p);
+  PRH_CHECK(keep(&p, N));
+  uint8_t *mask = static_cast<uint8_t *>(p);
+  PRH_CHECK(keep(&p, size_t(groups) * S * 4));
+  uint32_t *wbase = static_cast<uint32_t *>(p);
+  PRH_HIP(c, prh::launch_slice_scatter(g->l_in_off, g->l_in_src, N, n_huge, S, cnt, item, eoff, B, s_src, item_off,
+                                       mask, s));
+  *reinterpret_cast<uint32_t *>(c->pinned) = uint32_t(M);
+  PRH_HIP(c, hipMemcpyAsync(item_off + n_items, c->pinned, 4, hipMemcpyHostToDevice, s));
+  PRH_HIP(c, prh::launch_slice_wbase(item, N, n_huge, S, B, wbase, s));
+  P.src = s_src;
+  P.item_off = item_off;
+  P.mask = mask;
+  P.wbase = wbase;
+  // Chunk tables: chunk ids in (row, slice, part) order; chunk_list groups them by slice.
+  std::vector<uint32_t> hcnt(size_t(S) * n_huge);
+  for (int x = 0; x < S && n_huge; ++x)
+    PRH_HIP(c, hipMemcpy(hcnt.data() + size_t(x) * n_huge, cnt + size_t(x) * N, size_t(n_huge) * 4,
+                         hipMemcpyDeviceToHost));
+  std::vector<uint32_t> cbeg, cend, cslice, row_first(size_t(n_huge) + 1);
+  uint64_t off = 0;
+  for (uint32_t r = 0; r < n_huge; ++r) {
+    row_first[r] = uint32_t(cbeg.size());
+    for (int x = 0; x < S; ++x) {
+      const uint32_t len = hcnt[size_t(x) * n_huge + r];
+      for (uint32_t e = 0; e < len; e += chunk) {
+        cbeg.push_back(uint32_t(off + e));
+        cend.push_back(uint32_t(off + std::min(len, e + chunk)));
+        cslice.push_back(uint32_t(x));
+      }
+      off += len;
+    }
+  }
+  row_first[n_huge] = uint32_t(cbeg.size());
+  const uint32_t n_chunks = uint32_t(cbeg.size());
+  std::vector<uint32_t> clist;
+  std::vector<uint32_t> cfirst(size_t(S) + 1);
+  for (int x = 0; x < S; ++x) {
+    cfirst[x] = uint32_t(clist.size());
+    for (uint32_t i = 0; i < n_chunks; ++i)
+      if (cslice[i] == uint32_t(x)) clist.push_back(i);
+  }
+  cfirst[S] = uint32_t(clist.size());
+  PRH_CHECK(keep(&p, (size_t(n_chunks) * 3 + n_huge + 1) * 4));
+  uint32_t *ct = static_cast<uint32_t *>(p);
+  if (n_chunks) {
+    PRH_HIP(c, hipMemcpy(ct, clist.data(), size_t(n_chunks) * 4, hipMemcpyHostToDevice));
+    PRH_HIP(c, hipMemcpy(ct + n_chunks, cbeg.data(), size_t(n_chunks) * 4, hipMemcpyHostToDevice));
+    PRH_HIP(c, hipMemcpy(ct + 2 * size_t(n_chunks), cend.data(), size_t(n_chunks) * 4, hipMemcpyHostToDevice));
+  }
+  PRH_HIP(c, hipMemcpy(ct + 3 * size_t(n_chunks), row_first.data(), (size_t(n_huge) + 1) * 4, hipMemcpyHostToDevice));
+  PRH_CHECK(keep(&p, size_t(n_chunks) * 8));
+  P.chunk_partial = static_cast<double *>(p);
+  P.chunk_list = ct;
+  P.chunk_beg = ct + n_chunks;
+  P.chunk_end = ct + 2 * size_t(n_chunks);
+  P.row_chunk_first = ct + 3 * size_t(n_chunks);
+  // Section tables. Band b covers rows [rb[b], rb[b+1]); the first band takes a wave per item.
+  std::vector<uint32_t> rb{n_huge};
+  std::vector<int> width{64};
+  for (int w = 32; w >= 1; w /= 2) {
+    rb.push_back(std::max(n_huge, rows_above(deg, uint32_t(2 * w * S))));
+    width.push_back(w);
+  }
+  rb.push_back(N);
+  uint32_t max_blocks = 0;
+  for (int x = 0; x < S; ++x) {
+    uint32_t blocks = 0;
+    int ns = 0;
+    auto add = [&](int kind, int w, uint32_t b, uint32_t e, uint32_t per) {
+      if (e <= b) return;
+      P.sec[x][ns++] = prh::RankSection{kind, w, b, e, blocks, per};
+      blocks += (e - b + per - 1) / per;
+    };
+    add(prh::kChunkedRows, 0, cfirst[x], cfirst[x + 1], 1);
+    for (size_t b = 0; b + 1 < rb.size(); ++b) {
+      if (rb[b + 1] <= rb[b]) continue;
+      uint32_t k0, k1, e0, e1;
+      PRH_CHECK(at(item, x, rb[b], k0));
+      PRH_CHECK(at(item, x, rb[b + 1], k1));
+      PRH_CHECK(at(eoff, x, rb[b], e0));
+      PRH_CHECK(at(eoff, x, rb[b + 1], e1));
+      if (k1 <= k0) continue;
+      const int w = width[b];
+      const uint64_t unit = w == 64 ? prh::kWavesPerBlock : uint64_t(prh::kWavesPerBlock) * (prh::kWave / w);
+      const uint64_t nb = std::max<uint64_t>(1, (uint64_t(e1 - e0) + kEdgesPerBlock - 1) / kEdgesPerBlock);
+      uint64_t per = (uint64_t(k1 - k0) + nb - 1) / nb;
+      per = std::max<uint64_t>(unit, (per + unit - 1) / unit * unit);
+      add(w == 64 ? prh::kWaveRows : prh::kGroupRows, w, B.item[x] + k0, B.item[x] + k1, uint32_t(per));
+    }
+    P.nsec[x] = ns;
+    max_blocks = std::max(max_blocks, blocks);
+  }
+  P.blocks_per_slice = max_blocks;
+  PRH_HIP(c, hipStreamSynchronize(s));
+  g->sliced = P;
+  g->sliced_slices = S;
+  g->sliced_huge = t.huge_degree;
+  g->sliced_block_edges = t.block_edges;
+  g->sliced_valid = true;
+  return PRH_SUCCESS;
+}
+
 // Rank-kernel plan over the degree-ordered layout for the context's tuning (cached in the graph):
 //   rows with in-degree > huge_degree: chunks of huge_degree edges, one workgroup each;
 //   switch_degree < in-degree <= huge_degree: one wave per row;
 //   in-degree <= switch_degree: bands of equal lanes-per-row W, W = pow2ceil(d)/2 in 1..32 (or the fixed
 //   low_width), each band a contiguous row range.
-// Blocks of the row sections get about kEdgesPerBlock edges each.
-constexpr uint64_t kEdgesPerBlock = 4096;
-
 int build_rank_plan(prh_context *c, prh_graph *g) {
   const prh_tuning &t = c->tuning;
   if (g->plan_valid && std::memcmp(&g->plan_tuning, &t, sizeof(t)) == 0) return PRH_SUCCESS;
   const uint32_t N = g->N;
   const std::vector<uint32_t> &deg = g->sorted_deg;
+  const uint64_t kEdgesPerBlock = t.block_edges;
   PRH_HIP(c, hipStreamSynchronize(c->stream));  // the old tables may still be in use
-  // number of rows with degree > x (deg is descending: the first row with degree <= x)
-  auto above = [&](uint32_t x) -> uint32_t {
-    return uint32_t(std::lower_bound(deg.begin(), deg.end(), x, [](uint32_t a, uint32_t b) { return a > b; }) -
-                    deg.begin());
-  };
+  auto above = [&](uint32_t x) { return rows_above(deg, x); };
   auto edges = [&](uint32_t b, uint32_t e) {
     uint64_t s = 0;
     for (uint32_t i = b; i < e; ++i) s += deg[i];
@@ -404,7 +612,9 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   PRH_HIP(c, hipSetDevice(c->device));
   PRH_CHECK(ensure_workspace(c, N, dynamic, L));
   prh_graph *gm = const_cast<prh_graph *>(g);
-  PRH_CHECK(build_rank_plan(c, gm));
+  const bool sliced = variant == prh::kPlain && c->tuning.source_slices > 0;
+  if (sliced) PRH_CHECK(build_sliced_plan(c, gm));
+  else PRH_CHECK(build_rank_plan(c, gm));
   DevBatch b;
   if (dynamic) PRH_CHECK(stage_batch(c, g, du, dv, nd, iu, ni, host, b));
   if (q && !host) {  // device initial ranks: reject NaN/Inf (the reference would propagate them)
@@ -450,8 +660,9 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
                       c->naff.p, c->vaff.p, g->l_in_off, g->l_in_src, g->l_outdeg, C0, P, o->frontier_tolerance,
                       o->prune_tolerance};
       if (c->ktiming) PRH_HIP(c, hipEventRecord(c->kev[2 * l], s));
-      PRH_HIP(c, prh::launch_rank_update(variant, A, plan, err, done, l, o->tolerance, s,
-                                         c->ktiming ? c->kev[2 * l + 1] : nullptr));
+      hipEvent_t mark = c->ktiming ? c->kev[2 * l + 1] : nullptr;
+      if (sliced) PRH_HIP(c, prh::launch_sliced_update(A, g->sliced, err, done, l, o->tolerance, s, mark));
+      else PRH_HIP(c, prh::launch_rank_update(variant, A, plan, err, done, l, o->tolerance, s, mark));
       if (frontier) {
         prh::ExtendArgs E{c->vaff.p, c->naff.p, g->l_out_off, g->l_out_dst, out_s, nullptr, nullptr};
         PRH_HIP(c, prh::launch_extend(E, err, done, l, o->tolerance, s));
@@ -995,6 +1206,8 @@ void prh_default_tuning(prh_tuning *t) {
   t->huge_degree = 16384;
   t->low_width = 0;
   t->check_interval = 8;
+  t->source_slices = 0;
+  t->block_edges = 4096;
 }
 
 int prh_context_create(int device, prh_context **out) {
@@ -1085,8 +1298,10 @@ size_t prh_context_memory_used(const prh_context *c) { return c ? c->used : 0; }
 int prh_context_set_tuning(prh_context *c, const prh_tuning *t) {
   if (!c || !t) return PRH_ERROR_INVALID_ARGUMENT;
   const int w = t->low_width;
+  const int sl = t->source_slices;
   if (!(w == 0 || w == 1 || w == 2 || w == 4 || w == 8 || w == 16 || w == 32) || t->check_interval < 1 ||
-      t->huge_degree < t->switch_degree)
+      t->huge_degree < t->switch_degree || !(sl == 0 || sl == 1 || sl == 2 || sl == 4 || sl == 8) ||
+      t->block_edges < 256)
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad tuning");
   c->tuning = *t;
   return PRH_SUCCESS;
@@ -1174,6 +1389,7 @@ void prh_graph_destroy(prh_graph *g) {
   dev_free(c, g->l_out_dst, (M ? M : 4) * 4);
   dev_free(c, g->chunk_tables, g->chunk_tables_bytes);
   dev_free(c, g->chunk_partial, g->chunk_partial_bytes);
+  free_sliced(c, g);
   delete g;
 }
 

@@ -86,8 +86,12 @@ __device__ __forceinline__ double block_max(double x, double *lds4) {
 }
 
 // Non-negative doubles order like their bit patterns, so an unsigned 64-bit max is an exact L-inf reduction.
+// The slot only grows, so any value read from it is a lower bound of its final value: a workgroup whose maximum
+// does not exceed what it reads has nothing to add and skips the device-scope atomic (tens of thousands of
+// workgroups per launch would otherwise queue on one address).
 __device__ __forceinline__ void atomic_max_nonneg(unsigned long long *slot, double x) {
-  if (x > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(x));
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  if (x > 0.0 && b > *(const unsigned long long *)slot) atomicMax(slot, b);
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -103,22 +107,24 @@ struct LoopCtl {
 };
 
 // Entry test for the first kernel of iteration l; all threads of a block agree.
+// done and err[l-1] were last written by earlier launches, whose completion made them visible: plain loads (both
+// issued together) suffice. A volatile read here would be a system-scope load at the start of every wave.
 __device__ __forceinline__ bool iteration_begin(const LoopCtl &c) {
-  if (*(volatile int *)c.done) return false;
-  if (c.l > 0) {
-    const double e = __longlong_as_double((long long)c.err[c.l - 1]);
-    if (e < c.tol) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) *(volatile int *)c.done = c.l;
-      return false;
-    }
+  const int done = *c.done;
+  const double e = c.l > 0 ? __longlong_as_double((long long)c.err[c.l - 1]) : c.tol;
+  if (done) return false;
+  if (e < c.tol) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *(volatile int *)c.done = c.l;
+    return false;
   }
   return true;
 }
 
 // Entry test for kernels that follow the rank update within iteration l (frontier extension).
 __device__ __forceinline__ bool iteration_continues(const LoopCtl &c) {
-  if (*(volatile int *)c.done) return false;
-  return __longlong_as_double((long long)c.err[c.l]) >= c.tol;
+  const int done = *c.done;
+  const double e = __longlong_as_double((long long)c.err[c.l]);
+  return !done && e >= c.tol;
 }
 
 }  // namespace prh

@@ -85,6 +85,40 @@ struct ExtendArgs {
   int *changed;          // optional: set to 1 when a vertex is newly marked
 };
 
+// Source-sliced plan of the plain update (Static / Naive-dynamic). Source u belongs to slice
+// (u >> 4) & (slices - 1): 16 FP64 contributions = one 128-byte line, so every line of the contribution array
+// belongs to exactly one slice. Workgroup b of the gather launch works on slice b % slices; with 8 slices that is
+// the XCD the workgroup is dispatched to (round-robin), so each XCD's L2 caches only its own eighth of the lines.
+//   gather launch: per slice, one partial sum per (row, slice) item over the row's sources in that slice
+//                  (chunk items for rows above the chunk threshold);
+//   finish launch: per row, the sum of its (at most 8) partials in slice order, then the update and the L-inf.
+constexpr int kMaxSlices = 8;
+constexpr uint32_t kSliceLineShift = 4;
+
+struct SlicedPlan {
+  RankSection sec[kMaxSlices][kMaxSections];  // per slice: chunk section, then item bands (kWaveRows/kGroupRows)
+  int nsec[kMaxSlices];
+  int slices;                       // 1, 2, 4 or 8
+  uint32_t blocks_per_slice;        // gather grid = slices * blocks_per_slice
+  const uint32_t *src;              // [M] sources, slice-major (chunked rows first, then slice 0 items, slice 1 ...)
+  const uint32_t *item_off;         // [n_items + 1]: item k covers src[item_off[k] .. item_off[k+1])
+  double *partial;                  // [n_items]
+  const uint32_t *chunk_list;       // chunk ids grouped by slice (indexed by kChunkedRows sections)
+  const uint32_t *chunk_beg, *chunk_end;  // [n_chunks] by chunk id: row-major (row, slice, part)
+  double *chunk_partial;            // [n_chunks]
+  const uint32_t *row_chunk_first;  // [n_huge + 1]
+  uint32_t n_huge;                  // rows 0..n_huge-1 are summed from chunks
+  const uint8_t *mask;              // [N]: bit x = row has an item in slice x (0 for chunked rows)
+  const uint32_t *wbase;            // [ceil((N - n_huge) / 64) * slices]: item id of each 64-row group's first item
+  uint32_t N;
+};
+
+// Edge and item bases of the slices in the slice-major arrays.
+struct SliceBases {
+  uint32_t edge[kMaxSlices];  // first edge of slice x's items in SlicedPlan::src
+  uint32_t item[kMaxSlices];  // first item id of slice x
+};
+
 // ---- partition.hip ----
 // Stable 3-way partition of [0,N) by degree d = off[v+1]-off[v]: d <= t1, t1 < d <= t2, d > t2.
 // tiles: scratch of 3 * partition_tiles(N) uint32; counts_dev: 3 uint32 outputs (device).
@@ -99,6 +133,10 @@ hipError_t launch_init_ranks(double *r, double *a, double *cr, double *ca, const
 // when non-null, is recorded between the two, so that it times the rank kernel alone.
 hipError_t launch_rank_update(int variant, const RankArgs &A, const RankPlan &plan, unsigned long long *err,
                               int *done, int l, double tol, hipStream_t s, hipEvent_t mark = nullptr);
+// Plain update through the source-sliced plan: the gather launch, then the finish launch; `mark` (optional) is
+// recorded between them.
+hipError_t launch_sliced_update(const RankArgs &A, const SlicedPlan &plan, unsigned long long *err, int *done, int l,
+                                double tol, hipStream_t s, hipEvent_t mark = nullptr);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s);
@@ -184,6 +222,22 @@ hipError_t launch_gather_f64(const double *src, const uint32_t *idx, size_t n, d
 hipError_t launch_scatter_f64(const double *src, const uint32_t *idx, size_t n, double *dst, hipStream_t s);
 hipError_t launch_scatter_u8(const uint8_t *src, const uint32_t *idx, size_t n, uint8_t *dst, hipStream_t s);
 hipError_t launch_map_ids(uint32_t *ids, size_t n, const uint32_t *inv, hipStream_t s);
+// Sliced layout. cnt: [slices][N] edges of row v in slice x.
+hipError_t launch_slice_count(const uint32_t *off, const uint32_t *src, uint32_t N, int slices, uint32_t *cnt,
+                              hipStream_t s);
+// For slice x: has[v] = (v >= n_huge && cnt[x][v] > 0), ecnt[v] = (v >= n_huge ? cnt[x][v] : 0), for v < N, and
+// has[N] = ecnt[N] = 0 (so that exclusive scans of N + 1 values end with the totals).
+hipError_t launch_slice_flags(const uint32_t *cnt_x, uint32_t N, uint32_t n_huge, uint32_t *has, uint32_t *ecnt,
+                              hipStream_t s);
+// Exclusive prefix sum of n uint32 values (rocPRIM); call with tmp = nullptr to size tmp_bytes.
+hipError_t scan_u32(void *tmp, size_t &tmp_bytes, const uint32_t *in, uint32_t *out, size_t n, hipStream_t s);
+// Writes the slice-major sources, the item offsets of every (row, slice) item and the row masks.
+// item / eoff: [slices][N + 1] exclusive scans of has / ecnt.
+hipError_t launch_slice_scatter(const uint32_t *off, const uint32_t *src, uint32_t N, uint32_t n_huge, int slices,
+                                const uint32_t *cnt, const uint32_t *item, const uint32_t *eoff, const SliceBases &b,
+                                uint32_t *s_src, uint32_t *item_off, uint8_t *mask, hipStream_t s);
+hipError_t launch_slice_wbase(const uint32_t *item, uint32_t N, uint32_t n_huge, int slices, const SliceBases &b,
+                              uint32_t *wbase, hipStream_t s);
 
 // ---- frontier.hip ----
 hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *delu, const uint32_t *delv,

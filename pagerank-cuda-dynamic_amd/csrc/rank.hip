@@ -1,20 +1,26 @@
 // rank.hip -- the pull-based, atomic-free rank update with the fused L-inf reduction, plus small map kernels.
 //
-// One launch per iteration covers all vertices of the degree-ordered layout (rows sorted by in-degree,
-// descending; layout.hip) through a table of sections, heaviest first so that long rows start early:
-//   chunked rows  (in-degree above the chunk threshold): one workgroup per fixed-size edge chunk; each chunk's
-//                 sum goes to a partial slot and the workgroup that arrives last for the row adds the row's
-//                 partials in chunk order and finishes it (deterministic; the arrival count only picks who adds);
+// Row plan (every variant). One launch per iteration covers all vertices of the degree-ordered layout (rows
+// sorted by in-degree, descending; layout.hip) through a table of sections, heaviest first so that long rows
+// start early:
+//   chunked rows  (in-degree above the chunk threshold): one workgroup per fixed-size edge chunk, each writing its
+//                 partial sum; finish_chunked_kernel (the next launch) adds a row's partials in chunk order;
 //   wave rows     one wave per row, DPP/swizzle wave sum;
 //   group rows    W lanes per row (W = 1..32), one band per W over a contiguous row range, DPP group sum.
+//
+// Sliced plan (Static / Naive-dynamic; see SlicedPlan): sliced_gather_kernel sums each row's sources slice by
+// slice, slice x on the workgroups that land on XCD x, and sliced_finish_kernel adds a row's slice partials.
 //
 // Per vertex v (pagerankUpdateRanks{Thread,Block}CukU, pagerankCuda.hxx:87-109, :162-188):
 //   cv = sum_{u in in(v)} r[u]/deg[u]     -- gathered from the precomputed contributions cr[u] = r[u]/deg[u],
 //                                            the same FP64 value the reference divides per edge
 //   a[v] = C0 + P*cv                      -- or the DF-P closed form (C0 + P*(cv - r[v]/d))/(1 - P/d)
 //   prune / frontier flags                 -- vaff[v] = 0 if |a-r|/max(r,a) <= C; naff[v] = (|a-r|/max > D)
-// Each vertex has exactly one writer, so the update needs no atomics on ranks; the atomics are one 64-bit max
-// per workgroup for the L-inf change and one arrival count per chunk of a chunked row.
+// Each vertex has exactly one writer, so the update needs no atomics on ranks; the only atomic is one 64-bit max
+// per workgroup for the L-inf change. Every sum is taken in a fixed order, so results repeat bit for bit.
+#include <algorithm>
+#include <type_traits>
+
 #include "prh_device.hpp"
 #include "prh_internal.hpp"
 
@@ -29,12 +35,43 @@ struct Traits {
 };
 
 template <int VARIANT>
-__device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv) {
+__device__ __forceinline__ bool affected(const RankArgs &A, uint32_t v) {
+  if constexpr (Traits<VARIANT>::dynamic) return A.vaff[v] != 0;
+  else return true;
+}
+
+// Streamed arrays (source ids, offsets) are read with the non-temporal hint so that they do not displace the
+// gathered contributions from L2; the gathers themselves use the default policy.
+__device__ __forceinline__ uint32_t stream_u32(const uint32_t *p) { return __builtin_nontemporal_load(p); }
+
+// Everything a row's update reads besides its gathers, loaded in one batch before them so that the loads overlap
+// instead of adding round trips after the gathers.
+struct RowOperands {
+  uint32_t b, e;  // edge range
+  uint32_t d;     // out-degree
+  double rv;      // old rank
+  double crv;     // old contribution (DF-P's self term)
+  bool upd;       // the row is updated (always, except unaffected rows of the dynamic variants)
+};
+
+template <int VARIANT>
+__device__ __forceinline__ RowOperands row_operands(const RankArgs &A, uint32_t v) {
+  RowOperands o;
+  o.b = stream_u32(A.off + v);
+  o.e = stream_u32(A.off + v + 1);
+  o.d = A.deg[v];
+  o.rv = A.r[v];
+  o.crv = Traits<VARIANT>::dynamic ? A.cr[v] : 0.0;
+  o.upd = affected<VARIANT>(A, v);
+  return o;
+}
+
+template <int VARIANT>
+__device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv, uint32_t d, double rv,
+                                                double crv) {
   using T = Traits<VARIANT>;
-  const uint32_t d = A.deg[v];
-  const double rv = A.r[v];
   double av;
-  if constexpr (T::prune) av = fma(A.P, cv - A.cr[v], A.C0) * (1.0 / (1.0 - A.P / d));
+  if constexpr (T::prune) av = fma(A.P, cv - crv, A.C0) * (1.0 / (1.0 - A.P / d));
   else av = fma(A.P, cv, A.C0);
   A.a[v] = av;
   A.ca[v] = av / d;
@@ -48,36 +85,56 @@ __device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, d
 }
 
 template <int VARIANT>
+__device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv) {
+  return finish_vertex<VARIANT>(A, v, cv, A.deg[v], A.r[v], Traits<VARIANT>::prune ? A.cr[v] : 0.0);
+}
+
+// Unaffected row of a dynamic variant: carried over unchanged (rank and contribution).
+template <int VARIANT>
+__device__ __forceinline__ void copy_vertex(const RankArgs &A, uint32_t v, double rv, double crv) {
+  A.a[v] = rv;
+  A.ca[v] = crv;
+  if constexpr (Traits<VARIANT>::frontier) A.naff[v] = 0;
+}
+
+template <int VARIANT>
 __device__ __forceinline__ void copy_vertex(const RankArgs &A, uint32_t v) {
   A.a[v] = A.r[v];
   A.ca[v] = A.cr[v];
   if constexpr (Traits<VARIANT>::frontier) A.naff[v] = 0;
 }
 
-template <int VARIANT>
-__device__ __forceinline__ bool affected(const RankArgs &A, uint32_t v) {
-  if constexpr (Traits<VARIANT>::dynamic) return A.vaff[v] != 0;
-  else return true;
-}
-
-// Streamed arrays (source ids, offsets) are read with the non-temporal hint so that they do not displace the
-// gathered contributions from L2; the gathers themselves use the default policy.
-__device__ __forceinline__ uint32_t stream_u32(const uint32_t *p) { return __builtin_nontemporal_load(p); }
-
 // Sum of cr[src[e]] for e = first, first + STEP, ... < end, four gathers in flight per lane.
 template <int STEP>
-__device__ __forceinline__ double gather_sum(const RankArgs &A, uint32_t e, uint32_t end) {
+__device__ __forceinline__ double gather_sum(const uint32_t *src, const double *cr, uint32_t e, uint32_t end) {
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   for (; e + 3 * STEP < end; e += 4 * STEP) {
-    const uint32_t u0 = stream_u32(A.src + e), u1 = stream_u32(A.src + e + STEP);
-    const uint32_t u2 = stream_u32(A.src + e + 2 * STEP), u3 = stream_u32(A.src + e + 3 * STEP);
-    s0 += A.cr[u0];
-    s1 += A.cr[u1];
-    s2 += A.cr[u2];
-    s3 += A.cr[u3];
+    const uint32_t u0 = stream_u32(src + e), u1 = stream_u32(src + e + STEP);
+    const uint32_t u2 = stream_u32(src + e + 2 * STEP), u3 = stream_u32(src + e + 3 * STEP);
+    s0 += cr[u0];
+    s1 += cr[u1];
+    s2 += cr[u2];
+    s3 += cr[u3];
   }
-  for (; e < end; e += STEP) s0 += A.cr[stream_u32(A.src + e)];
+  for (; e < end; e += STEP) s0 += cr[stream_u32(src + e)];
   return (s0 + s1) + (s2 + s3);
+}
+
+template <int STEP>
+__device__ __forceinline__ double gather_sum(const RankArgs &A, uint32_t e, uint32_t end) {
+  return gather_sum<STEP>(A.src, A.cr, e, end);
+}
+
+// Sum of a workgroup-strided gather, valid in thread 0. Every thread must call it.
+__device__ __forceinline__ double block_gather_sum(const uint32_t *src, const double *cr, uint32_t b, uint32_t e,
+                                                   double *lds) {
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  double acc = group_sum<kWave>(gather_sum<kBlock>(src, cr, b + threadIdx.x, e));
+  if (lane == 0) lds[w] = acc;
+  __syncthreads();
+  const double t = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  __syncthreads();
+  return t;
 }
 
 // One workgroup per chunk item: its partial sum goes to the item's slot; rows are finished by
@@ -85,17 +142,12 @@ __device__ __forceinline__ double gather_sum(const RankArgs &A, uint32_t e, uint
 template <int VARIANT>
 __device__ __forceinline__ void chunked_rows(const RankArgs &A, const RankPlan &P, const RankSection &S, uint32_t blk,
                                              double *lds) {
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   const uint32_t ib = S.begin + blk * S.per_block, ie = min(S.end, ib + S.per_block);
   for (uint32_t it = ib; it < ie; ++it) {  // block-uniform
     const uint32_t v = P.item_row[it];
     if (!affected<VARIANT>(A, v)) continue;
-    double acc = gather_sum<kBlock>(A, P.item_beg[it] + threadIdx.x, P.item_end[it]);
-    acc = group_sum<kWave>(acc);
-    if (lane == 0) lds[w] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) P.partial[it] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
-    __syncthreads();
+    const double t = block_gather_sum(A.src, A.cr, P.item_beg[it], P.item_end[it], lds);
+    if (threadIdx.x == 0) P.partial[it] = t;
   }
 }
 
@@ -104,13 +156,14 @@ __device__ __forceinline__ void wave_rows(const RankArgs &A, const RankSection &
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   const uint32_t rb = S.begin + blk * S.per_block, re = min(S.end, rb + S.per_block);
   for (uint32_t v = rb + w; v < re; v += kWavesPerBlock) {  // wave-uniform
-    if (!affected<VARIANT>(A, v)) {
-      if (lane == 0) copy_vertex<VARIANT>(A, v);
+    const RowOperands o = row_operands<VARIANT>(A, v);
+    if (!o.upd) {
+      if (lane == 0) copy_vertex<VARIANT>(A, v, o.rv, o.crv);
       continue;
     }
-    double acc = gather_sum<kWave>(A, stream_u32(A.off + v) + lane, stream_u32(A.off + v + 1));
+    double acc = gather_sum<kWave>(A, o.b + lane, o.e);
     acc = group_sum<kWave>(acc);
-    if (lane == 0) emax = fmax(emax, finish_vertex<VARIANT>(A, v, acc));
+    if (lane == 0) emax = fmax(emax, finish_vertex<VARIANT>(A, v, acc, o.d, o.rv, o.crv));
   }
 }
 
@@ -123,16 +176,16 @@ __device__ __forceinline__ void group_rows(const RankArgs &A, const RankSection 
   for (uint32_t r0 = rb + w * G; r0 < re; r0 += kWavesPerBlock * G) {  // wave-uniform
     const uint32_t v = r0 + g;
     const bool valid = v < re;
-    bool upd = false;
+    RowOperands o{};
     double acc = 0.0;
     if (valid) {
-      upd = affected<VARIANT>(A, v);
-      if (upd) acc = gather_sum<W>(A, stream_u32(A.off + v) + j, stream_u32(A.off + v + 1));
+      o = row_operands<VARIANT>(A, v);
+      if (o.upd) acc = gather_sum<W>(A, o.b + j, o.e);
     }
     acc = group_sum<W>(acc);  // all lanes active again here
     if (valid && j == 0) {
-      if (upd) emax = fmax(emax, finish_vertex<VARIANT>(A, v, acc));
-      else copy_vertex<VARIANT>(A, v);
+      if (o.upd) emax = fmax(emax, finish_vertex<VARIANT>(A, v, acc, o.d, o.rv, o.crv));
+      else copy_vertex<VARIANT>(A, v, o.rv, o.crv);
     }
   }
 }
@@ -199,6 +252,117 @@ hipError_t launch_variant(const RankArgs &A, const RankPlan &P, const LoopCtl &c
     finish_chunked_kernel<VARIANT><<<g > 4096 ? 4096 : g, kBlock, 0, s>>>(A, P, P.n_chunked, ctl);
   }
   return hipGetLastError();
+}
+
+// ---- sliced plan ----
+
+// Gather launch: workgroup b works on slice x = b % slices (its XCD with 8 slices) and on block b / slices of that
+// slice's section table. Items are (row, slice) runs of sources; each leaves one partial sum.
+__global__ __launch_bounds__(kBlock) void sliced_gather_kernel(SlicedPlan P, const double *cr, LoopCtl ctl) {
+  if (!iteration_begin(ctl)) return;
+  __shared__ double lds_sum[kWavesPerBlock];
+  const uint32_t x = blockIdx.x & uint32_t(P.slices - 1);
+  const uint32_t j = blockIdx.x / uint32_t(P.slices);
+  const int ns = P.nsec[x];
+  if (ns == 0) return;
+  int s = 0;
+  while (s + 1 < ns && j >= P.sec[x][s + 1].block_begin) ++s;
+  const RankSection &S = P.sec[x][s];
+  const uint64_t first = uint64_t(j - S.block_begin) * S.per_block;
+  if (j < S.block_begin || first >= uint64_t(S.end - S.begin)) return;  // padding block of a shorter slice
+  const uint32_t ib = S.begin + uint32_t(first), ie = min(S.end, ib + S.per_block);
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  if (S.kind == kChunkedRows) {
+    for (uint32_t i = ib; i < ie; ++i) {  // block-uniform
+      const uint32_t c = P.chunk_list[i];
+      const double t = block_gather_sum(P.src, cr, P.chunk_beg[c], P.chunk_end[c], lds_sum);
+      if (threadIdx.x == 0) P.chunk_partial[c] = t;
+    }
+  } else if (S.kind == kWaveRows) {
+    for (uint32_t k = ib + w; k < ie; k += kWavesPerBlock) {  // wave-uniform
+      const double t = group_sum<kWave>(
+          gather_sum<kWave>(P.src, cr, stream_u32(P.item_off + k) + lane, stream_u32(P.item_off + k + 1)));
+      if (lane == 0) P.partial[k] = t;
+    }
+  } else {
+    auto items = [&](auto width) {
+      constexpr int W = decltype(width)::value, G = kWave / W;
+      const int g = lane / W, q = lane % W;
+      for (uint32_t k0 = ib + w * G; k0 < ie; k0 += kWavesPerBlock * G) {  // wave-uniform
+        const uint32_t k = k0 + g;
+        double acc = 0.0;
+        if (k < ie) acc = gather_sum<W>(P.src, cr, stream_u32(P.item_off + k) + q, stream_u32(P.item_off + k + 1));
+        acc = group_sum<W>(acc);
+        if (k < ie && q == 0) P.partial[k] = acc;
+      }
+    };
+    switch (S.width) {
+      case 1: items(std::integral_constant<int, 1>{}); break;
+      case 2: items(std::integral_constant<int, 2>{}); break;
+      case 4: items(std::integral_constant<int, 4>{}); break;
+      case 8: items(std::integral_constant<int, 8>{}); break;
+      case 16: items(std::integral_constant<int, 16>{}); break;
+      default: items(std::integral_constant<int, 32>{}); break;
+    }
+  }
+}
+
+// Finish launch: the first `huge_blocks` workgroups finish the chunked rows (a wave per row over its chunk
+// partials); the rest take one row per lane from row n_huge on, adding the row's slice partials in slice order.
+// A lane finds its partial in slice x from the 64-row group's first item id plus the number of lanes below it
+// whose rows also have an item in slice x (ballot + mbcnt). Every lane loads one value per slice, so that the S
+// loads are in flight together: a lane without an item in slice x reads slot 0 (always allocated) and drops it.
+template <int S>
+__global__ __launch_bounds__(kBlock) void sliced_finish_kernel(RankArgs A, SlicedPlan P, uint32_t huge_blocks,
+                                                               LoopCtl ctl) {
+  if (!iteration_begin(ctl)) return;
+  __shared__ double lds_max[kWavesPerBlock];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  double emax = 0.0;
+  if (blockIdx.x < huge_blocks) {
+    const uint32_t v = blockIdx.x * kWavesPerBlock + w;  // wave-uniform
+    if (v < P.n_huge) {
+      double t = 0.0;
+      for (uint32_t k = P.row_chunk_first[v] + lane; k < P.row_chunk_first[v + 1]; k += kWave) t += P.chunk_partial[k];
+      t = group_sum<kWave>(t);
+      if (lane == 0) emax = finish_vertex<kPlain>(A, v, t);
+    }
+  } else {
+    const uint32_t rows = P.N - P.n_huge;
+    const uint32_t stride = (gridDim.x - huge_blocks) * kBlock;
+    // Grid-stride over rows from n_huge, in 256-row tiles (a bounded grid: short workgroups by the ten thousand
+    // would be limited by workgroup dispatch). Waves stay aligned to 64-row groups.
+    for (uint32_t i0 = (blockIdx.x - huge_blocks) * kBlock; i0 < rows; i0 += stride) {
+      const uint32_t i = i0 + threadIdx.x;  // row offset from n_huge
+      const uint32_t v = P.n_huge + i;
+      const bool valid = v < P.N;
+      const uint32_t vv = valid ? v : P.N - 1;
+      const unsigned m = valid ? P.mask[vv] : 0u;
+      const uint32_t d = A.deg[vv];
+      const double rv = A.r[vv];
+      const uint32_t grp = __builtin_amdgcn_readfirstlane(i / kWave);
+      // The wave's 64-row group exists when its first row does; otherwise every m is 0 and no base is needed.
+      const bool group_ok = P.n_huge + grp * kWave < P.N;
+      const uint32_t *wb = P.wbase + size_t(group_ok ? grp : 0) * S;
+      uint32_t base[S];
+#pragma unroll
+      for (int x = 0; x < S; ++x) base[x] = wb[x];
+      double part[S];
+#pragma unroll
+      for (int x = 0; x < S; ++x) {
+        const bool has = (m >> x) & 1u;
+        const uint32_t k = has ? base[x] + lanes_below(__ballot(has)) : 0u;
+        const double t = P.partial[k];
+        part[x] = has ? t : 0.0;
+      }
+      double cv = 0.0;
+#pragma unroll
+      for (int x = 0; x < S; ++x) cv += part[x];  // slice order; absent slices add +0.0
+      if (valid) emax = fmax(emax, finish_vertex<kPlain>(A, v, cv, d, rv, 0.0));
+    }
+  }
+  emax = block_max(emax, lds_max);
+  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[ctl.l], emax);
 }
 
 // r = a = 1/N (or keep r as loaded and copy it to a); contributions cr = ca = r/deg.
@@ -293,6 +457,30 @@ hipError_t launch_rank_update(int variant, const RankArgs &A, const RankPlan &P,
     case kPrune: return launch_variant<kPrune>(A, P, ctl, s, mark);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_sliced_update(const RankArgs &A, const SlicedPlan &P, unsigned long long *err, int *done, int l,
+                                double tol, hipStream_t s, hipEvent_t mark) {
+  const LoopCtl ctl{err, done, l, tol};
+  const uint32_t gather_blocks = P.blocks_per_slice * uint32_t(P.slices);
+  if (gather_blocks) sliced_gather_kernel<<<gather_blocks, kBlock, 0, s>>>(P, A.cr, ctl);
+  if (mark) {
+    const hipError_t e = hipEventRecord(mark, s);
+    if (e != hipSuccess) return e;
+  }
+  const uint32_t huge_blocks = (P.n_huge + kWavesPerBlock - 1) / kWavesPerBlock;
+  const uint32_t row_blocks = std::min<uint32_t>((P.N - P.n_huge + kBlock - 1) / kBlock, 4096);
+  const uint32_t g = huge_blocks + row_blocks;
+  if (g) {
+    switch (P.slices) {
+      case 1: sliced_finish_kernel<1><<<g, kBlock, 0, s>>>(A, P, huge_blocks, ctl); break;
+      case 2: sliced_finish_kernel<2><<<g, kBlock, 0, s>>>(A, P, huge_blocks, ctl); break;
+      case 4: sliced_finish_kernel<4><<<g, kBlock, 0, s>>>(A, P, huge_blocks, ctl); break;
+      case 8: sliced_finish_kernel<8><<<g, kBlock, 0, s>>>(A, P, huge_blocks, ctl); break;
+      default: return hipErrorInvalidValue;
+    }
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_init_ranks(double *r, double *a, double *cr, double *ca, const uint32_t *deg, uint32_t N,

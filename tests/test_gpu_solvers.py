@@ -103,14 +103,17 @@ def test_dynamic_matches_oracle(ctx, oracle_mod, method, frac):
         assert np.abs(res.ranks - exact).max() <= RANK_TOL
 
 
-PLANS = [(128, 16384, 0), (8, 64, 0), (16, 100, 4), (2, 2, 1), (32, 33, 32), (64, 1000, 2)]
+# (switch_degree, huge_degree, low_width, source_slices): row plans (slices 0) and source-sliced plans.
+PLANS = [(128, 16384, 0, 0), (8, 64, 0, 0), (16, 100, 4, 0), (2, 2, 1, 0), (32, 33, 32, 0), (64, 1000, 2, 0),
+         (128, 16384, 0, 8), (64, 64, 0, 8), (128, 200, 0, 4), (2, 2, 0, 2), (128, 16384, 0, 1), (64, 64, 0, 1)]
 
 
-@pytest.mark.parametrize("plan", PLANS, ids=[f"sw{a}_huge{b}_w{c}" for a, b, c in PLANS])
-@pytest.mark.parametrize("method", [0, 4], ids=["static", "DFP"])
+@pytest.mark.parametrize("plan", PLANS, ids=[f"sw{a}_huge{b}_w{c}_sl{d}" for a, b, c, d in PLANS])
+@pytest.mark.parametrize("method", [0, 1, 4], ids=["static", "ND", "DFP"])
 def test_launch_plans_agree(ctx, oracle_mod, plan, method):
     """Every launch plan (chunked rows with their second combining launch, wave rows, fixed or banded lanes per
-    row) gives the oracle's ranks to 1e-9: the plan only changes the summation order."""
+    row; for Static and ND also the source-sliced gather with its per-row slice combine) gives the oracle's ranks
+    to 1e-9: the plan only changes the summation order."""
     from pagerank_amd import Batch, HostGraph, Tuning, apply_batch, solve
     x = HostGraph.rmat(12, 24, seed=4)
     q, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, x)
@@ -119,13 +122,38 @@ def test_launch_plans_agree(ctx, oracle_mod, plan, method):
     old = ctx.get_tuning()
     print(f"[plan] {plan} method={method} N={y.N} M={y.M} max_in={int(np.diff(y.in_off).max())}", flush=True)
     try:
-        ctx.set_tuning(Tuning(plan[0], plan[1], plan[2], old.check_interval))
+        ctx.set_tuning(Tuning(plan[0], plan[1], plan[2], old.check_interval, plan[3]))
         res = solve(ctx.upload(y), method, batch=b if method else None, q=q if method else None)
     finally:
         ctx.set_tuning(old)
     ref, it, _ = oracle_mod.pagerank_cuda(method, y, b if method else None, q if method else None)
     assert _close_iters(res.iterations, it), (res.iterations, it)
     assert np.abs(res.ranks - ref).max() <= RANK_TOL
+
+
+@pytest.mark.parametrize("slices", [0, 1, 8])
+def test_rows_without_in_edges(ctx, oracle_mod, slices):
+    """Vertices nobody links to (no self-loops here) keep rank (1-alpha)/N; a row with no sources has no item in
+    any slice and its sum must be exactly 0."""
+    from pagerank_amd import HostGraph, Tuning, pagerankStaticHip
+    n = 3000
+    u = np.arange(n, dtype=np.uint32)
+    src = np.concatenate([u[1:], [0], u[:100]])
+    dst = np.concatenate([np.zeros(n - 1, np.uint32), [1], (u[:100] * 7 + 3) % 50])
+    y = HostGraph.from_edges(n, src, dst, self_loops=False)
+    assert (np.diff(y.in_off.astype(np.int64)) == 0).sum() > n // 2
+    old = ctx.get_tuning()
+    print(f"[no-in-edges] slices={slices} N={y.N} M={y.M}", flush=True)
+    try:
+        ctx.set_tuning(Tuning(64, 64, old.low_width, old.check_interval, slices))
+        res = pagerankStaticHip(ctx.upload(y))
+    finally:
+        ctx.set_tuning(old)
+    ref, it, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, y)
+    assert _close_iters(res.iterations, it)
+    assert np.abs(res.ranks - ref).max() <= RANK_TOL
+    zero_in = np.diff(y.in_off.astype(np.int64)) == 0
+    assert np.all(res.ranks[zero_in] == (1 - 0.85) / n)  # C0 exactly: fma(alpha, 0, C0)
 
 
 @pytest.mark.parametrize("method", [2, 3, 4], ids=["DT", "DF", "DFP"])
