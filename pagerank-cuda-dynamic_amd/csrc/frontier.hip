@@ -24,6 +24,8 @@ __global__ __launch_bounds__(kBlock) void mark_partial_kernel(uint8_t *naff, uin
   }
 }
 
+// Marks w. Plain marking stores unconditionally: a byte store does not hold up the wave, a test before it would
+// (measured slower). Traversal levels test first, since only newly marked vertices join the next level.
 __device__ __forceinline__ void mark(const ExtendArgs &A, uint32_t w) {
   if (!A.fresh) {
     A.vaff[w] = 1;
@@ -34,6 +36,11 @@ __device__ __forceinline__ void mark(const ExtendArgs &A, uint32_t w) {
   }
 }
 
+constexpr int kLowLanes = 8;                     // lanes per low-out-degree vertex
+constexpr int kLowPerLane = 64 / kLowLanes;      // edges per lane (out-degree <= 64 in the low class)
+
+// High out-degree vertices: one wave per vertex. Low ones (out-degree <= 64): kLowLanes lanes per vertex, each
+// lane loading all of its (at most kLowPerLane) targets before marking any, so that the loads overlap.
 __global__ __launch_bounds__(kBlock) void extend_kernel(ExtendArgs A, uint32_t nb_high, uint32_t nb_low, LoopCtl ctl,
                                                         int use_ctl) {
   if (use_ctl && !iteration_continues(ctl)) return;
@@ -49,10 +56,23 @@ __global__ __launch_bounds__(kBlock) void extend_kernel(ExtendArgs A, uint32_t n
     }
   } else {
     const uint32_t n = A.sched.n_low;
-    for (uint32_t i = (blockIdx.x - nb_high) * kBlock + threadIdx.x; i < n; i += nb_low * kBlock) {
+    constexpr uint32_t kGroups = kBlock / kLowLanes;
+    const uint32_t g = threadIdx.x / kLowLanes, j = threadIdx.x % kLowLanes;
+    for (uint32_t i = (blockIdx.x - nb_high) * kGroups + g; i < n; i += nb_low * kGroups) {
       const uint32_t u = A.sched.order[i];
       if (!A.naff[u]) continue;
-      for (uint32_t e = A.off[u]; e < A.off[u + 1]; ++e) mark(A, A.dst[e]);
+      const uint32_t beg = A.off[u], end = A.off[u + 1];
+      uint32_t t[kLowPerLane];
+#pragma unroll
+      for (int k = 0; k < kLowPerLane; ++k) {
+        const uint32_t e = beg + j + uint32_t(k) * kLowLanes;
+        t[k] = e < end ? A.dst[e] : 0xFFFFFFFFu;
+      }
+#pragma unroll
+      for (int k = 0; k < kLowPerLane; ++k)
+        if (t[k] != 0xFFFFFFFFu) mark(A, t[k]);
+      // Only reached if the schedule's low class was split above 64.
+      for (uint32_t e = beg + kLowLanes * kLowPerLane + j; e < end; e += kLowLanes) mark(A, A.dst[e]);
     }
   }
 }
@@ -73,7 +93,7 @@ hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *del
 
 hipError_t launch_extend(const ExtendArgs &A, unsigned long long *err, int *done, int l, double tol, hipStream_t s) {
   const uint32_t nb_high = clamp_blocks(size_t(A.sched.n_high) + A.sched.n_huge, kWavesPerBlock, 4096);
-  const uint32_t nb_low = clamp_blocks(A.sched.n_low, kBlock, 8192);
+  const uint32_t nb_low = clamp_blocks(A.sched.n_low, kBlock / kLowLanes, 8192);
   if (nb_high + nb_low == 0) return hipSuccess;
   const LoopCtl ctl{err, done, l, tol};
   extend_kernel<<<nb_high + nb_low, kBlock, 0, s>>>(A, nb_high, nb_low, ctl, err ? 1 : 0);

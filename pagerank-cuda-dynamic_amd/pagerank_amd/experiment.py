@@ -1,0 +1,215 @@
+"""Experiment runner on synthetic graphs, following the reference's protocol and log format (main.cu:47-148).
+
+    python -m pagerank_amd.experiment dynamic --scale 22 --edge-factor 16 --fractions 1e-5,1e-4,1e-3 --batches 3
+    python -m pagerank_amd.experiment ppr --scale 22 --columns 64
+    python -m pagerank_amd.experiment adjoint --scale 20
+
+dynamic: as runExperiment (main.cu:47-148). A stream of update batches is applied to the graph one after another
+(80% insertions of uniform random vertex pairs, 20% deletions of uniform random existing non-self-loop edges,
+tidied as tidyBatchUpdateU does). After each batch every GPU method runs on the updated graph: Static, and ND /
+DT / DF / DF-P, each warm-started from its own result of the previous batch (main.cu:102-143). Each result is
+compared with a static solve of the same graph at tolerance 1e-100, which runs the full 500 iterations
+(main.cu:88). The reference computes that solve on the CPU; here it runs on the GPU (same Jacobi iteration, FP64).
+One line per method is printed in glog's format (main.cu:56-60). A summary of the speedups over Static follows.
+
+ppr: one batched personalized solve of k teleport columns, each with a seed set of uniform random size 1..1000,
+against k single-column solves and k plain static solves on the same graph.
+
+adjoint: weighted forward solve, adjoint solve with gradient assembly, and a plain static solve on the same graph.
+FP32 weights are uniform in [0.5, 1.5); the loss is L = sum_v c_v r_v^2 with c_v ~ N(0, 1).
+
+--csv FILE appends every measurement as a CSV row.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import sys
+import time
+
+import numpy as np
+
+from . import engine
+from .engine import Context, PagerankOptions, solve
+from .graphs import Batch, HostGraph, apply_batch
+
+METHODS = [
+    (engine.NAIVE_DYNAMIC, "pagerankNaiveDynamicHip"),
+    (engine.DYNAMIC_TRAVERSAL, "pagerankDynamicTraversalHip"),
+    (engine.DYNAMIC_FRONTIER, "pagerankDynamicFrontierHip"),
+    (engine.PRUNE_DYNAMIC_FRONTIER, "pagerankPruneDynamicFrontierHip"),
+]
+
+
+def log(msg: str) -> None:
+    print(f"# {msg}", flush=True)
+
+
+def l1_error(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.abs(a - b).sum())
+
+
+def glog(ans, ref: np.ndarray, technique: str, threads: int, deletionsf: float, insertionsf: float,
+         batch_index: int) -> str:
+    """One result line in the reference's format (main.cu:56-60)."""
+    return (f"{{-{deletionsf:.3e}/+{insertionsf:.3e} batchf, {batch_index:03d} batchi, {threads:03d} threads}} -> "
+            f"{{{ans.time:09.1f}ms, {ans.initializationTime:09.1f}ms init, {ans.markingTime:09.1f}ms mark, "
+            f"{ans.computationTime:09.1f}ms comp, {ans.iterations:03d} iter, {l1_error(ans.ranks, ref):.2e} err}} "
+            f"{technique}")
+
+
+class CsvLog:
+    FIELDS = ["mode", "graph", "vertices", "edges", "batch_fraction", "batch_index", "technique", "time_ms",
+              "init_ms", "mark_ms", "comp_ms", "iterations", "l1_error", "extra"]
+
+    def __init__(self, path: str | None):
+        self.f = open(path, "a", newline="") if path else None
+        self.w = csv.DictWriter(self.f, fieldnames=self.FIELDS) if self.f else None
+        if self.f and self.f.tell() == 0:
+            self.w.writeheader()
+
+    def row(self, **kw):
+        if self.w:
+            self.w.writerow({k: kw.get(k, "") for k in self.FIELDS})
+            self.f.flush()
+
+
+def make_rmat(scale: int, edge_factor: float, seed: int) -> tuple[HostGraph, str]:
+    t0 = time.time()
+    g = HostGraph.rmat(scale, edge_factor, 0.57, 0.19, 0.19, seed=seed)
+    name = f"rmat{scale}x{edge_factor:g}"
+    log(f"graph {name}: {g.N} vertices, {g.M} edges (dedup, self-loops), built in {time.time() - t0:.1f} s")
+    return g, name
+
+
+def run_dynamic(args, out: CsvLog) -> int:
+    x0, name = make_rmat(args.scale, args.edge_factor, args.seed)
+    ctx = Context(0)
+    o = PagerankOptions(repeat=args.repeat)
+    exact = PagerankOptions(repeat=1, tolerance=1e-100)
+    summary = []
+    for frac in [float(f) for f in args.fractions.split(",")]:
+        x = x0
+        xg = ctx.upload(x, with_out=False)
+        r0 = solve(xg, engine.STATIC, exact).ranks
+        xg.close()
+        prev = {m: r0 for m, _ in METHODS}
+        for bi in range(args.batches):
+            size = max(1, int(frac * x.M))
+            t0 = time.time()
+            b = Batch.random(x, size, 0.8, seed=args.seed * 1000 + bi + int(frac * 1e9))
+            y = apply_batch(x, b)
+            yg = ctx.upload(y)
+            log(f"batch {bi} of {frac:g}|E|: -{b.deletions}/+{b.insertions} edges, y has {y.M} edges "
+                f"({time.time() - t0:.1f} s to build and upload)")
+            s0 = solve(yg, engine.STATIC, exact).ranks
+            res = {engine.STATIC: solve(yg, engine.STATIC, o)}
+            for m, _ in METHODS:
+                res[m] = solve(yg, m, o, batch=b, q=prev[m])
+            for m, tech in [(engine.STATIC, "pagerankStaticHip")] + METHODS:
+                a = res[m]
+                print(glog(a, s0, tech, 1, 0.0, frac, bi), flush=True)
+                out.row(mode="dynamic", graph=name, vertices=y.N, edges=y.M, batch_fraction=frac, batch_index=bi,
+                        technique=tech, time_ms=a.time, init_ms=a.initializationTime, mark_ms=a.markingTime,
+                        comp_ms=a.computationTime, iterations=a.iterations, l1_error=l1_error(a.ranks, s0),
+                        extra=f"deletions={b.deletions};insertions={b.insertions}")
+            for m, _ in METHODS:
+                prev[m] = res[m].ranks
+            summary.append((frac, bi, {m: res[m].time for m in res}))
+            yg.close()
+            x = y
+    log("speedup over GPU Static (time of Static / time of the method), mean over batches:")
+    for frac in sorted({s[0] for s in summary}):
+        rows = [s[2] for s in summary if s[0] == frac]
+        st = np.mean([r[engine.STATIC] for r in rows])
+        parts = [f"Static {st:.3f} ms"]
+        for m, tech in METHODS:
+            t = np.mean([r[m] for r in rows])
+            parts.append(f"{tech.replace('pagerank', '').replace('Hip', '')} {t:.3f} ms ({st / t:.2f}x)")
+        log(f"batch {frac:g}|E|: " + ", ".join(parts))
+    return 0
+
+
+def seed_sets(n: int, k: int, rng: np.random.Generator, max_size: int = 1000) -> list[np.ndarray]:
+    return [rng.choice(n, size=int(rng.integers(1, max_size + 1)), replace=False).astype(np.uint32)
+            for _ in range(k)]
+
+
+def run_ppr(args, out: CsvLog) -> int:
+    g, name = make_rmat(args.scale, args.edge_factor, args.seed)
+    ctx = Context(0)
+    yg = ctx.upload(g, with_out=False)
+    rng = np.random.default_rng(args.seed)
+    sets = seed_sets(g.N, args.columns, rng)
+    o = PagerankOptions(repeat=args.repeat)
+    engine.ppr_batched(yg, sets[:2], PagerankOptions())  # warm-up
+    batched = engine.ppr_batched(yg, sets, o)
+    single_ms, single_iters, worst = 0.0, 0, 0.0
+    for j, s in enumerate(sets):
+        r = engine.ppr_batched(yg, [s], o)
+        single_ms += r.time
+        single_iters += int(r.iterations[0])
+        worst = max(worst, float(np.abs(r.ranks[:, 0] - batched.ranks[:, j]).max()))
+    static = engine.pagerankStaticHip(yg, o)
+    k = len(sets)
+    cols_iters = int(batched.iterations.sum())
+    log(f"batched {k} columns: {batched.time:.3f} ms, {cols_iters} column-iterations "
+        f"({g.M * cols_iters / (batched.time / 1e3) / 1e9:.1f} GEdges/s/iter per column)")
+    log(f"{k} single-column solves: {single_ms:.3f} ms ({g.M * single_iters / (single_ms / 1e3) / 1e9:.1f} "
+        f"GEdges/s/iter); batched speedup {single_ms / batched.time:.2f}x; max |batched - single| {worst:.2e}")
+    log(f"{k} plain static solves (estimated from one): {k * static.time:.3f} ms; batched speedup "
+        f"{k * static.time / batched.time:.2f}x")
+    for tech, t, its, extra in (("ppr_batched", batched.time, cols_iters, f"columns={k}"),
+                                ("ppr_single_x{}".format(k), single_ms, single_iters, f"linf_vs_batched={worst:.3e}"),
+                                ("static_x{}".format(k), k * static.time, k * static.iterations, "")):
+        out.row(mode="ppr", graph=name, vertices=g.N, edges=g.M, technique=tech, time_ms=t, iterations=its,
+                extra=extra)
+    return 0
+
+
+def run_adjoint(args, out: CsvLog) -> int:
+    g, name = make_rmat(args.scale, args.edge_factor, args.seed)
+    ctx = Context(0)
+    yg = ctx.upload(g)
+    rng = np.random.default_rng(args.seed)
+    w = rng.uniform(0.5, 1.5, g.M).astype(np.float32)
+    s = np.full(g.N, 1.0 / g.N)
+    c = rng.standard_normal(g.N)
+    o = PagerankOptions(repeat=args.repeat)
+    engine.weighted_pagerank(yg, w, s, PagerankOptions())  # warm-up
+    fw = engine.weighted_pagerank(yg, w, s, o)
+    t0 = time.perf_counter()
+    reps = args.repeat
+    for _ in range(reps):
+        lam, gw, gs, it = engine.weighted_adjoint(yg, w, fw.ranks, 2 * c * fw.ranks, o)
+    adj_ms = (time.perf_counter() - t0) / reps * 1e3  # host wall time including the host<->device copies
+    static = engine.pagerankStaticHip(yg, o)
+    log(f"forward weighted solve: {fw.time:.3f} ms, {fw.iterations} iterations "
+        f"({g.M * fw.iterations / (fw.computationTime / 1e3) / 1e9:.1f} GEdges/s/iter)")
+    log(f"adjoint solve + gradient assembly: {adj_ms:.3f} ms wall incl. copies, {it} iterations")
+    log(f"plain static solve: {static.time:.3f} ms, {static.iterations} iterations; forward+adjoint / static "
+        f"{(fw.time + adj_ms) / static.time:.2f}x")
+    for tech, t, its in (("weighted_forward", fw.time, fw.iterations), ("weighted_adjoint_wall", adj_ms, it),
+                         ("static", static.time, static.iterations)):
+        out.row(mode="adjoint", graph=name, vertices=g.N, edges=g.M, technique=tech, time_ms=t, iterations=its)
+    return 0
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("mode", choices=["dynamic", "ppr", "adjoint"])
+    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--edge-factor", type=float, default=16.0)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--fractions", default="1e-5,1e-4,1e-3")
+    ap.add_argument("--batches", type=int, default=3)
+    ap.add_argument("--repeat", type=int, default=5)
+    ap.add_argument("--columns", type=int, default=64)
+    ap.add_argument("--csv", default=None)
+    args = ap.parse_args(argv)
+    out = CsvLog(args.csv)
+    return {"dynamic": run_dynamic, "ppr": run_ppr, "adjoint": run_adjoint}[args.mode](args, out)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
