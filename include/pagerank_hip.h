@@ -83,6 +83,9 @@ typedef struct prh_tuning {
                              source u in slice (u / 16) mod source_slices, slice x on the workgroups b with
                              b mod source_slices = x (8: one slice per XCD and its L2) [0] */
   uint32_t block_edges;   /* edges per workgroup of the rank kernels (>= 256) [4096] */
+  int extension_direction; /* DF / DF-P extension: 0 push along out-edges of frontier vertices; 1 push or pull
+                              (unaffected rows look for a frontier in-neighbour), whichever the last rank launch
+                              found cheaper; 2 pull. The flags, hence the results, are the same for all [1] */
 } prh_tuning;
 
 typedef struct prh_context prh_context;

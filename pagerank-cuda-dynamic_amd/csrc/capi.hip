@@ -41,7 +41,8 @@ struct prh_context {
   DevBuf<uint8_t> vaff, naff, buff;
   DevBuf<uint32_t> sched_in, sched_out, tiles, counts, batch, scratch_u32;
   DevBuf<unsigned long long> loop;  // [L] error bits, then one slot holding `done`
-  DevBuf<int> flags;                // [0] validation bits, [1] traversal "changed"
+  DevBuf<int> flags;                // [0] validation bits, [1] traversal "changed", [2] extension direction
+  DevBuf<double> ext_stats;         // per rank workgroup: pull and push costs of the next extension
   // batched personalized PageRank
   DevBuf<double> ppr_rank[2], ppr_contrib[2], ppr_partial, ppr_mass, ppr_out;
   DevBuf<int32_t> ppr_row;
@@ -638,6 +639,15 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   int *done = reinterpret_cast<int *>(c->loop.p + L);
   const int check = std::max(1, c->tuning.check_interval);
   const prh::RankPlan &plan = g->plan;
+  // Extension direction (frontier variants): 0 push only, 1 chosen per iteration from the rank launch's costs,
+  // 2 pull only. The pull form splits rows at in-degree 64 (a wave each) and 4 (8 lanes each).
+  const int dir_mode = frontier ? c->tuning.extension_direction : 0;
+  const bool dir_auto = dir_mode == 1;
+  const uint32_t stat_slots = plan.blocks + (plan.n_chunked ? prh::finish_chunked_grid(plan.n_chunked) : 0);
+  if (dir_auto) PRH_CHECK(ensure(c, c->ext_stats, 2 * size_t(std::max<uint32_t>(stat_slots, 1))));
+  int *dir = frontier ? c->flags.p + 2 : nullptr;
+  const prh::PullArgs pull{c->vaff.p, c->naff.p, g->l_in_off, g->l_in_src, rows_above(g->sorted_deg, 64),
+                           rows_above(g->sorted_deg, 4), N, dir};
   for (int rep = 0; rep < o->repeat; ++rep) {
     if (q) PRH_CHECK(load_initial_ranks(c, g, q, host));  // untimed, as the reference's q copy (pagerankCuda.hxx:810)
     PRH_HIP(c, hipEventRecord(c->ev[0], s));
@@ -652,20 +662,23 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
     PRH_HIP(c, hipEventRecord(c->ev[2], s));
     // Iterations.
     PRH_HIP(c, prh::launch_fill_u64(err, 0ull, size_t(L) + 1, s));
+    if (frontier && dir_mode != 1) PRH_HIP(c, prh::launch_fill_i32(dir, dir_mode == 2 ? 1 : 0, 1, s));
     const double P = o->damping;
     const double C0 = (1 - P) / N;
     int n = -1, launched = 0;
     for (int l = 0; l < L; ++l) {
       prh::RankArgs A{c->rank[(l + 1) & 1].p, c->rank[l & 1].p, c->contrib[(l + 1) & 1].p, c->contrib[l & 1].p,
                       c->naff.p, c->vaff.p, g->l_in_off, g->l_in_src, g->l_outdeg, C0, P, o->frontier_tolerance,
-                      o->prune_tolerance};
+                      o->prune_tolerance, dir_auto ? c->ext_stats.p : nullptr};
       if (c->ktiming) PRH_HIP(c, hipEventRecord(c->kev[2 * l], s));
       hipEvent_t mark = c->ktiming ? c->kev[2 * l + 1] : nullptr;
       if (sliced) PRH_HIP(c, prh::launch_sliced_update(A, g->sliced, err, done, l, o->tolerance, s, mark));
       else PRH_HIP(c, prh::launch_rank_update(variant, A, plan, err, done, l, o->tolerance, s, mark));
       if (frontier) {
-        prh::ExtendArgs E{c->vaff.p, c->naff.p, g->l_out_off, g->l_out_dst, out_s, nullptr, nullptr};
+        if (dir_auto) PRH_HIP(c, prh::launch_direction(c->ext_stats.p, stat_slots, dir, err, done, l, o->tolerance, s));
+        prh::ExtendArgs E{c->vaff.p, c->naff.p, g->l_out_off, g->l_out_dst, out_s, nullptr, nullptr, dir};
         PRH_HIP(c, prh::launch_extend(E, err, done, l, o->tolerance, s));
+        if (dir_mode != 0) PRH_HIP(c, prh::launch_extend_pull(pull, err, done, l, o->tolerance, s));
       }
       launched = l + 1;
       if (launched % check == 0 || launched == L) {
@@ -1208,6 +1221,7 @@ void prh_default_tuning(prh_tuning *t) {
   t->check_interval = 8;
   t->source_slices = 0;
   t->block_edges = 4096;
+  t->extension_direction = 1;
 }
 
 int prh_context_create(int device, prh_context **out) {
@@ -1250,6 +1264,7 @@ void prh_context_destroy(prh_context *c) {
   release(c, c->sched_out);
   release(c, c->tiles);
   release(c, c->counts);
+  release(c, c->ext_stats);
   release(c, c->batch);
   release(c, c->scratch_u32);
   release(c, c->loop);
@@ -1301,7 +1316,7 @@ int prh_context_set_tuning(prh_context *c, const prh_tuning *t) {
   const int sl = t->source_slices;
   if (!(w == 0 || w == 1 || w == 2 || w == 4 || w == 8 || w == 16 || w == 32) || t->check_interval < 1 ||
       t->huge_degree < t->switch_degree || !(sl == 0 || sl == 1 || sl == 2 || sl == 4 || sl == 8) ||
-      t->block_edges < 256)
+      t->block_edges < 256 || t->extension_direction < 0 || t->extension_direction > 2)
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad tuning");
   c->tuning = *t;
   return PRH_SUCCESS;

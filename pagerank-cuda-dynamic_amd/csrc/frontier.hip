@@ -4,9 +4,13 @@
 //                 (pagerankAffectedFrontierPartialCukU, pagerankCuda.hxx:272-281).
 //   extend:       vaff[w] = 1 for every out-neighbour w of a vertex u with naff[u]
 //                 (pagerankAffectedExtend{Thread,Block}CukU + pagerankMarkNeighborsCudU, :55-63, :313-366),
-//                 split by out-degree: one lane per low-out-degree vertex, one wave per high-out-degree vertex.
+//                 split by out-degree: 8 lanes per low-out-degree vertex, one wave per high-out-degree vertex.
 //                 With `fresh` set it is one breadth-first level: only newly marked vertices enter `fresh`
 //                 and `changed` records whether any vertex was newly marked (the DT closure, :438-456).
+//   direction /   the same extension in pull form (the direction-optimizing idea of Beamer et al., SC 2012):
+//   extend_pull   every unaffected row w takes vaff[w] = 1 when an in-neighbour is in the frontier. The rank
+//                 launch leaves per-workgroup costs of both forms (in-edges of unaffected rows, out-edges of
+//                 frontier rows); `direction` totals them and the cheaper form runs. Both give the same flags.
 // Concurrent writers only ever store 1, so the flags are the same for any interleaving.
 #include "prh_device.hpp"
 #include "prh_internal.hpp"
@@ -44,6 +48,7 @@ constexpr int kLowPerLane = 64 / kLowLanes;      // edges per lane (out-degree <
 __global__ __launch_bounds__(kBlock) void extend_kernel(ExtendArgs A, uint32_t nb_high, uint32_t nb_low, LoopCtl ctl,
                                                         int use_ctl) {
   if (use_ctl && !iteration_continues(ctl)) return;
+  if (A.direction && *A.direction != 0) return;
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   if (blockIdx.x < nb_high) {
     const uint32_t n = A.sched.n_high + A.sched.n_huge;
@@ -77,6 +82,87 @@ __global__ __launch_bounds__(kBlock) void extend_kernel(ExtendArgs A, uint32_t n
   }
 }
 
+constexpr int kDirThreads = 1024;
+
+// One workgroup: totals of the per-workgroup costs, then the choice (pull only when strictly cheaper).
+__global__ __launch_bounds__(kDirThreads) void direction_kernel(const double *stats, uint32_t n, int *direction,
+                                                                LoopCtl ctl) {
+  if (!iteration_continues(ctl)) return;
+  __shared__ double lds[2 * (kDirThreads / kWave)];
+  double pull = 0.0, push = 0.0;
+  for (uint32_t i = threadIdx.x; i < n; i += kDirThreads) {
+    pull += stats[2 * size_t(i)];
+    push += stats[2 * size_t(i) + 1];
+  }
+  pull = group_sum<kWave>(pull);
+  push = group_sum<kWave>(push);
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    lds[w] = pull;
+    lds[kDirThreads / kWave + w] = push;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < kDirThreads / kWave; ++k) {
+      a += lds[k];
+      b += lds[kDirThreads / kWave + k];
+    }
+    *direction = a < b ? 1 : 0;
+  }
+}
+
+constexpr int kPullLanes = 8;
+
+// Pull form of the extension over the in-CSR rows (rows sorted by in-degree, descending): a wave per row for
+// rows [0, n_wave), kPullLanes lanes per row for [n_wave, n_group), one lane per row after that. Only rows not yet
+// affected look at their in-neighbours, and stop at the first one in the frontier.
+__global__ __launch_bounds__(kBlock) void extend_pull_kernel(PullArgs A, uint32_t nb_wave, uint32_t nb_group,
+                                                             uint32_t nb_one, LoopCtl ctl) {
+  if (!iteration_continues(ctl)) return;
+  if (*A.direction != 1) return;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  if (blockIdx.x < nb_wave) {
+    for (uint32_t v = blockIdx.x * kWavesPerBlock + w; v < A.n_wave; v += nb_wave * kWavesPerBlock) {
+      if (A.vaff[v]) continue;  // wave-uniform
+      const uint32_t b = A.off[v], e = A.off[v + 1];
+      for (uint32_t k = b; k < e; k += kWave) {
+        const bool hit = k + lane < e && A.naff[A.src[k + lane]];
+        if (__ballot(hit)) {
+          if (lane == 0) A.vaff[v] = 1;
+          break;
+        }
+      }
+    }
+  } else if (blockIdx.x < nb_wave + nb_group) {
+    const uint32_t g = threadIdx.x / kPullLanes, j = threadIdx.x % kPullLanes;
+    constexpr uint32_t kGroups = kBlock / kPullLanes;
+    const uint32_t first = A.n_wave;
+    for (uint32_t r0 = first + (blockIdx.x - nb_wave) * kGroups; r0 < A.n_group; r0 += nb_group * kGroups) {
+      const uint32_t v = r0 + g;  // rows r0 .. r0 + kGroups - 1 in this round (block-uniform loop)
+      bool hit = false;
+      if (v < A.n_group && !A.vaff[v]) {
+        const uint32_t b = A.off[v], e = A.off[v + 1];
+        for (uint32_t k = b + j; k < e && !hit; k += kPullLanes) hit = A.naff[A.src[k]] != 0;
+      }
+      const unsigned long long bal = __ballot(hit);
+      const unsigned group_bits = unsigned(bal >> ((lane / kPullLanes) * kPullLanes)) & ((1u << kPullLanes) - 1);
+      if (j == 0 && group_bits) A.vaff[v] = 1;
+    }
+  } else {
+    for (uint32_t v = A.n_group + (blockIdx.x - nb_wave - nb_group) * kBlock + threadIdx.x; v < A.N;
+         v += nb_one * kBlock) {
+      if (A.vaff[v]) continue;
+      const uint32_t b = A.off[v], e = A.off[v + 1];
+      for (uint32_t k = b; k < e; ++k)
+        if (A.naff[A.src[k]]) {
+          A.vaff[v] = 1;
+          break;
+        }
+    }
+  }
+}
+
 inline uint32_t clamp_blocks(size_t n, size_t per, uint32_t cap) {
   const size_t g = (n + per - 1) / per;
   return uint32_t(g > cap ? cap : g);
@@ -97,6 +183,24 @@ hipError_t launch_extend(const ExtendArgs &A, unsigned long long *err, int *done
   if (nb_high + nb_low == 0) return hipSuccess;
   const LoopCtl ctl{err, done, l, tol};
   extend_kernel<<<nb_high + nb_low, kBlock, 0, s>>>(A, nb_high, nb_low, ctl, err ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_direction(const double *stats, uint32_t n, int *direction, unsigned long long *err, int *done, int l,
+                            double tol, hipStream_t s) {
+  const LoopCtl ctl{err, done, l, tol};
+  direction_kernel<<<1, kDirThreads, 0, s>>>(stats, n, direction, ctl);
+  return hipGetLastError();
+}
+
+hipError_t launch_extend_pull(const PullArgs &A, unsigned long long *err, int *done, int l, double tol,
+                              hipStream_t s) {
+  const uint32_t nb_wave = clamp_blocks(A.n_wave, kWavesPerBlock, 2048);
+  const uint32_t nb_group = clamp_blocks(size_t(A.n_group) - A.n_wave, kBlock / kPullLanes, 4096);
+  const uint32_t nb_one = clamp_blocks(size_t(A.N) - A.n_group, kBlock, 4096);
+  if (nb_wave + nb_group + nb_one == 0) return hipSuccess;
+  const LoopCtl ctl{err, done, l, tol};
+  extend_pull_kernel<<<nb_wave + nb_group + nb_one, kBlock, 0, s>>>(A, nb_wave, nb_group, nb_one, ctl);
   return hipGetLastError();
 }
 

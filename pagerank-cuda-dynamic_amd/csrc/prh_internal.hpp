@@ -73,6 +73,9 @@ struct RankArgs {
   const uint32_t *src;  // in-CSR sources [M]
   const uint32_t *deg;  // out-degree [N]
   double C0, P, D, C;  // teleport share (1-P)/N, damping, frontier and prune tolerances
+  // Frontier variants, optional: per workgroup, [2 b] = in-edges of the rows left unaffected (the cost of pulling
+  // the next extension) and [2 b + 1] = out-edges of the rows put in the frontier (the cost of pushing it).
+  double *stats;
 };
 
 struct ExtendArgs {
@@ -83,6 +86,17 @@ struct ExtendArgs {
   Schedule sched;        // out-degree partition
   uint8_t *fresh;        // optional: also set for newly marked vertices (traversal levels)
   int *changed;          // optional: set to 1 when a vertex is newly marked
+  const int *direction;  // optional: the push runs only while *direction == 0 (see launch_direction)
+};
+
+// The pull form of the extension: every unaffected row w gets vaff[w] = 1 if one of its in-neighbours is in the
+// frontier. Same flags as the push. Rows [0, n_wave) take a wave each, [n_wave, n_group) 8 lanes, the rest 1 lane.
+struct PullArgs {
+  uint8_t *vaff;
+  const uint8_t *naff;
+  const uint32_t *off, *src;  // in-CSR (degree-ordered layout)
+  uint32_t n_wave, n_group, N;
+  const int *direction;       // runs only while *direction == 1
 };
 
 // Source-sliced plan of the plain update (Static / Naive-dynamic). Source u belongs to slice
@@ -129,6 +143,11 @@ hipError_t launch_partition(const uint32_t *off, uint32_t N, uint32_t t1, uint32
 // ---- rank.hip ----
 hipError_t launch_init_ranks(double *r, double *a, double *cr, double *ca, const uint32_t *deg, uint32_t N,
                              bool from_r, hipStream_t s);
+// Workgroups of the chunked rows' combining launch.
+inline uint32_t finish_chunked_grid(uint32_t n_chunked) {
+  const uint32_t g = (n_chunked + kWavesPerBlock - 1) / kWavesPerBlock;
+  return g > 4096 ? 4096 : g;
+}
 // Rank update of iteration l: the rank kernel, then (if the plan has chunked rows) their combining kernel. `mark`,
 // when non-null, is recorded between the two, so that it times the rank kernel alone.
 hipError_t launch_rank_update(int variant, const RankArgs &A, const RankPlan &plan, unsigned long long *err,
@@ -245,5 +264,11 @@ hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *del
 // Extension; when err/done are non-null the kernel honours the iteration control of iteration l.
 hipError_t launch_extend(const ExtendArgs &A, unsigned long long *err, int *done, int l, double tol,
                          hipStream_t s);
+// Sums the per-workgroup costs the rank launches left in stats[0 .. 2 n) and sets *direction to 1 (pull) when
+// pulling is cheaper, else 0 (push). Honours iteration l's control like the extension.
+hipError_t launch_direction(const double *stats, uint32_t n, int *direction, unsigned long long *err, int *done, int l,
+                            double tol, hipStream_t s);
+hipError_t launch_extend_pull(const PullArgs &A, unsigned long long *err, int *done, int l, double tol,
+                              hipStream_t s);
 
 }  // namespace prh

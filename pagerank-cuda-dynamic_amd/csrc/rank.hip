@@ -66,9 +66,14 @@ __device__ __forceinline__ RowOperands row_operands(const RankArgs &A, uint32_t 
   return o;
 }
 
+// Flags a frontier variant leaves on a row: still affected (not pruned), and in the frontier.
+struct RowFlags {
+  bool affected = true, frontier = false;
+};
+
 template <int VARIANT>
 __device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv, uint32_t d, double rv,
-                                                double crv) {
+                                                double crv, RowFlags &f) {
   using T = Traits<VARIANT>;
   double av;
   if constexpr (T::prune) av = fma(A.P, cv - crv, A.C0) * (1.0 / (1.0 - A.P / d));
@@ -78,10 +83,49 @@ __device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, d
   const double ev = fabs(av - rv);
   if constexpr (T::frontier) {
     const double rel = ev / fmax(rv, av);
-    if (T::prune && rel <= A.C) A.vaff[v] = 0;
-    A.naff[v] = rel > A.D ? 1 : 0;
+    if (T::prune && rel <= A.C) {
+      A.vaff[v] = 0;
+      f.affected = false;
+    }
+    f.frontier = rel > A.D;
+    A.naff[v] = f.frontier ? 1 : 0;
   }
   return ev;
+}
+
+template <int VARIANT>
+__device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv, uint32_t d, double rv,
+                                                double crv) {
+  RowFlags f;
+  return finish_vertex<VARIANT>(A, v, cv, d, rv, crv, f);
+}
+
+// Per-lane sums of the extension's two costs (frontier variants with A.stats set).
+struct CostAcc {
+  double pull = 0.0, push = 0.0;
+  __device__ void add(const RowFlags &f, uint32_t in_deg, uint32_t out_deg) {
+    if (!f.affected) pull += in_deg;
+    if (f.frontier) push += out_deg;
+  }
+};
+
+// Workgroup totals of the costs to A.stats[2 slot], [2 slot + 1]. Every thread must call it.
+template <int VARIANT>
+__device__ __forceinline__ void store_costs(const RankArgs &A, CostAcc c, uint32_t slot, double *lds8) {
+  if constexpr (Traits<VARIANT>::frontier) {
+    if (!A.stats) return;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const double pull = group_sum<kWave>(c.pull), push = group_sum<kWave>(c.push);
+    if (lane == 0) {
+      lds8[w] = pull;
+      lds8[kWavesPerBlock + w] = push;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      A.stats[2 * size_t(slot)] = (lds8[0] + lds8[1]) + (lds8[2] + lds8[3]);
+      A.stats[2 * size_t(slot) + 1] = (lds8[4] + lds8[5]) + (lds8[6] + lds8[7]);
+    }
+  }
 }
 
 template <int VARIANT>
@@ -152,23 +196,33 @@ __device__ __forceinline__ void chunked_rows(const RankArgs &A, const RankPlan &
 }
 
 template <int VARIANT>
-__device__ __forceinline__ void wave_rows(const RankArgs &A, const RankSection &S, uint32_t blk, double &emax) {
+__device__ __forceinline__ void wave_rows(const RankArgs &A, const RankSection &S, uint32_t blk, double &emax,
+                                          CostAcc &cost) {
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   const uint32_t rb = S.begin + blk * S.per_block, re = min(S.end, rb + S.per_block);
   for (uint32_t v = rb + w; v < re; v += kWavesPerBlock) {  // wave-uniform
     const RowOperands o = row_operands<VARIANT>(A, v);
+    RowFlags f;
     if (!o.upd) {
-      if (lane == 0) copy_vertex<VARIANT>(A, v, o.rv, o.crv);
+      if (lane == 0) {
+        copy_vertex<VARIANT>(A, v, o.rv, o.crv);
+        f.affected = false;
+        cost.add(f, o.e - o.b, o.d);
+      }
       continue;
     }
     double acc = gather_sum<kWave>(A, o.b + lane, o.e);
     acc = group_sum<kWave>(acc);
-    if (lane == 0) emax = fmax(emax, finish_vertex<VARIANT>(A, v, acc, o.d, o.rv, o.crv));
+    if (lane == 0) {
+      emax = fmax(emax, finish_vertex<VARIANT>(A, v, acc, o.d, o.rv, o.crv, f));
+      cost.add(f, o.e - o.b, o.d);
+    }
   }
 }
 
 template <int VARIANT, int W>
-__device__ __forceinline__ void group_rows(const RankArgs &A, const RankSection &S, uint32_t blk, double &emax) {
+__device__ __forceinline__ void group_rows(const RankArgs &A, const RankSection &S, uint32_t blk, double &emax,
+                                           CostAcc &cost) {
   constexpr int G = kWave / W;  // rows per wave per round
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   const int g = lane / W, j = lane % W;
@@ -184,8 +238,14 @@ __device__ __forceinline__ void group_rows(const RankArgs &A, const RankSection 
     }
     acc = group_sum<W>(acc);  // all lanes active again here
     if (valid && j == 0) {
-      if (o.upd) emax = fmax(emax, finish_vertex<VARIANT>(A, v, acc, o.d, o.rv, o.crv));
-      else copy_vertex<VARIANT>(A, v, o.rv, o.crv);
+      RowFlags f;
+      if (o.upd) {
+        emax = fmax(emax, finish_vertex<VARIANT>(A, v, acc, o.d, o.rv, o.crv, f));
+      } else {
+        copy_vertex<VARIANT>(A, v, o.rv, o.crv);
+        f.affected = false;
+      }
+      cost.add(f, o.e - o.b, o.d);
     }
   }
 }
@@ -200,20 +260,23 @@ __global__ __launch_bounds__(kBlock) void rank_update_kernel(RankArgs A, RankPla
   const RankSection &S = P.sec[s];
   const uint32_t blk = blockIdx.x - S.block_begin;
   double emax = 0.0;
+  CostAcc cost;
   if (S.kind == kChunkedRows) {
     chunked_rows<VARIANT>(A, P, S, blk, lds_sum);
   } else if (S.kind == kWaveRows) {
-    wave_rows<VARIANT>(A, S, blk, emax);
+    wave_rows<VARIANT>(A, S, blk, emax, cost);
   } else {
     switch (S.width) {
-      case 1: group_rows<VARIANT, 1>(A, S, blk, emax); break;
-      case 2: group_rows<VARIANT, 2>(A, S, blk, emax); break;
-      case 4: group_rows<VARIANT, 4>(A, S, blk, emax); break;
-      case 8: group_rows<VARIANT, 8>(A, S, blk, emax); break;
-      case 16: group_rows<VARIANT, 16>(A, S, blk, emax); break;
-      default: group_rows<VARIANT, 32>(A, S, blk, emax); break;
+      case 1: group_rows<VARIANT, 1>(A, S, blk, emax, cost); break;
+      case 2: group_rows<VARIANT, 2>(A, S, blk, emax, cost); break;
+      case 4: group_rows<VARIANT, 4>(A, S, blk, emax, cost); break;
+      case 8: group_rows<VARIANT, 8>(A, S, blk, emax, cost); break;
+      case 16: group_rows<VARIANT, 16>(A, S, blk, emax, cost); break;
+      default: group_rows<VARIANT, 32>(A, S, blk, emax, cost); break;
     }
   }
+  __shared__ double lds_cost[2 * kWavesPerBlock];
+  store_costs<VARIANT>(A, cost, blockIdx.x, lds_cost);
   emax = block_max(emax, lds_max);
   if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[ctl.l], emax);
 }
@@ -224,18 +287,30 @@ template <int VARIANT>
 __global__ __launch_bounds__(kBlock) void finish_chunked_kernel(RankArgs A, RankPlan P, uint32_t n_rows, LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;  // same outcome as the rank kernel's entry test of this iteration
   __shared__ double lds_max[kWavesPerBlock];
+  __shared__ double lds_cost[2 * kWavesPerBlock];
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   double emax = 0.0;
+  CostAcc cost;
   for (uint32_t v = blockIdx.x * kWavesPerBlock + w; v < n_rows; v += gridDim.x * kWavesPerBlock) {
+    const uint32_t in_deg = A.off[v + 1] - A.off[v], d = A.deg[v];
+    RowFlags f;
     if (!affected<VARIANT>(A, v)) {
-      if (lane == 0) copy_vertex<VARIANT>(A, v);
+      if (lane == 0) {
+        copy_vertex<VARIANT>(A, v);
+        f.affected = false;
+        cost.add(f, in_deg, d);
+      }
       continue;
     }
     double s = 0.0;
     for (uint32_t k = P.row_first[v] + lane; k < P.row_first[v + 1]; k += kWave) s += P.partial[k];
     s = group_sum<kWave>(s);
-    if (lane == 0) emax = fmax(emax, finish_vertex<VARIANT>(A, v, s));
+    if (lane == 0) {
+      emax = fmax(emax, finish_vertex<VARIANT>(A, v, s, d, A.r[v], Traits<VARIANT>::prune ? A.cr[v] : 0.0, f));
+      cost.add(f, in_deg, d);
+    }
   }
+  store_costs<VARIANT>(A, cost, P.blocks + blockIdx.x, lds_cost);
   emax = block_max(emax, lds_max);
   if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[ctl.l], emax);
 }
@@ -247,10 +322,7 @@ hipError_t launch_variant(const RankArgs &A, const RankPlan &P, const LoopCtl &c
     const hipError_t e = hipEventRecord(mark, s);
     if (e != hipSuccess) return e;
   }
-  if (P.n_chunked) {
-    const uint32_t g = (P.n_chunked + kWavesPerBlock - 1) / kWavesPerBlock;
-    finish_chunked_kernel<VARIANT><<<g > 4096 ? 4096 : g, kBlock, 0, s>>>(A, P, P.n_chunked, ctl);
-  }
+  if (P.n_chunked) finish_chunked_kernel<VARIANT><<<finish_chunked_grid(P.n_chunked), kBlock, 0, s>>>(A, P, P.n_chunked, ctl);
   return hipGetLastError();
 }
 

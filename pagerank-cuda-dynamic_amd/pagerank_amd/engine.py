@@ -68,6 +68,7 @@ class Tuning:
     check_interval: int = 8
     source_slices: int = 0
     block_edges: int = 4096
+    extension_direction: int = 1
 
 
 class Context:
@@ -118,12 +119,13 @@ class Context:
     def set_tuning(self, t: Tuning):
         self.check(self._lib.prh_context_set_tuning(
             self._h, C.byref(prh_tuning(t.switch_degree, t.huge_degree, t.low_width, t.check_interval,
-                                        t.source_slices, t.block_edges))))
+                                        t.source_slices, t.block_edges, t.extension_direction))))
 
     def get_tuning(self) -> Tuning:
         n = prh_tuning()
         self.check(self._lib.prh_context_get_tuning(self._h, C.byref(n)))
-        return Tuning(n.switch_degree, n.huge_degree, n.low_width, n.check_interval, n.source_slices, n.block_edges)
+        return Tuning(n.switch_degree, n.huge_degree, n.low_width, n.check_interval, n.source_slices, n.block_edges,
+                      n.extension_direction)
 
     def upload(self, y: HostGraph, with_out: bool = True) -> "DeviceGraph":
         return DeviceGraph(self, y, with_out)

@@ -222,3 +222,33 @@ def test_grid_traversal_long_chain(ctx, oracle_mod):
     y = apply_batch(x, b)
     got = ctx.mark_affected(ctx.upload(y), 2, b)
     assert np.array_equal(got, oracle_mod.mark(2, y, b))
+
+
+@pytest.mark.parametrize("frac", [1e-4, 1e-2])
+@pytest.mark.parametrize("method", [3, 4], ids=["DF", "DFP"])
+def test_extension_directions_agree(ctx, oracle_mod, method, frac):
+    """Push, pull and the per-iteration choice between them set the same affected flags, so DF and DF-P give
+    bit-identical ranks and iteration counts with every extension_direction, and the oracle's ranks to 1e-9."""
+    from pagerank_amd import Batch, HostGraph, Tuning, apply_batch, solve
+    x = HostGraph.rmat(13, 16, seed=8)
+    q, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, x)
+    b = Batch.random(x, max(1, int(frac * x.M)), 0.8, seed=12)
+    y = apply_batch(x, b)
+    yg = ctx.upload(y)
+    old = ctx.get_tuning()
+    out = {}
+    try:
+        for d in (0, 1, 2):
+            print(f"[direction] method={method} frac={frac} direction={d}", flush=True)
+            t = ctx.get_tuning()
+            t.extension_direction = d
+            ctx.set_tuning(t)
+            out[d] = solve(yg, method, batch=b, q=q)
+    finally:
+        ctx.set_tuning(old)
+    ref, it, _ = oracle_mod.pagerank_cuda(method, y, b, q)
+    for d in (1, 2):
+        assert out[d].iterations == out[0].iterations
+        assert np.array_equal(out[d].ranks, out[0].ranks), d
+    assert _close_iters(out[0].iterations, it)
+    assert np.abs(out[0].ranks - ref).max() <= RANK_TOL
