@@ -2,10 +2,13 @@
 """Benchmark: static PageRank on one MI355X, printed as one JSON line.
 
 A step is one complete static solve (tolerance 1e-10 on the L-inf change, damping 0.85, FP64 ranks, at most 500
-iterations) of a synthetic graph that is already resident in HBM. Default workload: directed RMAT scale 24
-(16,777,216 vertices, 268,435,456 samples, a/b/c/d = 0.57/0.19/0.19/0.05, seed 42, duplicates removed, one
-self-loop per vertex). `value` is GEdges/s per iteration over the whole timed region
-(|E| x iterations / wall time of the K solves, including partition and initialisation).
+iterations) of a synthetic graph that is already resident in HBM, writing its ranks to a device buffer
+(prh_solve_device). Default workload: directed RMAT scale 24 (16,777,216 vertices, 268,435,456 samples,
+a/b/c/d = 0.57/0.19/0.19/0.05, seed 42, duplicates removed, one self-loop per vertex). Before timing, the
+autotuner picks the launch plan (prh_autotune; --no-autotune keeps the defaults). `value` is GEdges/s per
+iteration over the whole timed region (|E| x iterations / wall time of the K solves, including initialisation
+and the ranks' return to the caller's vertex order). config.gedges_per_s_iter_host_buffers is the same solve with
+host buffers (ranks copied over PCIe), reported beside it.
 
 roofline: the dominant kernel is the rank update (one launch per iteration). Its algorithmic bytes per launch are
 12|E| + 24|V| (per edge a 4-byte source id and an 8-byte gathered FP64 value; per vertex a 4-byte offset, 4-byte
@@ -102,6 +105,9 @@ def main() -> int:
     ap.add_argument("--cols", type=int, default=4000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch plan instead of timing the autotuner's candidates first")
+    ap.add_argument("--tuning-cache", default=None, help="JSON cache file for the autotuner's choice")
     ap.add_argument("--source-slices", type=int, default=None, help="override the tuning's source_slices")
     ap.add_argument("--huge-degree", type=int, default=None, help="override the tuning's huge_degree")
     ap.add_argument("--copy-calibration", type=float, metavar="GIB", default=0.0,
@@ -114,7 +120,9 @@ def main() -> int:
     if args.copy_calibration > 0:
         return copy_calibration(args.copy_calibration)
 
-    from pagerank_amd import Context, PagerankOptions, pagerankStaticHip
+    import torch  # device buffer for the ranks; imported before the engine library (shared HIP runtime)
+
+    from pagerank_amd import STATIC, Context, PagerankOptions, pagerankStaticHip, solve_device
 
     t0 = time.time()
     g, desc = make_graph(args)
@@ -127,25 +135,40 @@ def main() -> int:
         if args.huge_degree is not None:
             t.huge_degree = args.huge_degree
         ctx.set_tuning(t)
-    tuning = ctx.get_tuning()
     yg = ctx.upload(g, with_out=False)
+    autotuned = False
+    if not args.no_autotune and args.source_slices is None and args.huge_degree is None:
+        t0 = time.time()
+        t, ms, hit = ctx.autotune(yg, args.tuning_cache, iterations=10)
+        autotuned = True
+        log(f"autotune: {t} at {ms:.3f} ms/iteration ({'cache' if hit else f'{time.time() - t0:.1f} s'})")
+    tuning = ctx.get_tuning()
     o = PagerankOptions()
+    ranks = torch.empty(g.N, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
     for i in range(args.warmup):
-        r = pagerankStaticHip(yg, o)
-        log(f"warmup {i}: {r.iterations} iterations, {r.time:.2f} ms")
+        st = solve_device(yg, STATIC, o, None, ranks.data_ptr())
+        log(f"warmup {i}: {st.iterations} iterations, {st.time:.2f} ms")
     ctx.kernel_timing(True)
     ctx.kernel_timing_read()
     iters = 0
     tcomp = 0.0
-    t_start = time.perf_counter()  # every solve returns after its stream has drained
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()  # every solve returns after the engine's stream has drained
     for _ in range(args.steps):
-        r = pagerankStaticHip(yg, o)
-        iters += r.iterations
-        tcomp += r.computationTime
+        st = solve_device(yg, STATIC, o, None, ranks.data_ptr())
+        iters += st.iterations
+        tcomp += st.computation_time
+    torch.cuda.synchronize()
     t_total = time.perf_counter() - t_start
     launches, kms = ctx.kernel_timing_read()
     ctx.kernel_timing(False)
     log(f"timed {args.steps} solves: {iters} iterations, {t_total * 1e3:.1f} ms, rank-update launches {launches}")
+    # The same solve through host buffers (ranks copied back over PCIe, as the C++ drop-in does): reported beside
+    # the headline, never as it.
+    t0 = time.perf_counter()
+    r = pagerankStaticHip(yg, o)
+    host_rate = g.M * r.iterations / (time.perf_counter() - t0) / 1e9
 
     value = g.M * iters / t_total / 1e9
     kavg_s = kms / max(launches, 1) / 1e3
@@ -166,8 +189,10 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": desc, "vertices": g.N, "edges": g.M, "tolerance": 1e-10, "damping": 0.85,
                    "tuning": {"switch_degree": tuning.switch_degree, "huge_degree": tuning.huge_degree,
-                              "source_slices": tuning.source_slices},
+                              "source_slices": tuning.source_slices, "block_edges": tuning.block_edges,
+                              "autotuned": autotuned},
                    "iterations_per_solve": iters / args.steps,
+                   "gedges_per_s_iter_host_buffers": round(host_rate, 4),
                    "gedges_per_s_iter_compute_only": round(g.M * iters / (tcomp / 1e3) / 1e9, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(desc),

@@ -73,7 +73,7 @@ typedef struct prh_stats {
 /* Launch tuning (no counterpart in the reference, whose values are fixed: switch degree 64, 256-thread blocks).
  * Results do not depend on these beyond floating-point summation order. */
 typedef struct prh_tuning {
-  uint32_t switch_degree; /* in-degree <= this: several vertices per wave; above: one wave per vertex [128] */
+  uint32_t switch_degree; /* in-degree <= this: several vertices per wave; above: one wave per vertex [64] */
   uint32_t huge_degree;   /* in-degree > this: split into chunks of this many edges, one workgroup per chunk,
                              summed by a second launch [16384] */
   int low_width;          /* lanes per vertex of in-degree <= switch_degree: 1, 2, 4, 8, 16 or 32; 0 = by degree
@@ -82,7 +82,7 @@ typedef struct prh_tuning {
   int source_slices;      /* Static / Naive-dynamic: 0 = row kernel; 1, 2, 4 or 8 = sum each row slice by slice,
                              source u in slice (u / 16) mod source_slices, slice x on the workgroups b with
                              b mod source_slices = x (8: one slice per XCD and its L2) [0] */
-  uint32_t block_edges;   /* edges per workgroup of the rank kernels (>= 256) [4096] */
+  uint32_t block_edges;   /* edges per workgroup of the rank kernels (>= 256) [2048] */
   int extension_direction; /* DF / DF-P extension: 0 push along out-edges of frontier vertices; 1 push or pull
                               (unaffected rows look for a frontier in-neighbour), whichever the last rank launch
                               found cheaper; 2 pull. The flags, hence the results, are the same for all [1] */
@@ -181,6 +181,17 @@ int prh_weighted_pagerank(prh_context *context, const prh_graph *graph, const fl
 int prh_weighted_adjoint(prh_context *context, const prh_graph *graph, const float *weights, const double *ranks,
                          const double *grad_ranks, const prh_options *options, double *lambda,
                          double *grad_weights, double *grad_teleport, prh_stats *stats, int device_pointers);
+
+/* Autotuner (no counterpart in the reference, whose launch constants are fixed: switch degree 64, 256-thread
+ * blocks, pagerankCuda.hxx:93, _cuda.hxx:62-120). Times candidate launch plans of the static update on this graph
+ * (row plan: switch degree 64/128/256 x chunk threshold 4096/16384 x 2048/4096 edges per workgroup; sliced plan:
+ * 1 or 8 source slices x 2048/4096 edges per workgroup), `iterations` iterations each after a warm-up, and sets
+ * the fastest as the context's tuning (other fields keep their values). With cache_path (may be NULL) the choice
+ * is read from / written to a JSON file keyed by the device architecture, its compute-unit count, N, M and a hash
+ * of the graph's degree sequence; a hit skips the timing (*from_cache = 1). Results of every plan agree within
+ * rounding, so tuning changes run time only. */
+int prh_autotune(prh_context *context, const prh_graph *graph, const char *cache_path, int iterations,
+                 prh_tuning *chosen, double *ms_per_iteration, int *from_cache);
 
 /* Per-launch timing of the rank-update kernel with HIP events on the context's stream (no counterpart in the
  * reference). While enabled, every solve accumulates the summed duration and count of the rank-update

@@ -75,6 +75,8 @@ ENGINE_SIGNATURES = {
     "prh_weighted_pagerank": (C.c_int, [vp, vp, vp, vp, C.POINTER(prh_options), vp, C.POINTER(prh_stats), C.c_int]),
     "prh_weighted_adjoint": (C.c_int, [vp, vp, vp, vp, vp, C.POINTER(prh_options), vp, vp, vp,
                                        C.POINTER(prh_stats), C.c_int]),
+    "prh_autotune": (C.c_int, [vp, vp, C.c_char_p, C.c_int, C.POINTER(prh_tuning), C.POINTER(C.c_double),
+                              C.POINTER(C.c_int)]),
     "prh_kernel_timing_enable": (C.c_int, [vp, C.c_int]),
     "prh_kernel_timing_read": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
 }

@@ -62,12 +62,12 @@ class PagerankResult:
 
 @dataclass
 class Tuning:
-    switch_degree: int = 128
+    switch_degree: int = 64
     huge_degree: int = 16384
     low_width: int = 0
     check_interval: int = 8
     source_slices: int = 0
-    block_edges: int = 4096
+    block_edges: int = 2048
     extension_direction: int = 1
 
 
@@ -126,6 +126,18 @@ class Context:
         self.check(self._lib.prh_context_get_tuning(self._h, C.byref(n)))
         return Tuning(n.switch_degree, n.huge_degree, n.low_width, n.check_interval, n.source_slices, n.block_edges,
                       n.extension_direction)
+
+    def autotune(self, yg: "DeviceGraph", cache_path: str | None = None, iterations: int = 10):
+        """prh_autotune: time candidate launch plans on yg and keep the fastest as this context's tuning.
+        Returns (Tuning, ms per iteration, whether the choice came from the cache)."""
+        t = prh_tuning()
+        ms = C.c_double()
+        hit = C.c_int()
+        path = cache_path.encode() if cache_path else None
+        self.check(self._lib.prh_autotune(self._h, yg.handle, path, iterations, C.byref(t), C.byref(ms),
+                                          C.byref(hit)))
+        return (Tuning(t.switch_degree, t.huge_degree, t.low_width, t.check_interval, t.source_slices, t.block_edges,
+                       t.extension_direction), ms.value, bool(hit.value))
 
     def upload(self, y: HostGraph, with_out: bool = True) -> "DeviceGraph":
         return DeviceGraph(self, y, with_out)
