@@ -193,6 +193,32 @@ int prh_weighted_adjoint(prh_context *context, const prh_graph *graph, const flo
 int prh_autotune(prh_context *context, const prh_graph *graph, const char *cache_path, int iterations,
                  prh_tuning *chosen, double *ms_per_iteration, int *from_cache);
 
+/* Streaming dynamic session (no counterpart in the reference, whose experiment rebuilds y on the host and uploads
+ * it for every batch, main.cu:80-143): a device-resident copy of `graph` (uploaded with its out-CSR) that update
+ * batches modify on the device, with the ranks kept on the device between batches. Each update tidies the batch as
+ * tidyBatchUpdateU does (batch.hxx:201-208: deletions of existing edges only, insertions of absent edges only,
+ * duplicates dropped), applies it (applyBatchUpdateU, batch.hxx:219-247), and solves with `method` from the previous
+ * ranks: PRH_STATIC (from 1/N), PRH_NAIVE_DYNAMIC, PRH_DYNAMIC_FRONTIER or PRH_PRUNE_DYNAMIC_FRONTIER (the traversal
+ * closure's host-checked levels are not streamed: PRH_ERROR_UNSUPPORTED). The batch application and marking run as
+ * one captured HIP graph and the iterations as a captured graph of 8 iterations replayed until the device reports
+ * convergence. Batch ids are the caller's vertex ids (host arrays); each side holds at most max_batch edges;
+ * max_edges bounds the graph's growth (0: |E| + 16 max_batch). initial_ranks (host, caller order) may be NULL: a
+ * static solve of `graph` then provides them. A batch that names a vertex outside the graph or leaves a vertex
+ * without out-edges fails and leaves the stream unusable. stats->initialization_time covers the batch upload,
+ * application and marking; computation_time the iterations. The context's tuning at the first update of each
+ * buffer parity is captured with the graphs. */
+typedef struct prh_stream prh_stream;
+int prh_stream_create(prh_context *context, const prh_graph *graph, prh_method method, const prh_options *options,
+                      uint32_t max_batch, uint64_t max_edges, const double *initial_ranks, prh_stream **stream);
+int prh_stream_update(prh_stream *stream, const uint32_t *deletion_sources, const uint32_t *deletion_targets,
+                      size_t n_deletions, const uint32_t *insertion_sources, const uint32_t *insertion_targets,
+                      size_t n_insertions, double *ranks, prh_stats *stats);
+/* The current ranks in the caller's vertex order (host). */
+int prh_stream_ranks(prh_stream *stream, double *ranks);
+/* The current number of edges. */
+uint64_t prh_stream_size(const prh_stream *stream);
+void prh_stream_destroy(prh_stream *stream);
+
 /* Per-launch timing of the rank-update kernel with HIP events on the context's stream (no counterpart in the
  * reference). While enabled, every solve accumulates the summed duration and count of the rank-update
  * launches that did work; reading resets the accumulators. */

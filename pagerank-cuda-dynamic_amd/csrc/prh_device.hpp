@@ -102,19 +102,23 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long *slot, doub
 struct LoopCtl {
   unsigned long long *err;  // [max iterations] L-inf bits, zeroed before the loop
   int *done;                // 0 while running, else the number of iterations performed
-  int l;                    // this launch's iteration index
+  int l;                    // this launch's iteration index, or -1: read it from done[1] (replayed graphs, where
+                            // the same launch serves many iterations and a device counter says which)
   double tol;
 };
+
+__device__ __forceinline__ int iter_index(const LoopCtl &c) { return c.l >= 0 ? c.l : c.done[1]; }
 
 // Entry test for the first kernel of iteration l; all threads of a block agree.
 // done and err[l-1] were last written by earlier launches, whose completion made them visible: plain loads (both
 // issued together) suffice. A volatile read here would be a system-scope load at the start of every wave.
 __device__ __forceinline__ bool iteration_begin(const LoopCtl &c) {
   const int done = *c.done;
-  const double e = c.l > 0 ? __longlong_as_double((long long)c.err[c.l - 1]) : c.tol;
+  const int l = iter_index(c);
+  const double e = l > 0 ? __longlong_as_double((long long)c.err[l - 1]) : c.tol;
   if (done) return false;
   if (e < c.tol) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *(volatile int *)c.done = c.l;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *(volatile int *)c.done = l;
     return false;
   }
   return true;
@@ -123,7 +127,7 @@ __device__ __forceinline__ bool iteration_begin(const LoopCtl &c) {
 // Entry test for kernels that follow the rank update within iteration l (frontier extension).
 __device__ __forceinline__ bool iteration_continues(const LoopCtl &c) {
   const int done = *c.done;
-  const double e = __longlong_as_double((long long)c.err[c.l]);
+  const double e = __longlong_as_double((long long)c.err[iter_index(c)]);
   return !done && e >= c.tol;
 }
 

@@ -258,6 +258,35 @@ hipError_t launch_slice_scatter(const uint32_t *off, const uint32_t *src, uint32
 hipError_t launch_slice_wbase(const uint32_t *item, uint32_t N, uint32_t n_huge, int slices, const SliceBases &b,
                               uint32_t *wbase, hipStream_t s);
 
+// ---- stream.hip (device-side batch application; see the file's header) ----
+hipError_t launch_batch_keys(const uint32_t *bu, const uint32_t *bv, const uint32_t *count, uint32_t cap,
+                             const uint32_t *inv, uint32_t N, unsigned long long *in_keys,
+                             unsigned long long *out_keys, int *flags, int bad_bit, hipStream_t s);
+hipError_t sort_keys_u64(void *tmp, size_t &tmp_bytes, const unsigned long long *in, unsigned long long *out,
+                         uint32_t n, hipStream_t s);
+hipError_t launch_batch_valid(const unsigned long long *keys, uint32_t cap, const uint32_t *off, const uint32_t *idx,
+                              bool want_exists, uint32_t *valid, hipStream_t s);
+hipError_t launch_batch_compact(const unsigned long long *keys, const uint32_t *valid, const uint32_t *pos,
+                                uint32_t cap, unsigned long long *out, uint32_t *count, hipStream_t s);
+hipError_t launch_row_delta(const unsigned long long *dkeys, const uint32_t *nd, const unsigned long long *ikeys,
+                            const uint32_t *ni, uint32_t cap_total, int *delta, uint8_t *changed, hipStream_t s);
+hipError_t launch_new_degrees(const uint32_t *off, const int *delta, uint32_t N, uint32_t *deg, int *flags,
+                              int empty_bit, hipStream_t s);
+hipError_t launch_rewrite(const uint32_t *old_off, const uint32_t *old_idx, const uint32_t *new_off,
+                          const uint8_t *changed, const unsigned long long *dkeys, const uint32_t *nd,
+                          const unsigned long long *ikeys, const uint32_t *ni, uint32_t N, uint64_t max_edges,
+                          uint32_t *new_idx, hipStream_t s);
+hipError_t launch_insert(const uint32_t *old_off, const uint32_t *old_idx, const uint32_t *new_off,
+                         const unsigned long long *dkeys, const uint32_t *nd, const unsigned long long *ikeys,
+                         const uint32_t *ni, uint32_t cap, uint32_t *new_idx, hipStream_t s);
+hipError_t launch_degrees(const uint32_t *off, uint32_t N, uint32_t *deg, hipStream_t s);
+hipError_t launch_refresh_chunks(const uint32_t *item_row, const uint32_t *row_first, uint32_t n_items,
+                                 const uint32_t *off, uint32_t chunk, uint32_t *item_beg, uint32_t *item_end,
+                                 hipStream_t s);
+hipError_t launch_mark_batch(uint8_t *naff, uint8_t *vaff, const unsigned long long *dkeys, const uint32_t *nd,
+                             const unsigned long long *ikeys, const uint32_t *ni, uint32_t cap_total, hipStream_t s);
+hipError_t launch_advance(int *l, int *done, const unsigned long long *err, double tol, int max_iter, hipStream_t s);
+
 // ---- frontier.hip ----
 hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *delu, const uint32_t *delv,
                                size_t ND, const uint32_t *insu, size_t NI, hipStream_t s);

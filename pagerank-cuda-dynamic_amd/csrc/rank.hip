@@ -278,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void rank_update_kernel(RankArgs A, RankPla
   __shared__ double lds_cost[2 * kWavesPerBlock];
   store_costs<VARIANT>(A, cost, blockIdx.x, lds_cost);
   emax = block_max(emax, lds_max);
-  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[ctl.l], emax);
+  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
 }
 
 // Chunked rows, after the rank kernel of the same iteration: one wave per row adds the row's chunk partials (in a
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void finish_chunked_kernel(RankArgs A, Rank
   }
   store_costs<VARIANT>(A, cost, P.blocks + blockIdx.x, lds_cost);
   emax = block_max(emax, lds_max);
-  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[ctl.l], emax);
+  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
 }
 
 template <int VARIANT>
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void sliced_finish_kernel(RankArgs A, Slice
     }
   }
   emax = block_max(emax, lds_max);
-  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[ctl.l], emax);
+  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
 }
 
 // r = a = 1/N (or keep r as loaded and copy it to a); contributions cr = ca = r/deg.

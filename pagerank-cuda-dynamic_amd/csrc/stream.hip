@@ -1,0 +1,295 @@
+// stream.hip -- applying an edge batch to a device-resident graph (the streaming dynamic session).
+//
+// The session keeps the degree-ordered layout of its base graph (the vertex permutation stays fixed) and updates
+// both CSRs of that layout in place of the host's tidy-and-apply (tidyBatchUpdateU, applyBatchUpdateU,
+// batch.hxx:152-247): deletions keep only existing edges, insertions only absent ones, both sorted and unique;
+// deletions are removed and insertions added. Every row stays sorted by column.
+//
+// Per CSR (rows = targets for the in-CSR, sources for the out-CSR), with key = row << 32 | column:
+//   batch_keys     ids mapped to layout ids, keys built (entries past the batch's size or with a bad id: ~0)
+//   (sort)         rocPRIM merge sort of the keys, at the session's fixed capacity
+//   batch_valid    an entry survives if it is the first of its key and its edge exists (deletions) or is
+//                  absent (insertions) in the current rows (binary search)
+//   (scan)         exclusive scan of the survivor flags
+//   batch_compact  survivors packed in key order, their count to a device word
+//   row_delta      per-row degree change and a "changed" flag
+//   new_degrees    old degree + change (the out-CSR also flags rows left with no out-edge)
+//   (scan)         new offsets
+//   rewrite        every old edge to its new place: unchanged rows shift, changed rows skip deleted columns and
+//                  make room for inserted ones (ranks found by binary search in the row's deletions/insertions)
+//   insert         every inserted column to its place in its row
+// Sizes are read from device words, so one captured sequence serves every batch up to the capacity.
+#include <rocprim/device/device_merge_sort.hpp>
+
+#include "prh_device.hpp"
+#include "prh_internal.hpp"
+
+namespace prh {
+namespace {
+
+constexpr unsigned long long kNoKey = ~0ull;
+
+inline uint32_t grid_for(size_t n, size_t per, uint32_t cap) {
+  const size_t g = (n + per - 1) / per;
+  return uint32_t(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+// First index in a[lo, hi) with a[i] >= x.
+template <class T>
+__device__ __forceinline__ uint32_t lower_bound(const T *a, uint32_t lo, uint32_t hi, T x) {
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void batch_keys_kernel(const uint32_t *bu, const uint32_t *bv,
+                                                            const uint32_t *count, uint32_t cap, const uint32_t *inv,
+                                                            uint32_t N, unsigned long long *in_keys,
+                                                            unsigned long long *out_keys, int *flags, int bad_bit) {
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cap; i += gridDim.x * kBlock) {
+    unsigned long long ki = kNoKey, ko = kNoKey;
+    if (i < n) {
+      const uint32_t u = bu[i], v = bv[i];
+      if (u < N && v < N) {
+        const unsigned long long lu = inv[u], lv = inv[v];
+        ki = lv << 32 | lu;
+        ko = lu << 32 | lv;
+      } else {
+        atomicOr(flags, bad_bit);
+      }
+    }
+    in_keys[i] = ki;
+    out_keys[i] = ko;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void batch_valid_kernel(const unsigned long long *keys, uint32_t cap,
+                                                             const uint32_t *off, const uint32_t *idx, int want_exists,
+                                                             uint32_t *valid) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cap; i += gridDim.x * kBlock) {
+    const unsigned long long k = keys[i];
+    bool ok = k != kNoKey && (i == 0 || keys[i - 1] != k);
+    if (ok) {
+      const uint32_t row = uint32_t(k >> 32), col = uint32_t(k);
+      const uint32_t b = off[row], e = off[row + 1];
+      const uint32_t at = lower_bound(idx, b, e, col);
+      const bool exists = at < e && idx[at] == col;
+      ok = exists == (want_exists != 0);
+    }
+    valid[i] = ok ? 1u : 0u;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void batch_compact_kernel(const unsigned long long *keys, const uint32_t *valid,
+                                                               const uint32_t *pos, uint32_t cap,
+                                                               unsigned long long *out, uint32_t *count) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cap; i += gridDim.x * kBlock) {
+    if (valid[i]) out[pos[i]] = keys[i];
+    if (i == cap - 1) *count = pos[i] + valid[i];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void row_delta_kernel(const unsigned long long *dkeys, const uint32_t *nd,
+                                                           const unsigned long long *ikeys, const uint32_t *ni,
+                                                           int *delta, uint8_t *changed) {
+  const uint32_t a = *nd, b = *ni;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a + b; i += gridDim.x * kBlock) {
+    const uint32_t row = uint32_t((i < a ? dkeys[i] : ikeys[i - a]) >> 32);
+    atomicAdd(&delta[row], i < a ? -1 : 1);
+    changed[row] = 1;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void new_degrees_kernel(const uint32_t *off, const int *delta, uint32_t N,
+                                                             uint32_t *deg, int *flags, int empty_bit) {
+  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v <= N; v += gridDim.x * kBlock) {
+    if (v == N) {
+      deg[v] = 0;
+      continue;
+    }
+    const uint32_t d = uint32_t(int(off[v + 1] - off[v]) + delta[v]);
+    deg[v] = d;
+    if (empty_bit && d == 0) atomicOr(flags, empty_bit);
+  }
+}
+
+constexpr uint32_t kRewriteTile = 4096;  // old edges per workgroup
+
+// Workgroup b moves old edges [b * tile, (b + 1) * tile): each edge finds its row by binary search in the offsets
+// of the rows that overlap the tile.
+__global__ __launch_bounds__(kBlock) void rewrite_kernel(const uint32_t *old_off, const uint32_t *old_idx,
+                                                         const uint32_t *new_off, const uint8_t *changed,
+                                                         const unsigned long long *dkeys, const uint32_t *nd,
+                                                         const unsigned long long *ikeys, const uint32_t *ni,
+                                                         uint32_t N, uint32_t *new_idx) {
+  const uint32_t M = old_off[N];
+  const uint32_t e0 = blockIdx.x * kRewriteTile;
+  if (e0 >= M) return;
+  const uint32_t e1 = min(M, e0 + kRewriteTile);
+  // rows overlapping [e0, e1): from the row holding e0 to the row holding e1 - 1
+  const uint32_t r_first = lower_bound(old_off, 0u, N + 1, e0 + 1) - 1;
+  const uint32_t r_last = lower_bound(old_off, r_first, N + 1, e1) - 1;
+  const uint32_t D = *nd, I = *ni;
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
+    const uint32_t row = lower_bound(old_off, r_first, r_last + 1, e + 1) - 1;
+    const uint32_t col = old_idx[e];
+    const uint32_t k = e - old_off[row];
+    if (!changed[row]) {
+      new_idx[new_off[row] + k] = col;
+      continue;
+    }
+    const unsigned long long rk = (unsigned long long)row << 32, key = rk | col;
+    const uint32_t d_lo = lower_bound(dkeys, 0u, D, rk), d_at = lower_bound(dkeys, d_lo, D, key);
+    if (d_at < D && dkeys[d_at] == key) continue;  // deleted
+    const uint32_t i_lo = lower_bound(ikeys, 0u, I, rk), i_at = lower_bound(ikeys, i_lo, I, key);
+    new_idx[new_off[row] + k - (d_at - d_lo) + (i_at - i_lo)] = col;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void insert_kernel(const uint32_t *old_off, const uint32_t *old_idx,
+                                                        const uint32_t *new_off, const unsigned long long *dkeys,
+                                                        const uint32_t *nd, const unsigned long long *ikeys,
+                                                        const uint32_t *ni, uint32_t *new_idx) {
+  const uint32_t D = *nd, I = *ni;
+  for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < I; j += gridDim.x * kBlock) {
+    const unsigned long long key = ikeys[j], rk = key & 0xFFFFFFFF00000000ull;
+    const uint32_t row = uint32_t(key >> 32), col = uint32_t(key);
+    const uint32_t i_lo = lower_bound(ikeys, 0u, I, rk);
+    const uint32_t b = old_off[row], e = old_off[row + 1];
+    const uint32_t o_at = lower_bound(old_idx, b, e, col) - b;
+    const uint32_t d_lo = lower_bound(dkeys, 0u, D, rk), d_at = lower_bound(dkeys, d_lo, D, key);
+    new_idx[new_off[row] + o_at - (d_at - d_lo) + (j - i_lo)] = col;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void degrees_kernel(const uint32_t *off, uint32_t N, uint32_t *deg) {
+  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < N; v += gridDim.x * kBlock) deg[v] = off[v + 1] - off[v];
+}
+
+// Item i of chunked row r = item_row[i], the k-th of the row's items, covers [off[r] + k chunk, ...); the row's
+// last item also takes any remainder, so a row that grew past its items is still covered.
+__global__ __launch_bounds__(kBlock) void refresh_chunks_kernel(const uint32_t *item_row, const uint32_t *row_first,
+                                                                uint32_t n_items, const uint32_t *off, uint32_t chunk,
+                                                                uint32_t *item_beg, uint32_t *item_end) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_items; i += gridDim.x * kBlock) {
+    const uint32_t r = item_row[i], k = i - row_first[r];
+    const bool last = i + 1 == row_first[r + 1];
+    const uint32_t b = off[r], e = off[r + 1];
+    const uint64_t lo = uint64_t(b) + uint64_t(k) * chunk;
+    item_beg[i] = uint32_t(lo < e ? lo : e);
+    item_end[i] = last ? e : uint32_t(lo + chunk < e ? lo + chunk : e);
+  }
+}
+
+// DF initial marking from the compacted batch in layout ids (out-CSR keys: source << 32 | target):
+// naff[u] for every deletion and insertion source, vaff[v] for every deletion target (pagerankCuda.hxx:272-281).
+__global__ __launch_bounds__(kBlock) void mark_batch_kernel(uint8_t *naff, uint8_t *vaff,
+                                                            const unsigned long long *dkeys, const uint32_t *nd,
+                                                            const unsigned long long *ikeys, const uint32_t *ni) {
+  const uint32_t a = *nd, b = *ni;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a + b; i += gridDim.x * kBlock) {
+    const unsigned long long k = i < a ? dkeys[i] : ikeys[i - a];
+    naff[uint32_t(k >> 32)] = 1;
+    if (i < a) vaff[uint32_t(k)] = 1;
+  }
+}
+
+// After iteration l's launches: stop when its L-inf change is below tolerance (done = l + 1 iterations) or when
+// the iteration cap is reached; otherwise move to iteration l + 1.
+__global__ void advance_kernel(int *l, int *done, const unsigned long long *err, double tol, int max_iter) {
+  if (*done) return;
+  const int i = *l;
+  const double e = __longlong_as_double((long long)err[i]);
+  if (e < tol || i + 1 >= max_iter) *done = i + 1;
+  else *l = i + 1;
+}
+
+}  // namespace
+
+hipError_t launch_batch_keys(const uint32_t *bu, const uint32_t *bv, const uint32_t *count, uint32_t cap,
+                             const uint32_t *inv, uint32_t N, unsigned long long *in_keys,
+                             unsigned long long *out_keys, int *flags, int bad_bit, hipStream_t s) {
+  batch_keys_kernel<<<grid_for(cap, kBlock, 1024), kBlock, 0, s>>>(bu, bv, count, cap, inv, N, in_keys, out_keys,
+                                                                    flags, bad_bit);
+  return hipGetLastError();
+}
+
+// Merge sort: its launches are kernels only (rocPRIM's radix sort clears state with hipMemsetAsync, which must not
+// be recorded into the captured graphs that replay this).
+hipError_t sort_keys_u64(void *tmp, size_t &tmp_bytes, const unsigned long long *in, unsigned long long *out,
+                         uint32_t n, hipStream_t s) {
+  return rocprim::merge_sort(tmp, tmp_bytes, in, out, n, rocprim::less<unsigned long long>(), s);
+}
+
+hipError_t launch_batch_valid(const unsigned long long *keys, uint32_t cap, const uint32_t *off, const uint32_t *idx,
+                              bool want_exists, uint32_t *valid, hipStream_t s) {
+  batch_valid_kernel<<<grid_for(cap, kBlock, 1024), kBlock, 0, s>>>(keys, cap, off, idx, want_exists ? 1 : 0, valid);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_compact(const unsigned long long *keys, const uint32_t *valid, const uint32_t *pos,
+                                uint32_t cap, unsigned long long *out, uint32_t *count, hipStream_t s) {
+  batch_compact_kernel<<<grid_for(cap, kBlock, 1024), kBlock, 0, s>>>(keys, valid, pos, cap, out, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_delta(const unsigned long long *dkeys, const uint32_t *nd, const unsigned long long *ikeys,
+                            const uint32_t *ni, uint32_t cap_total, int *delta, uint8_t *changed, hipStream_t s) {
+  row_delta_kernel<<<grid_for(cap_total, kBlock, 1024), kBlock, 0, s>>>(dkeys, nd, ikeys, ni, delta, changed);
+  return hipGetLastError();
+}
+
+hipError_t launch_new_degrees(const uint32_t *off, const int *delta, uint32_t N, uint32_t *deg, int *flags,
+                              int empty_bit, hipStream_t s) {
+  new_degrees_kernel<<<grid_for(size_t(N) + 1, kBlock, 8192), kBlock, 0, s>>>(off, delta, N, deg, flags, empty_bit);
+  return hipGetLastError();
+}
+
+hipError_t launch_rewrite(const uint32_t *old_off, const uint32_t *old_idx, const uint32_t *new_off,
+                          const uint8_t *changed, const unsigned long long *dkeys, const uint32_t *nd,
+                          const unsigned long long *ikeys, const uint32_t *ni, uint32_t N, uint64_t max_edges,
+                          uint32_t *new_idx, hipStream_t s) {
+  const uint64_t g = (max_edges + kRewriteTile - 1) / kRewriteTile;
+  rewrite_kernel<<<uint32_t(g ? g : 1), kBlock, 0, s>>>(old_off, old_idx, new_off, changed, dkeys, nd, ikeys, ni, N,
+                                                        new_idx);
+  return hipGetLastError();
+}
+
+hipError_t launch_insert(const uint32_t *old_off, const uint32_t *old_idx, const uint32_t *new_off,
+                         const unsigned long long *dkeys, const uint32_t *nd, const unsigned long long *ikeys,
+                         const uint32_t *ni, uint32_t cap, uint32_t *new_idx, hipStream_t s) {
+  insert_kernel<<<grid_for(cap, kBlock, 1024), kBlock, 0, s>>>(old_off, old_idx, new_off, dkeys, nd, ikeys, ni,
+                                                               new_idx);
+  return hipGetLastError();
+}
+
+hipError_t launch_degrees(const uint32_t *off, uint32_t N, uint32_t *deg, hipStream_t s) {
+  degrees_kernel<<<grid_for(N, kBlock, 8192), kBlock, 0, s>>>(off, N, deg);
+  return hipGetLastError();
+}
+
+hipError_t launch_refresh_chunks(const uint32_t *item_row, const uint32_t *row_first, uint32_t n_items,
+                                 const uint32_t *off, uint32_t chunk, uint32_t *item_beg, uint32_t *item_end,
+                                 hipStream_t s) {
+  if (!n_items) return hipSuccess;
+  refresh_chunks_kernel<<<grid_for(n_items, kBlock, 1024), kBlock, 0, s>>>(item_row, row_first, n_items, off, chunk,
+                                                                           item_beg, item_end);
+  return hipGetLastError();
+}
+
+hipError_t launch_mark_batch(uint8_t *naff, uint8_t *vaff, const unsigned long long *dkeys, const uint32_t *nd,
+                             const unsigned long long *ikeys, const uint32_t *ni, uint32_t cap_total, hipStream_t s) {
+  mark_batch_kernel<<<grid_for(cap_total, kBlock, 1024), kBlock, 0, s>>>(naff, vaff, dkeys, nd, ikeys, ni);
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(int *l, int *done, const unsigned long long *err, double tol, int max_iter, hipStream_t s) {
+  advance_kernel<<<1, 1, 0, s>>>(l, done, err, tol, max_iter);
+  return hipGetLastError();
+}
+
+}  // namespace prh

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void weighted_step_kernel(WeightedArgs A, u
     }
   }
   emax = block_max(emax, lds);
-  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[ctl.l], emax);
+  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
 }
 
 // W_u = sum of u's out-weights (FP64), one wave per row.

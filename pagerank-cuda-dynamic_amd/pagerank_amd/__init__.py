@@ -5,14 +5,14 @@ this package is its Python mirror plus host-side graph preparation.
 """
 from .engine import (DYNAMIC_FRONTIER, DYNAMIC_TRAVERSAL, METHOD_NAMES, NAIVE_DYNAMIC, PRUNE_DYNAMIC_FRONTIER,
                      STATIC, BatchedResult, Context, DeviceGraph, PagerankError, PagerankOptions, PagerankResult,
-                     Tuning, ppr_batched, weighted_adjoint, weighted_pagerank,
+                     Stream, Tuning, ppr_batched, weighted_adjoint, weighted_pagerank,
                      pagerankDynamicFrontierHip, pagerankDynamicTraversalHip, pagerankNaiveDynamicHip,
                      pagerankPruneDynamicFrontierHip, pagerankStaticHip, solve, solve_device)
 from .graphs import Batch, HostGraph, apply_batch
 
 __all__ = [
     "STATIC", "NAIVE_DYNAMIC", "DYNAMIC_TRAVERSAL", "DYNAMIC_FRONTIER", "PRUNE_DYNAMIC_FRONTIER", "METHOD_NAMES",
-    "Context", "DeviceGraph", "PagerankError", "PagerankOptions", "PagerankResult", "Tuning", "solve",
+    "Context", "DeviceGraph", "PagerankError", "PagerankOptions", "PagerankResult", "Stream", "Tuning", "solve",
     "solve_device", "BatchedResult", "ppr_batched", "weighted_pagerank", "weighted_adjoint", "pagerankStaticHip", "pagerankNaiveDynamicHip", "pagerankDynamicTraversalHip",
     "pagerankDynamicFrontierHip", "pagerankPruneDynamicFrontierHip", "Batch", "HostGraph", "apply_batch",
 ]

@@ -77,6 +77,12 @@ ENGINE_SIGNATURES = {
                                        C.POINTER(prh_stats), C.c_int]),
     "prh_autotune": (C.c_int, [vp, vp, C.c_char_p, C.c_int, C.POINTER(prh_tuning), C.POINTER(C.c_double),
                               C.POINTER(C.c_int)]),
+    "prh_stream_create": (C.c_int, [vp, vp, C.c_int, C.POINTER(prh_options), C.c_uint32, C.c_uint64, f64p,
+                                   C.POINTER(vp)]),
+    "prh_stream_update": (C.c_int, [vp, u32p, u32p, C.c_size_t, u32p, u32p, C.c_size_t, f64p, C.POINTER(prh_stats)]),
+    "prh_stream_ranks": (C.c_int, [vp, f64p]),
+    "prh_stream_size": (C.c_uint64, [vp]),
+    "prh_stream_destroy": (None, [vp]),
     "prh_kernel_timing_enable": (C.c_int, [vp, C.c_int]),
     "prh_kernel_timing_read": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
 }

@@ -299,6 +299,59 @@ def weighted_adjoint(yg: DeviceGraph, weights: np.ndarray, ranks: np.ndarray, gr
     return lam, gw, gs, st.iterations
 
 
+class Stream:
+    """Streaming dynamic session (prh_stream_*): a device copy of yg that batches update on the device, with each
+    batch's solve (method from the previous ranks) replayed from captured HIP graphs."""
+
+    def __init__(self, yg: DeviceGraph, method: int, o: PagerankOptions | None = None, max_batch: int = 1 << 16,
+                 max_edges: int = 0, initial_ranks: np.ndarray | None = None):
+        self.ctx = yg.ctx
+        self.graph = yg  # keeps the base graph (its layout and permutation) alive
+        self.N = yg.N
+        o = o or PagerankOptions()
+        q = None
+        if initial_ranks is not None:
+            q = np.ascontiguousarray(initial_ranks, dtype=np.float64)
+            if len(q) != yg.N:
+                raise ValueError("initial ranks need one value per vertex")
+        h = C.c_void_p()
+        self.ctx.check(self.ctx._lib.prh_stream_create(self.ctx.handle, yg.handle, method, C.byref(o.native()),
+                                                       max_batch, max_edges,
+                                                       q.ctypes.data_as(f64p) if q is not None else None,
+                                                       C.byref(h)))
+        self._h = h
+
+    def update(self, batch: Batch, want_ranks: bool = True) -> PagerankResult:
+        ranks = np.zeros(self.N) if want_ranks else np.zeros(0)
+        st = prh_stats()
+        self.ctx.check(self.ctx._lib.prh_stream_update(self._h, _p(batch.del_u), _p(batch.del_v), batch.deletions,
+                                                       _p(batch.ins_u), _p(batch.ins_v), batch.insertions,
+                                                       ranks.ctypes.data_as(f64p) if want_ranks else None,
+                                                       C.byref(st)))
+        return PagerankResult(ranks, st.iterations, st.time, st.initialization_time, st.marking_time,
+                              st.computation_time)
+
+    def ranks(self) -> np.ndarray:
+        r = np.zeros(self.N)
+        self.ctx.check(self.ctx._lib.prh_stream_ranks(self._h, r.ctypes.data_as(f64p)))
+        return r
+
+    @property
+    def size(self) -> int:
+        return int(self.ctx._lib.prh_stream_size(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.ctx._lib.prh_stream_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # Reference-named wrappers (pagerankCuda.hxx:864-988). x is accepted for signature parity and, like the
 # reference, only its emptiness matters.
 def pagerankStaticHip(yg: DeviceGraph, o: PagerankOptions | None = None, partition: bool = True) -> PagerankResult:
