@@ -517,6 +517,7 @@ int stage_batch(prh_context *c, const prh_graph *g, const uint32_t *du, const ui
 // reference reads its partition point back (pagerankCuda.hxx:633).
 int out_schedule(prh_context *c, const prh_graph *g, prh::Schedule &out_s) {
   const uint32_t N = g->N;
+  PRH_CHECK(ensure(c, c->sched_out, N));  // the partition below writes N ids into it
   PRH_CHECK(ensure(c, c->tiles, size_t(3) * prh::partition_tiles(N)));
   PRH_CHECK(ensure(c, c->counts, 8));
   PRH_HIP(c, prh::launch_partition(g->l_out_off, N, 64u, 0xFFFFFFFFu, c->sched_out.p, c->tiles.p, c->counts.p,

@@ -1,6 +1,7 @@
 """Experiment runner on synthetic graphs, following the reference's protocol and log format (main.cu:47-148).
 
     python -m pagerank_amd.experiment dynamic --scale 22 --edge-factor 16 --fractions 1e-5,1e-4,1e-3 --batches 3
+    python -m pagerank_amd.experiment stream --scale 22 --fractions 1e-4 --batches 16
     python -m pagerank_amd.experiment ppr --scale 22 --columns 64
     python -m pagerank_amd.experiment adjoint --scale 20
 
@@ -11,6 +12,11 @@ DT / DF / DF-P, each warm-started from its own result of the previous batch (mai
 compared with a static solve of the same graph at tolerance 1e-100, which runs the full 500 iterations
 (main.cu:88). The reference computes that solve on the CPU; here it runs on the GPU (same Jacobi iteration, FP64).
 One line per method is printed in glog's format (main.cu:56-60). A summary of the speedups over Static follows.
+
+stream: 16 (--batches) consecutive DF-P batches of one size (--fractions, first value) through the streaming session
+(prh_stream: batch applied on the device, apply and iterations replayed from captured HIP graphs), against the same
+batches as separate calls (host-side apply, upload of the new graph, DF-P solve) and against the DF-P solve times
+alone. Every stream result is compared with the separate call's.
 
 ppr: one batched personalized solve of k teleport columns, each with a seed set of uniform random size 1..1000,
 against k single-column solves and k plain static solves on the same graph.
@@ -130,6 +136,61 @@ def run_dynamic(args, out: CsvLog) -> int:
     return 0
 
 
+def run_stream(args, out: CsvLog) -> int:
+    x, name = make_rmat(args.scale, args.edge_factor, args.seed)
+    frac = float(args.fractions.split(",")[0])
+    size = max(1, int(frac * x.M))
+    ctx = Context(0)
+    xg = ctx.upload(x)
+    q0 = solve(xg, engine.STATIC, PagerankOptions()).ranks
+    seq = []
+    cur = x
+    for i in range(args.batches):
+        b = Batch.random(cur, size, 0.8, seed=args.seed * 7 + i)
+        y = apply_batch(cur, b)
+        seq.append((b, y))
+        cur = y
+    log(f"{args.batches} batches of {frac:g}|E| ({size} edges each) prepared")
+    # streaming session: one warm-up pass (graph capture) on a throwaway session, then the timed pass
+    st = engine.Stream(xg, engine.PRUNE_DYNAMIC_FRONTIER, max_batch=size + 16, initial_ranks=q0)
+    st.update(seq[0][0], want_ranks=False)
+    st.close()
+    st = engine.Stream(xg, engine.PRUNE_DYNAMIC_FRONTIER, max_batch=size + 16, initial_ranks=q0)
+    t0 = time.perf_counter()
+    stream_res = [st.update(b, want_ranks=False) for b, _ in seq]
+    stream_wall = (time.perf_counter() - t0) * 1e3
+    final_stream = st.ranks()
+    st.close()
+    # separate calls: host apply (prepared above, not timed), upload, DF-P solve
+    q = q0
+    t0 = time.perf_counter()
+    sep_solve_ms, sep_iters, worst = 0.0, 0, 0.0
+    results = []
+    for b, y in seq:
+        yg = ctx.upload(y)
+        r = solve(yg, engine.PRUNE_DYNAMIC_FRONTIER, PagerankOptions(), batch=b, q=q)
+        yg.close()
+        sep_solve_ms += r.time
+        sep_iters += r.iterations
+        q = r.ranks
+        results.append(r)
+    sep_wall = (time.perf_counter() - t0) * 1e3
+    worst = float(np.abs(final_stream - q).max())
+    s_iters = sum(r.iterations for r in stream_res)
+    s_apply = sum(r.initializationTime for r in stream_res)
+    s_comp = sum(r.computationTime for r in stream_res)
+    log(f"stream: {stream_wall:.2f} ms wall for {args.batches} batches ({stream_wall / args.batches:.3f} ms/batch); "
+        f"device apply+mark {s_apply:.2f} ms, iterations {s_comp:.2f} ms, {s_iters} iterations")
+    log(f"separate calls: {sep_wall:.2f} ms wall (upload + layout + solve; host-side apply not counted), "
+        f"DF-P solves alone {sep_solve_ms:.2f} ms, {sep_iters} iterations; stream speedup {sep_wall / stream_wall:.2f}x "
+        f"(vs solves alone {sep_solve_ms / stream_wall:.2f}x); L-inf(final stream - final separate) {worst:.2e}")
+    for tech, t, its in (("dfp_stream", stream_wall, s_iters), ("dfp_separate_calls", sep_wall, sep_iters),
+                         ("dfp_solves_only", sep_solve_ms, sep_iters)):
+        out.row(mode="stream", graph=name, vertices=x.N, edges=x.M, batch_fraction=frac, technique=tech, time_ms=t,
+                iterations=its, extra=f"batches={args.batches};linf={worst:.3e}")
+    return 0
+
+
 def seed_sets(n: int, k: int, rng: np.random.Generator, max_size: int = 1000) -> list[np.ndarray]:
     return [rng.choice(n, size=int(rng.integers(1, max_size + 1)), replace=False).astype(np.uint32)
             for _ in range(k)]
@@ -197,7 +258,7 @@ def run_adjoint(args, out: CsvLog) -> int:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("mode", choices=["dynamic", "ppr", "adjoint"])
+    ap.add_argument("mode", choices=["dynamic", "stream", "ppr", "adjoint"])
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--edge-factor", type=float, default=16.0)
     ap.add_argument("--seed", type=int, default=42)
@@ -208,7 +269,7 @@ def main(argv=None) -> int:
     ap.add_argument("--csv", default=None)
     args = ap.parse_args(argv)
     out = CsvLog(args.csv)
-    return {"dynamic": run_dynamic, "ppr": run_ppr, "adjoint": run_adjoint}[args.mode](args, out)
+    return {"dynamic": run_dynamic, "stream": run_stream, "ppr": run_ppr, "adjoint": run_adjoint}[args.mode](args, out)
 
 
 if __name__ == "__main__":

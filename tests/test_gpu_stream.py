@@ -95,3 +95,19 @@ def test_stream_rejects_bad_batches(ctx):
     st.close()
     with pytest.raises(PagerankError, match="traversal"):
         Stream(ctx.upload(x), 2, max_batch=8)
+
+
+def test_stream_on_fresh_context(oracle_mod):
+    """A context whose first dynamic work is a stream (no earlier dynamic solve has sized its workspace)."""
+    from pagerank_amd import Batch, Context, HostGraph, Stream, apply_batch, solve
+    x = HostGraph.rmat(11, 16, seed=9)
+    b = Batch.random(x, 50, 0.8, seed=1)
+    y = apply_batch(x, b)
+    with Context(0) as fresh:
+        xg = fresh.upload(x)
+        q = solve(xg, 0).ranks  # a static solve only
+        st = Stream(xg, 4, max_batch=64, initial_ranks=q)
+        got = st.update(b)
+        st.close()
+        ref, it, _ = oracle_mod.pagerank_cuda(oracle_mod.PRUNE_DYNAMIC_FRONTIER, y, b, q)
+        assert abs(got.iterations - it) <= 1 and np.abs(got.ranks - ref).max() <= RANK_TOL
