@@ -13,6 +13,8 @@
 // Random numbers: SplitMix64 (Steele, Lea and Flood, "Fast splittable pseudorandom number generators",
 // OOPSLA 2014), one stream per sample index, so results do not depend on the thread count.
 #include <omp.h>
+#include <cstdio>
+#include <cstdlib>
 
 #include <algorithm>
 #include <atomic>
@@ -184,6 +186,14 @@ void build_transpose(prhh_graph *g) {
   }
 }
 
+// Phase lines on stderr when PRHH_PROGRESS is set (large builds take minutes).
+void progress(const char *phase, uint64_t n) {
+  static const bool on = std::getenv("PRHH_PROGRESS") != nullptr;
+  if (!on) return;
+  std::fprintf(stderr, "[graphgen] %s (%llu)\n", phase, (unsigned long long)n);
+  std::fflush(stderr);
+}
+
 // Generic bucketed build from a sample function f(i) -> (u, v) for i in [0, S).
 template <class F>
 prhh_graph *build_from_samples(uint32_t N, uint64_t S, bool self_loops, F f) {
@@ -197,6 +207,7 @@ prhh_graph *build_from_samples(uint32_t N, uint64_t S, bool self_loops, F f) {
     f(uint64_t(i), u, v);
     cnt[u].fetch_add(1, std::memory_order_relaxed);
   }
+  progress("sources counted", S);
   uint32_t *bcount = alloc_array<uint32_t>(N);
   #pragma omp parallel for schedule(static)
   for (int64_t u = 0; u < int64_t(N); ++u) {
@@ -213,12 +224,15 @@ prhh_graph *build_from_samples(uint32_t N, uint64_t S, bool self_loops, F f) {
     raw[bstart[u] + cnt[u].fetch_add(1, std::memory_order_relaxed)] = v;
   }
   cnt.reset();
+  progress("samples bucketed by source", S);
   prhh_graph *g = new prhh_graph();
   bool ok = finalize_rows(g, N, raw, bstart.get(), bcount, self_loops);
   ::operator delete[](raw);
   ::operator delete[](bcount);
   if (!ok) { delete g; return nullptr; }
+  progress("rows sorted and deduplicated", g->M);
   build_transpose(g);
+  progress("transpose built", g->M);
   return g;
 }
 

@@ -88,9 +88,20 @@ def make_rmat(scale: int, edge_factor: float, seed: int) -> tuple[HostGraph, str
     return g, name
 
 
+def autotune(ctx: Context, g: HostGraph, args) -> None:
+    if args.no_autotune:
+        return
+    yg = ctx.upload(g, with_out=False)
+    t, ms, hit = ctx.autotune(yg, args.tuning_cache, iterations=10)
+    yg.close()
+    log(f"autotune: switch {t.switch_degree}, chunk {t.huge_degree}, slices {t.source_slices}, "
+        f"block edges {t.block_edges}: {ms:.3f} ms per static iteration{' (cache)' if hit else ''}")
+
+
 def run_dynamic(args, out: CsvLog) -> int:
     x0, name = make_rmat(args.scale, args.edge_factor, args.seed)
     ctx = Context(0)
+    autotune(ctx, x0, args)
     o = PagerankOptions(repeat=args.repeat)
     exact = PagerankOptions(repeat=1, tolerance=1e-100)
     summary = []
@@ -267,6 +278,8 @@ def main(argv=None) -> int:
     ap.add_argument("--repeat", type=int, default=5)
     ap.add_argument("--columns", type=int, default=64)
     ap.add_argument("--csv", default=None)
+    ap.add_argument("--no-autotune", action="store_true", help="dynamic mode: keep the default launch plan")
+    ap.add_argument("--tuning-cache", default=None, help="dynamic mode: the autotuner's JSON cache")
     args = ap.parse_args(argv)
     out = CsvLog(args.csv)
     return {"dynamic": run_dynamic, "stream": run_stream, "ppr": run_ppr, "adjoint": run_adjoint}[args.mode](args, out)
