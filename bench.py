@@ -58,15 +58,25 @@ def make_graph(args):
 
 
 def cpu_baseline(g, budget_s: float):
+    """Static solves (tolerance 1e-10) of the same graph by the OpenMP port, bounded to about budget_s seconds:
+    iterations of one solve are cut at the budget, and a solve that converges early is repeated."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     _, it, ms = oracle.static_omp(g, max_iterations=2, threads=threads)
     per_iter = ms / max(it, 1) / 1e3
     k = int(max(2, min(500, budget_s / max(per_iter, 1e-6))))
-    _, it, ms = oracle.static_omp(g, max_iterations=k, threads=threads)
-    value = g.M * it / (ms / 1e3) / 1e9
+    iters, total_ms, solves = 0, 0.0, 0
+    while total_ms / 1e3 < 0.8 * budget_s or solves == 0:
+        _, it, ms = oracle.static_omp(g, max_iterations=k, threads=threads)
+        iters += it
+        total_ms += ms
+        solves += 1
+        if it < 2:
+            break
+    value = g.M * iters / (total_ms / 1e3) / 1e9
     return {"value": round(value, 4), "unit": "GEdges/s/iter", "cores": threads, "kind": "port",
-            "sample": f"{it} synchronous iterations of the OpenMP static solver on the same graph ({ms / 1e3:.1f} s)"}
+            "sample": f"{solves} static solve(s) of the OpenMP port on the same graph, {iters} synchronous iterations "
+                      f"in {total_ms / 1e3:.1f} s"}
 
 
 def copy_calibration(gib: float) -> int:
