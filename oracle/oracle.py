@@ -49,6 +49,9 @@ _REF_SIG = {
                                u8p]),
     "ref_apply_batch": (C.c_int64, [C.c_uint32, u32p, u32p, u32p, u32p, C.c_size_t, u32p, u32p, C.c_size_t, u32p,
                                     u32p]),
+    "ref_read_mtx": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), u32p, u32p]),
+    "ref_read_temporal": (C.c_int, [C.c_char_p, C.c_uint64, C.c_uint64, C.c_int, C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_uint64), u32p, u32p]),
 }
 
 _lib = None
@@ -240,3 +243,26 @@ def ref_apply_batch(x, batch):
                               _u32(iu)[1], _u32(iv)[1], len(iu), y_off.ctypes.data_as(u32p),
                               y_edg.ctypes.data_as(u32p))
     return y_off, y_edg[:m]
+
+
+def _ref_csr(call):
+    n, m = C.c_uint32(), C.c_uint64()
+    if call(C.byref(n), C.byref(m), None, None) != 0:
+        raise ValueError("reference reader failed")
+    off = np.zeros(n.value + 1, dtype=np.uint32)
+    edg = np.zeros(max(m.value, 1), dtype=np.uint32)
+    call(C.byref(n), C.byref(m), _u32(off)[1], _u32(edg)[1])
+    return off, edg[:m.value]
+
+
+def ref_read_mtx(path):
+    """The reference's readMtxW (mtx.hxx) as an out-CSR over the existing vertices in key order."""
+    p = str(path).encode()
+    return _ref_csr(lambda n, m, o, e: ref().ref_read_mtx(p, n, m, o, e))
+
+
+def ref_read_temporal(path, rows, lines, symmetric=False):
+    """The reference's readTemporalW (snap.hxx) on the first `lines` lines, as ref_read_mtx."""
+    p = str(path).encode()
+    return _ref_csr(lambda n, m, o, e: ref().ref_read_temporal(p, rows, lines, 1 if symmetric else 0, n, m, o, e))
+

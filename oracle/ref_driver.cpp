@@ -24,6 +24,8 @@
 #include "batch.hxx"
 #include "pagerank.hxx"
 #include "pagerankPrune.hxx"
+#include "mtx.hxx"
+#include "snap.hxx"
 
 using K = uint32_t;
 using V = double;
@@ -137,6 +139,43 @@ int64_t ref_apply_batch(uint32_t N, const uint32_t *xoff, const uint32_t *xedg, 
   }
   y_off[N] = o;
   return int64_t(o);
+}
+
+// The reference's readers (mtx.hxx readMtxW, snap.hxx readTemporalW) into its DiGraph, exported as an out-CSR over
+// the graph's existing vertices in ascending key order (the compaction pagerankInvokeCuda applies,
+// pagerankCuda.hxx:756-763). Call with off = edg = NULL to get *n and *m, then again with arrays of n + 1 and m.
+static int export_csr(const G &x, uint32_t *n, uint64_t *m, uint32_t *off, uint32_t *edg) {
+  vector<K> keys;
+  x.forEachVertexKey([&](auto u) { keys.push_back(u); });
+  std::sort(keys.begin(), keys.end());
+  vector<uint32_t> id(x.span(), 0);
+  for (size_t i = 0; i < keys.size(); ++i) id[keys[i]] = uint32_t(i);
+  *n = uint32_t(keys.size());
+  *m = x.size();
+  if (!off || !edg) return 0;
+  uint64_t k = 0;
+  for (size_t i = 0; i < keys.size(); ++i) {
+    off[i] = uint32_t(k);
+    vector<uint32_t> row;
+    x.forEachEdgeKey(keys[i], [&](auto v) { row.push_back(id[v]); });
+    std::sort(row.begin(), row.end());
+    for (uint32_t v : row) edg[k++] = v;
+  }
+  off[keys.size()] = uint32_t(k);
+  return 0;
+}
+
+int ref_read_mtx(const char *path, uint32_t *n, uint64_t *m, uint32_t *off, uint32_t *edg) {
+  G x;
+  readMtxW(x, path);
+  return export_csr(x, n, m, off, edg);
+}
+
+int ref_read_temporal(const char *path, uint64_t rows, uint64_t lines, int symmetric, uint32_t *n, uint64_t *m,
+                      uint32_t *off, uint32_t *edg) {
+  G x;
+  readTemporalW(x, path, false, symmetric != 0, rows, lines);
+  return export_csr(x, n, m, off, edg);
 }
 
 }  // extern "C"

@@ -448,4 +448,126 @@ prhh_graph *prhh_apply_batch(const prhh_graph *x, const prhh_batch *b) {
   return y;
 }
 
+
+// ---- readers (mtx.hxx, snap.hxx) ---------------------------------------------------------------------------------
+// Edge lists with 0-based ids: file vertex k (1-based, as the reference's DiGraph keeps it) becomes k - 1, which is
+// where pagerankInvokeCuda's compaction of the existing vertices puts it (pagerankCuda.hxx:756-763, properties.hxx).
+
+struct prhh_edges {
+  std::vector<uint32_t> src, dst;
+  uint64_t order = 0;  // vertices 1..order in the file
+  int symmetric = 0;
+};
+
+struct prhh_temporal {
+  FILE *f = nullptr;
+  ~prhh_temporal() {
+    if (f) std::fclose(f);
+  }
+};
+
+namespace {
+
+// Parses "u v ..." (1-based ids); false on a line that does not start with two integers.
+bool parse_pair(const char *line, uint64_t &u, uint64_t &v) {
+  char *end = nullptr;
+  u = std::strtoull(line, &end, 10);
+  if (end == line) return false;
+  const char *p = end;
+  v = std::strtoull(p, &end, 10);
+  return end != p;
+}
+
+}  // namespace
+
+// Matrix Market coordinate file as a directed edge list (readMtxDo, mtx.hxx:120-140): comment lines start with
+// '%'; the banner's symmetry field "symmetric" or "skew-symmetric" adds each edge's mirror; the size line is
+// "rows cols nnz"; body lines "u v [w]" stop at the first line without two integers. Returns NULL if the file
+// cannot be read, is not "matrix coordinate", or names a vertex 0 or above max(rows, cols).
+prhh_edges *prhh_read_mtx(const char *path) {
+  FILE *f = std::fopen(path, "r");
+  if (!f) return nullptr;
+  std::unique_ptr<prhh_edges> e(new prhh_edges());
+  char line[4096];
+  char h[5][64] = {};
+  bool banner = false, ok = true;
+  uint64_t rows = 0, cols = 0, size = 0;
+  while (std::fgets(line, sizeof(line), f)) {
+    if (line[0] == '%') {
+      if (line[1] == '%') {
+        banner = std::sscanf(line, "%63s %63s %63s %63s %63s", h[0], h[1], h[2], h[3], h[4]) >= 4;
+      }
+      continue;
+    }
+    ok = banner && std::strcmp(h[1], "matrix") == 0 && std::strcmp(h[2], "coordinate") == 0 &&
+         std::sscanf(line, "%llu %llu %llu", (unsigned long long *)&rows, (unsigned long long *)&cols,
+                     (unsigned long long *)&size) == 3;
+    break;
+  }
+  if (!ok) {
+    std::fclose(f);
+    return nullptr;
+  }
+  e->symmetric = std::strcmp(h[4], "symmetric") == 0 || std::strcmp(h[4], "skew-symmetric") == 0;
+  e->order = std::max(rows, cols);
+  e->src.reserve(size * (e->symmetric ? 2 : 1));
+  e->dst.reserve(size * (e->symmetric ? 2 : 1));
+  while (std::fgets(line, sizeof(line), f)) {
+    uint64_t u, v;
+    if (!parse_pair(line, u, v)) break;
+    if (u == 0 || v == 0 || u > e->order || v > e->order) {
+      std::fclose(f);
+      return nullptr;
+    }
+    e->src.push_back(uint32_t(u - 1));
+    e->dst.push_back(uint32_t(v - 1));
+    if (e->symmetric) {
+      e->src.push_back(uint32_t(v - 1));
+      e->dst.push_back(uint32_t(u - 1));
+    }
+  }
+  std::fclose(f);
+  return e.release();
+}
+
+// SNAP temporal edge file read in consecutive pieces, as readTemporalDo does (snap.hxx:40-60): each call reads up to
+// `lines` further lines "u v [t]" (1-based ids <= rows), adding mirrors when symmetric, and stops early at the end of
+// the file or at a line without two integers. Returns NULL on an id 0 or above rows.
+prhh_temporal *prhh_temporal_open(const char *path) {
+  FILE *f = std::fopen(path, "r");
+  if (!f) return nullptr;
+  prhh_temporal *t = new prhh_temporal();
+  t->f = f;
+  return t;
+}
+
+prhh_edges *prhh_temporal_read(prhh_temporal *t, uint64_t rows, uint64_t lines, int symmetric) {
+  if (!t || !t->f) return nullptr;
+  std::unique_ptr<prhh_edges> e(new prhh_edges());
+  e->order = rows;
+  e->symmetric = symmetric != 0;
+  char line[4096];
+  for (; lines > 0 && std::fgets(line, sizeof(line), t->f); --lines) {
+    uint64_t u, v;
+    if (!parse_pair(line, u, v)) break;
+    if (u == 0 || v == 0 || u > rows || v > rows) return nullptr;
+    e->src.push_back(uint32_t(u - 1));
+    e->dst.push_back(uint32_t(v - 1));
+    if (symmetric) {
+      e->src.push_back(uint32_t(v - 1));
+      e->dst.push_back(uint32_t(u - 1));
+    }
+  }
+  return e.release();
+}
+
+void prhh_temporal_close(prhh_temporal *t) { delete t; }
+
+uint64_t prhh_edges_count(const prhh_edges *e) { return e ? e->src.size() : 0; }
+uint64_t prhh_edges_order(const prhh_edges *e) { return e ? e->order : 0; }
+int prhh_edges_symmetric(const prhh_edges *e) { return e ? e->symmetric : 0; }
+const uint32_t *prhh_edges_src(const prhh_edges *e) { return e ? e->src.data() : nullptr; }
+const uint32_t *prhh_edges_dst(const prhh_edges *e) { return e ? e->dst.data() : nullptr; }
+void prhh_edges_free(prhh_edges *e) { delete e; }
+
 }  // extern "C"

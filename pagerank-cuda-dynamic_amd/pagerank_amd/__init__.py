@@ -8,11 +8,11 @@ from .engine import (DYNAMIC_FRONTIER, DYNAMIC_TRAVERSAL, METHOD_NAMES, NAIVE_DY
                      Stream, Tuning, ppr_batched, weighted_adjoint, weighted_pagerank,
                      pagerankDynamicFrontierHip, pagerankDynamicTraversalHip, pagerankNaiveDynamicHip,
                      pagerankPruneDynamicFrontierHip, pagerankStaticHip, solve, solve_device)
-from .graphs import Batch, HostGraph, apply_batch
+from .graphs import Batch, HostGraph, TemporalReader, apply_batch, read_mtx_edges
 
 __all__ = [
     "STATIC", "NAIVE_DYNAMIC", "DYNAMIC_TRAVERSAL", "DYNAMIC_FRONTIER", "PRUNE_DYNAMIC_FRONTIER", "METHOD_NAMES",
     "Context", "DeviceGraph", "PagerankError", "PagerankOptions", "PagerankResult", "Stream", "Tuning", "solve",
     "solve_device", "BatchedResult", "ppr_batched", "weighted_pagerank", "weighted_adjoint", "pagerankStaticHip", "pagerankNaiveDynamicHip", "pagerankDynamicTraversalHip",
-    "pagerankDynamicFrontierHip", "pagerankPruneDynamicFrontierHip", "Batch", "HostGraph", "apply_batch",
+    "pagerankDynamicFrontierHip", "pagerankPruneDynamicFrontierHip", "Batch", "HostGraph", "TemporalReader", "apply_batch", "read_mtx_edges",
 ]

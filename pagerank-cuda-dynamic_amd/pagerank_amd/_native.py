@@ -91,6 +91,16 @@ HOST_SIGNATURES = {
     "prhh_rmat": (vp, [C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int]),
     "prhh_grid": (vp, [C.c_uint32, C.c_uint32, C.c_double, C.c_uint64]),
     "prhh_from_edges": (vp, [C.c_uint32, C.c_uint64, u32p, u32p, C.c_int]),
+    "prhh_read_mtx": (vp, [C.c_char_p]),
+    "prhh_temporal_open": (vp, [C.c_char_p]),
+    "prhh_temporal_read": (vp, [vp, C.c_uint64, C.c_uint64, C.c_int]),
+    "prhh_temporal_close": (None, [vp]),
+    "prhh_edges_count": (C.c_uint64, [vp]),
+    "prhh_edges_order": (C.c_uint64, [vp]),
+    "prhh_edges_symmetric": (C.c_int, [vp]),
+    "prhh_edges_src": (vp, [vp]),
+    "prhh_edges_dst": (vp, [vp]),
+    "prhh_edges_free": (None, [vp]),
     "prhh_free": (None, [vp]),
     "prhh_order": (C.c_uint32, [vp]),
     "prhh_size": (C.c_uint64, [vp]),

@@ -1,6 +1,8 @@
-"""Experiment runner on synthetic graphs, following the reference's protocol and log format (main.cu:47-148).
+"""Experiment runner on synthetic graphs and SNAP temporal files, following the reference's protocol and log format (main.cu:47-148).
 
     python -m pagerank_amd.experiment dynamic --scale 22 --edge-factor 16 --fractions 1e-5,1e-4,1e-3 --batches 3
+    python -m pagerank_amd.experiment temporal --file sx-stackoverflow.txt --rows 26019770 --size 63497050 \\
+        --fractions 1e-5 --batches 5
     python -m pagerank_amd.experiment stream --scale 22 --fractions 1e-4 --batches 16
     python -m pagerank_amd.experiment ppr --scale 22 --columns 64
     python -m pagerank_amd.experiment adjoint --scale 20
@@ -12,6 +14,10 @@ DT / DF / DF-P, each warm-started from its own result of the previous batch (mai
 compared with a static solve of the same graph at tolerance 1e-100, which runs the full 500 iterations
 (main.cu:88). The reference computes that solve on the CPU; here it runs on the GPU (same Jacobi iteration, FP64).
 One line per method is printed in glog's format (main.cu:56-60). A summary of the speedups over Static follows.
+
+temporal: the same loop on the reference's input (main.cu:157-174): the first 90% of a SNAP temporal file's lines
+make the base graph (self-loop on every vertex), and each batch is the next fraction*size lines as insertions,
+tidied against the current graph. --rows and --size are the reference's arguments 2 and 3 (main.sh:46-50).
 
 stream: 16 (--batches) consecutive DF-P batches of one size (--fractions, first value) through the streaming session
 (prh_stream: batch applied on the device, apply and iterations replayed from captured HIP graphs), against the same
@@ -37,7 +43,7 @@ import numpy as np
 
 from . import engine
 from .engine import Context, PagerankOptions, solve
-from .graphs import Batch, HostGraph, apply_batch
+from .graphs import Batch, HostGraph, TemporalReader, apply_batch
 
 METHODS = [
     (engine.NAIVE_DYNAMIC, "pagerankNaiveDynamicHip"),
@@ -98,43 +104,46 @@ def autotune(ctx: Context, g: HostGraph, args) -> None:
         f"block edges {t.block_edges}: {ms:.3f} ms per static iteration{' (cache)' if hit else ''}")
 
 
-def run_dynamic(args, out: CsvLog) -> int:
-    x0, name = make_rmat(args.scale, args.edge_factor, args.seed)
-    ctx = Context(0)
-    autotune(ctx, x0, args)
+def run_batches(ctx: Context, x: HostGraph, name: str, frac: float, next_batch, args, out: CsvLog,
+                summary: list) -> None:
+    """runExperiment's loop (main.cu:79-146) on one batch source: next_batch(x, bi) gives the tidied batch bi
+    against the current graph x (None ends the stream early). Every method runs on each updated graph and is
+    compared with a static solve of it at tolerance 1e-100."""
     o = PagerankOptions(repeat=args.repeat)
     exact = PagerankOptions(repeat=1, tolerance=1e-100)
-    summary = []
-    for frac in [float(f) for f in args.fractions.split(",")]:
-        x = x0
-        xg = ctx.upload(x, with_out=False)
-        r0 = solve(xg, engine.STATIC, exact).ranks
-        xg.close()
-        prev = {m: r0 for m, _ in METHODS}
-        for bi in range(args.batches):
-            size = max(1, int(frac * x.M))
-            t0 = time.time()
-            b = Batch.random(x, size, 0.8, seed=args.seed * 1000 + bi + int(frac * 1e9))
-            y = apply_batch(x, b)
-            yg = ctx.upload(y)
-            log(f"batch {bi} of {frac:g}|E|: -{b.deletions}/+{b.insertions} edges, y has {y.M} edges "
-                f"({time.time() - t0:.1f} s to build and upload)")
-            s0 = solve(yg, engine.STATIC, exact).ranks
-            res = {engine.STATIC: solve(yg, engine.STATIC, o)}
-            for m, _ in METHODS:
-                res[m] = solve(yg, m, o, batch=b, q=prev[m])
-            for m, tech in [(engine.STATIC, "pagerankStaticHip")] + METHODS:
-                a = res[m]
-                print(glog(a, s0, tech, 1, 0.0, frac, bi), flush=True)
-                out.row(mode="dynamic", graph=name, vertices=y.N, edges=y.M, batch_fraction=frac, batch_index=bi,
-                        technique=tech, time_ms=a.time, init_ms=a.initializationTime, mark_ms=a.markingTime,
-                        comp_ms=a.computationTime, iterations=a.iterations, l1_error=l1_error(a.ranks, s0),
-                        extra=f"deletions={b.deletions};insertions={b.insertions}")
-            for m, _ in METHODS:
-                prev[m] = res[m].ranks
-            summary.append((frac, bi, {m: res[m].time for m in res}))
-            yg.close()
-            x = y
+    xg = ctx.upload(x, with_out=False)
+    r0 = solve(xg, engine.STATIC, exact).ranks
+    xg.close()
+    prev = {m: r0 for m, _ in METHODS}
+    for bi in range(args.batches):
+        t0 = time.time()
+        b = next_batch(x, bi)
+        if b is None:
+            log(f"input ended before batch {bi}")
+            break
+        y = apply_batch(x, b)
+        yg = ctx.upload(y)
+        log(f"batch {bi} of {frac:g}|E|: -{b.deletions}/+{b.insertions} edges, y has {y.M} edges "
+            f"({time.time() - t0:.1f} s to build and upload)")
+        s0 = solve(yg, engine.STATIC, exact).ranks
+        res = {engine.STATIC: solve(yg, engine.STATIC, o)}
+        for m, _ in METHODS:
+            res[m] = solve(yg, m, o, batch=b, q=prev[m])
+        for m, tech in [(engine.STATIC, "pagerankStaticHip")] + METHODS:
+            a = res[m]
+            print(glog(a, s0, tech, 1, 0.0, frac, bi), flush=True)
+            out.row(mode=args.mode, graph=name, vertices=y.N, edges=y.M, batch_fraction=frac, batch_index=bi,
+                    technique=tech, time_ms=a.time, init_ms=a.initializationTime, mark_ms=a.markingTime,
+                    comp_ms=a.computationTime, iterations=a.iterations, l1_error=l1_error(a.ranks, s0),
+                    extra=f"deletions={b.deletions};insertions={b.insertions}")
+        for m, _ in METHODS:
+            prev[m] = res[m].ranks
+        summary.append((frac, bi, {m: res[m].time for m in res}))
+        yg.close()
+        x = y
+
+
+def print_summary(summary: list) -> None:
     log("speedup over GPU Static (time of Static / time of the method), mean over batches:")
     for frac in sorted({s[0] for s in summary}):
         rows = [s[2] for s in summary if s[0] == frac]
@@ -144,6 +153,49 @@ def run_dynamic(args, out: CsvLog) -> int:
             t = np.mean([r[m] for r in rows])
             parts.append(f"{tech.replace('pagerank', '').replace('Hip', '')} {t:.3f} ms ({st / t:.2f}x)")
         log(f"batch {frac:g}|E|: " + ", ".join(parts))
+
+
+def run_dynamic(args, out: CsvLog) -> int:
+    x0, name = make_rmat(args.scale, args.edge_factor, args.seed)
+    ctx = Context(0)
+    autotune(ctx, x0, args)
+    summary = []
+    for frac in [float(f) for f in args.fractions.split(",")]:
+        def random_batch(x, bi, frac=frac):
+            return Batch.random(x, max(1, int(frac * x.M)), 0.8, seed=args.seed * 1000 + bi + int(frac * 1e9))
+        run_batches(ctx, x0, name, frac, random_batch, args, out, summary)
+    print_summary(summary)
+    return 0
+
+
+def run_temporal(args, out: CsvLog) -> int:
+    """main.cu:157-174 on a SNAP temporal file: the first 90% of --size lines (vertices 1..--rows) form the base
+    graph, a self-loop is added to every vertex, and each batch is the next fraction*size lines, as insertions
+    tidied against the current graph (main.cu:84-87)."""
+    if not args.file or not args.rows or not args.size:
+        raise SystemExit("temporal mode needs --file, --rows and --size")
+    frac = float(args.fractions.split(",")[0])
+    t0 = time.time()
+    rd = TemporalReader(args.file, args.rows)
+    su, sv = rd.read(int(0.90 * args.size))
+    x0 = HostGraph.from_edges(args.rows, su, sv, self_loops=True)
+    name = args.file.rsplit("/", 1)[-1]
+    log(f"graph {name}: {x0.N} vertices, {x0.M} edges (90% of {args.size} lines, self-loops), "
+        f"read in {time.time() - t0:.1f} s")
+    ctx = Context(0)
+    autotune(ctx, x0, args)
+    lines = max(1, int(frac * args.size))
+
+    def next_batch(x, bi):
+        iu, iv = rd.read(lines)
+        if len(iu) == 0:
+            return None
+        return Batch.tidy(x, np.zeros((0, 2), np.uint32), np.stack([iu, iv], axis=1))
+
+    summary = []
+    run_batches(ctx, x0, name, frac, next_batch, args, out, summary)
+    rd.close()
+    print_summary(summary)
     return 0
 
 
@@ -269,7 +321,10 @@ def run_adjoint(args, out: CsvLog) -> int:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("mode", choices=["dynamic", "stream", "ppr", "adjoint"])
+    ap.add_argument("mode", choices=["dynamic", "temporal", "stream", "ppr", "adjoint"])
+    ap.add_argument("--file", default=None, help="temporal mode: SNAP temporal edge file (1-based ids)")
+    ap.add_argument("--rows", type=int, default=0, help="temporal mode: number of vertices")
+    ap.add_argument("--size", type=int, default=0, help="temporal mode: number of lines (temporal edges)")
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--edge-factor", type=float, default=16.0)
     ap.add_argument("--seed", type=int, default=42)
@@ -282,7 +337,7 @@ def main(argv=None) -> int:
     ap.add_argument("--tuning-cache", default=None, help="dynamic mode: the autotuner's JSON cache")
     args = ap.parse_args(argv)
     out = CsvLog(args.csv)
-    return {"dynamic": run_dynamic, "stream": run_stream, "ppr": run_ppr, "adjoint": run_adjoint}[args.mode](args, out)
+    return {"dynamic": run_dynamic, "temporal": run_temporal, "stream": run_stream, "ppr": run_ppr, "adjoint": run_adjoint}[args.mode](args, out)
 
 
 if __name__ == "__main__":

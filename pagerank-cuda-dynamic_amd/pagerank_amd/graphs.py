@@ -81,6 +81,13 @@ class HostGraph:
         return cls(h)
 
     @classmethod
+    def from_mtx(cls, path: str, self_loops: bool = False) -> "HostGraph":
+        """A Matrix Market file as a graph (readMtxW, mtx.hxx), duplicates removed; with self_loops, one self-loop
+        per vertex (addSelfLoopsOmp, as main.cu:71 does before solving)."""
+        src, dst, order = read_mtx_edges(path)
+        return cls.from_edges(order, src, dst, self_loops=self_loops)
+
+    @classmethod
     def from_csr(cls, out_off, out_edg) -> "HostGraph":
         """From an out-CSR given as numpy arrays (rows ascending, no self-loops added). Builds the in-CSR."""
         out_off = np.ascontiguousarray(out_off, dtype=np.uint32)
@@ -123,6 +130,54 @@ class HostGraph:
         src = self.out_sources()
         A[self.out_edg, src] = 1.0 / self.outdeg[src]
         return A
+
+
+def _take_edges(eh) -> tuple[np.ndarray, np.ndarray, int]:
+    h = _native.host()
+    n = int(h.prhh_edges_count(eh))
+    src = _view(C.cast(h.prhh_edges_src(eh), _native.u32p), n).copy()
+    dst = _view(C.cast(h.prhh_edges_dst(eh), _native.u32p), n).copy()
+    order = int(h.prhh_edges_order(eh))
+    h.prhh_edges_free(eh)
+    return src, dst, order
+
+
+def read_mtx_edges(path: str) -> tuple[np.ndarray, np.ndarray, int]:
+    """Matrix Market coordinate file -> (src, dst, order) with 0-based ids (file vertex k -> k - 1), mirrors added
+    for symmetric files (readMtxDo, mtx.hxx:120-140)."""
+    eh = _native.host().prhh_read_mtx(str(path).encode())
+    if not eh:
+        raise ValueError(f"cannot read {path} as a Matrix Market coordinate graph")
+    return _take_edges(eh)
+
+
+class TemporalReader:
+    """SNAP temporal edge file read in consecutive pieces (readTemporalDo, snap.hxx:40-60): read(n) returns the
+    next n lines as 0-based (src, dst), with mirrors when symmetric."""
+
+    def __init__(self, path: str, rows: int, symmetric: bool = False):
+        self._h = _native.host().prhh_temporal_open(str(path).encode())
+        if not self._h:
+            raise ValueError(f"cannot open {path}")
+        self.rows, self.symmetric = rows, symmetric
+
+    def read(self, lines: int) -> tuple[np.ndarray, np.ndarray]:
+        eh = _native.host().prhh_temporal_read(self._h, self.rows, lines, 1 if self.symmetric else 0)
+        if not eh:
+            raise ValueError("vertex id outside 1..rows in the temporal file")
+        src, dst, _ = _take_edges(eh)
+        return src, dst
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _native.host().prhh_temporal_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass

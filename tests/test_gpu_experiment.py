@@ -1,0 +1,57 @@
+"""The experiment runner end to end on the GPU: temporal mode on a small SNAP temporal file (main.cu:157-174's
+protocol) and dynamic mode on a small RMAT graph. Checks the graph sizes each batch reaches (recomputed with numpy
+from the file) and the reference experiment's L1 error metric (main.cu:55) for every method."""
+from __future__ import annotations
+
+import csv
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EXACT = {"pagerankStaticHip", "pagerankNaiveDynamicHip", "pagerankDynamicTraversalHip",
+         "pagerankDynamicFrontierHip"}
+
+
+def _rows(path):
+    with open(path) as f:
+        return list(csv.DictReader(f))
+
+
+def test_temporal_mode(tmp_path, capsys):
+    from pagerank_amd import experiment
+    rng = np.random.default_rng(5)
+    rows, size = 600, 20000
+    u = (rng.zipf(1.6, size) % rows) + 1
+    v = rng.integers(1, rows + 1, size)
+    p = tmp_path / "t.txt"
+    p.write_text("".join(f"{a} {b} {1000 + i}\n" for i, (a, b) in enumerate(zip(u.tolist(), v.tolist()))))
+    out = tmp_path / "r.csv"
+    assert experiment.main(["temporal", "--file", str(p), "--rows", str(rows), "--size", str(size),
+                            "--fractions", "1e-3", "--batches", "3", "--repeat", "1", "--no-autotune",
+                            "--csv", str(out)]) == 0
+    got = _rows(out)
+    assert len(got) == 3 * 5
+    base = int(0.9 * size)
+    for bi in range(3):
+        k = base + (bi + 1) * 20
+        keys = np.unique((u[:k] - 1).astype(np.int64) * rows + (v[:k] - 1))
+        keys = np.union1d(keys, np.arange(rows, dtype=np.int64) * (rows + 1))
+        for r in (r for r in got if int(r["batch_index"]) == bi):
+            assert int(r["vertices"]) == rows and int(r["edges"]) == len(keys)
+            bound = 1e-8 if r["technique"] in EXACT else 1e-5
+            assert float(r["l1_error"]) < bound, r
+    assert "pagerankPruneDynamicFrontierHip" in capsys.readouterr().out
+
+
+def test_dynamic_mode(tmp_path):
+    from pagerank_amd import experiment
+    out = tmp_path / "r.csv"
+    assert experiment.main(["dynamic", "--scale", "12", "--fractions", "1e-4,1e-3", "--batches", "2",
+                            "--repeat", "1", "--no-autotune", "--csv", str(out)]) == 0
+    got = _rows(out)
+    assert len(got) == 2 * 2 * 5
+    for r in got:
+        bound = 1e-8 if r["technique"] in EXACT else 1e-5
+        assert float(r["l1_error"]) < bound, r
