@@ -1,6 +1,9 @@
 """The experiment runner end to end on the GPU: temporal mode on a small SNAP temporal file (main.cu:157-174's
 protocol) and dynamic mode on a small RMAT graph. Checks the graph sizes each batch reaches (recomputed with numpy
-from the file) and the reference experiment's L1 error metric (main.cu:55) for every method."""
+from the file) and the reference experiment's L1 error metric (main.cu:55) for every method: at most 1e-9 per vertex
+for the methods that iterate to tau = 1e-10 in L-inf (with no dead ends the iteration contracts by a in L1, so the
+last step's L1 change, at most N * tau, bounds the L1 distance to the fixed point by N * tau * a / (1 - a), about
+5.7e-10 per vertex), and 1e-8 per vertex for DF-P, whose pruned vertices stop early."""
 from __future__ import annotations
 
 import csv
@@ -12,6 +15,10 @@ pytestmark = pytest.mark.gpu
 
 EXACT = {"pagerankStaticHip", "pagerankNaiveDynamicHip", "pagerankDynamicTraversalHip",
          "pagerankDynamicFrontierHip"}
+
+
+def _bound(r) -> float:
+    return int(r["vertices"]) * (1e-9 if r["technique"] in EXACT else 1e-8)
 
 
 def _rows(path):
@@ -40,8 +47,7 @@ def test_temporal_mode(tmp_path, capsys):
         keys = np.union1d(keys, np.arange(rows, dtype=np.int64) * (rows + 1))
         for r in (r for r in got if int(r["batch_index"]) == bi):
             assert int(r["vertices"]) == rows and int(r["edges"]) == len(keys)
-            bound = 1e-8 if r["technique"] in EXACT else 1e-5
-            assert float(r["l1_error"]) < bound, r
+            assert float(r["l1_error"]) < _bound(r), r
     assert "pagerankPruneDynamicFrontierHip" in capsys.readouterr().out
 
 
@@ -53,5 +59,4 @@ def test_dynamic_mode(tmp_path):
     got = _rows(out)
     assert len(got) == 2 * 2 * 5
     for r in got:
-        bound = 1e-8 if r["technique"] in EXACT else 1e-5
-        assert float(r["l1_error"]) < bound, r
+        assert float(r["l1_error"]) < _bound(r), r
