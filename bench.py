@@ -120,6 +120,7 @@ def main() -> int:
     ap.add_argument("--tuning-cache", default=None, help="JSON cache file for the autotuner's choice")
     ap.add_argument("--source-slices", type=int, default=None, help="override the tuning's source_slices")
     ap.add_argument("--huge-degree", type=int, default=None, help="override the tuning's huge_degree")
+    ap.add_argument("--blocking", type=int, default=None, help="override the tuning's blocking (propagation blocking)")
     ap.add_argument("--copy-calibration", type=float, metavar="GIB", default=0.0,
                     help="only run a plain device copy of GIB GiB (for scaling the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -138,16 +139,19 @@ def main() -> int:
     g, desc = make_graph(args)
     log(f"graph N={g.N} M={g.M} built in {time.time() - t0:.1f}s")
     ctx = Context(0)
-    if args.source_slices is not None or args.huge_degree is not None:
+    overridden = any(v is not None for v in (args.source_slices, args.huge_degree, args.blocking))
+    if overridden:
         t = ctx.get_tuning()
         if args.source_slices is not None:
             t.source_slices = args.source_slices
         if args.huge_degree is not None:
             t.huge_degree = args.huge_degree
+        if args.blocking is not None:
+            t.blocking = args.blocking
         ctx.set_tuning(t)
     yg = ctx.upload(g, with_out=False)
     autotuned = False
-    if not args.no_autotune and args.source_slices is None and args.huge_degree is None:
+    if not args.no_autotune and not overridden:
         t0 = time.time()
         t, ms, hit = ctx.autotune(yg, args.tuning_cache, iterations=10)
         autotuned = True

@@ -1,0 +1,409 @@
+// binned.hip -- the propagation-blocked plain update (Static / Naive-dynamic) and the build of its layout.
+//
+// The pull kernel gathers cr[u] = r[u]/deg[u] at random; every gather from a source outside the L2-resident hot
+// set costs a 128-byte line fill for 8 useful bytes. Propagation blocking (Beamer, Asanovic and Patterson,
+// "Reducing PageRank Communication via Propagation Blocking", IPDPS 2017) moves that randomness into LDS. Vertices
+// are cut into runs of B consecutive layout rows: source blocks on one side, destination bins on the other. Each
+// iteration makes two streaming passes through X, one FP64 slot per edge:
+//   expand      one workgroup per item of a source block: the block's B contributions are loaded into LDS, and
+//               every edge leaving the block copies its source's contribution into its slot of X;
+//   accumulate  one workgroup per item of a destination bin: the bin's slots are read in order and added into B
+//               LDS sums (ds_add_f64, one addition per wave when its 64 slots share a destination); a bin covered
+//               by one item is finished right there, a split bin leaves one row of B partial sums per item;
+//   finish      split bins only: a vertex's partials are added in item order, then the update.
+// Expand stores pairs of entries (16 bytes a lane); accumulate takes 64-slot groups, most of which lie inside one
+// run and need only scalar loads of the run table.
+// Slot order is bin-major: bin b, then source block s, then the in-CSR order (destination, then source) inside the
+// (s, b) tile; accumulate items are ranges of it. Expand runs block-major (block s, then bin b, same order inside
+// a tile) and writes X in that order, sequentially, with a 16-bit local source id per entry. Each nonempty tile is
+// one run of consecutive slots and of consecutive X entries; accumulate maps slot j to its X entry through the run
+// table (run_j = first slot, run_x = first X entry; grp_run = the run holding slot 64 q), and reads X and the 16-bit
+// local destination ids (stored in X's order) run by run.
+// Per edge and iteration: expand reads 2 bytes and writes 8, accumulate reads 8 + 2 (plus a few bytes per run).
+// LDS additions are made in arrival order, so the sums repeat only to rounding, not bit for bit; the update itself
+// is the plain one (pagerankUpdateRanksThreadCukU, pagerankCuda.hxx:87-109): a[v] = C0 + P * sum, ca[v] = a[v] /
+// deg[v].
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include "prh_device.hpp"
+#include "prh_internal.hpp"
+
+namespace prh {
+namespace {
+
+constexpr int kBinWaves = kBinBlock / kWave;
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t ld_nt(const uint32_t *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ uint32_t ld_nt(const uint16_t *p) { return __builtin_nontemporal_load(p); }
+
+// Plain update of vertex v from its in-sum; returns the L-inf term.
+__device__ __forceinline__ double finish_plain(const RankArgs &A, uint32_t v, double cv) {
+  const double av = fma(A.P, cv, A.C0);
+  A.a[v] = av;
+  A.ca[v] = av / A.deg[v];
+  return fabs(av - A.r[v]);
+}
+
+// Workgroup-wide max over kBinBlock threads, valid in thread 0. Every thread must call it.
+__device__ __forceinline__ double bin_block_max(double x, double *lds) {
+  x = group_max<kWave>(x);
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[w] = x;
+  __syncthreads();
+  double m = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBinWaves; ++i) m = fmax(m, lds[i]);
+  return m;
+}
+
+template <int B>
+__global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const double *cr, LoopCtl ctl) {
+  if (!iteration_begin(ctl)) return;
+  __shared__ double c[B];
+  const uint4 it = P.ex_item[blockIdx.x];  // block, first entry, end entry
+  const uint32_t base = it.x * uint32_t(B);
+  const uint32_t n = min(uint32_t(B), P.N - base);
+  for (uint32_t i = threadIdx.x; i < n; i += kBinBlock) c[i] = cr[base + i];
+  __syncthreads();
+  // Pairs of entries: thread t takes pairs p = t + m * kBinBlock of the even-aligned range (16-byte stores); a lone
+  // first or last entry is written on its own.
+  uint32_t k0 = it.y, k1 = it.z;
+  if (k0 & 1u) {
+    if (threadIdx.x == 0) P.X[k0] = c[ld_nt(P.src16 + k0)];
+    ++k0;
+  }
+  if (k1 > k0 && (k1 & 1u)) {
+    if (threadIdx.x == 0) P.X[k1 - 1] = c[ld_nt(P.src16 + k1 - 1)];
+    --k1;
+  }
+  const uint32_t *s32 = reinterpret_cast<const uint32_t *>(P.src16);
+  d2 *x2 = reinterpret_cast<d2 *>(P.X);
+  const uint32_t p0 = k0 / 2, p1 = k1 / 2;
+  uint32_t p = p0 + threadIdx.x;
+  for (; p + 3 * kBinBlock < p1; p += 4 * kBinBlock) {
+    const uint32_t a0 = __builtin_nontemporal_load(s32 + p), a1 = __builtin_nontemporal_load(s32 + p + kBinBlock);
+    const uint32_t a2 = __builtin_nontemporal_load(s32 + p + 2 * kBinBlock);
+    const uint32_t a3 = __builtin_nontemporal_load(s32 + p + 3 * kBinBlock);
+    x2[p] = d2{c[a0 & 0xFFFFu], c[a0 >> 16]};
+    x2[p + kBinBlock] = d2{c[a1 & 0xFFFFu], c[a1 >> 16]};
+    x2[p + 2 * kBinBlock] = d2{c[a2 & 0xFFFFu], c[a2 >> 16]};
+    x2[p + 3 * kBinBlock] = d2{c[a3 & 0xFFFFu], c[a3 >> 16]};
+  }
+  for (; p < p1; p += kBinBlock) {
+    const uint32_t a = __builtin_nontemporal_load(s32 + p);
+    x2[p] = d2{c[a & 0xFFFFu], c[a >> 16]};
+  }
+}
+
+__device__ __forceinline__ uint32_t or_dpp_step(uint32_t x, int ctrl_sel) {
+  switch (ctrl_sel) {
+    case 0: return x | uint32_t(__builtin_amdgcn_update_dpp(0, int(x), kDppQuadSwap1, 0xF, 0xF, false));
+    case 1: return x | uint32_t(__builtin_amdgcn_update_dpp(0, int(x), kDppQuadSwap2, 0xF, 0xF, false));
+    case 2: return x | uint32_t(__builtin_amdgcn_update_dpp(0, int(x), kDppRowHalfMirror, 0xF, 0xF, false));
+    case 3: return x | uint32_t(__builtin_amdgcn_update_dpp(0, int(x), kDppRowMirror, 0xF, 0xF, false));
+    default: return x | uint32_t(__builtin_amdgcn_ds_swizzle(int(x), 0x401F));
+  }
+}
+
+// OR of a 32-bit value over the wave (the pairing of group_sum), wave-uniform result. All lanes must be active.
+__device__ __forceinline__ uint32_t wave_or(uint32_t x) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) x = or_dpp_step(x, k);
+  return uint32_t(__builtin_amdgcn_readlane(int(x), 0)) | uint32_t(__builtin_amdgcn_readlane(int(x), 32));
+}
+
+__device__ __forceinline__ uint32_t lane_pull(uint32_t x, uint32_t src_lane) {
+  return uint32_t(__builtin_amdgcn_ds_bpermute(int(src_lane << 2), int(x)));
+}
+
+// X entry of slot g + lane, for a wave-uniform g that is a multiple of 64. r0 = the run holding slot g (scalar
+// loads: g is made uniform). If the next run starts at or after g + 64, the group lies in r0. Otherwise lane L
+// loads run r0 + L; the runs that start inside (g, g + 63] mark their offsets in a 64-bit mask, and slot g + i lies
+// in run r0 + (number of marks at offsets <= i). All lanes must be active; slots past the last run map to garbage
+// that the caller masks.
+__device__ __forceinline__ uint32_t slot_entry(const BinnedPlan &P, uint32_t g, uint32_t lane) {
+  g = __builtin_amdgcn_readfirstlane(g);
+  const uint32_t r0 = __builtin_amdgcn_readfirstlane(P.grp_run[g >> 6]);
+  const uint32_t nxt = r0 + 1 < P.n_runs ? __builtin_amdgcn_readfirstlane(P.run_j[r0 + 1]) : 0xFFFFFFFFu;
+  if (nxt - g >= 64u) {  // the whole group lies in run r0 (nxt > g always)
+    const uint32_t x0 = __builtin_amdgcn_readfirstlane(P.run_x[r0]);
+    const uint32_t j0 = __builtin_amdgcn_readfirstlane(P.run_j[r0]);
+    return x0 + (g + lane - j0);
+  }
+  const uint32_t rl = r0 + lane;
+  const bool valid = rl < P.n_runs;
+  const uint32_t rj = valid ? P.run_j[rl] : 0xFFFFFFFFu;
+  const uint32_t rx = valid ? P.run_x[rl] : 0u;
+  const uint32_t o = rj - g;  // >= 1 for lanes >= 1 (runs after r0 start after g)
+  const bool mark = lane >= 1 && valid && o < 64;
+  const uint32_t lo = wave_or(mark && o < 32 ? 1u << o : 0u);
+  const uint32_t hi = wave_or(mark && o >= 32 ? 1u << (o - 32) : 0u);
+  const unsigned long long m = (static_cast<unsigned long long>(hi) << 32) | lo;
+  const uint32_t c = uint32_t(__popcll(m & ((2ull << lane) - 1)));  // lane 63: 2 << 63 wraps to 0, mask all
+  const uint32_t sj = lane_pull(rj, c), sx = lane_pull(rx, c);
+  return sx + (g + lane - sj);
+}
+
+// Adds one wave's 64 slots into the LDS sums. All lanes must be active; lanes past the item carry ok = false.
+__device__ __forceinline__ void add_slots(double *sum, uint32_t d, double x, bool ok) {
+  const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+  if (__ballot(d == d0 || !ok) == ~0ull) {  // one destination for the whole wave: reduce first
+    const double t = group_sum<kWave>(ok ? x : 0.0);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(&sum[d0], t);
+  } else if (ok) {
+    atomicAdd(&sum[d], x);
+  }
+}
+
+template <int B>
+__global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, BinnedPlan P, LoopCtl ctl) {
+  if (!iteration_begin(ctl)) return;
+  __shared__ double sum[B];
+  __shared__ double lds_max[kBinWaves];
+  const uint4 it = P.ac_item[blockIdx.x];  // bin, first slot, end slot, partial row (kWholeBin: none)
+  for (uint32_t i = threadIdx.x; i < uint32_t(B); i += kBinBlock) sum[i] = 0.0;
+  __syncthreads();
+  const int w = threadIdx.x / kWave;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  // Groups of 64 slots aligned to multiples of 64, dealt to the waves in turn; slots outside the item are masked.
+  uint32_t g = __builtin_amdgcn_readfirstlane((it.y & ~63u) + uint32_t(w) * kWave);  // wave-uniform
+  for (; g + 3 * kBinBlock < it.z; g += 4 * kBinBlock) {
+    const uint32_t k0 = slot_entry(P, g, lane), k1 = slot_entry(P, g + kBinBlock, lane);
+    const uint32_t k2 = slot_entry(P, g + 2 * kBinBlock, lane), k3 = slot_entry(P, g + 3 * kBinBlock, lane);
+    const bool ok0 = g + lane >= it.y;  // groups 1..3 start past it.y; all four end before it.z except possibly 3
+    const bool ok3 = g + 3 * kBinBlock + lane < it.z;
+    const uint32_t d0 = ok0 ? ld_nt(P.dst16 + k0) : 0u, d1 = ld_nt(P.dst16 + k1), d2 = ld_nt(P.dst16 + k2);
+    const uint32_t d3 = ok3 ? ld_nt(P.dst16 + k3) : 0u;
+    const double x0 = ok0 ? __builtin_nontemporal_load(P.X + k0) : 0.0;
+    const double x1 = __builtin_nontemporal_load(P.X + k1), x2 = __builtin_nontemporal_load(P.X + k2);
+    const double x3 = ok3 ? __builtin_nontemporal_load(P.X + k3) : 0.0;
+    add_slots(sum, d0, x0, ok0);
+    add_slots(sum, d1, x1, true);
+    add_slots(sum, d2, x2, true);
+    add_slots(sum, d3, x3, ok3);
+  }
+  for (; g < it.z; g += kBinBlock) {
+    const uint32_t k = slot_entry(P, g, lane);
+    const bool ok = g + lane >= it.y && g + lane < it.z;
+    add_slots(sum, ok ? ld_nt(P.dst16 + k) : 0u, ok ? __builtin_nontemporal_load(P.X + k) : 0.0, ok);
+  }
+  __syncthreads();
+  const uint32_t b0 = it.x * uint32_t(B);
+  const uint32_t nb = min(uint32_t(B), P.N - b0);
+  if (it.w == kWholeBin) {
+    double emax = 0.0;
+    for (uint32_t i = threadIdx.x; i < nb; i += kBinBlock) emax = fmax(emax, finish_plain(A, b0 + i, sum[i]));
+    emax = bin_block_max(emax, lds_max);
+    if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
+  } else {
+    double *p = P.partial + size_t(it.w) * B;
+    for (uint32_t i = threadIdx.x; i < nb; i += kBinBlock) p[i] = sum[i];
+  }
+}
+
+// Split bins, one workgroup per unit (bin, first partial row, rows, first vertex | mode << 31):
+//   mode 0: vertices first .. first + 1023 of the bin, a thread per vertex adding all its partial rows;
+//   mode 1: vertices first .. first + 63, wave w adding rows w, w + 16, ... of its lane's vertex, then the 16 wave
+//           sums in wave order (bins with many rows).
+__global__ __launch_bounds__(kBinBlock) void finish_bins_kernel(RankArgs A, BinnedPlan P, LoopCtl ctl) {
+  if (!iteration_begin(ctl)) return;
+  __shared__ double part[kBinWaves][kWave];
+  __shared__ double lds_max[kBinWaves];
+  const uint4 m = P.multi[blockIdx.x];
+  const bool split = (m.w >> 31) != 0;
+  const uint32_t first = m.w & 0x7FFFFFFFu;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const uint32_t i = first + (split ? uint32_t(lane) : threadIdx.x);  // vertex offset in the bin
+  const uint32_t v = m.x * uint32_t(P.B) + i;
+  const bool ok = i < uint32_t(P.B) && v < P.N;
+  const double *p = P.partial + size_t(m.y) * P.B + i;
+  double emax = 0.0;
+  if (!split) {
+    if (ok) {
+      double s0 = 0.0, s1 = 0.0;
+      uint32_t k = 0;
+      for (; k + 1 < m.z; k += 2) {
+        s0 += p[size_t(k) * P.B];
+        s1 += p[size_t(k + 1) * P.B];
+      }
+      if (k < m.z) s0 += p[size_t(k) * P.B];
+      emax = finish_plain(A, v, s0 + s1);
+    }
+  } else {
+    double s0 = 0.0, s1 = 0.0;
+    if (ok) {
+      uint32_t k = w;
+      for (; k + kBinWaves < m.z; k += 2 * kBinWaves) {
+        s0 += p[size_t(k) * P.B];
+        s1 += p[size_t(k + kBinWaves) * P.B];
+      }
+      if (k < m.z) s0 += p[size_t(k) * P.B];
+    }
+    part[w][lane] = s0 + s1;
+    __syncthreads();
+    if (w == 0 && ok) {
+      double t = 0.0;
+      for (int q = 0; q < kBinWaves; ++q) t += part[q][lane];
+      emax = finish_plain(A, v, t);
+    }
+  }
+  emax = bin_block_max(emax, lds_max);
+  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
+}
+
+template <int B>
+hipError_t binned_update(const RankArgs &A, const BinnedPlan &P, const LoopCtl &ctl, hipStream_t s, hipEvent_t mark) {
+  if (P.n_ex) expand_kernel<B><<<P.n_ex, kBinBlock, 0, s>>>(P, A.cr, ctl);
+  accumulate_kernel<B><<<P.n_ac, kBinBlock, 0, s>>>(A, P, ctl);
+  if (P.n_multi) finish_bins_kernel<<<P.n_multi, kBinBlock, 0, s>>>(A, P, ctl);
+  if (mark) {
+    const hipError_t e = hipEventRecord(mark, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
+// ---- build ----
+
+inline uint32_t grid(size_t n, size_t per, uint32_t cap) {
+  const size_t g = (n + per - 1) / per;
+  return uint32_t(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+// key[e] = bin(v) * nblocks + block(src[e]) and row[e] = v for every in-CSR slot e of row v; one wave per row.
+__global__ __launch_bounds__(kBlock) void bin_keys_kernel(const uint32_t *off, const uint32_t *src, uint32_t N,
+                                                          uint32_t B, uint32_t nblocks, uint32_t *key, uint32_t *row) {
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  for (uint32_t v = blockIdx.x * kWavesPerBlock + w; v < N; v += gridDim.x * kWavesPerBlock) {
+    const uint32_t kb = (v / B) * nblocks;
+    for (uint32_t e = off[v] + lane; e < off[v + 1]; e += kWave) {
+      key[e] = kb + src[e] / B;
+      row[e] = v;
+    }
+  }
+}
+
+// Slot j holds in-CSR slot e = perm2[j]: dst[j] = row - bin start, u2[j] = source, key1[j] = source block.
+__global__ __launch_bounds__(kBlock) void bin_slots_kernel(const uint32_t *perm2, const uint32_t *src,
+                                                           const uint32_t *row, size_t M, uint32_t B, uint16_t *dst,
+                                                           uint32_t *u2, uint32_t *key1) {
+  for (size_t j = size_t(blockIdx.x) * kBlock + threadIdx.x; j < M; j += size_t(gridDim.x) * kBlock) {
+    const uint32_t e = perm2[j], u = src[e], v = row[e];
+    dst[j] = uint16_t(v % B);
+    u2[j] = u;
+    key1[j] = u / B;
+  }
+}
+
+// flag[j] = 1 where slot j starts a run (a new tile in the sorted tile keys).
+__global__ __launch_bounds__(kBlock) void run_flags_kernel(const uint32_t *key2s, size_t M, uint32_t *flag) {
+  for (size_t j = size_t(blockIdx.x) * kBlock + threadIdx.x; j < M; j += size_t(gridDim.x) * kBlock)
+    flag[j] = (j == 0 || key2s[j] != key2s[j - 1]) ? 1u : 0u;
+}
+
+// X entry k is slot perm1[k]: src16[k], dst16[k] (its destination), inv[perm1[k]] = k.
+__global__ __launch_bounds__(kBlock) void bin_expand_kernel(const uint32_t *perm1, const uint32_t *u2,
+                                                            const uint16_t *dst, size_t M, uint32_t B,
+                                                            uint16_t *src16, uint16_t *dst16, uint32_t *inv) {
+  for (size_t k = size_t(blockIdx.x) * kBlock + threadIdx.x; k < M; k += size_t(gridDim.x) * kBlock) {
+    const uint32_t j = perm1[k];
+    src16[k] = uint16_t(u2[j] % B);
+    dst16[k] = dst[j];
+    inv[j] = uint32_t(k);
+  }
+}
+
+// Run tables from the run flags and their exclusive scan rid: run rid[j] starts at slot j (flagged) and X entry
+// inv[j]; grp_run[j / 64] = run of slot j for j a multiple of 64.
+__global__ __launch_bounds__(kBlock) void bin_runs_kernel(const uint32_t *flag, const uint32_t *rid,
+                                                          const uint32_t *inv, size_t M, uint32_t *run_j,
+                                                          uint32_t *run_x, uint32_t *grp_run) {
+  for (size_t j = size_t(blockIdx.x) * kBlock + threadIdx.x; j < M; j += size_t(gridDim.x) * kBlock) {
+    const uint32_t f = flag[j], r = rid[j];
+    if (f) {
+      run_j[r] = uint32_t(j);
+      run_x[r] = inv[j];
+    }
+    if ((j & 63) == 0) grp_run[j >> 6] = r + f - 1;
+  }
+}
+
+// start[s] = first k with key[k] >= s, for s = 0 .. n (key sorted ascending).
+__global__ __launch_bounds__(kBlock) void lower_bounds_kernel(const uint32_t *key, size_t M, uint32_t n,
+                                                              uint32_t *start) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s > n) return;
+  size_t lo = 0, hi = M;
+  while (lo < hi) {
+    const size_t mid = lo + (hi - lo) / 2;
+    if (key[mid] < s) lo = mid + 1;
+    else hi = mid;
+  }
+  start[s] = uint32_t(lo);
+}
+
+}  // namespace
+
+hipError_t launch_binned_update(const RankArgs &A, const BinnedPlan &P, unsigned long long *err, int *done, int l,
+                                double tol, hipStream_t s, hipEvent_t mark) {
+  const LoopCtl ctl{err, done, l, tol};
+  switch (P.B) {
+    case 1024: return binned_update<1024>(A, P, ctl, s, mark);
+    case 2048: return binned_update<2048>(A, P, ctl, s, mark);
+    case 4096: return binned_update<4096>(A, P, ctl, s, mark);
+    case 8192: return binned_update<8192>(A, P, ctl, s, mark);
+    case 16384: return binned_update<16384>(A, P, ctl, s, mark);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_bin_keys(const uint32_t *off, const uint32_t *src, uint32_t N, uint32_t B, uint32_t nblocks,
+                           uint32_t *key, uint32_t *row, hipStream_t s) {
+  if (!N) return hipSuccess;
+  bin_keys_kernel<<<grid(N, kWavesPerBlock, 16384), kBlock, 0, s>>>(off, src, N, B, nblocks, key, row);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_slots(const uint32_t *perm2, const uint32_t *src, const uint32_t *row, size_t M, uint32_t B,
+                            uint16_t *dst, uint32_t *u2, uint32_t *key1, hipStream_t s) {
+  if (!M) return hipSuccess;
+  bin_slots_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(perm2, src, row, M, B, dst, u2, key1);
+  return hipGetLastError();
+}
+
+hipError_t launch_run_flags(const uint32_t *key2s, size_t M, uint32_t *flag, hipStream_t s) {
+  if (!M) return hipSuccess;
+  run_flags_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(key2s, M, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_expand(const uint32_t *perm1, const uint32_t *u2, const uint16_t *dst, size_t M, uint32_t B,
+                             uint16_t *src16, uint16_t *dst16, uint32_t *inv, hipStream_t s) {
+  if (!M) return hipSuccess;
+  bin_expand_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(perm1, u2, dst, M, B, src16, dst16, inv);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_runs(const uint32_t *flag, const uint32_t *rid, const uint32_t *inv, size_t M, uint32_t *run_j,
+                           uint32_t *run_x, uint32_t *grp_run, hipStream_t s) {
+  if (!M) return hipSuccess;
+  bin_runs_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(flag, rid, inv, M, run_j, run_x, grp_run);
+  return hipGetLastError();
+}
+
+hipError_t launch_lower_bounds(const uint32_t *key, size_t M, uint32_t n, uint32_t *start, hipStream_t s) {
+  lower_bounds_kernel<<<(n + kBlock) / kBlock, kBlock, 0, s>>>(key, M, n, start);
+  return hipGetLastError();
+}
+
+hipError_t sort_pairs_u32(void *tmp, size_t &tmp_bytes, const uint32_t *key_in, uint32_t *key_out,
+                          const uint32_t *val_in, uint32_t *val_out, size_t n, int end_bit, hipStream_t s) {
+  if (val_in)
+    return rocprim::radix_sort_pairs(tmp, tmp_bytes, key_in, key_out, val_in, val_out, n, 0, unsigned(end_bit), s);
+  return rocprim::radix_sort_pairs(tmp, tmp_bytes, key_in, key_out, rocprim::counting_iterator<uint32_t>(0u), val_out,
+                                   n, 0, unsigned(end_bit), s);
+}
+
+}  // namespace prh

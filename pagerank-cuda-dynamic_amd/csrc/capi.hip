@@ -89,6 +89,11 @@ struct prh_graph {
   bool sliced_valid = false;
   prh::SlicedPlan sliced{};
   std::vector<std::pair<void *, size_t>> sliced_bufs;  // its device allocations
+  // Propagation-blocked plan of the plain update, built on first use for the current block width.
+  int binned_B = 0;
+  bool binned_valid = false;
+  prh::BinnedPlan binned{};
+  std::vector<std::pair<void *, size_t>> binned_bufs;
 };
 
 namespace {
@@ -380,6 +385,173 @@ int build_sliced_plan(prh_context *c, prh_graph *g) {
   return PRH_SUCCESS;
 }
 
+void free_binned(prh_context *c, prh_graph *g) {
+  for (auto &b : g->binned_bufs) dev_free(c, b.first, b.second);
+  g->binned_bufs.clear();
+  g->binned_valid = false;
+}
+
+// Propagation-blocked plan (prh::BinnedPlan) for blocks and bins of B = tuning.blocking layout rows:
+//   1. key2[e] = bin(row) * nblocks + block(source) per in-CSR slot; a stable sort by key2 gives the slot order
+//      (bin, block, in-CSR order), perm2[j] = in-CSR slot of slot j; runs start where the sorted key2 changes;
+//   2. per slot: its destination, its source, and key1 = its block; a stable sort of the slots by key1 gives the
+//      X order (block, bin, in-CSR order), perm1[k] = slot of X entry k;
+//   3. src16 / dst16 in X order, the run tables (first slot, first X entry; the run of every 64th slot);
+//   4. block starts in X order (lower bounds of the sorted key1) and bin starts from the in-degrees cut the two
+//      orders into items; split bins get partial rows.
+int build_binned_plan(prh_context *c, prh_graph *g) {
+  const int B = c->tuning.blocking;
+  if (g->binned_valid && g->binned_B == B) return PRH_SUCCESS;
+  hipStream_t s = c->stream;
+  PRH_HIP(c, hipStreamSynchronize(s));
+  free_binned(c, g);
+  const uint32_t N = g->N;
+  const uint64_t M = g->M;
+  const uint64_t nb = (uint64_t(N) + B - 1) / B;
+  if (nb * nb > (uint64_t(1) << 32))
+    return fail(c, PRH_ERROR_UNSUPPORTED, "blocking %d: %llu blocks per side exceed 32-bit tile keys", B,
+                (unsigned long long)nb);
+  int key2_bits = 0, key1_bits = 0;
+  while ((uint64_t(1) << key2_bits) < nb * nb) ++key2_bits;
+  while ((uint64_t(1) << key1_bits) < nb) ++key1_bits;
+  key2_bits = std::max(key2_bits, 1);
+  key1_bits = std::max(key1_bits, 1);
+  struct Temps {
+    prh_context *c;
+    std::vector<std::pair<void *, size_t>> v;
+    ~Temps() {
+      for (auto &b : v) dev_free(c, b.first, b.second);
+    }
+  } tmp{c, {}};
+  auto temp = [&](auto **p, size_t bytes) -> int {
+    void *q = nullptr;
+    PRH_CHECK(dev_alloc(c, &q, bytes ? bytes : 16));
+    tmp.v.emplace_back(q, bytes ? bytes : 16);
+    *p = static_cast<std::remove_pointer_t<decltype(p)>>(q);
+    return PRH_SUCCESS;
+  };
+  auto keep = [&](auto **p, size_t bytes) -> int {
+    void *q = nullptr;
+    PRH_CHECK(dev_alloc(c, &q, bytes ? bytes : 16));
+    g->binned_bufs.emplace_back(q, bytes ? bytes : 16);
+    *p = static_cast<std::remove_pointer_t<decltype(p)>>(q);
+    return PRH_SUCCESS;
+  };
+  prh::BinnedPlan P{};
+  P.B = B;
+  P.N = N;
+  uint16_t *src16, *dst16;
+  uint32_t *grp_run, *run_j = nullptr, *run_x = nullptr;
+  double *X;
+  PRH_CHECK(keep(&src16, M * 2));
+  PRH_CHECK(keep(&dst16, M * 2));
+  PRH_CHECK(keep(&X, M * 8));
+  PRH_CHECK(keep(&grp_run, (M + 63) / 64 * 4));
+  std::vector<uint32_t> bstart(nb + 1, 0);
+  uint32_t n_runs = 0;
+  if (M) {
+    uint32_t *key, *row, *k2s, *perm2, *u2, *rid, *k1s, *perm1, *starts;
+    uint16_t *dst;
+    for (uint32_t **p : {&key, &row, &k2s, &perm2, &u2, &rid, &k1s, &perm1}) PRH_CHECK(temp(p, M * 4));
+    PRH_CHECK(temp(&dst, M * 2));
+    PRH_CHECK(temp(&starts, (nb + 1) * 4));
+    size_t sort_bytes = 0, sort1_bytes = 0, scan_bytes = 0;
+    PRH_HIP(c, prh::sort_pairs_u32(nullptr, sort_bytes, key, k2s, nullptr, perm2, M, key2_bits, s));
+    PRH_HIP(c, prh::sort_pairs_u32(nullptr, sort1_bytes, key, k1s, nullptr, perm1, M, key1_bits, s));
+    PRH_HIP(c, prh::scan_u32(nullptr, scan_bytes, row, rid, M, s));
+    const size_t work_bytes = std::max({sort_bytes, sort1_bytes, scan_bytes});
+    uint8_t *work;
+    PRH_CHECK(temp(&work, work_bytes));
+    size_t wb = work_bytes;
+    PRH_HIP(c, prh::launch_bin_keys(g->l_in_off, g->l_in_src, N, uint32_t(B), uint32_t(nb), key, row, s));
+    PRH_HIP(c, prh::sort_pairs_u32(work, wb, key, k2s, nullptr, perm2, M, key2_bits, s));
+    PRH_HIP(c, prh::launch_bin_slots(perm2, g->l_in_src, row, M, uint32_t(B), dst, u2, key, s));
+    // row now holds the run flags, rid their exclusive scan; key holds key1.
+    PRH_HIP(c, prh::launch_run_flags(k2s, M, row, s));
+    wb = work_bytes;
+    PRH_HIP(c, prh::scan_u32(work, wb, row, rid, M, s));
+    wb = work_bytes;
+    PRH_HIP(c, prh::sort_pairs_u32(work, wb, key, k1s, nullptr, perm1, M, key1_bits, s));
+    // perm2 is free again: it takes inv (slot -> X entry).
+    PRH_HIP(c, prh::launch_bin_expand(perm1, u2, dst, M, uint32_t(B), src16, dst16, perm2, s));
+    PRH_HIP(c, prh::launch_lower_bounds(k1s, M, uint32_t(nb), starts, s));
+    PRH_HIP(c, hipMemcpyAsync(bstart.data(), starts, (nb + 1) * 4, hipMemcpyDeviceToHost, s));
+    uint32_t last[2];
+    PRH_HIP(c, hipMemcpyAsync(&last[0], rid + (M - 1), 4, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipMemcpyAsync(&last[1], row + (M - 1), 4, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipStreamSynchronize(s));
+    n_runs = last[0] + last[1];
+    PRH_CHECK(keep(&run_j, size_t(n_runs) * 4));
+    PRH_CHECK(keep(&run_x, size_t(n_runs) * 4));
+    PRH_HIP(c, prh::launch_bin_runs(row, rid, perm2, M, run_j, run_x, grp_run, s));
+    PRH_HIP(c, hipStreamSynchronize(s));
+  }
+  // Items: expand items of at most 8 per CU over the whole graph, accumulate items of at most half that many
+  // (fewer split bins, hence fewer partial rows); both at least 16K entries.
+  hipDeviceProp_t prop{};
+  PRH_HIP(c, hipGetDeviceProperties(&prop, c->device));
+  const uint64_t cus = uint64_t(std::max(prop.multiProcessorCount, 1));
+  const uint64_t span_ex = std::max<uint64_t>(M / (cus * 8), 16384);
+  const uint64_t span = std::max<uint64_t>(M / (cus * 4), 16384);
+  std::vector<uint4> ex, ac, multi;
+  for (uint64_t b = 0; b < nb; ++b)
+    for (uint64_t k = bstart[b]; k < bstart[b + 1]; k += span_ex)
+      ex.push_back(make_uint4(uint32_t(b), uint32_t(k), uint32_t(std::min<uint64_t>(bstart[b + 1], k + span_ex)), 0));
+  uint32_t rows = 0;
+  uint64_t j = 0;
+  for (uint64_t b = 0; b < nb; ++b) {
+    const uint64_t r1 = std::min<uint64_t>(N, (b + 1) * uint64_t(B));
+    uint64_t e = 0;
+    for (uint64_t r = b * uint64_t(B); r < r1; ++r) e += g->sorted_deg[r];
+    const uint64_t n = std::max<uint64_t>(1, (e + span - 1) / span);
+    if (n == 1) {
+      ac.push_back(make_uint4(uint32_t(b), uint32_t(j), uint32_t(j + e), prh::kWholeBin));
+    } else {
+      // Finish units: a thread per vertex over 1024 vertices, or (more than 16 rows) 16 waves over 64 vertices.
+      const uint32_t bin_rows = uint32_t(r1 - b * uint64_t(B));
+      const bool wide = n > 16;
+      const uint32_t step = wide ? 64u : uint32_t(prh::kBinBlock);
+      for (uint32_t f = 0; f < bin_rows; f += step)
+        multi.push_back(make_uint4(uint32_t(b), rows, uint32_t(n), f | (wide ? 0x80000000u : 0u)));
+      for (uint64_t q = 0; q < n; ++q) {
+        const uint64_t j0 = j + e * q / n, j1 = j + e * (q + 1) / n;
+        ac.push_back(make_uint4(uint32_t(b), uint32_t(j0), uint32_t(j1), rows++));
+      }
+    }
+    j += e;
+  }
+  if (j != M) return fail(c, PRH_ERROR_DEVICE, "binned layout: %llu slots of %llu", (unsigned long long)j,
+                          (unsigned long long)M);
+  uint4 *tables;
+  PRH_CHECK(keep(&tables, (ex.size() + ac.size() + multi.size()) * sizeof(uint4)));
+  if (!ex.empty()) PRH_HIP(c, hipMemcpy(tables, ex.data(), ex.size() * sizeof(uint4), hipMemcpyHostToDevice));
+  PRH_HIP(c, hipMemcpy(tables + ex.size(), ac.data(), ac.size() * sizeof(uint4), hipMemcpyHostToDevice));
+  if (!multi.empty())
+    PRH_HIP(c, hipMemcpy(tables + ex.size() + ac.size(), multi.data(), multi.size() * sizeof(uint4),
+                         hipMemcpyHostToDevice));
+  double *partial;
+  PRH_CHECK(keep(&partial, size_t(rows) * B * 8));
+  P.src16 = src16;
+  P.dst16 = dst16;
+  P.X = X;
+  P.run_j = run_j;
+  P.run_x = run_x;
+  P.grp_run = grp_run;
+  P.n_runs = n_runs;
+  P.ex_item = tables;
+  P.ac_item = tables + ex.size();
+  P.multi = tables + ex.size() + ac.size();
+  P.partial = partial;
+  P.n_ex = uint32_t(ex.size());
+  P.n_ac = uint32_t(ac.size());
+  P.n_multi = uint32_t(multi.size());
+  PRH_HIP(c, hipStreamSynchronize(s));
+  g->binned = P;
+  g->binned_B = B;
+  g->binned_valid = true;
+  return PRH_SUCCESS;
+}
+
 // Rank-kernel plan over the degree-ordered layout for the context's tuning (cached in the graph):
 //   rows with in-degree > huge_degree: chunks of huge_degree edges, one workgroup each;
 //   switch_degree < in-degree <= huge_degree: one wave per row;
@@ -614,8 +786,10 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   PRH_HIP(c, hipSetDevice(c->device));
   PRH_CHECK(ensure_workspace(c, N, dynamic, L));
   prh_graph *gm = const_cast<prh_graph *>(g);
-  const bool sliced = variant == prh::kPlain && c->tuning.source_slices > 0;
-  if (sliced) PRH_CHECK(build_sliced_plan(c, gm));
+  const bool binned = variant == prh::kPlain && c->tuning.blocking > 0;
+  const bool sliced = !binned && variant == prh::kPlain && c->tuning.source_slices > 0;
+  if (binned) PRH_CHECK(build_binned_plan(c, gm));
+  else if (sliced) PRH_CHECK(build_sliced_plan(c, gm));
   else PRH_CHECK(build_rank_plan(c, gm));
   DevBatch b;
   if (dynamic) PRH_CHECK(stage_batch(c, g, du, dv, nd, iu, ni, host, b));
@@ -673,7 +847,8 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
                       o->prune_tolerance, dir_auto ? c->ext_stats.p : nullptr};
       if (c->ktiming) PRH_HIP(c, hipEventRecord(c->kev[2 * l], s));
       hipEvent_t mark = c->ktiming ? c->kev[2 * l + 1] : nullptr;
-      if (sliced) PRH_HIP(c, prh::launch_sliced_update(A, g->sliced, err, done, l, o->tolerance, s, mark));
+      if (binned) PRH_HIP(c, prh::launch_binned_update(A, g->binned, err, done, l, o->tolerance, s, mark));
+      else if (sliced) PRH_HIP(c, prh::launch_sliced_update(A, g->sliced, err, done, l, o->tolerance, s, mark));
       else PRH_HIP(c, prh::launch_rank_update(variant, A, plan, err, done, l, o->tolerance, s, mark));
       if (frontier) {
         if (dir_auto) PRH_HIP(c, prh::launch_direction(c->ext_stats.p, stat_slots, dir, err, done, l, o->tolerance, s));
@@ -1223,6 +1398,7 @@ void prh_default_tuning(prh_tuning *t) {
   t->source_slices = 0;
   t->block_edges = 2048;
   t->extension_direction = 1;
+  t->blocking = 0;
 }
 
 int prh_context_create(int device, prh_context **out) {
@@ -1317,7 +1493,9 @@ int prh_context_set_tuning(prh_context *c, const prh_tuning *t) {
   const int sl = t->source_slices;
   if (!(w == 0 || w == 1 || w == 2 || w == 4 || w == 8 || w == 16 || w == 32) || t->check_interval < 1 ||
       t->huge_degree < t->switch_degree || !(sl == 0 || sl == 1 || sl == 2 || sl == 4 || sl == 8) ||
-      t->block_edges < 256 || t->extension_direction < 0 || t->extension_direction > 2)
+      t->block_edges < 256 || t->extension_direction < 0 || t->extension_direction > 2 ||
+      !(t->blocking == 0 || t->blocking == 1024 || t->blocking == 2048 || t->blocking == 4096 ||
+        t->blocking == 8192 || t->blocking == 16384))
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad tuning");
   c->tuning = *t;
   return PRH_SUCCESS;
@@ -1406,6 +1584,7 @@ void prh_graph_destroy(prh_graph *g) {
   dev_free(c, g->chunk_tables, g->chunk_tables_bytes);
   dev_free(c, g->chunk_partial, g->chunk_partial_bytes);
   free_sliced(c, g);
+  free_binned(c, g);
   delete g;
 }
 
@@ -1453,10 +1632,11 @@ std::string tuning_key(prh_context *c, const prh_graph *g) {
   return buf;
 }
 
-const char *const kTuningFields[] = {"switch_degree", "huge_degree", "low_width", "check_interval",
-                                     "source_slices", "block_edges", "extension_direction"};
+constexpr int kTuningCount = 8;
+const char *const kTuningFields[kTuningCount] = {"switch_degree", "huge_degree", "low_width", "check_interval",
+                                                 "source_slices", "block_edges", "extension_direction", "blocking"};
 
-void tuning_to_values(const prh_tuning &t, long long v[7]) {
+void tuning_to_values(const prh_tuning &t, long long v[kTuningCount]) {
   v[0] = t.switch_degree;
   v[1] = t.huge_degree;
   v[2] = t.low_width;
@@ -1464,9 +1644,10 @@ void tuning_to_values(const prh_tuning &t, long long v[7]) {
   v[4] = t.source_slices;
   v[5] = t.block_edges;
   v[6] = t.extension_direction;
+  v[7] = t.blocking;
 }
 
-void values_to_tuning(const long long v[7], prh_tuning &t) {
+void values_to_tuning(const long long v[kTuningCount], prh_tuning &t) {
   t.switch_degree = uint32_t(v[0]);
   t.huge_degree = uint32_t(v[1]);
   t.low_width = int(v[2]);
@@ -1474,6 +1655,7 @@ void values_to_tuning(const long long v[7], prh_tuning &t) {
   t.source_slices = int(v[4]);
   t.block_edges = uint32_t(v[5]);
   t.extension_direction = int(v[6]);
+  t.blocking = int(v[7]);
 }
 
 // The cache is a JSON object with one entry per line:
@@ -1484,7 +1666,7 @@ void values_to_tuning(const long long v[7], prh_tuning &t) {
 // Lines that do not parse as an entry are ignored (a hand-edited file degrades to a cache miss).
 struct CacheEntry {
   std::string key;
-  long long v[7];
+  long long v[kTuningCount];
   double ms;
 };
 
@@ -1500,7 +1682,7 @@ std::vector<CacheEntry> read_cache(const char *path) {
     CacheEntry e;
     e.key.assign(q0 + 1, q1);
     bool ok = true;
-    for (int k = 0; k < 7 && ok; ++k) {
+    for (int k = 0; k < kTuningCount && ok; ++k) {
       const std::string pat = std::string("\"") + kTuningFields[k] + "\":";
       const char *p = std::strstr(q1, pat.c_str());
       ok = p && std::sscanf(p + pat.size(), "%lld", &e.v[k]) == 1;
@@ -1521,7 +1703,7 @@ bool write_cache(const char *path, const std::vector<CacheEntry> &entries) {
   for (size_t i = 0; i < entries.size(); ++i) {
     const CacheEntry &e = entries[i];
     std::fprintf(f, "\"%s\": {", e.key.c_str());
-    for (int k = 0; k < 7; ++k) std::fprintf(f, "\"%s\": %lld, ", kTuningFields[k], e.v[k]);
+    for (int k = 0; k < kTuningCount; ++k) std::fprintf(f, "\"%s\": %lld, ", kTuningFields[k], e.v[k]);
     std::fprintf(f, "\"ms_per_iteration\": %.6f}%s\n", e.ms, i + 1 < entries.size() ? "," : "");
   }
   std::fprintf(f, "}\n");
@@ -1558,8 +1740,10 @@ int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int
         return PRH_SUCCESS;
       }
   }
-  // Candidates: the row plan over switch degree x chunk threshold x block size, and the sliced plan over slice
-  // count x block size. Each is timed on `iterations` static iterations (tolerance 0) after one warm-up solve.
+  // Candidates: the row plan over switch degree x chunk threshold x block size, the sliced plan over slice
+  // count x block size, and the propagation-blocked plan over its block width (graphs of 2^20 edges and more,
+  // where its streaming passes pay). Each is timed on `iterations` static iterations (tolerance 0) after one
+  // warm-up solve.
   std::vector<prh_tuning> cand;
   for (uint32_t sw : {64u, 128u, 256u})
     for (uint32_t hu : {4096u, 16384u})
@@ -1570,6 +1754,7 @@ int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int
         t.low_width = 0;
         t.source_slices = 0;
         t.block_edges = be;
+        t.blocking = 0;
         cand.push_back(t);
       }
   for (int sl : {1, 8})
@@ -1580,6 +1765,20 @@ int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int
       t.low_width = 0;
       t.source_slices = sl;
       t.block_edges = be;
+      t.blocking = 0;
+      cand.push_back(t);
+    }
+  if (g->M >= (uint64_t(1) << 20))
+    for (int bw : {4096, 8192, 16384}) {
+      const uint64_t nb = (uint64_t(g->N) + bw - 1) / bw;
+      if (nb * nb > (uint64_t(1) << 32)) continue;
+      prh_tuning t = base;
+      t.switch_degree = 64;
+      t.huge_degree = 16384;
+      t.low_width = 0;
+      t.source_slices = 0;
+      t.block_edges = 2048;
+      t.blocking = bw;
       cand.push_back(t);
     }
   double *ranks = nullptr;

@@ -133,6 +133,35 @@ struct SliceBases {
   uint32_t item[kMaxSlices];  // first item id of slice x
 };
 
+// Propagation-blocked plan of the plain update (Static / Naive-dynamic; binned.hip). Layout rows are cut into runs
+// of B vertices: source blocks and destination bins, both numbered s, b = vertex / B. Slots (one per edge) are
+// ordered bin-major (bin, source block, destination, source); X holds the slots' contributions block-major (block,
+// bin, same order within a tile), so each nonempty tile is one run, consecutive in both orders.
+//   expand item     (s, k0, k1, 0): X entries k0..k1-1, all leaving block s;
+//   accumulate item (b, j0, j1, row): slots j0..j1-1, all entering bin b; row = kWholeBin when the item is the
+//                   bin's only one (it finishes the bin), else its row of B partial sums;
+//   finish unit     (b, first row, rows, first vertex | mode << 31): part of a bin with several items, finished
+//                   by finish_bins_kernel (mode 0: 1024 vertices, a thread each; 1: 64 vertices, 16 waves each).
+constexpr int kBinBlock = 1024;
+constexpr uint32_t kWholeBin = 0xFFFFFFFFu;
+
+struct BinnedPlan {
+  int B;                   // vertices per block and bin: 1024, 2048, 4096, 8192 or 16384
+  uint32_t N;
+  const uint16_t *src16;   // [M] X order: source - s * B
+  const uint16_t *dst16;   // [M] X order: destination - b * B
+  double *X;               // [M]
+  const uint32_t *run_j;   // [n_runs] first slot of each run (ascending)
+  const uint32_t *run_x;   // [n_runs] its X entry
+  const uint32_t *grp_run; // [ceil(M / 64)] run holding slot 64 q
+  uint32_t n_runs;
+  const uint4 *ex_item;    // [n_ex]
+  const uint4 *ac_item;    // [n_ac]
+  const uint4 *multi;      // [n_multi] finish units
+  double *partial;         // [partial rows][B]
+  uint32_t n_ex, n_ac, n_multi;
+};
+
 // ---- partition.hip ----
 // Stable 3-way partition of [0,N) by degree d = off[v+1]-off[v]: d <= t1, t1 < d <= t2, d > t2.
 // tiles: scratch of 3 * partition_tiles(N) uint32; counts_dev: 3 uint32 outputs (device).
@@ -169,6 +198,31 @@ hipError_t launch_check_finite(const double *x, size_t n, int *flags, int bit, h
 // beg[i] = off[ids[i]], end[i] = off[ids[i] + 1]
 hipError_t launch_gather_ranges(const uint32_t *off, const uint32_t *ids, size_t n, uint32_t *beg, uint32_t *end,
                                 hipStream_t s);
+
+// ---- binned.hip ----
+// One iteration through the propagation-blocked plan: expand, accumulate, and (with split bins) finish. `mark`,
+// when non-null, is recorded after the last of them.
+hipError_t launch_binned_update(const RankArgs &A, const BinnedPlan &P, unsigned long long *err, int *done, int l,
+                                double tol, hipStream_t s, hipEvent_t mark = nullptr);
+// Build steps. key[e] = bin(row) * nblocks + block(src[e]), row[e] = row of in-CSR slot e.
+hipError_t launch_bin_keys(const uint32_t *off, const uint32_t *src, uint32_t N, uint32_t B, uint32_t nblocks,
+                           uint32_t *key, uint32_t *row, hipStream_t s);
+// For slot j (in-CSR slot perm2[j]): dst[j], u2[j] = source, key1[j] = source block.
+hipError_t launch_bin_slots(const uint32_t *perm2, const uint32_t *src, const uint32_t *row, size_t M, uint32_t B,
+                            uint16_t *dst, uint32_t *u2, uint32_t *key1, hipStream_t s);
+// flag[j] = slot j starts a run (its sorted tile key differs from slot j-1's).
+hipError_t launch_run_flags(const uint32_t *key2s, size_t M, uint32_t *flag, hipStream_t s);
+// For X entry k (slot perm1[k]): src16[k], dst16[k], inv[perm1[k]] = k.
+hipError_t launch_bin_expand(const uint32_t *perm1, const uint32_t *u2, const uint16_t *dst, size_t M, uint32_t B,
+                             uint16_t *src16, uint16_t *dst16, uint32_t *inv, hipStream_t s);
+// Run tables from the flags and their exclusive scan rid.
+hipError_t launch_bin_runs(const uint32_t *flag, const uint32_t *rid, const uint32_t *inv, size_t M, uint32_t *run_j,
+                           uint32_t *run_x, uint32_t *grp_run, hipStream_t s);
+// start[s] = lower bound of s in the sorted key[0..M), s = 0..n.
+hipError_t launch_lower_bounds(const uint32_t *key, size_t M, uint32_t n, uint32_t *start, hipStream_t s);
+// Stable radix sort of (key, value) on key bits [0, end_bit); val_in null means values 0, 1, 2, ...
+hipError_t sort_pairs_u32(void *tmp, size_t &tmp_bytes, const uint32_t *key_in, uint32_t *key_out,
+                          const uint32_t *val_in, uint32_t *val_out, size_t n, int end_bit, hipStream_t s);
 
 // ---- ppr.hip ----
 constexpr int kPprCols = 64;  // teleport columns per batch (one per lane)

@@ -69,6 +69,12 @@ class Tuning:
     source_slices: int = 0
     block_edges: int = 2048
     extension_direction: int = 1
+    blocking: int = 0
+
+
+def _tuning_of(n) -> Tuning:
+    return Tuning(n.switch_degree, n.huge_degree, n.low_width, n.check_interval, n.source_slices, n.block_edges,
+                  n.extension_direction, n.blocking)
 
 
 class Context:
@@ -119,13 +125,12 @@ class Context:
     def set_tuning(self, t: Tuning):
         self.check(self._lib.prh_context_set_tuning(
             self._h, C.byref(prh_tuning(t.switch_degree, t.huge_degree, t.low_width, t.check_interval,
-                                        t.source_slices, t.block_edges, t.extension_direction))))
+                                        t.source_slices, t.block_edges, t.extension_direction, t.blocking))))
 
     def get_tuning(self) -> Tuning:
         n = prh_tuning()
         self.check(self._lib.prh_context_get_tuning(self._h, C.byref(n)))
-        return Tuning(n.switch_degree, n.huge_degree, n.low_width, n.check_interval, n.source_slices, n.block_edges,
-                      n.extension_direction)
+        return _tuning_of(n)
 
     def autotune(self, yg: "DeviceGraph", cache_path: str | None = None, iterations: int = 10):
         """prh_autotune: time candidate launch plans on yg and keep the fastest as this context's tuning.
@@ -136,8 +141,7 @@ class Context:
         path = cache_path.encode() if cache_path else None
         self.check(self._lib.prh_autotune(self._h, yg.handle, path, iterations, C.byref(t), C.byref(ms),
                                           C.byref(hit)))
-        return (Tuning(t.switch_degree, t.huge_degree, t.low_width, t.check_interval, t.source_slices, t.block_edges,
-                       t.extension_direction), ms.value, bool(hit.value))
+        return _tuning_of(t), ms.value, bool(hit.value)
 
     def upload(self, y: HostGraph, with_out: bool = True) -> "DeviceGraph":
         return DeviceGraph(self, y, with_out)
