@@ -10,15 +10,18 @@ iteration over the whole timed region (|E| x iterations / wall time of the K sol
 and the ranks' return to the caller's vertex order). config.gedges_per_s_iter_host_buffers is the same solve with
 host buffers (ranks copied over PCIe), reported beside it.
 
-roofline: the dominant kernel is the rank update (one launch per iteration). Its algorithmic bytes per launch are
-12|E| + 24|V| (per edge a 4-byte source id and an 8-byte gathered FP64 value; per vertex a 4-byte offset, 4-byte
-out-degree, 8-byte old-rank read and 8-byte new-rank write; the L-inf reduction is fused). Its duration is measured
-with HIP events recorded on the engine's stream around every launch that did work inside the timed region.
+roofline: the dominant work is the rank update, once per iteration: one rank_update_kernel launch with the row
+plan, or expand_kernel + accumulate_kernel (+ finish_bins_kernel) with the propagation-blocked plan
+(tuning.blocking), timed together. Its algorithmic bytes per iteration are 12|E| + 24|V| (per edge a 4-byte source
+id and an 8-byte FP64 contribution; per vertex a 4-byte offset, 4-byte out-degree, 8-byte old-rank read and 8-byte
+new-rank write; the L-inf reduction is fused), whatever the plan moves in addition. Its duration is measured with HIP
+events recorded on the engine's stream around every iteration's update that did work inside the timed region.
 
-roofline.traffic: HBM bytes per rank-kernel launch, from rocprofv3 counter runs of this script on the same workload
-(FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor that the same counter needs to give the known byte
-count of a plain device copy, measured with --copy-calibration). The record, with the commands that produced it, is
-profiles/rank_update_traffic.json; it is reported only when its workload string equals this run's.
+roofline.traffic: HBM bytes per iteration's update, from rocprofv3 counter runs of this script on the same workload
+and plan (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor that the same counter needs to give the
+known byte count of a plain device copy, measured with --copy-calibration). The records, with the commands that
+produced them, are in profiles/rank_update_traffic.json; one is reported only when its workload and plan strings
+equal this run's.
 
 cpu_baseline: the oracle's OpenMP restatement of the reference's pagerankStaticOmp (bit-identical results to the
 compiled reference on the test graphs), run on this host's cores on the same graph for a bounded number of
@@ -93,15 +96,28 @@ def copy_calibration(gib: float) -> int:
     return 0
 
 
-def measured_traffic(desc: str):
+def measured_traffic(desc: str, plan: str):
     path = REPO / "profiles" / "rank_update_traffic.json"
     try:
-        rec = json.loads(path.read_text())
-    except (OSError, ValueError):
+        recs = json.loads(path.read_text()).get("records", [])
+    except (OSError, ValueError, AttributeError):
         return None
-    if rec.get("workload") != desc:
-        return None
-    return rec.get("hbm_bytes_per_launch")
+    for rec in recs:
+        if rec.get("workload") == desc and rec.get("plan") == plan:
+            return rec.get("hbm_bytes_per_launch")
+    return None
+
+
+def plan_name(t) -> str:
+    """The rank-update plan a static solve runs with tuning t, and the kernels of one iteration."""
+    if t.blocking > 0:
+        return (f"propagation blocking {t.blocking}: expand_kernel<{t.blocking}> + accumulate_kernel<{t.blocking}> "
+                f"+ finish_bins_kernel")
+    if t.source_slices > 0:
+        return (f"source slices {t.source_slices} (huge_degree {t.huge_degree}, block_edges {t.block_edges}): "
+                f"sliced_gather_kernel + sliced_finish_kernel")
+    return (f"row plan (switch_degree {t.switch_degree}, huge_degree {t.huge_degree}, block_edges {t.block_edges}): "
+            f"rank_update_kernel<0>")
 
 
 def main() -> int:
@@ -184,6 +200,7 @@ def main() -> int:
     r = pagerankStaticHip(yg, o)
     host_rate = g.M * r.iterations / (time.perf_counter() - t0) / 1e9
 
+    plan = plan_name(tuning)
     value = g.M * iters / t_total / 1e9
     kavg_s = kms / max(launches, 1) / 1e3
     bytes_alg = 12 * g.M + 24 * g.N
@@ -204,13 +221,13 @@ def main() -> int:
         "config": {"workload": desc, "vertices": g.N, "edges": g.M, "tolerance": 1e-10, "damping": 0.85,
                    "tuning": {"switch_degree": tuning.switch_degree, "huge_degree": tuning.huge_degree,
                               "source_slices": tuning.source_slices, "block_edges": tuning.block_edges,
-                              "autotuned": autotuned},
+                              "blocking": tuning.blocking, "autotuned": autotuned},
                    "iterations_per_solve": iters / args.steps,
                    "gedges_per_s_iter_host_buffers": round(host_rate, 4),
                    "gedges_per_s_iter_compute_only": round(g.M * iters / (tcomp / 1e3) / 1e9, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(desc),
-                     "kernel": "rank_update_kernel", "kernel_avg_us": round(kavg_s * 1e6, 2),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(desc, plan),
+                     "kernel": plan, "kernel_avg_us": round(kavg_s * 1e6, 2),
                      "bytes_per_launch": bytes_alg},
     }
     if not args.no_cpu:
