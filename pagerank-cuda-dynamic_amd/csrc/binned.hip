@@ -1,4 +1,4 @@
-// binned.hip -- the propagation-blocked plain update (Static / Naive-dynamic) and the build of its layout.
+// binned.hip -- the propagation-blocked rank update (every solver variant) and the build of its layout.
 //
 // The pull kernel gathers cr[u] = r[u]/deg[u] at random; every gather from a source outside the L2-resident hot
 // set costs a 128-byte line fill for 8 useful bytes. Propagation blocking (Beamer, Asanovic and Patterson,
@@ -20,14 +20,15 @@
 // table (run_j = first slot, run_x = first X entry; grp_run = the run holding slot 64 q), and reads X and the 16-bit
 // local destination ids (stored in X's order) run by run.
 // Per edge and iteration: expand reads 2 bytes and writes 8, accumulate reads 8 + 2 (plus a few bytes per run).
-// LDS additions are made in arrival order, so the sums repeat only to rounding, not bit for bit; the update itself
-// is the plain one (pagerankUpdateRanksThreadCukU, pagerankCuda.hxx:87-109): a[v] = C0 + P * sum, ca[v] = a[v] /
-// deg[v].
+// LDS additions are made in arrival order, so the sums repeat only to rounding, not bit for bit. Every vertex gets
+// its full in-sum; the update from it is the other plans' (update.hpp), so the dynamic variants carry unaffected
+// vertices over and set their prune and frontier flags as the row kernel does.
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 
 #include "prh_device.hpp"
 #include "prh_internal.hpp"
+#include "update.hpp"
 
 namespace prh {
 namespace {
@@ -38,12 +39,49 @@ typedef double d2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t ld_nt(const uint32_t *p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ uint32_t ld_nt(const uint16_t *p) { return __builtin_nontemporal_load(p); }
 
-// Plain update of vertex v from its in-sum; returns the L-inf term.
-__device__ __forceinline__ double finish_plain(const RankArgs &A, uint32_t v, double cv) {
-  const double av = fma(A.P, cv, A.C0);
-  A.a[v] = av;
-  A.ca[v] = av / A.deg[v];
-  return fabs(av - A.r[v]);
+// Vertex v of a bin from its in-sum: updated (update.hpp), or carried over when a dynamic variant left it
+// unaffected; its extension costs are added to `cost` (frontier variants with A.stats). Returns the L-inf term.
+template <int VARIANT>
+__device__ __forceinline__ double bin_finish(const RankArgs &A, uint32_t v, double cv, CostAcc &cost) {
+  const uint32_t d = A.deg[v];
+  const double rv = A.r[v];
+  const double crv = Traits<VARIANT>::dynamic ? A.cr[v] : 0.0;
+  RowFlags f;
+  double e = 0.0;
+  if (affected<VARIANT>(A, v)) {
+    e = finish_vertex<VARIANT>(A, v, cv, d, rv, crv, f);
+  } else {
+    copy_vertex<VARIANT>(A, v, rv, crv);
+    f.affected = false;
+  }
+  if constexpr (Traits<VARIANT>::frontier)
+    if (A.stats) cost.add(f, A.off[v + 1] - A.off[v], d);
+  return e;
+}
+
+// Workgroup totals of the extension costs to A.stats[2 slot], [2 slot + 1] (frontier variants). Every thread must
+// call it.
+template <int VARIANT>
+__device__ __forceinline__ void bin_store_costs(const RankArgs &A, const CostAcc &c, uint32_t slot, double *lds) {
+  if constexpr (Traits<VARIANT>::frontier) {
+    if (!A.stats) return;
+    const int w = threadIdx.x / kWave;
+    const double pull = group_sum<kWave>(c.pull), push = group_sum<kWave>(c.push);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      lds[w] = pull;
+      lds[kBinWaves + w] = push;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double a = 0.0, b = 0.0;
+      for (int i = 0; i < kBinWaves; ++i) {
+        a += lds[i];
+        b += lds[kBinWaves + i];
+      }
+      A.stats[2 * size_t(slot)] = a;
+      A.stats[2 * size_t(slot) + 1] = b;
+    }
+  }
 }
 
 // Workgroup-wide max over kBinBlock threads, valid in thread 0. Every thread must call it.
@@ -157,11 +195,12 @@ __device__ __forceinline__ void add_slots(double *sum, uint32_t d, double x, boo
   }
 }
 
-template <int B>
+template <int B, int VARIANT>
 __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, BinnedPlan P, LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;
   __shared__ double sum[B];
   __shared__ double lds_max[kBinWaves];
+  __shared__ double lds_cost[2 * kBinWaves];
   const uint4 it = P.ac_item[blockIdx.x];  // bin, first slot, end slot, partial row (kWholeBin: none)
   for (uint32_t i = threadIdx.x; i < uint32_t(B); i += kBinBlock) sum[i] = 0.0;
   __syncthreads();
@@ -192,25 +231,30 @@ __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, Binne
   __syncthreads();
   const uint32_t b0 = it.x * uint32_t(B);
   const uint32_t nb = min(uint32_t(B), P.N - b0);
+  CostAcc cost;
+  double emax = 0.0;
   if (it.w == kWholeBin) {
-    double emax = 0.0;
-    for (uint32_t i = threadIdx.x; i < nb; i += kBinBlock) emax = fmax(emax, finish_plain(A, b0 + i, sum[i]));
-    emax = bin_block_max(emax, lds_max);
-    if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
+    for (uint32_t i = threadIdx.x; i < nb; i += kBinBlock) emax = fmax(emax, bin_finish<VARIANT>(A, b0 + i, sum[i], cost));
   } else {
     double *p = P.partial + size_t(it.w) * B;
     for (uint32_t i = threadIdx.x; i < nb; i += kBinBlock) p[i] = sum[i];
   }
+  bin_store_costs<VARIANT>(A, cost, blockIdx.x, lds_cost);
+  emax = bin_block_max(emax, lds_max);
+  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
 }
 
 // Split bins, one workgroup per unit (bin, first partial row, rows, first vertex | mode << 31):
 //   mode 0: vertices first .. first + 1023 of the bin, a thread per vertex adding all its partial rows;
 //   mode 1: vertices first .. first + 63, wave w adding rows w, w + 16, ... of its lane's vertex, then the 16 wave
 //           sums in wave order (bins with many rows).
+template <int VARIANT>
 __global__ __launch_bounds__(kBinBlock) void finish_bins_kernel(RankArgs A, BinnedPlan P, LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;
   __shared__ double part[kBinWaves][kWave];
   __shared__ double lds_max[kBinWaves];
+  __shared__ double lds_cost[2 * kBinWaves];
+  CostAcc cost;
   const uint4 m = P.multi[blockIdx.x];
   const bool split = (m.w >> 31) != 0;
   const uint32_t first = m.w & 0x7FFFFFFFu;
@@ -229,7 +273,7 @@ __global__ __launch_bounds__(kBinBlock) void finish_bins_kernel(RankArgs A, Binn
         s1 += p[size_t(k + 1) * P.B];
       }
       if (k < m.z) s0 += p[size_t(k) * P.B];
-      emax = finish_plain(A, v, s0 + s1);
+      emax = bin_finish<VARIANT>(A, v, s0 + s1, cost);
     }
   } else {
     double s0 = 0.0, s1 = 0.0;
@@ -246,18 +290,19 @@ __global__ __launch_bounds__(kBinBlock) void finish_bins_kernel(RankArgs A, Binn
     if (w == 0 && ok) {
       double t = 0.0;
       for (int q = 0; q < kBinWaves; ++q) t += part[q][lane];
-      emax = finish_plain(A, v, t);
+      emax = bin_finish<VARIANT>(A, v, t, cost);
     }
   }
+  bin_store_costs<VARIANT>(A, cost, P.n_ac + blockIdx.x, lds_cost);
   emax = bin_block_max(emax, lds_max);
   if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
 }
 
-template <int B>
+template <int B, int VARIANT>
 hipError_t binned_update(const RankArgs &A, const BinnedPlan &P, const LoopCtl &ctl, hipStream_t s, hipEvent_t mark) {
   if (P.n_ex) expand_kernel<B><<<P.n_ex, kBinBlock, 0, s>>>(P, A.cr, ctl);
-  accumulate_kernel<B><<<P.n_ac, kBinBlock, 0, s>>>(A, P, ctl);
-  if (P.n_multi) finish_bins_kernel<<<P.n_multi, kBinBlock, 0, s>>>(A, P, ctl);
+  accumulate_kernel<B, VARIANT><<<P.n_ac, kBinBlock, 0, s>>>(A, P, ctl);
+  if (P.n_multi) finish_bins_kernel<VARIANT><<<P.n_multi, kBinBlock, 0, s>>>(A, P, ctl);
   if (mark) {
     const hipError_t e = hipEventRecord(mark, s);
     if (e != hipSuccess) return e;
@@ -344,17 +389,29 @@ __global__ __launch_bounds__(kBlock) void lower_bounds_kernel(const uint32_t *ke
   start[s] = uint32_t(lo);
 }
 
+template <int B>
+hipError_t binned_update_variant(int variant, const RankArgs &A, const BinnedPlan &P, const LoopCtl &ctl,
+                                 hipStream_t s, hipEvent_t mark) {
+  switch (variant) {
+    case kPlain: return binned_update<B, kPlain>(A, P, ctl, s, mark);
+    case kTraversal: return binned_update<B, kTraversal>(A, P, ctl, s, mark);
+    case kFrontier: return binned_update<B, kFrontier>(A, P, ctl, s, mark);
+    case kPrune: return binned_update<B, kPrune>(A, P, ctl, s, mark);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 }  // namespace
 
-hipError_t launch_binned_update(const RankArgs &A, const BinnedPlan &P, unsigned long long *err, int *done, int l,
-                                double tol, hipStream_t s, hipEvent_t mark) {
+hipError_t launch_binned_update(int variant, const RankArgs &A, const BinnedPlan &P, unsigned long long *err,
+                                int *done, int l, double tol, hipStream_t s, hipEvent_t mark) {
   const LoopCtl ctl{err, done, l, tol};
   switch (P.B) {
-    case 1024: return binned_update<1024>(A, P, ctl, s, mark);
-    case 2048: return binned_update<2048>(A, P, ctl, s, mark);
-    case 4096: return binned_update<4096>(A, P, ctl, s, mark);
-    case 8192: return binned_update<8192>(A, P, ctl, s, mark);
-    case 16384: return binned_update<16384>(A, P, ctl, s, mark);
+    case 1024: return binned_update_variant<1024>(variant, A, P, ctl, s, mark);
+    case 2048: return binned_update_variant<2048>(variant, A, P, ctl, s, mark);
+    case 4096: return binned_update_variant<4096>(variant, A, P, ctl, s, mark);
+    case 8192: return binned_update_variant<8192>(variant, A, P, ctl, s, mark);
+    case 16384: return binned_update_variant<16384>(variant, A, P, ctl, s, mark);
     default: return hipErrorInvalidValue;
   }
 }

@@ -786,7 +786,7 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   PRH_HIP(c, hipSetDevice(c->device));
   PRH_CHECK(ensure_workspace(c, N, dynamic, L));
   prh_graph *gm = const_cast<prh_graph *>(g);
-  const bool binned = variant == prh::kPlain && c->tuning.blocking > 0;
+  const bool binned = c->tuning.blocking > 0;
   const bool sliced = !binned && variant == prh::kPlain && c->tuning.source_slices > 0;
   if (binned) PRH_CHECK(build_binned_plan(c, gm));
   else if (sliced) PRH_CHECK(build_sliced_plan(c, gm));
@@ -818,7 +818,9 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   // 2 pull only. The pull form splits rows at in-degree 64 (a wave each) and 4 (8 lanes each).
   const int dir_mode = frontier ? c->tuning.extension_direction : 0;
   const bool dir_auto = dir_mode == 1;
-  const uint32_t stat_slots = plan.blocks + (plan.n_chunked ? prh::finish_chunked_grid(plan.n_chunked) : 0);
+  // Per-workgroup extension costs: one slot per workgroup of the launches that finish vertices.
+  const uint32_t stat_slots = binned ? g->binned.n_ac + g->binned.n_multi
+                                     : plan.blocks + (plan.n_chunked ? prh::finish_chunked_grid(plan.n_chunked) : 0);
   if (dir_auto) PRH_CHECK(ensure(c, c->ext_stats, 2 * size_t(std::max<uint32_t>(stat_slots, 1))));
   int *dir = frontier ? c->flags.p + 2 : nullptr;
   const prh::PullArgs pull{c->vaff.p, c->naff.p, g->l_in_off, g->l_in_src, rows_above(g->sorted_deg, 64),
@@ -847,7 +849,7 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
                       o->prune_tolerance, dir_auto ? c->ext_stats.p : nullptr};
       if (c->ktiming) PRH_HIP(c, hipEventRecord(c->kev[2 * l], s));
       hipEvent_t mark = c->ktiming ? c->kev[2 * l + 1] : nullptr;
-      if (binned) PRH_HIP(c, prh::launch_binned_update(A, g->binned, err, done, l, o->tolerance, s, mark));
+      if (binned) PRH_HIP(c, prh::launch_binned_update(variant, A, g->binned, err, done, l, o->tolerance, s, mark));
       else if (sliced) PRH_HIP(c, prh::launch_sliced_update(A, g->sliced, err, done, l, o->tolerance, s, mark));
       else PRH_HIP(c, prh::launch_rank_update(variant, A, plan, err, done, l, o->tolerance, s, mark));
       if (frontier) {

@@ -133,7 +133,7 @@ struct SliceBases {
   uint32_t item[kMaxSlices];  // first item id of slice x
 };
 
-// Propagation-blocked plan of the plain update (Static / Naive-dynamic; binned.hip). Layout rows are cut into runs
+// Propagation-blocked plan of the rank update (every variant; binned.hip). Layout rows are cut into runs
 // of B vertices: source blocks and destination bins, both numbered s, b = vertex / B. Slots (one per edge) are
 // ordered bin-major (bin, source block, destination, source); X holds the slots' contributions block-major (block,
 // bin, same order within a tile), so each nonempty tile is one run, consecutive in both orders.
@@ -202,8 +202,8 @@ hipError_t launch_gather_ranges(const uint32_t *off, const uint32_t *ids, size_t
 // ---- binned.hip ----
 // One iteration through the propagation-blocked plan: expand, accumulate, and (with split bins) finish. `mark`,
 // when non-null, is recorded after the last of them.
-hipError_t launch_binned_update(const RankArgs &A, const BinnedPlan &P, unsigned long long *err, int *done, int l,
-                                double tol, hipStream_t s, hipEvent_t mark = nullptr);
+hipError_t launch_binned_update(int variant, const RankArgs &A, const BinnedPlan &P, unsigned long long *err,
+                                int *done, int l, double tol, hipStream_t s, hipEvent_t mark = nullptr);
 // Build steps. key[e] = bin(row) * nblocks + block(src[e]), row[e] = row of in-CSR slot e.
 hipError_t launch_bin_keys(const uint32_t *off, const uint32_t *src, uint32_t N, uint32_t B, uint32_t nblocks,
                            uint32_t *key, uint32_t *row, hipStream_t s);

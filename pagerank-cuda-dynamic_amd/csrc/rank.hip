@@ -23,22 +23,10 @@
 
 #include "prh_device.hpp"
 #include "prh_internal.hpp"
+#include "update.hpp"
 
 namespace prh {
 namespace {
-
-template <int VARIANT>
-struct Traits {
-  static constexpr bool dynamic = VARIANT >= kTraversal;
-  static constexpr bool frontier = VARIANT >= kFrontier;
-  static constexpr bool prune = VARIANT == kPrune;
-};
-
-template <int VARIANT>
-__device__ __forceinline__ bool affected(const RankArgs &A, uint32_t v) {
-  if constexpr (Traits<VARIANT>::dynamic) return A.vaff[v] != 0;
-  else return true;
-}
 
 // Streamed arrays (source ids, offsets) are read with the non-temporal hint so that they do not displace the
 // gathered contributions from L2; the gathers themselves use the default policy.
@@ -66,49 +54,6 @@ __device__ __forceinline__ RowOperands row_operands(const RankArgs &A, uint32_t 
   return o;
 }
 
-// Flags a frontier variant leaves on a row: still affected (not pruned), and in the frontier.
-struct RowFlags {
-  bool affected = true, frontier = false;
-};
-
-template <int VARIANT>
-__device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv, uint32_t d, double rv,
-                                                double crv, RowFlags &f) {
-  using T = Traits<VARIANT>;
-  double av;
-  if constexpr (T::prune) av = fma(A.P, cv - crv, A.C0) * (1.0 / (1.0 - A.P / d));
-  else av = fma(A.P, cv, A.C0);
-  A.a[v] = av;
-  A.ca[v] = av / d;
-  const double ev = fabs(av - rv);
-  if constexpr (T::frontier) {
-    const double rel = ev / fmax(rv, av);
-    if (T::prune && rel <= A.C) {
-      A.vaff[v] = 0;
-      f.affected = false;
-    }
-    f.frontier = rel > A.D;
-    A.naff[v] = f.frontier ? 1 : 0;
-  }
-  return ev;
-}
-
-template <int VARIANT>
-__device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv, uint32_t d, double rv,
-                                                double crv) {
-  RowFlags f;
-  return finish_vertex<VARIANT>(A, v, cv, d, rv, crv, f);
-}
-
-// Per-lane sums of the extension's two costs (frontier variants with A.stats set).
-struct CostAcc {
-  double pull = 0.0, push = 0.0;
-  __device__ void add(const RowFlags &f, uint32_t in_deg, uint32_t out_deg) {
-    if (!f.affected) pull += in_deg;
-    if (f.frontier) push += out_deg;
-  }
-};
-
 // Workgroup totals of the costs to A.stats[2 slot], [2 slot + 1]. Every thread must call it.
 template <int VARIANT>
 __device__ __forceinline__ void store_costs(const RankArgs &A, CostAcc c, uint32_t slot, double *lds8) {
@@ -126,26 +71,6 @@ __device__ __forceinline__ void store_costs(const RankArgs &A, CostAcc c, uint32
       A.stats[2 * size_t(slot) + 1] = (lds8[4] + lds8[5]) + (lds8[6] + lds8[7]);
     }
   }
-}
-
-template <int VARIANT>
-__device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv) {
-  return finish_vertex<VARIANT>(A, v, cv, A.deg[v], A.r[v], Traits<VARIANT>::prune ? A.cr[v] : 0.0);
-}
-
-// Unaffected row of a dynamic variant: carried over unchanged (rank and contribution).
-template <int VARIANT>
-__device__ __forceinline__ void copy_vertex(const RankArgs &A, uint32_t v, double rv, double crv) {
-  A.a[v] = rv;
-  A.ca[v] = crv;
-  if constexpr (Traits<VARIANT>::frontier) A.naff[v] = 0;
-}
-
-template <int VARIANT>
-__device__ __forceinline__ void copy_vertex(const RankArgs &A, uint32_t v) {
-  A.a[v] = A.r[v];
-  A.ca[v] = A.cr[v];
-  if constexpr (Traits<VARIANT>::frontier) A.naff[v] = 0;
 }
 
 // Sum of cr[src[e]] for e = first, first + STEP, ... < end, four gathers in flight per lane.

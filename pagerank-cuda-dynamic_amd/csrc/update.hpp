@@ -1,0 +1,90 @@
+// update.hpp -- the per-vertex update shared by every plan of the rank update (rank.hip, binned.hip).
+//
+// Per vertex v (pagerankUpdateRanks{Thread,Block}CukU, pagerankCuda.hxx:87-109, :162-188), given its in-sum
+// cv = sum_{u in in(v)} r[u]/deg[u]:
+//   a[v] = C0 + P*cv                      -- or the DF-P closed form (C0 + P*(cv - r[v]/d))/(1 - P/d)
+//   prune / frontier flags                 -- vaff[v] = 0 if |a-r|/max(r,a) <= C; naff[v] = (|a-r|/max > D)
+// Unaffected rows of the dynamic variants are carried over unchanged.
+#pragma once
+#include "prh_device.hpp"
+#include "prh_internal.hpp"
+
+namespace prh {
+
+template <int VARIANT>
+struct Traits {
+  static constexpr bool dynamic = VARIANT >= kTraversal;
+  static constexpr bool frontier = VARIANT >= kFrontier;
+  static constexpr bool prune = VARIANT == kPrune;
+};
+
+template <int VARIANT>
+__device__ __forceinline__ bool affected(const RankArgs &A, uint32_t v) {
+  if constexpr (Traits<VARIANT>::dynamic) return A.vaff[v] != 0;
+  else return true;
+}
+
+// Flags a frontier variant leaves on a row: still affected (not pruned), and in the frontier.
+struct RowFlags {
+  bool affected = true, frontier = false;
+};
+
+template <int VARIANT>
+__device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv, uint32_t d, double rv,
+                                                double crv, RowFlags &f) {
+  using T = Traits<VARIANT>;
+  double av;
+  if constexpr (T::prune) av = fma(A.P, cv - crv, A.C0) * (1.0 / (1.0 - A.P / d));
+  else av = fma(A.P, cv, A.C0);
+  A.a[v] = av;
+  A.ca[v] = av / d;
+  const double ev = fabs(av - rv);
+  if constexpr (T::frontier) {
+    const double rel = ev / fmax(rv, av);
+    if (T::prune && rel <= A.C) {
+      A.vaff[v] = 0;
+      f.affected = false;
+    }
+    f.frontier = rel > A.D;
+    A.naff[v] = f.frontier ? 1 : 0;
+  }
+  return ev;
+}
+
+template <int VARIANT>
+__device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv, uint32_t d, double rv,
+                                                double crv) {
+  RowFlags f;
+  return finish_vertex<VARIANT>(A, v, cv, d, rv, crv, f);
+}
+
+// Per-lane sums of the extension's two costs (frontier variants with A.stats set).
+struct CostAcc {
+  double pull = 0.0, push = 0.0;
+  __device__ void add(const RowFlags &f, uint32_t in_deg, uint32_t out_deg) {
+    if (!f.affected) pull += in_deg;
+    if (f.frontier) push += out_deg;
+  }
+};
+
+template <int VARIANT>
+__device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv) {
+  return finish_vertex<VARIANT>(A, v, cv, A.deg[v], A.r[v], Traits<VARIANT>::prune ? A.cr[v] : 0.0);
+}
+
+// Unaffected row of a dynamic variant: carried over unchanged (rank and contribution).
+template <int VARIANT>
+__device__ __forceinline__ void copy_vertex(const RankArgs &A, uint32_t v, double rv, double crv) {
+  A.a[v] = rv;
+  A.ca[v] = crv;
+  if constexpr (Traits<VARIANT>::frontier) A.naff[v] = 0;
+}
+
+template <int VARIANT>
+__device__ __forceinline__ void copy_vertex(const RankArgs &A, uint32_t v) {
+  A.a[v] = A.r[v];
+  A.ca[v] = A.cr[v];
+  if constexpr (Traits<VARIANT>::frontier) A.naff[v] = 0;
+}
+
+}  // namespace prh

@@ -101,7 +101,8 @@ def autotune(ctx: Context, g: HostGraph, args) -> None:
     t, ms, hit = ctx.autotune(yg, args.tuning_cache, iterations=10)
     yg.close()
     log(f"autotune: switch {t.switch_degree}, chunk {t.huge_degree}, slices {t.source_slices}, "
-        f"block edges {t.block_edges}: {ms:.3f} ms per static iteration{' (cache)' if hit else ''}")
+        f"block edges {t.block_edges}, blocking {t.blocking}: {ms:.3f} ms per static iteration"
+        f"{' (cache)' if hit else ''}")
 
 
 def run_batches(ctx: Context, x: HostGraph, name: str, frac: float, next_batch, args, out: CsvLog,

@@ -77,6 +77,26 @@ def test_binned_naive_dynamic(ctx, oracle_mod):
     assert np.abs(res.ranks - ref).max() <= RANK_TOL
 
 
+DYN = [(2, 1024, 1e-3), (3, 2048, 1e-3), (4, 1024, 1e-3), (3, 4096, 1e-2), (4, 2048, 1e-4)]
+
+
+@pytest.mark.parametrize("method,blocking,frac", DYN, ids=[f"m{m}_B{b}_{f:g}" for m, b, f in DYN])
+def test_binned_dynamic_methods(ctx, oracle_mod, method, blocking, frac):
+    """DT, DF and DF-P through the binned plan: every vertex gets its full in-sum, unaffected ones are carried over,
+    prune and frontier flags as the row kernel sets them, extension direction chosen from the costs the binned
+    kernels record. Ranks within 1e-9 of the oracle's restatement of the same method, iterations +-1."""
+    from pagerank_amd import Batch, HostGraph, apply_batch, solve
+    x = HostGraph.rmat(13, 16, seed=21)
+    q, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, x)
+    b = Batch.random(x, max(1, int(frac * x.M)), 0.8, seed=5)
+    y = apply_batch(x, b)
+    print(f"[binned-dyn] method={method} B={blocking} frac={frac} N={y.N} M={y.M}", flush=True)
+    res = _with_blocking(ctx, blocking, lambda: solve(ctx.upload(y), method, batch=b, q=q))
+    ref, it, _ = oracle_mod.pagerank_cuda(method, y, b, q)
+    assert abs(res.iterations - it) <= 1, (res.iterations, it)
+    assert np.abs(res.ranks - ref).max() <= RANK_TOL
+
+
 def test_binned_repeats_to_rounding(ctx):
     """Two solves with the plan agree to rounding (LDS additions may arrive in another order)."""
     from pagerank_amd import HostGraph, pagerankStaticHip
