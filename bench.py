@@ -26,6 +26,13 @@ equal this run's.
 cpu_baseline: the oracle's OpenMP restatement of the reference's pagerankStaticOmp (bit-identical results to the
 compiled reference on the test graphs), run on this host's cores on the same graph for a bounded number of
 iterations (about 10 s).
+
+--workload dfp: the dynamic side instead. Each step applies a fresh seeded batch of --batch-fraction |E| (80%
+insertions, 20% deletions) to the RMAT base graph and times a DF-P solve warm-started from the base graph's ranks
+(engine-measured solve time: initialisation, marking and iterations, as PagerankResult.time), beside a GPU static
+solve of the same updated graph. Its cpu_baseline is the oracle's port of the reference's asynchronous OpenMP DF-P
+(pagerankPruneDynamicFrontierOmp<true>, the CPU solver the reference's experiment runs; bit-identical to the
+compiled reference on one thread) on the same batch, on this host's cores.
 """
 from __future__ import annotations
 
@@ -50,6 +57,11 @@ def log(msg: str) -> None:
 
 def make_graph(args):
     from pagerank_amd import HostGraph
+    if args.workload == "dfp":
+        g = HostGraph.rmat(args.scale, 16.0, 0.57, 0.19, 0.19, seed=42)
+        desc = (f"DF-P on directed RMAT scale {args.scale} (edge factor 16, a/b/c/d=.57/.19/.19/.05, seed 42, dedup, "
+                f"self-loops), batches of {args.batch_fraction:g}|E| (80% insertions)")
+        return g, desc
     if args.workload == "rmat":
         g = HostGraph.rmat(args.scale, 16.0, 0.57, 0.19, 0.19, seed=42)
         desc = (f"static PageRank, directed RMAT scale {args.scale} (edge factor 16, a/b/c/d=.57/.19/.19/.05, "
@@ -80,6 +92,70 @@ def cpu_baseline(g, budget_s: float):
     return {"value": round(value, 4), "unit": "GEdges/s/iter", "cores": threads, "kind": "port",
             "sample": f"{solves} static solve(s) of the OpenMP port on the same graph, {iters} synchronous iterations "
                       f"in {total_ms / 1e3:.1f} s"}
+
+
+def bench_dfp(args, ctx, g, desc) -> int:
+    """GPU DF-P per update batch against GPU static on the same updated graph; CPU: the async DF-P port."""
+    from pagerank_amd import PRUNE_DYNAMIC_FRONTIER, STATIC, Batch, PagerankOptions, apply_batch, solve
+    o = PagerankOptions()
+    xg = ctx.upload(g, with_out=False)
+    autotuned = False
+    if not args.no_autotune:
+        t, ms, hit = ctx.autotune(xg, args.tuning_cache, iterations=10)
+        autotuned = True
+        log(f"autotune: {t} at {ms:.3f} ms/iteration")
+    q = solve(xg, STATIC, o).ranks
+    xg.close()
+    size = max(1, int(args.batch_fraction * g.M))
+    dfp_ms, st_ms, its, st_its, last = [], [], [], [], None
+    for i in range(args.warmup + args.steps):
+        b = Batch.random(g, size, 0.8, seed=1000 + i)
+        y = apply_batch(g, b)
+        yg = ctx.upload(y)
+        r = solve(yg, PRUNE_DYNAMIC_FRONTIER, o, batch=b, q=q)
+        s = solve(yg, STATIC, o)
+        yg.close()
+        log(f"batch {i}: -{b.deletions}/+{b.insertions}: DF-P {r.time:.2f} ms ({r.iterations} it), static "
+            f"{s.time:.2f} ms ({s.iterations} it), L-inf DF-P vs static {abs(r.ranks - s.ranks).max():.2e}")
+        if i >= args.warmup:
+            dfp_ms.append(r.time)
+            st_ms.append(s.time)
+            its.append(r.iterations)
+            st_its.append(s.iterations)
+            last = (y, b)
+    value = sum(dfp_ms) / len(dfp_ms)
+    static = sum(st_ms) / len(st_ms)
+    tuning = ctx.get_tuning()
+    out = {
+        "metric": f"DF-P solve time per {args.batch_fraction:g}|E| update batch (ms), and speedup over GPU static",
+        "value": round(value, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(value, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "vertices": g.N, "edges": g.M, "batch_edges": size,
+                   "dfp_iterations": its, "static_ms": round(static, 3), "static_iterations": st_its,
+                   "speedup_over_gpu_static": round(static / value, 3), "plan": plan_name(tuning),
+                   "autotuned": autotuned},
+        "roofline": None,
+    }
+    if not args.no_cpu:
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        y, b = last
+        _, it, ms = oracle.pagerank_async(PRUNE_DYNAMIC_FRONTIER, y, b, q, threads=threads)
+        out["cpu_baseline"] = {"value": round(ms, 3), "unit": "ms", "cores": threads, "kind": "port",
+                               "sample": f"one asynchronous OpenMP DF-P solve (the reference's "
+                                         f"pagerankPruneDynamicFrontierOmp<true>, ported) of the last timed batch, "
+                                         f"{it} sweeps"}
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def copy_calibration(gib: float) -> int:
@@ -125,7 +201,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["rmat", "grid"], default="rmat")
+    ap.add_argument("--workload", choices=["rmat", "grid", "dfp"], default="rmat")
+    ap.add_argument("--batch-fraction", type=float, default=1e-5, help="dfp workload: batch size as a fraction of |E|")
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--rows", type=int, default=6000)
     ap.add_argument("--cols", type=int, default=4000)
@@ -155,6 +232,8 @@ def main() -> int:
     g, desc = make_graph(args)
     log(f"graph N={g.N} M={g.M} built in {time.time() - t0:.1f}s")
     ctx = Context(0)
+    if args.workload == "dfp":
+        return bench_dfp(args, ctx, g, desc)
     overridden = any(v is not None for v in (args.source_slices, args.huge_degree, args.blocking))
     if overridden:
         t = ctx.get_tuning()

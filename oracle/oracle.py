@@ -33,6 +33,9 @@ _SIG = {
                                     C.c_double, f64p, ip, u8p]),
     "orc_static_omp": (C.c_int, [C.c_uint32, u32p, u32p, u32p, C.c_double, C.c_double, C.c_int, C.c_int, f64p, ip,
                                  f64p]),
+    "orc_pagerank_async": (C.c_int, [C.c_int, C.c_uint32, u32p, u32p, u32p, u32p, u32p, u32p, u32p, C.c_size_t,
+                                     u32p, C.c_size_t, f64p, C.c_double, C.c_double, C.c_int, C.c_double,
+                                     C.c_double, C.c_int, f64p, ip, f64p]),
     "orc_power_iteration": (C.c_int, [C.c_uint32, u32p, u32p, u32p, f64p, C.c_double, C.c_int, f64p, ip]),
     "orc_ppr": (C.c_int, [C.c_uint32, u32p, u32p, u32p, f64p, C.c_double, C.c_double, C.c_int, f64p, ip]),
     "orc_weighted_forward": (C.c_int, [C.c_uint32, u32p, u32p, f32p, f64p, f64p, C.c_double, C.c_double, C.c_int,
@@ -151,6 +154,25 @@ def static_omp(y, damping=0.85, tolerance=1e-10, max_iterations=500, threads=0):
     ms = C.c_double()
     lib().orc_static_omp(y.N, _u32(y.in_off)[1], _u32(y.in_edg)[1], _u32(y.outdeg)[1], damping, tolerance,
                          max_iterations, threads, ranks.ctypes.data_as(f64p), C.byref(it), C.byref(ms))
+    return ranks, int(it.value), float(ms.value)
+
+
+def pagerank_async(method, y, batch=None, q=None, threads=0, damping=0.85, tolerance=1e-10, max_iterations=500,
+                   frontier_tolerance=1e-6, prune_tolerance=1e-6):
+    """The reference's asynchronous OpenMP solver `method` on the updated graph y (a HostGraph; batch = the tidied
+    batch that produced it, q = previous ranks). Returns (ranks, iterations, ms)."""
+    du, dv, iu, _ = _batch(batch)
+    ranks = np.zeros(y.N)
+    it = C.c_int()
+    ms = C.c_double()
+    qq = None if q is None else np.ascontiguousarray(q, dtype=np.float64)
+    st = lib().orc_pagerank_async(
+        method, y.N, _u32(y.in_off)[1], _u32(y.in_edg)[1], _u32(y.outdeg)[1], _u32(y.out_off)[1], _u32(y.out_edg)[1],
+        _u32(du)[1], _u32(dv)[1], len(du), _u32(iu)[1], len(iu), qq.ctypes.data_as(f64p) if qq is not None else None,
+        damping, tolerance, max_iterations, frontier_tolerance, prune_tolerance, threads, ranks.ctypes.data_as(f64p),
+        C.byref(it), C.byref(ms))
+    if st != 0:
+        raise ValueError("oracle rejected arguments")
     return ranks, int(it.value), float(ms.value)
 
 

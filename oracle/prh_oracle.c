@@ -6,6 +6,8 @@
  *     sequentially, with the same iteration structure, flag rules and summation order
  *     (thread kernel: sequential ascending sum; block kernel: 256 strided partial sums + tree);
  *   - orc_static_omp restates the OpenMP static path of inc/pagerank.hxx (the CPU baseline "port");
+ *   - orc_pagerank_async restates the asynchronous OpenMP solvers that the reference's experiment runs on dynamic
+ *     graphs (ASYNC=true: ranks updated in place), the CPU baseline of the dynamic methods;
  *   - orc_power_iteration is a compensated-sum FP64 power iteration run to stagnation (the "true" fixed point);
  *   - orc_ppr / orc_weighted_* are the FP64 oracles for the personalized (batched) and adjoint forms.
  *
@@ -231,6 +233,86 @@ int orc_static_omp(uint32_t N, const uint32_t *xtoff, const uint32_t *xtedg, con
   *iterations = l;
   *time_ms = (t1 - t0) * 1e3;
   free(a); free(r);
+  return 0;
+}
+
+/* The reference's asynchronous OpenMP solvers (template switch ASYNC=true), as main.cu:90-145 runs them:
+ *   mode 0 pagerankStaticOmp<true> (pagerank.hxx:473-482), 1 pagerankNaiveDynamicOmp<true> (:517-525),
+ *   2 pagerankDynamicTraversalOmp<true> (:619-627), 3 pagerankDynamicFrontierOmp<true> (:718-727),
+ *   4 pagerankPruneDynamicFrontierOmp<true> (pagerankPrune.hxx:300-307 via pagerankPruneInvokeOmp, :211-246).
+ * Ranks live in one array updated in place, vertex by vertex (pagerankUpdateRanksAsyncOmp, pagerank.hxx:240-253;
+ * pagerankPruneUpdateRanksAsyncOmp, pagerankPrune.hxx:140-158) under schedule(dynamic, 2048), and a sweep's L-inf
+ * is the largest |old - new| it saw; the loop stops after the first sweep below E (pagerankInvokeOmp, :412-440).
+ * With several threads the in-place reads race, as in the reference; with one thread the order is 0..N-1 and the
+ * result is deterministic. Marking: DT the BFS closure (the same set as pagerankAffectedTraversalOmpW,
+ * :562-578), DF and DF-P the out-neighbours of the batch sources and the deleted edges' targets (the set of
+ * pagerankAffectedFrontierOmpW, :662-676). During the sweeps DF marks every out-neighbour of a vertex whose
+ * relative change exceeds D (:725); DF-P uses the closed-form self-loop update, prunes at <= C and marks
+ * out-neighbours other than itself that are not yet affected (pagerankPrune.hxx:29-39, :148-155).
+ * C0 = (1 - P) / N: the reference's N is the vertex count, and its DF-P span equals it for dense ids.
+ * xt*: in-CSR of the updated graph with out-degrees; yoff/yedg: its out-CSR. time_ms covers initialisation,
+ * marking and the sweeps, like PagerankResult.time. Returns 0, or -1 on bad arguments. */
+int orc_pagerank_async(int mode, uint32_t N, const uint32_t *xtoff, const uint32_t *xtedg, const uint32_t *xtdat,
+                       const uint32_t *yoff, const uint32_t *yedg, const uint32_t *delu, const uint32_t *delv,
+                       size_t ND, const uint32_t *insu, size_t NI, const double *q, double P, double E, int L,
+                       double D, double C, int threads, double *ranks, int *iterations, double *time_ms) {
+  if (mode < 0 || mode > 4 || (mode != ORC_STATIC && !q)) return -1;
+  if (N == 0) { *iterations = 0; *time_ms = 0; return 0; }
+  if (threads > 0) omp_set_num_threads(threads);
+  double *r = (double *)malloc(sizeof(double) * N);
+  uint8_t *vaff = (uint8_t *)malloc(N), *naff = (uint8_t *)malloc(N);
+  double t0 = omp_get_wtime();
+  #pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v < (int64_t)N; ++v) r[v] = mode == ORC_STATIC ? 1.0 / N : q[v];
+  if (mode == ORC_DYNAMIC_TRAVERSAL) orc_mark_traversal(N, yoff, yedg, delu, delv, ND, insu, NI, vaff);
+  else if (mode >= ORC_DYNAMIC_FRONTIER) orc_mark_frontier(N, yoff, yedg, delu, delv, ND, insu, NI, vaff, naff);
+  else memset(vaff, 1, N);
+  const double C0 = (1 - P) / N;
+  int l = 0;
+  while (l < L) {
+    double el = 0.0;
+    #pragma omp parallel for schedule(dynamic, 2048) reduction(max : el)
+    for (int64_t v = 0; v < (int64_t)N; ++v) {
+      if (!vaff[v]) continue;
+      const double rv = r[v];
+      double cv = 0.0;
+      for (uint32_t i = xtoff[v]; i < xtoff[v + 1]; ++i) {
+        uint32_t u = xtedg[i];
+        cv += r[u] / xtdat[u];
+      }
+      double av;
+      if (mode == ORC_PRUNE_DYNAMIC_FRONTIER) {
+        const uint32_t d = xtdat[v];
+        volatile double pc = P * (cv - rv / d); /* the host's separate multiply and add */
+        av = (C0 + pc) * (1 / (1 - (P / d)));
+      } else {
+        volatile double pc = P * cv;
+        av = C0 + pc;
+      }
+      r[v] = av;
+      const double ev = fabs(rv - av);
+      if (ev > el) el = ev;
+      const double rel = ev / (rv > av ? rv : av);
+      if (mode == ORC_DYNAMIC_FRONTIER) {
+        if (rel > D)
+          for (uint32_t i = yoff[v]; i < yoff[v + 1]; ++i) vaff[yedg[i]] = 1;
+      } else if (mode == ORC_PRUNE_DYNAMIC_FRONTIER) {
+        if (rel <= C) vaff[v] = 0;
+        if (rel > D)
+          for (uint32_t i = yoff[v]; i < yoff[v + 1]; ++i) {
+            uint32_t w = yedg[i];
+            if (w != (uint32_t)v && !vaff[w]) vaff[w] = 1;
+          }
+      }
+    }
+    ++l;
+    if (el < E) break;
+  }
+  double t1 = omp_get_wtime();
+  memcpy(ranks, r, sizeof(double) * N);
+  *iterations = l;
+  *time_ms = (t1 - t0) * 1e3;
+  free(r); free(vaff); free(naff);
   return 0;
 }
 

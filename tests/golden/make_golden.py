@@ -7,6 +7,8 @@ reference headers by oracle/Makefile). Run in the container that has the referen
 Each fixture holds a small graph x (out-CSR), a tidied batch, and what the reference computes on them:
   static_*  pagerankStaticOmp (synchronous)             on y = x + batch
   ND/DT/DF/DFP_*  the synchronous CPU dynamic solvers   on (x, y, batch, q = static ranks of x)
+  async_{static,ND,DT,DF,DFP}_*  the asynchronous OpenMP solvers (ASYNC=true, as main.cu runs the dynamic ones)
+                  on one thread, where their in-place sweep order is deterministic
   affected_df / affected_dt  pagerankAffected{Frontier,Traversal}OmpW flags
   y_off / y_edg  tidyBatchUpdateU + applyBatchUpdateOmpU result
 The fixtures are data only (inputs and expected outputs)."""
@@ -55,6 +57,9 @@ def main():
         for m, tag in ((1, "ND"), (2, "DT"), (3, "DF"), (4, "DFP")):
             r, it, _ = oracle.ref_pagerank(m, x, b, q)
             out[f"{tag}_ranks"], out[f"{tag}_iterations"] = r, it
+        for m, tag in ((0, "static"), (1, "ND"), (2, "DT"), (3, "DF"), (4, "DFP")):
+            r, it, _ = oracle.ref_pagerank(m, x, b, None if m == 0 else q, asynchronous=True, threads=1)
+            out[f"async_{tag}_ranks"], out[f"async_{tag}_iterations"] = r, it
         out["affected_df"] = oracle.ref_affected(0, x, b)
         out["affected_dt"] = oracle.ref_affected(1, x, b)
         np.savez_compressed(HERE / f"{name}.npz", **out)

@@ -55,6 +55,37 @@ def test_static_port_bit_exact_with_reference(oracle_mod, name):
     assert np.array_equal(r, z["static_ranks"])
 
 
+ASYNC = [(0, "static"), (1, "ND"), (2, "DT"), (3, "DF"), (4, "DFP")]
+
+
+@pytest.mark.parametrize("method,tag", ASYNC, ids=[t for _, t in ASYNC])
+@pytest.mark.parametrize("name", FIXTURES)
+def test_async_port_bit_exact_with_reference(oracle_mod, name, method, tag):
+    """On one thread the asynchronous OpenMP solvers (the dynamic CPU baselines of the reference's experiment)
+    sweep in vertex order, and the port reproduces the reference's ranks and iteration counts bit for bit."""
+    from pagerank_amd import apply_batch
+    z, x, b = _load(name)
+    y = apply_batch(x, b)
+    r, it, _ = oracle_mod.pagerank_async(method, y, b if method else None, z["q"] if method else None, threads=1)
+    assert it == int(z[f"async_{tag}_iterations"])
+    assert np.array_equal(r, z[f"async_{tag}_ranks"])
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 4])
+def test_async_port_reaches_fixed_point(oracle_mod, method):
+    """With several threads the in-place reads race, as in the reference, but every sweep is still a contraction:
+    Static/ND/DT/DF end within 1e-8 of the exact fixed point, DF-P within its pruning bound (1e-5 x max rank)."""
+    from pagerank_amd import Batch, HostGraph, apply_batch
+    x = HostGraph.rmat(11, 16, seed=17)
+    q, _, _ = oracle_mod.static_omp(x)
+    b = Batch.random(x, 200, 0.8, seed=8)
+    y = apply_batch(x, b)
+    exact, _ = oracle_mod.power_iteration(y)
+    r, it, _ = oracle_mod.pagerank_async(method, y, b if method else None, q if method else None, threads=4)
+    bound = 1e-5 * exact.max() if method == 4 else 1e-8
+    assert 0 < it < 500 and np.abs(r - exact).max() <= bound
+
+
 @pytest.mark.parametrize("name", FIXTURES)
 def test_cuda_restatement_static_nd(oracle_mod, name):
     """The GPU-path restatement differs from the CPU path only by fused multiply-adds and the 256-way block sum."""
@@ -173,6 +204,11 @@ def test_live_reference_agrees(oracle_mod):
     r, it, _ = oracle_mod.static_omp(y)
     assert it == itr and np.array_equal(r, rr)
     assert np.array_equal(oracle_mod.mark(2, y, b), oracle_mod.ref_affected(1, x, b))
+    q, _, _ = oracle_mod.ref_pagerank(0, x)
+    for m in (1, 4):
+        rr, itr, _ = oracle_mod.ref_pagerank(m, x, b, q, asynchronous=True, threads=1)
+        r, it, _ = oracle_mod.pagerank_async(m, y, b, q, threads=1)
+        assert it == itr and np.array_equal(r, rr), m
 
 
 def test_pins_reject_planted_mistakes(oracle_mod):
