@@ -263,6 +263,7 @@ def seed_sets(n: int, k: int, rng: np.random.Generator, max_size: int = 1000) ->
 def run_ppr(args, out: CsvLog) -> int:
     g, name = make_rmat(args.scale, args.edge_factor, args.seed)
     ctx = Context(0)
+    autotune(ctx, g, args)  # the plain static solves compared against use the best launch plan
     yg = ctx.upload(g, with_out=False)
     rng = np.random.default_rng(args.seed)
     sets = seed_sets(g.N, args.columns, rng)
@@ -295,6 +296,7 @@ def run_ppr(args, out: CsvLog) -> int:
 def run_adjoint(args, out: CsvLog) -> int:
     g, name = make_rmat(args.scale, args.edge_factor, args.seed)
     ctx = Context(0)
+    autotune(ctx, g, args)  # the plain static solve compared against uses the best launch plan
     yg = ctx.upload(g)
     rng = np.random.default_rng(args.seed)
     w = rng.uniform(0.5, 1.5, g.M).astype(np.float32)
@@ -334,8 +336,8 @@ def main(argv=None) -> int:
     ap.add_argument("--repeat", type=int, default=5)
     ap.add_argument("--columns", type=int, default=64)
     ap.add_argument("--csv", default=None)
-    ap.add_argument("--no-autotune", action="store_true", help="dynamic mode: keep the default launch plan")
-    ap.add_argument("--tuning-cache", default=None, help="dynamic mode: the autotuner's JSON cache")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch plan (dynamic, temporal, ppr, adjoint)")
+    ap.add_argument("--tuning-cache", default=None, help="the autotuner's JSON cache")
     args = ap.parse_args(argv)
     out = CsvLog(args.csv)
     return {"dynamic": run_dynamic, "temporal": run_temporal, "stream": run_stream, "ppr": run_ppr, "adjoint": run_adjoint}[args.mode](args, out)
