@@ -8,8 +8,9 @@
 //   expand      one workgroup per item of a source block: the block's B contributions are loaded into LDS, and
 //               every edge leaving the block copies its source's contribution into its slot of X;
 //   accumulate  one workgroup per item of a destination bin: the bin's slots are read in order and added into B
-//               LDS sums (ds_add_f64, one addition per wave when its 64 slots share a destination); a bin covered
-//               by one item is finished right there, a split bin leaves one row of B partial sums per item;
+//               LDS sums (ds_add_f64; neighbouring slots with the same destination are summed first, by a wave
+//               prefix sum, so that they do not collide on one address); a bin covered by one item is finished
+//               right there, a split bin leaves one row of B partial sums per item;
 //   finish      split bins only: a vertex's partials are added in item order, then the update.
 // Expand stores pairs of entries (16 bytes a lane); accumulate takes 64-slot groups, most of which lie inside one
 // run and need only scalar loads of the run table.
@@ -184,15 +185,56 @@ __device__ __forceinline__ uint32_t slot_entry(const BinnedPlan &P, uint32_t g, 
   return sx + (g + lane - sj);
 }
 
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_rows(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, ROWS, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Inclusive prefix sum over the 64 lanes (row shifts within 16-lane rows, then the row broadcasts of lanes 15 and
+// 31). All lanes must be active.
+__device__ __forceinline__ double wave_prefix_sum(double x) {
+  x += dpp_rows<0x111, 0xF>(x);  // row_shr:1
+  x += dpp_rows<0x112, 0xF>(x);  // row_shr:2
+  x += dpp_rows<0x114, 0xF>(x);  // row_shr:4
+  x += dpp_rows<0x118, 0xF>(x);  // row_shr:8
+  x += dpp_rows<0x142, 0xA>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_rows<0x143, 0xC>(x);  // row_bcast:31 into rows 2 and 3
+  return x;
+}
+
+__device__ __forceinline__ double pull_f64(double x, uint32_t src_lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_ds_bpermute(int(src_lane << 2), int(b));
+  const int hi = __builtin_amdgcn_ds_bpermute(int(src_lane << 2), int(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // Adds one wave's 64 slots into the LDS sums. All lanes must be active; lanes past the item carry ok = false.
+// Lanes with the same destination as their left neighbour (hot destinations repeat within a destination-sorted
+// run) are summed first, so that each such segment costs one LDS addition instead of colliding on one address.
 __device__ __forceinline__ void add_slots(double *sum, uint32_t d, double x, bool ok) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-  if (__ballot(d == d0 || !ok) == ~0ull) {  // one destination for the whole wave: reduce first
+  if (__ballot(d == d0 || !ok) == ~0ull) {  // one destination for the whole wave
     const double t = group_sum<kWave>(ok ? x : 0.0);
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(&sum[d0], t);
-  } else if (ok) {
-    atomicAdd(&sum[d], x);
+    if (lane == 0) atomicAdd(&sum[d0], t);
+    return;
   }
+  const uint32_t dp = uint32_t(__builtin_amdgcn_update_dpp(0, int(d), 0x138, 0xF, 0xF, false));  // wave_shr:1
+  const unsigned long long heads = __ballot(lane == 0 || d != dp || !ok);
+  if (heads == ~0ull || __ballot(ok) != ~0ull) {  // no repeats, or a partial group: one addition per lane
+    if (ok) atomicAdd(&sum[d], x);
+    return;
+  }
+  const double s = wave_prefix_sum(x);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  const uint32_t h = 63u - uint32_t(__clzll(heads & upto));  // head of this lane's segment
+  const double before = pull_f64(s, h == 0 ? 0u : h - 1);
+  const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+  if (tail) atomicAdd(&sum[d], h == 0 ? s : s - before);
 }
 
 template <int B, int VARIANT>
