@@ -50,8 +50,9 @@ struct prh_context {
   DevBuf<unsigned long long> ppr_err;
   DevBuf<prh::PprState> ppr_state;
   // weighted forward / adjoint
-  DevBuf<double> wt_x[2], wt_z[2], wt_W, wt_term, wt_r, wt_gw, wt_gs;
-  DevBuf<float> wt_wout, wt_win;
+  // weighted forward / adjoint, on the degree-ordered layout ("l": layout order, else the caller's order)
+  DevBuf<double> wt_x[2], wt_z[2], wt_W, wt_Wl, wt_term, wt_terml, wt_r, wt_rl, wt_gwl, wt_gsl, wt_outM, wt_outN;
+  DevBuf<float> wt_wout, wt_wl;
   uint64_t *pinned = nullptr;       // small host-pinned readback area (kPinnedBytes)
   hipEvent_t ev[4] = {};
   bool ktiming = false;
@@ -66,7 +67,8 @@ struct prh_graph {
   uint64_t M = 0;
   // As uploaded (caller's ids): used by the batched and weighted solvers.
   uint32_t *in_off = nullptr, *in_src = nullptr, *outdeg = nullptr, *out_off = nullptr, *out_dst = nullptr;
-  uint32_t *in_eid = nullptr;  // out-CSR position of every in-CSR slot, built on first weighted use
+  // Raw out-CSR slot of every slot of the layout's in-CSR and out-CSR, built on first weighted use.
+  uint32_t *l_in_eid = nullptr, *l_out_eid = nullptr;
   // Degree-ordered layout used by the five solvers: layout row i is vertex perm[i] (in-degree descending, stable),
   // inv is the inverse; in-rows sorted by new source id.
   uint32_t *perm = nullptr, *inv = nullptr;
@@ -1215,20 +1217,47 @@ int stage(prh_context *c, const T *src, size_t n, bool device, DevBuf<T> &buf, c
   return PRH_SUCCESS;
 }
 
+// Shared set-up of the weighted solves: the raw weights staged, W (out-weight sums) in the caller's order and in
+// layout order, the layout edge maps (once per graph), and the iteration buffers.
 int weighted_common(prh_context *c, const prh_graph *g, const float *w, const prh_options *o, bool device,
                     const float **w_out) {
   if (!g || !o || !w) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null argument");
   if (g->ctx != c) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "graph belongs to another context");
-  if (!g->out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "weighted solves need the graph's out-CSR");
+  if (!g->out_off || !g->l_out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "weighted solves need the graph's out-CSR");
   if (o->max_iterations < 0 || !(o->tolerance >= 0) || !(o->damping > 0 && o->damping < 1))
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad options");
   PRH_HIP(c, hipSetDevice(c->device));
-  PRH_CHECK(stage(c, w, size_t(g->M), device, c->wt_wout, w_out));
-  PRH_CHECK(ensure(c, c->wt_W, g->N));
-  PRH_HIP(c, prh::launch_out_weight_sum(g->out_off, *w_out, g->N, c->wt_W.p, c->stream));
+  const uint32_t N = g->N;
+  const uint64_t M = g->M;
+  hipStream_t hs = c->stream;
+  if (!g->l_in_eid) {
+    prh_graph *gm = const_cast<prh_graph *>(g);
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&gm->l_in_eid), (M ? M : 4) * 4));
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&gm->l_out_eid), (M ? M : 4) * 4));
+    PRH_CHECK(ensure(c, c->flags, 4));
+    PRH_HIP(c, prh::launch_fill_i32(c->flags.p, 0, 1, hs));
+    PRH_HIP(c, prh::launch_layout_edge_map(g->l_in_off, g->l_in_src, g->perm, g->out_off, g->out_dst, N, true,
+                                           gm->l_in_eid, c->flags.p, hs));
+    PRH_HIP(c, prh::launch_layout_edge_map(g->l_out_off, g->l_out_dst, g->perm, g->out_off, g->out_dst, N, false,
+                                           gm->l_out_eid, c->flags.p, hs));
+    PRH_HIP(c, hipMemcpyAsync(c->pinned, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, hs));
+    PRH_HIP(c, hipStreamSynchronize(hs));
+    if (*reinterpret_cast<int *>(c->pinned)) {
+      dev_free(c, gm->l_in_eid, (M ? M : 4) * 4);
+      dev_free(c, gm->l_out_eid, (M ? M : 4) * 4);
+      gm->l_in_eid = gm->l_out_eid = nullptr;
+      return fail(c, PRH_ERROR_INVALID_GRAPH, "in-CSR and out-CSR disagree or out-rows are not ascending");
+    }
+  }
+  PRH_CHECK(stage(c, w, size_t(M), device, c->wt_wout, w_out));
+  PRH_CHECK(ensure(c, c->wt_W, N));
+  PRH_CHECK(ensure(c, c->wt_Wl, N));
+  PRH_HIP(c, prh::launch_out_weight_sum(g->out_off, *w_out, N, c->wt_W.p, hs));
+  PRH_HIP(c, prh::launch_gather_f64(c->wt_W.p, g->perm, N, c->wt_Wl.p, hs));
+  PRH_CHECK(ensure(c, c->wt_wl, std::max<uint64_t>(1, M)));
   for (int i = 0; i < 2; ++i) {
-    PRH_CHECK(ensure(c, c->wt_x[i], g->N));
-    PRH_CHECK(ensure(c, c->wt_z[i], g->N));
+    PRH_CHECK(ensure(c, c->wt_x[i], N));
+    PRH_CHECK(ensure(c, c->wt_z[i], N));
   }
   return PRH_SUCCESS;
 }
@@ -1246,6 +1275,26 @@ int two_way_schedule(prh_context *c, const uint32_t *off, uint32_t N, DevBuf<uin
   return PRH_SUCCESS;
 }
 
+// A caller-order vector of N doubles in layout order: dst[i] = src[perm[i]].
+int to_layout(prh_context *c, const prh_graph *g, const double *src, DevBuf<double> &dst) {
+  PRH_CHECK(ensure(c, dst, g->N));
+  PRH_HIP(c, prh::launch_gather_f64(src, g->perm, g->N, dst.p, c->stream));
+  return PRH_SUCCESS;
+}
+
+// A layout-order result back to the caller's order (through `tmp` when the destination is on the host).
+int from_layout(prh_context *c, const double *src, const uint32_t *map, size_t n, double *dst, bool device,
+                DevBuf<double> &tmp) {
+  if (device) {
+    PRH_HIP(c, prh::launch_scatter_f64(src, map, n, dst, c->stream));
+    return PRH_SUCCESS;
+  }
+  PRH_CHECK(ensure(c, tmp, std::max<size_t>(1, n)));
+  PRH_HIP(c, prh::launch_scatter_f64(src, map, n, tmp.p, c->stream));
+  if (n) PRH_HIP(c, hipMemcpyAsync(dst, tmp.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+  return PRH_SUCCESS;
+}
+
 int weighted_forward_impl(prh_context *c, const prh_graph *g, const float *w, const double *s, const prh_options *o,
                           double *ranks, prh_stats *st, bool device) {
   if (!c) return PRH_ERROR_INVALID_ARGUMENT;
@@ -1257,38 +1306,23 @@ int weighted_forward_impl(prh_context *c, const prh_graph *g, const float *w, co
   hipStream_t hs = c->stream;
   const double *term = nullptr;
   PRH_CHECK(stage(c, s, N, device, c->wt_term, &term));
-  if (!g->in_eid) {  // the out-CSR slot of every in-CSR slot, once per graph
-    prh_graph *gm = const_cast<prh_graph *>(g);
-    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&gm->in_eid), M * 4));
-    PRH_CHECK(ensure(c, c->flags, 4));
-    PRH_HIP(c, prh::launch_fill_i32(c->flags.p, 0, 1, hs));
-    PRH_HIP(c, prh::launch_edge_map(g->in_off, g->in_src, g->out_off, g->out_dst, N, gm->in_eid, c->flags.p, hs));
-    PRH_HIP(c, hipMemcpyAsync(c->pinned, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, hs));
-    PRH_HIP(c, hipStreamSynchronize(hs));
-    if (*reinterpret_cast<int *>(c->pinned)) {
-      dev_free(c, gm->in_eid, M * 4);
-      gm->in_eid = nullptr;
-      return fail(c, PRH_ERROR_INVALID_GRAPH, "in-CSR and out-CSR disagree or out-rows are not ascending");
-    }
-  }
-  PRH_CHECK(ensure(c, c->wt_win, std::max<uint64_t>(1, M)));
-  PRH_HIP(c, prh::launch_permute_weights(w_out, g->in_eid, M, c->wt_win.p, hs));
+  PRH_CHECK(to_layout(c, g, term, c->wt_terml));
+  PRH_HIP(c, prh::launch_permute_weights(w_out, g->l_in_eid, M, c->wt_wl.p, hs));
   prh::Schedule sc{};
-  PRH_CHECK(two_way_schedule(c, g->in_off, N, c->sched_in, sc));
+  PRH_CHECK(two_way_schedule(c, g->l_in_off, N, c->sched_in, sc));
   PRH_HIP(c, hipEventRecord(c->ev[0], hs));
-  PRH_HIP(c, prh::launch_weighted_init(term, c->wt_W.p, N, false, c->wt_x[0].p, c->wt_z[0].p, hs));
+  PRH_HIP(c, prh::launch_weighted_init(c->wt_terml.p, c->wt_Wl.p, N, false, c->wt_x[0].p, c->wt_z[0].p, hs));
   PRH_HIP(c, hipEventRecord(c->ev[1], hs));
   const int width = low_width_for(c->tuning, N, M);
   int n = 0;
   PRH_CHECK(run_iterations(c, o->max_iterations, o->tolerance, [&](int l, unsigned long long *err, int *done) -> int {
-    const prh::WeightedArgs A{g->in_off, g->in_src, c->wt_win.p, c->wt_z[l & 1].p, c->wt_x[(l + 1) & 1].p,
-                              c->wt_z[(l + 1) & 1].p, c->wt_x[l & 1].p, term, c->wt_W.p, sc, o->damping};
+    const prh::WeightedArgs A{g->l_in_off, g->l_in_src, c->wt_wl.p, c->wt_z[l & 1].p, c->wt_x[(l + 1) & 1].p,
+                              c->wt_z[(l + 1) & 1].p, c->wt_x[l & 1].p, c->wt_terml.p, c->wt_Wl.p, sc, o->damping};
     PRH_HIP(c, prh::launch_weighted_step(false, A, width, err, done, l, o->tolerance, hs));
     return PRH_SUCCESS;
   }, &n));
   PRH_HIP(c, hipEventRecord(c->ev[2], hs));
-  PRH_HIP(c, hipMemcpyAsync(ranks, c->wt_x[n & 1].p, size_t(N) * 8,
-                            device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, hs));
+  PRH_CHECK(from_layout(c, c->wt_x[n & 1].p, g->perm, N, ranks, device, c->wt_outN));
   PRH_HIP(c, hipStreamSynchronize(hs));
   float t01 = 0, t12 = 0;
   PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
@@ -1310,35 +1344,31 @@ int weighted_adjoint_impl(prh_context *c, const prh_graph *g, const float *w, co
   const double *term = nullptr, *rr = nullptr;
   PRH_CHECK(stage(c, gr, N, device, c->wt_term, &term));
   PRH_CHECK(stage(c, r, N, device, c->wt_r, &rr));
+  PRH_CHECK(to_layout(c, g, term, c->wt_terml));
+  PRH_CHECK(to_layout(c, g, rr, c->wt_rl));
+  PRH_HIP(c, prh::launch_permute_weights(w_out, g->l_out_eid, M, c->wt_wl.p, hs));
   prh::Schedule sc{};
-  PRH_CHECK(two_way_schedule(c, g->out_off, N, c->sched_out, sc));
+  PRH_CHECK(two_way_schedule(c, g->l_out_off, N, c->sched_out, sc));
   PRH_HIP(c, hipEventRecord(c->ev[0], hs));
-  PRH_HIP(c, prh::launch_weighted_init(term, c->wt_W.p, N, true, c->wt_x[0].p, c->wt_z[0].p, hs));
+  PRH_HIP(c, prh::launch_weighted_init(c->wt_terml.p, c->wt_Wl.p, N, true, c->wt_x[0].p, c->wt_z[0].p, hs));
   PRH_HIP(c, hipEventRecord(c->ev[1], hs));
   const int width = low_width_for(c->tuning, N, M);
   int n = 0;
   PRH_CHECK(run_iterations(c, o->max_iterations, o->tolerance, [&](int l, unsigned long long *err, int *done) -> int {
-    const prh::WeightedArgs A{g->out_off, g->out_dst, w_out, c->wt_x[l & 1].p, c->wt_x[(l + 1) & 1].p,
-                              c->wt_x[(l + 1) & 1].p, c->wt_x[l & 1].p, term, c->wt_W.p, sc, o->damping};
+    const prh::WeightedArgs A{g->l_out_off, g->l_out_dst, c->wt_wl.p, c->wt_x[l & 1].p, c->wt_x[(l + 1) & 1].p,
+                              c->wt_x[(l + 1) & 1].p, c->wt_x[l & 1].p, c->wt_terml.p, c->wt_Wl.p, sc, o->damping};
     PRH_HIP(c, prh::launch_weighted_step(true, A, width, err, done, l, o->tolerance, hs));
     return PRH_SUCCESS;
   }, &n));
-  const double *lam_d = c->wt_x[n & 1].p;
-  double *gw = grad_w, *gs = grad_s;
-  if (!device) {
-    PRH_CHECK(ensure(c, c->wt_gw, std::max<uint64_t>(1, M)));
-    PRH_CHECK(ensure(c, c->wt_gs, N));
-    gw = c->wt_gw.p;
-    gs = c->wt_gs.p;
-  }
-  PRH_HIP(c, prh::launch_weighted_grad(g->out_off, g->out_dst, c->wt_W.p, rr, lam_d, term, o->damping, N, gw, gs, hs));
+  const double *lam_l = c->wt_x[n & 1].p;
+  PRH_CHECK(ensure(c, c->wt_gwl, std::max<uint64_t>(1, M)));
+  PRH_CHECK(ensure(c, c->wt_gsl, N));
+  PRH_HIP(c, prh::launch_weighted_grad(g->l_out_off, g->l_out_dst, c->wt_Wl.p, c->wt_rl.p, lam_l, c->wt_terml.p,
+                                       o->damping, N, c->wt_gwl.p, c->wt_gsl.p, hs));
   PRH_HIP(c, hipEventRecord(c->ev[2], hs));
-  const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  if (lam) PRH_HIP(c, hipMemcpyAsync(lam, lam_d, size_t(N) * 8, kind, hs));
-  if (!device) {
-    PRH_HIP(c, hipMemcpyAsync(grad_w, gw, size_t(M) * 8, kind, hs));
-    PRH_HIP(c, hipMemcpyAsync(grad_s, gs, size_t(N) * 8, kind, hs));
-  }
+  if (lam) PRH_CHECK(from_layout(c, lam_l, g->perm, N, lam, device, c->wt_z[0]));
+  PRH_CHECK(from_layout(c, c->wt_gwl.p, g->l_out_eid, M, grad_w, device, c->wt_outM));
+  PRH_CHECK(from_layout(c, c->wt_gsl.p, g->perm, N, grad_s, device, c->wt_outN));
   PRH_HIP(c, hipStreamSynchronize(hs));
   float t01 = 0, t12 = 0;
   PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
@@ -1463,13 +1493,11 @@ void prh_context_destroy(prh_context *c) {
     release(c, c->wt_x[i]);
     release(c, c->wt_z[i]);
   }
-  release(c, c->wt_W);
-  release(c, c->wt_term);
-  release(c, c->wt_r);
-  release(c, c->wt_gw);
-  release(c, c->wt_gs);
+  for (DevBuf<double> *b : {&c->wt_W, &c->wt_Wl, &c->wt_term, &c->wt_terml, &c->wt_r, &c->wt_rl, &c->wt_gwl,
+                            &c->wt_gsl, &c->wt_outM, &c->wt_outN})
+    release(c, *b);
   release(c, c->wt_wout);
-  release(c, c->wt_win);
+  release(c, c->wt_wl);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c->kev)
@@ -1575,7 +1603,8 @@ void prh_graph_destroy(prh_graph *g) {
   dev_free(c, g->outdeg, N * 4);
   dev_free(c, g->out_off, (N + 1) * 4);
   dev_free(c, g->out_dst, (M ? M : 4) * 4);
-  dev_free(c, g->in_eid, (M ? M : 4) * 4);
+  dev_free(c, g->l_in_eid, (M ? M : 4) * 4);
+  dev_free(c, g->l_out_eid, (M ? M : 4) * 4);
   dev_free(c, g->perm, N * 4);
   dev_free(c, g->inv, N * 4);
   dev_free(c, g->l_in_off, (N + 1) * 4);
@@ -1608,6 +1637,8 @@ int prh_solve_device(prh_context *c, const prh_graph *g, prh_method method, int 
 }
 
 // ---- autotuner ----------------------------------------------------------------------------------------------
+
+}  // extern "C"
 
 namespace {
 
@@ -1714,6 +1745,8 @@ bool write_cache(const char *path, const std::vector<CacheEntry> &entries) {
 }
 
 }  // namespace
+
+extern "C" {
 
 int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int iterations, prh_tuning *chosen,
                  double *ms_per_iteration, int *from_cache) {

@@ -276,6 +276,10 @@ hipError_t launch_weighted_step(bool adjoint, const WeightedArgs &A, int width, 
 hipError_t launch_out_weight_sum(const uint32_t *out_off, const float *w, uint32_t N, double *W, hipStream_t s);
 hipError_t launch_edge_map(const uint32_t *in_off, const uint32_t *in_src, const uint32_t *out_off,
                            const uint32_t *out_dst, uint32_t N, uint32_t *in_eid, int *missing, hipStream_t s);
+// eid[k] = raw out-CSR slot of slot k of a degree-ordered layout CSR (transpose: its in-CSR, else its out-CSR).
+hipError_t launch_layout_edge_map(const uint32_t *l_off, const uint32_t *l_idx, const uint32_t *perm,
+                                  const uint32_t *out_off, const uint32_t *out_dst, uint32_t N, bool transpose,
+                                  uint32_t *eid, int *missing, hipStream_t s);
 hipError_t launch_permute_weights(const float *w_out, const uint32_t *in_eid, size_t M, float *w_in, hipStream_t s);
 hipError_t launch_weighted_init(const double *start, const double *W, uint32_t N, bool adjoint, double *x, double *z,
                                 hipStream_t s);

@@ -4,7 +4,9 @@
 //   forward  (pull over the in-CSR):  r_v   <- (1-a) s_v + a * sum_{u in in(v)}  w_uv * (r_u / W_u)
 //   adjoint  (pull over the out-CSR): lam_u <- g_u + (a / W_u) * sum_{v in out(u)} w_uv * lam_v
 // i.e. (I - a P^T) r = (1-a) s and its transpose (I - a P) lam = g, both by the same Jacobi pull with the
-// convergence control of the rank update (fused L-inf, device-side stop). The gradient of L(r) follows without
+// convergence control of the rank update (fused L-inf, device-side stop). The solves run on the degree-ordered
+// layout (its in-CSR forward, its out-CSR for the adjoint), with the weights carried to layout slots through
+// layout_edge_map_kernel, so that the most-gathered values share cache lines as in the rank update. The gradient of L(r) follows without
 // storing per-edge terms: dL/ds = (1-a) lam and dL/dw_uv = (a r_u / W_u) (lam_v - (lam_u - g_u)/a), where
 // (lam_u - g_u)/a = sum_v P[u,v] lam_v is recomputed from the adjoint fixed point.
 #include "prh_device.hpp"
@@ -112,6 +114,33 @@ __global__ __launch_bounds__(kBlock) void edge_map_kernel(const uint32_t *in_off
   if (bad) atomicOr(missing, 1);
 }
 
+// The same map for a CSR of the degree-ordered layout (rows and ids in new numbering, perm: new -> old): eid[k] =
+// raw out-CSR slot of slot k. transpose: rows are destinations (the layout in-CSR), else sources (its out-CSR).
+__global__ __launch_bounds__(kBlock) void layout_edge_map_kernel(const uint32_t *l_off, const uint32_t *l_idx,
+                                                                 const uint32_t *perm, const uint32_t *out_off,
+                                                                 const uint32_t *out_dst, uint32_t N, int transpose,
+                                                                 uint32_t *eid, int *missing) {
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  bool bad = false;
+  for (uint32_t i = blockIdx.x * kWavesPerBlock + wv; i < N; i += gridDim.x * kWavesPerBlock) {
+    const uint32_t a = perm[i];
+    for (uint32_t k = l_off[i] + lane; k < l_off[i + 1]; k += kWave) {
+      const uint32_t b = perm[l_idx[k]];
+      const uint32_t u = transpose ? b : a, v = transpose ? a : b;
+      uint32_t lo = out_off[u], hi = out_off[u + 1];
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (out_dst[mid] < v) lo = mid + 1;
+        else hi = mid;
+      }
+      const bool found = lo < out_off[u + 1] && out_dst[lo] == v;
+      bad |= !found;
+      eid[k] = found ? lo : 0u;
+    }
+  }
+  if (bad) atomicOr(missing, 1);
+}
+
 __global__ __launch_bounds__(kBlock) void permute_weights_kernel(const float *w_out, const uint32_t *in_eid, size_t M,
                                                                  float *w_in) {
   for (size_t k = size_t(blockIdx.x) * kBlock + threadIdx.x; k < M; k += size_t(gridDim.x) * kBlock)
@@ -186,6 +215,15 @@ hipError_t launch_edge_map(const uint32_t *in_off, const uint32_t *in_src, const
   if (!N) return hipSuccess;
   edge_map_kernel<<<blocks(N, kWavesPerBlock, 16384), kBlock, 0, s>>>(in_off, in_src, out_off, out_dst, N, in_eid,
                                                                       missing);
+  return hipGetLastError();
+}
+
+hipError_t launch_layout_edge_map(const uint32_t *l_off, const uint32_t *l_idx, const uint32_t *perm,
+                                  const uint32_t *out_off, const uint32_t *out_dst, uint32_t N, bool transpose,
+                                  uint32_t *eid, int *missing, hipStream_t s) {
+  if (!N) return hipSuccess;
+  layout_edge_map_kernel<<<blocks(N, kWavesPerBlock, 16384), kBlock, 0, s>>>(l_off, l_idx, perm, out_off, out_dst, N,
+                                                                              transpose ? 1 : 0, eid, missing);
   return hipGetLastError();
 }
 
