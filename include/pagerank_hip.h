@@ -77,7 +77,7 @@ typedef struct prh_tuning {
   uint32_t huge_degree;   /* in-degree > this: split into chunks of this many edges, one workgroup per chunk,
                              summed by a second launch [16384] */
   int low_width;          /* lanes per vertex of in-degree <= switch_degree: 1, 2, 4, 8, 16 or 32; 0 = by degree
-                             band, W lanes for in-degree in (W, 2W] (weighted kernels: from |E|/|V|) [0] */
+                             band, W lanes for in-degree in (W, 2W] (the weighted solves always use bands) [0] */
   int check_interval;     /* iterations launched between host reads of the device convergence flag [8] */
   int source_slices;      /* Static / Naive-dynamic: 0 = row kernel; 1, 2, 4 or 8 = sum each row slice by slice,
                              source u in slice (u / 16) mod source_slices, slice x on the workgroups b with

@@ -53,6 +53,9 @@ struct prh_context {
   // weighted forward / adjoint, on the degree-ordered layout ("l": layout order, else the caller's order)
   DevBuf<double> wt_x[2], wt_z[2], wt_W, wt_Wl, wt_term, wt_terml, wt_r, wt_rl, wt_gwl, wt_gsl, wt_outM, wt_outN;
   DevBuf<float> wt_wout, wt_wl;
+  DevBuf<uint32_t> wt_chunks;       // heavy-row chunk tables of the weighted pull
+  DevBuf<uint32_t> wt_sort, wt_sort_tmp;  // degree sort of its low rows
+  DevBuf<double> wt_partial;        // their sums
   uint64_t *pinned = nullptr;       // small host-pinned readback area (kPinnedBytes)
   hipEvent_t ev[4] = {};
   bool ktiming = false;
@@ -168,12 +171,6 @@ void release(prh_context *c, DevBuf<T> &b) {
 }
 
 bool valid_method(int m) { return m >= PRH_STATIC && m <= PRH_PRUNE_DYNAMIC_FRONTIER; }
-
-int low_width_for(const prh_tuning &t, uint32_t N, uint64_t M) {
-  if (t.low_width) return t.low_width;
-  const double avg = N ? double(M) / N : 1.0;
-  return avg < 3 ? 1 : avg < 6 ? 2 : avg < 12 ? 4 : avg < 24 ? 8 : 16;
-}
 
 uint32_t cap_blocks(uint64_t n, uint64_t per, uint32_t cap) {
   const uint64_t g = (n + per - 1) / per;
@@ -1262,16 +1259,88 @@ int weighted_common(prh_context *c, const prh_graph *g, const float *w, const pr
   return PRH_SUCCESS;
 }
 
-// Two-way degree schedule (<= 64 | rest) of a CSR into `order`.
-int two_way_schedule(prh_context *c, const uint32_t *off, uint32_t N, DevBuf<uint32_t> &order, prh::Schedule &sc) {
+// The low rows of a weighted schedule (order[0, n_low), degree <= 64) sorted by degree, descending, and cut into
+// bands of W lanes per row for degree in (W, 2W] (W = 32 .. 2; W = 1 up to degree 2), each band on its own
+// blocks of about block_edges edges.
+int weighted_bands(prh_context *c, const uint32_t *off, DevBuf<uint32_t> &order, prh::WeightedArgs &A) {
+  hipStream_t hs = c->stream;
+  const uint32_t n = A.sched.n_low;
+  A.n_bands = 0;
+  if (!n) return PRH_SUCCESS;
+  size_t sort_bytes = 0;
+  PRH_CHECK(ensure(c, c->wt_sort, size_t(3) * n + 65));
+  uint32_t *key = c->wt_sort.p, *key_s = key + n, *val_s = key_s + n, *starts = val_s + n;
+  PRH_HIP(c, prh::sort_pairs_u32(nullptr, sort_bytes, key, key_s, order.p, val_s, n, 7, hs));
+  PRH_CHECK(ensure(c, c->wt_sort_tmp, sort_bytes / 4 + 1));
+  PRH_HIP(c, prh::launch_degree_keys(off, order.p, n, key, hs));
+  PRH_HIP(c, prh::sort_pairs_u32(c->wt_sort_tmp.p, sort_bytes, key, key_s, order.p, val_s, n, 7, hs));
+  PRH_HIP(c, hipMemcpyAsync(order.p, val_s, size_t(n) * 4, hipMemcpyDeviceToDevice, hs));
+  PRH_HIP(c, prh::launch_lower_bounds(key_s, n, 64, starts, hs));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, starts, 65 * 4, hipMemcpyDeviceToHost, hs));
+  PRH_HIP(c, hipStreamSynchronize(hs));
+  const uint32_t *lb = reinterpret_cast<const uint32_t *>(c->pinned);
+  auto above = [&](uint32_t t) { return lb[64 - t]; };  // rows with degree > t
+  const uint64_t edges_per_block = std::max<uint32_t>(c->tuning.block_edges, 256);
+  uint32_t blocks = 0, prev = 0;
+  for (int W = 32; W >= 1; W /= 2) {
+    const uint32_t end = W == 1 ? n : above(uint32_t(W));
+    if (end > prev) {
+      const uint64_t unit = uint64_t(prh::kWavesPerBlock) * (prh::kWave / W);
+      const uint64_t per = std::max<uint64_t>(unit, (edges_per_block / (2 * W) + unit - 1) / unit * unit);
+      A.band[A.n_bands++] = prh::RankSection{prh::kGroupRows, W, prev, end, blocks, uint32_t(per)};
+      blocks += uint32_t((end - prev + per - 1) / per);
+      prev = end;
+    }
+  }
+  return PRH_SUCCESS;
+}
+
+// Schedule of a weighted pull CSR: rows stably partitioned by degree into [<= 64 | <= kWeightedChunk | heavier]
+// (`order`), and the heavier rows cut into chunks of kWeightedChunk edges (tables in wt_chunks, sums in
+// wt_partial). Fills the schedule and chunk fields of A.
+int weighted_schedule(prh_context *c, const uint32_t *off, uint32_t N, DevBuf<uint32_t> &order, prh::WeightedArgs &A) {
+  hipStream_t hs = c->stream;
   PRH_CHECK(ensure(c, order, N));
   PRH_CHECK(ensure(c, c->tiles, size_t(3) * prh::partition_tiles(N)));
   PRH_CHECK(ensure(c, c->counts, 8));
-  PRH_HIP(c, prh::launch_partition(off, N, 64u, 0xFFFFFFFFu, order.p, c->tiles.p, c->counts.p, c->stream));
-  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->counts.p, 8, hipMemcpyDeviceToHost, c->stream));
-  PRH_HIP(c, hipStreamSynchronize(c->stream));
+  PRH_HIP(c, prh::launch_partition(off, N, 64u, prh::kWeightedChunk, order.p, c->tiles.p, c->counts.p, hs));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->counts.p, 12, hipMemcpyDeviceToHost, hs));
+  PRH_HIP(c, hipStreamSynchronize(hs));
   const uint32_t *cnt = reinterpret_cast<const uint32_t *>(c->pinned);
-  sc = prh::Schedule{order.p, cnt[0], cnt[1], 0};
+  A.sched = prh::Schedule{order.p, cnt[0], cnt[1], cnt[2]};
+  A.n_chunks = 0;
+  A.chunk_beg = A.chunk_end = A.heavy_first = nullptr;
+  A.partial = nullptr;
+  PRH_CHECK(weighted_bands(c, off, order, A));
+  const uint32_t nh = A.sched.n_huge;
+  if (!nh) return PRH_SUCCESS;
+  PRH_CHECK(ensure(c, c->wt_chunks, size_t(2) * nh));
+  PRH_HIP(c, prh::launch_gather_ranges(off, order.p + A.sched.n_low + A.sched.n_high, nh, c->wt_chunks.p,
+                                       c->wt_chunks.p + nh, hs));
+  std::vector<uint32_t> be(size_t(2) * nh);
+  PRH_HIP(c, hipMemcpyAsync(be.data(), c->wt_chunks.p, be.size() * 4, hipMemcpyDeviceToHost, hs));
+  PRH_HIP(c, hipStreamSynchronize(hs));
+  std::vector<uint32_t> cb, ce, first(size_t(nh) + 1);
+  for (uint32_t h = 0; h < nh; ++h) {
+    first[h] = uint32_t(cb.size());
+    for (uint64_t e = be[h]; e < be[nh + h]; e += prh::kWeightedChunk) {
+      cb.push_back(uint32_t(e));
+      ce.push_back(uint32_t(std::min<uint64_t>(be[nh + h], e + prh::kWeightedChunk)));
+    }
+  }
+  first[nh] = uint32_t(cb.size());
+  const size_t nc = cb.size();
+  PRH_CHECK(ensure(c, c->wt_chunks, 2 * nc + nh + 1));
+  PRH_HIP(c, hipMemcpyAsync(c->wt_chunks.p, cb.data(), nc * 4, hipMemcpyHostToDevice, hs));
+  PRH_HIP(c, hipMemcpyAsync(c->wt_chunks.p + nc, ce.data(), nc * 4, hipMemcpyHostToDevice, hs));
+  PRH_HIP(c, hipMemcpyAsync(c->wt_chunks.p + 2 * nc, first.data(), (size_t(nh) + 1) * 4, hipMemcpyHostToDevice, hs));
+  PRH_CHECK(ensure(c, c->wt_partial, nc));
+  PRH_HIP(c, hipStreamSynchronize(hs));
+  A.chunk_beg = c->wt_chunks.p;
+  A.chunk_end = c->wt_chunks.p + nc;
+  A.heavy_first = c->wt_chunks.p + 2 * nc;
+  A.partial = c->wt_partial.p;
+  A.n_chunks = uint32_t(nc);
   return PRH_SUCCESS;
 }
 
@@ -1308,17 +1377,25 @@ int weighted_forward_impl(prh_context *c, const prh_graph *g, const float *w, co
   PRH_CHECK(stage(c, s, N, device, c->wt_term, &term));
   PRH_CHECK(to_layout(c, g, term, c->wt_terml));
   PRH_HIP(c, prh::launch_permute_weights(w_out, g->l_in_eid, M, c->wt_wl.p, hs));
-  prh::Schedule sc{};
-  PRH_CHECK(two_way_schedule(c, g->l_in_off, N, c->sched_in, sc));
+  prh::WeightedArgs A0{};
+  PRH_CHECK(weighted_schedule(c, g->l_in_off, N, c->sched_in, A0));
   PRH_HIP(c, hipEventRecord(c->ev[0], hs));
   PRH_HIP(c, prh::launch_weighted_init(c->wt_terml.p, c->wt_Wl.p, N, false, c->wt_x[0].p, c->wt_z[0].p, hs));
   PRH_HIP(c, hipEventRecord(c->ev[1], hs));
-  const int width = low_width_for(c->tuning, N, M);
   int n = 0;
   PRH_CHECK(run_iterations(c, o->max_iterations, o->tolerance, [&](int l, unsigned long long *err, int *done) -> int {
-    const prh::WeightedArgs A{g->l_in_off, g->l_in_src, c->wt_wl.p, c->wt_z[l & 1].p, c->wt_x[(l + 1) & 1].p,
-                              c->wt_z[(l + 1) & 1].p, c->wt_x[l & 1].p, c->wt_terml.p, c->wt_Wl.p, sc, o->damping};
-    PRH_HIP(c, prh::launch_weighted_step(false, A, width, err, done, l, o->tolerance, hs));
+    prh::WeightedArgs A = A0;
+    A.off = g->l_in_off;
+    A.idx = g->l_in_src;
+    A.w = c->wt_wl.p;
+    A.z = c->wt_z[l & 1].p;
+    A.x_new = c->wt_x[(l + 1) & 1].p;
+    A.z_new = c->wt_z[(l + 1) & 1].p;
+    A.x_old = c->wt_x[l & 1].p;
+    A.term = c->wt_terml.p;
+    A.W = c->wt_Wl.p;
+    A.P = o->damping;
+    PRH_HIP(c, prh::launch_weighted_step(false, A, err, done, l, o->tolerance, hs));
     return PRH_SUCCESS;
   }, &n));
   PRH_HIP(c, hipEventRecord(c->ev[2], hs));
@@ -1347,17 +1424,25 @@ int weighted_adjoint_impl(prh_context *c, const prh_graph *g, const float *w, co
   PRH_CHECK(to_layout(c, g, term, c->wt_terml));
   PRH_CHECK(to_layout(c, g, rr, c->wt_rl));
   PRH_HIP(c, prh::launch_permute_weights(w_out, g->l_out_eid, M, c->wt_wl.p, hs));
-  prh::Schedule sc{};
-  PRH_CHECK(two_way_schedule(c, g->l_out_off, N, c->sched_out, sc));
+  prh::WeightedArgs A0{};
+  PRH_CHECK(weighted_schedule(c, g->l_out_off, N, c->sched_out, A0));
   PRH_HIP(c, hipEventRecord(c->ev[0], hs));
   PRH_HIP(c, prh::launch_weighted_init(c->wt_terml.p, c->wt_Wl.p, N, true, c->wt_x[0].p, c->wt_z[0].p, hs));
   PRH_HIP(c, hipEventRecord(c->ev[1], hs));
-  const int width = low_width_for(c->tuning, N, M);
   int n = 0;
   PRH_CHECK(run_iterations(c, o->max_iterations, o->tolerance, [&](int l, unsigned long long *err, int *done) -> int {
-    const prh::WeightedArgs A{g->l_out_off, g->l_out_dst, c->wt_wl.p, c->wt_x[l & 1].p, c->wt_x[(l + 1) & 1].p,
-                              c->wt_x[(l + 1) & 1].p, c->wt_x[l & 1].p, c->wt_terml.p, c->wt_Wl.p, sc, o->damping};
-    PRH_HIP(c, prh::launch_weighted_step(true, A, width, err, done, l, o->tolerance, hs));
+    prh::WeightedArgs A = A0;
+    A.off = g->l_out_off;
+    A.idx = g->l_out_dst;
+    A.w = c->wt_wl.p;
+    A.z = c->wt_x[l & 1].p;
+    A.x_new = c->wt_x[(l + 1) & 1].p;
+    A.z_new = c->wt_x[(l + 1) & 1].p;
+    A.x_old = c->wt_x[l & 1].p;
+    A.term = c->wt_terml.p;
+    A.W = c->wt_Wl.p;
+    A.P = o->damping;
+    PRH_HIP(c, prh::launch_weighted_step(true, A, err, done, l, o->tolerance, hs));
     return PRH_SUCCESS;
   }, &n));
   const double *lam_l = c->wt_x[n & 1].p;
@@ -1498,6 +1583,10 @@ void prh_context_destroy(prh_context *c) {
     release(c, *b);
   release(c, c->wt_wout);
   release(c, c->wt_wl);
+  release(c, c->wt_chunks);
+  release(c, c->wt_sort);
+  release(c, c->wt_sort_tmp);
+  release(c, c->wt_partial);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : c->kev)

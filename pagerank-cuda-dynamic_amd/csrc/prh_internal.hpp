@@ -259,6 +259,8 @@ hipError_t launch_ppr_iteration(const PprArgs &a, hipStream_t s);  // identical 
 hipError_t launch_ppr_collect(const PprArgs &a, double *out, hipStream_t s);
 
 // ---- weighted.hip ----
+constexpr int kWeightedBands = 6;  // lanes per low row: 32, 16, 8, 4, 2, 1
+
 struct WeightedArgs {
   const uint32_t *off, *idx;  // pull CSR: in-CSR (forward) or out-CSR (adjoint)
   const float *w;             // FP32 weights aligned with idx
@@ -267,12 +269,28 @@ struct WeightedArgs {
   const double *x_old;        // previous values (for the L-inf change)
   const double *term;         // teleport s (forward, scaled by 1-P in the kernel) or g (adjoint)
   const double *W;            // out-weight sums
-  Schedule sched;             // degree schedule of the pull CSR (two-way)
+  Schedule sched;             // degree schedule of the pull CSR: [<= 64 | <= kWeightedChunk | heavier]
   double P;
+  // Rows of the third class are split into chunks of at most kWeightedChunk edges: chunk i covers edges
+  // [chunk_beg[i], chunk_end[i]) and leaves its sum in partial[i]; heavy row h (order[n_low + n_high + h]) owns
+  // chunks [heavy_first[h], heavy_first[h + 1]).
+  const uint32_t *chunk_beg, *chunk_end, *heavy_first;
+  double *partial;
+  uint32_t n_chunks;
+  // The low rows (order[0, n_low)) sorted by degree, descending, in bands of equal lanes per row: band b covers
+  // order[band[b].begin, band[b].end) with band[b].width lanes per row, on blocks from band[b].block_begin,
+  // band[b].per_block rows each (kind unused).
+  RankSection band[kWeightedBands];
+  int n_bands;
 };
 
-hipError_t launch_weighted_step(bool adjoint, const WeightedArgs &A, int width, unsigned long long *err, int *done,
-                                int l, double tol, hipStream_t s);
+constexpr uint32_t kWeightedChunk = 4096;
+
+// One Jacobi step: the chunk sums, the wave rows, the low-row bands, then (with chunks) the heavy rows' finish.
+hipError_t launch_weighted_step(bool adjoint, const WeightedArgs &A, unsigned long long *err, int *done, int l,
+                                double tol, hipStream_t s);
+// key[i] = 64 - degree of row order[i], for the first n rows (degrees <= 64).
+hipError_t launch_degree_keys(const uint32_t *off, const uint32_t *order, uint32_t n, uint32_t *key, hipStream_t s);
 hipError_t launch_out_weight_sum(const uint32_t *out_off, const float *w, uint32_t N, double *W, hipStream_t s);
 hipError_t launch_edge_map(const uint32_t *in_off, const uint32_t *in_src, const uint32_t *out_off,
                            const uint32_t *out_dst, uint32_t N, uint32_t *in_eid, int *missing, hipStream_t s);

@@ -42,37 +42,84 @@ __device__ __forceinline__ double wgather(const WeightedArgs &A, uint32_t e, uin
   return s0 + s1;
 }
 
-// One Jacobi step: low-degree rows with W lanes each, the rest one wave per row; fused L-inf.
+// Rows order[rb .. re) of one band, W lanes per row, 64/W rows per wave per round.
 template <bool ADJOINT, int W>
+__device__ __forceinline__ void band_rows(const WeightedArgs &A, uint32_t rb, uint32_t re, double &emax) {
+  constexpr int G = kWave / W;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const int g = lane / W, j = lane % W;
+  for (uint32_t i0 = rb + w * G; i0 < re; i0 += kWavesPerBlock * G) {  // wave-uniform
+    const uint32_t i = i0 + g;
+    double s = 0.0;
+    uint32_t v = 0;
+    if (i < re) {
+      v = A.sched.order[i];
+      s = wgather<W>(A, A.off[v] + j, A.off[v + 1]);
+    }
+    s = group_sum<W>(s);
+    if (i < re && j == 0) emax = fmax(emax, wfinish<ADJOINT>(A, v, s));
+  }
+}
+
+// One Jacobi step. Blocks [0, n_chunks) sum one chunk of a heavy row each (its partial); the next nb_high blocks
+// take the rows of degree in (64, kWeightedChunk], a wave per row; the rest the low-row bands. Fused L-inf for the
+// rows finished here.
+template <bool ADJOINT>
 __global__ __launch_bounds__(kBlock) void weighted_step_kernel(WeightedArgs A, uint32_t nb_high, LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;
   __shared__ double lds[kWavesPerBlock];
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  if (blockIdx.x < A.n_chunks) {  // block-uniform
+    double s = group_sum<kWave>(wgather<kBlock>(A, A.chunk_beg[blockIdx.x] + threadIdx.x, A.chunk_end[blockIdx.x]));
+    if (lane == 0) lds[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) A.partial[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+    return;
+  }
+  const uint32_t b = blockIdx.x - A.n_chunks;
   double emax = 0.0;
-  if (blockIdx.x < nb_high) {
+  if (b < nb_high) {
     const uint32_t *order = A.sched.order + A.sched.n_low;
-    const uint32_t n = A.sched.n_high + A.sched.n_huge;
-    for (uint32_t i = blockIdx.x * kWavesPerBlock + w; i < n; i += nb_high * kWavesPerBlock) {
+    const uint32_t n = A.sched.n_high;
+    for (uint32_t i = b * kWavesPerBlock + w; i < n; i += nb_high * kWavesPerBlock) {
       const uint32_t v = order[i];
       double s = wgather<kWave>(A, A.off[v] + lane, A.off[v + 1]);
       s = group_sum<kWave>(s);
       if (lane == 0) emax = fmax(emax, wfinish<ADJOINT>(A, v, s));
     }
   } else {
-    constexpr int G = kWave / W;
-    const int g = lane / W, j = lane % W;
-    const uint32_t nb = gridDim.x - nb_high, n = A.sched.n_low;
-    for (uint32_t i0 = ((blockIdx.x - nb_high) * kWavesPerBlock + w) * G; i0 < n; i0 += nb * kWavesPerBlock * G) {
-      const uint32_t i = i0 + g;
-      double s = 0.0;
-      uint32_t v = 0;
-      if (i < n) {
-        v = A.sched.order[i];
-        s = wgather<W>(A, A.off[v] + j, A.off[v + 1]);
-      }
-      s = group_sum<W>(s);
-      if (i < n && j == 0) emax = fmax(emax, wfinish<ADJOINT>(A, v, s));
+    const uint32_t lb = b - nb_high;
+    int k = 0;
+    while (k + 1 < A.n_bands && lb >= A.band[k + 1].block_begin) ++k;
+    const RankSection &S = A.band[k];
+    const uint32_t rb = S.begin + (lb - S.block_begin) * S.per_block;
+    const uint32_t re = min(S.end, rb + S.per_block);
+    switch (S.width) {
+      case 32: band_rows<ADJOINT, 32>(A, rb, re, emax); break;
+      case 16: band_rows<ADJOINT, 16>(A, rb, re, emax); break;
+      case 8: band_rows<ADJOINT, 8>(A, rb, re, emax); break;
+      case 4: band_rows<ADJOINT, 4>(A, rb, re, emax); break;
+      case 2: band_rows<ADJOINT, 2>(A, rb, re, emax); break;
+      default: band_rows<ADJOINT, 1>(A, rb, re, emax); break;
     }
+  }
+  emax = block_max(emax, lds);
+  if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
+}
+
+// Heavy rows, after the step of the same iteration: a wave per row adds its chunk sums in chunk order.
+template <bool ADJOINT>
+__global__ __launch_bounds__(kBlock) void weighted_heavy_kernel(WeightedArgs A, LoopCtl ctl) {
+  if (!iteration_begin(ctl)) return;
+  __shared__ double lds[kWavesPerBlock];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const uint32_t *order = A.sched.order + A.sched.n_low + A.sched.n_high;
+  double emax = 0.0;
+  for (uint32_t h = blockIdx.x * kWavesPerBlock + w; h < A.sched.n_huge; h += gridDim.x * kWavesPerBlock) {
+    double s = 0.0;
+    for (uint32_t k = A.heavy_first[h] + lane; k < A.heavy_first[h + 1]; k += kWave) s += A.partial[k];
+    s = group_sum<kWave>(s);
+    if (lane == 0) emax = fmax(emax, wfinish<ADJOINT>(A, order[h], s));
   }
   emax = block_max(emax, lds);
   if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
@@ -177,31 +224,41 @@ inline uint32_t blocks(size_t n, size_t per, uint32_t cap) {
   return uint32_t(g < 1 ? 1 : (g > cap ? cap : g));
 }
 
-template <bool ADJOINT>
-hipError_t launch_step(const WeightedArgs &A, int width, const LoopCtl &ctl, hipStream_t s) {
-  const uint32_t nb_high = A.sched.n_high + A.sched.n_huge
-                               ? blocks(size_t(A.sched.n_high) + A.sched.n_huge, kWavesPerBlock, 16384) : 0;
-  const uint32_t nb_low = A.sched.n_low ? blocks(A.sched.n_low, size_t(kWavesPerBlock) * (kWave / width), 32768) : 0;
-  const uint32_t grid = nb_high + nb_low;
-  if (!grid) return hipSuccess;
-  switch (width) {
-    case 1: weighted_step_kernel<ADJOINT, 1><<<grid, kBlock, 0, s>>>(A, nb_high, ctl); break;
-    case 2: weighted_step_kernel<ADJOINT, 2><<<grid, kBlock, 0, s>>>(A, nb_high, ctl); break;
-    case 4: weighted_step_kernel<ADJOINT, 4><<<grid, kBlock, 0, s>>>(A, nb_high, ctl); break;
-    case 8: weighted_step_kernel<ADJOINT, 8><<<grid, kBlock, 0, s>>>(A, nb_high, ctl); break;
-    case 16: weighted_step_kernel<ADJOINT, 16><<<grid, kBlock, 0, s>>>(A, nb_high, ctl); break;
-    case 32: weighted_step_kernel<ADJOINT, 32><<<grid, kBlock, 0, s>>>(A, nb_high, ctl); break;
-    default: return hipErrorInvalidValue;
+// key[i] = 64 - degree of row order[i] (low rows only: degree <= 64), for the degree sort of the low rows.
+__global__ __launch_bounds__(kBlock) void degree_keys_kernel(const uint32_t *off, const uint32_t *order, uint32_t n,
+                                                             uint32_t *key) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const uint32_t v = order[i];
+    key[i] = 64u - (off[v + 1] - off[v]);
   }
+}
+
+template <bool ADJOINT>
+hipError_t launch_step(const WeightedArgs &A, const LoopCtl &ctl, hipStream_t s) {
+  const uint32_t nb_high = A.sched.n_high ? blocks(A.sched.n_high, kWavesPerBlock, 16384) : 0;
+  uint32_t nb_low = 0;
+  for (int k = 0; k < A.n_bands; ++k)
+    nb_low = max(nb_low, A.band[k].block_begin +
+                             (A.band[k].end - A.band[k].begin + A.band[k].per_block - 1) / A.band[k].per_block);
+  const uint32_t grid = A.n_chunks + nb_high + nb_low;
+  if (grid) weighted_step_kernel<ADJOINT><<<grid, kBlock, 0, s>>>(A, nb_high, ctl);
+  if (A.sched.n_huge)
+    weighted_heavy_kernel<ADJOINT><<<blocks(A.sched.n_huge, kWavesPerBlock, 4096), kBlock, 0, s>>>(A, ctl);
   return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_weighted_step(bool adjoint, const WeightedArgs &A, int width, unsigned long long *err, int *done,
-                                int l, double tol, hipStream_t s) {
+hipError_t launch_weighted_step(bool adjoint, const WeightedArgs &A, unsigned long long *err, int *done, int l,
+                                double tol, hipStream_t s) {
   const LoopCtl ctl{err, done, l, tol};
-  return adjoint ? launch_step<true>(A, width, ctl, s) : launch_step<false>(A, width, ctl, s);
+  return adjoint ? launch_step<true>(A, ctl, s) : launch_step<false>(A, ctl, s);
+}
+
+hipError_t launch_degree_keys(const uint32_t *off, const uint32_t *order, uint32_t n, uint32_t *key, hipStream_t s) {
+  if (!n) return hipSuccess;
+  degree_keys_kernel<<<blocks(n, kBlock, 8192), kBlock, 0, s>>>(off, order, n, key);
+  return hipGetLastError();
 }
 
 hipError_t launch_out_weight_sum(const uint32_t *out_off, const float *w, uint32_t N, double *W, hipStream_t s) {

@@ -33,11 +33,20 @@ def test_uniform_weights_equal_static(ctx):
     assert np.abs(r.ranks - st.ranks).max() <= 1e-9
 
 
-@pytest.mark.parametrize("scale", [11, 14])
-def test_forward_and_adjoint_match_oracle(ctx, oracle_mod, scale):
+def _star_case(n=12000, seed=6):
+    from conftest import star_graph
+    y = star_graph(n)
+    rng = np.random.default_rng(seed)
+    return y, rng.uniform(0.5, 1.5, y.M).astype(np.float32), rng.standard_normal(y.N), rng.dirichlet(np.ones(y.N))
+
+
+@pytest.mark.parametrize("case", ["rmat11", "rmat14", "star12000"])
+def test_forward_and_adjoint_match_oracle(ctx, oracle_mod, case):
+    """rmat11/rmat14: rows of every degree class; star12000: a hub whose in-row (forward) and out-row (adjoint)
+    hold 12000 edges, summed in chunks of 4096 by separate workgroups and combined by a second launch."""
     from pagerank_amd import weighted_adjoint, weighted_pagerank
-    y, w, c, s = _case(scale)
-    print(f"[adjoint] scale={scale} N={y.N} M={y.M}", flush=True)
+    y, w, c, s = _star_case() if case.startswith("star") else _case(int(case[4:]))
+    print(f"[adjoint] case={case} N={y.N} M={y.M}", flush=True)
     yg = ctx.upload(y)
     fw = weighted_pagerank(yg, w, s)
     r, it, W = oracle_mod.weighted_forward(y, w, s)
