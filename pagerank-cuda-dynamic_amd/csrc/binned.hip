@@ -276,7 +276,8 @@ __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, Binne
   CostAcc cost;
   double emax = 0.0;
   if (it.w == kWholeBin) {
-    for (uint32_t i = threadIdx.x; i < nb; i += kBinBlock) emax = fmax(emax, bin_finish<VARIANT>(A, b0 + i, sum[i], cost));
+    for (uint32_t i = threadIdx.x; i < nb; i += kBinBlock)
+      emax = fmax(emax, bin_finish<VARIANT>(A, b0 + i, sum[i], cost));
   } else {
     double *p = P.partial + size_t(it.w) * B;
     for (uint32_t i = threadIdx.x; i < nb; i += kBinBlock) p[i] = sum[i];

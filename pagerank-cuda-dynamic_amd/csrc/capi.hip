@@ -172,12 +172,6 @@ void release(prh_context *c, DevBuf<T> &b) {
 
 bool valid_method(int m) { return m >= PRH_STATIC && m <= PRH_PRUNE_DYNAMIC_FRONTIER; }
 
-uint32_t cap_blocks(uint64_t n, uint64_t per, uint32_t cap) {
-  const uint64_t g = (n + per - 1) / per;
-  return uint32_t(std::min<uint64_t>(g, cap));
-}
-
-
 // Number of rows with in-degree > x in the descending degree list.
 uint32_t rows_above(const std::vector<uint32_t> &deg, uint32_t x) {
   return uint32_t(std::lower_bound(deg.begin(), deg.end(), x, [](uint32_t a, uint32_t b) { return a > b; }) -

@@ -292,8 +292,6 @@ hipError_t launch_weighted_step(bool adjoint, const WeightedArgs &A, unsigned lo
 // key[i] = 64 - degree of row order[i], for the first n rows (degrees <= 64).
 hipError_t launch_degree_keys(const uint32_t *off, const uint32_t *order, uint32_t n, uint32_t *key, hipStream_t s);
 hipError_t launch_out_weight_sum(const uint32_t *out_off, const float *w, uint32_t N, double *W, hipStream_t s);
-hipError_t launch_edge_map(const uint32_t *in_off, const uint32_t *in_src, const uint32_t *out_off,
-                           const uint32_t *out_dst, uint32_t N, uint32_t *in_eid, int *missing, hipStream_t s);
 // eid[k] = raw out-CSR slot of slot k of a degree-ordered layout CSR (transpose: its in-CSR, else its out-CSR).
 hipError_t launch_layout_edge_map(const uint32_t *l_off, const uint32_t *l_idx, const uint32_t *perm,
                                   const uint32_t *out_off, const uint32_t *out_dst, uint32_t N, bool transpose,

@@ -3,12 +3,12 @@
 // Transition probabilities P[u,v] = w_uv / W_u with FP32 edge weights w and FP64 out-weight sums W_u.
 //   forward  (pull over the in-CSR):  r_v   <- (1-a) s_v + a * sum_{u in in(v)}  w_uv * (r_u / W_u)
 //   adjoint  (pull over the out-CSR): lam_u <- g_u + (a / W_u) * sum_{v in out(u)} w_uv * lam_v
-// i.e. (I - a P^T) r = (1-a) s and its transpose (I - a P) lam = g, both by the same Jacobi pull with the
-// convergence control of the rank update (fused L-inf, device-side stop). The solves run on the degree-ordered
-// layout (its in-CSR forward, its out-CSR for the adjoint), with the weights carried to layout slots through
-// layout_edge_map_kernel, so that the most-gathered values share cache lines as in the rank update. The gradient of L(r) follows without
-// storing per-edge terms: dL/ds = (1-a) lam and dL/dw_uv = (a r_u / W_u) (lam_v - (lam_u - g_u)/a), where
-// (lam_u - g_u)/a = sum_v P[u,v] lam_v is recomputed from the adjoint fixed point.
+// i.e. (I - a P^T) r = (1-a) s and its transpose (I - a P) lam = g, both by the same Jacobi pull with the convergence
+// control of the rank update (fused L-inf, device-side stop). The solves run on the degree-ordered layout (its in-CSR
+// forward, its out-CSR for the adjoint), with the weights carried to layout slots through layout_edge_map_kernel, so
+// that the most-gathered values share cache lines as in the rank update. The gradient of L(r) follows without storing
+// per-edge terms: dL/ds = (1-a) lam and dL/dw_uv = (a r_u / W_u) (lam_v - (lam_u - g_u)/a), where (lam_u - g_u)/a =
+// sum_v P[u,v] lam_v is recomputed from the adjoint fixed point.
 #include "prh_device.hpp"
 #include "prh_internal.hpp"
 
@@ -137,32 +137,9 @@ __global__ __launch_bounds__(kBlock) void out_weight_sum_kernel(const uint32_t *
   }
 }
 
-// in_eid[k] = out-CSR position of in-CSR slot k: for row v and source u = in_src[k], the index of v in u's
-// ascending out-row (binary search). One wave per in-row.
-__global__ __launch_bounds__(kBlock) void edge_map_kernel(const uint32_t *in_off, const uint32_t *in_src,
-                                                          const uint32_t *out_off, const uint32_t *out_dst, uint32_t N,
-                                                          uint32_t *in_eid, int *missing) {
-  const int wv = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  bool bad = false;
-  for (uint32_t v = blockIdx.x * kWavesPerBlock + wv; v < N; v += gridDim.x * kWavesPerBlock) {
-    for (uint32_t k = in_off[v] + lane; k < in_off[v + 1]; k += kWave) {
-      const uint32_t u = in_src[k];
-      uint32_t lo = out_off[u], hi = out_off[u + 1];
-      while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (out_dst[mid] < v) lo = mid + 1;
-        else hi = mid;
-      }
-      const bool found = lo < out_off[u + 1] && out_dst[lo] == v;
-      bad |= !found;
-      in_eid[k] = found ? lo : 0u;
-    }
-  }
-  if (bad) atomicOr(missing, 1);
-}
-
-// The same map for a CSR of the degree-ordered layout (rows and ids in new numbering, perm: new -> old): eid[k] =
-// raw out-CSR slot of slot k. transpose: rows are destinations (the layout in-CSR), else sources (its out-CSR).
+// For a CSR of the degree-ordered layout (rows and ids in new numbering, perm: new -> old): eid[k] = the raw
+// out-CSR slot of the same edge as slot k, by binary search in the source's ascending out-row. transpose: rows are
+// destinations (the layout in-CSR), else sources (its out-CSR).
 __global__ __launch_bounds__(kBlock) void layout_edge_map_kernel(const uint32_t *l_off, const uint32_t *l_idx,
                                                                  const uint32_t *perm, const uint32_t *out_off,
                                                                  const uint32_t *out_dst, uint32_t N, int transpose,
@@ -264,14 +241,6 @@ hipError_t launch_degree_keys(const uint32_t *off, const uint32_t *order, uint32
 hipError_t launch_out_weight_sum(const uint32_t *out_off, const float *w, uint32_t N, double *W, hipStream_t s) {
   if (!N) return hipSuccess;
   out_weight_sum_kernel<<<blocks(N, kWavesPerBlock, 16384), kBlock, 0, s>>>(out_off, w, N, W);
-  return hipGetLastError();
-}
-
-hipError_t launch_edge_map(const uint32_t *in_off, const uint32_t *in_src, const uint32_t *out_off,
-                           const uint32_t *out_dst, uint32_t N, uint32_t *in_eid, int *missing, hipStream_t s) {
-  if (!N) return hipSuccess;
-  edge_map_kernel<<<blocks(N, kWavesPerBlock, 16384), kBlock, 0, s>>>(in_off, in_src, out_off, out_dst, N, in_eid,
-                                                                      missing);
   return hipGetLastError();
 }
 

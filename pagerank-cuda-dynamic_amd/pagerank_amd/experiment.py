@@ -1,4 +1,5 @@
-"""Experiment runner on synthetic graphs and SNAP temporal files, following the reference's protocol and log format (main.cu:47-148).
+"""Experiment runner on synthetic graphs and SNAP temporal files, following the reference's protocol and log format
+(main.cu:47-148).
 
     python -m pagerank_amd.experiment dynamic --scale 22 --edge-factor 16 --fractions 1e-5,1e-4,1e-3 --batches 3
     python -m pagerank_amd.experiment temporal --file sx-stackoverflow.txt --rows 26019770 --size 63497050 \\

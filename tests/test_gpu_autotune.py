@@ -46,3 +46,25 @@ def test_autotune_choice_cache_and_parity(ctx, oracle_mod, tmp_path):
         assert np.abs(res2.ranks - ref2).max() <= 1e-9
     finally:
         ctx.set_tuning(old)
+
+
+def test_autotune_large_graph_considers_blocking(ctx, oracle_mod, tmp_path):
+    """From 2^20 edges on, the propagation-blocked plan (B = 4096, 8192, 16384) is among the candidates; whichever
+    plan wins, the cache records its block width and the tuned solve gives the oracle's ranks."""
+    from pagerank_amd import HostGraph, pagerankStaticHip
+    cache = tmp_path / "tuning.json"
+    old = ctx.get_tuning()
+    try:
+        g = HostGraph.rmat(16, 20, seed=13)
+        assert g.M >= 1 << 20
+        yg = ctx.upload(g, with_out=False)
+        print(f"[autotune-large] N={g.N} M={g.M}", flush=True)
+        t, ms, hit = ctx.autotune(yg, str(cache), iterations=4)
+        assert not hit and ms > 0 and t.blocking in (0, 4096, 8192, 16384)
+        (rec,) = json.loads(cache.read_text()).values()
+        assert rec["blocking"] == t.blocking
+        res = pagerankStaticHip(yg)
+        ref, it, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, g)
+        assert abs(res.iterations - it) <= 1 and np.abs(res.ranks - ref).max() <= 1e-9
+    finally:
+        ctx.set_tuning(old)
