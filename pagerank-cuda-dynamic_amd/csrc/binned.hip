@@ -106,8 +106,9 @@ __global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const d
   const uint32_t n = min(uint32_t(B), P.N - base);
   for (uint32_t i = threadIdx.x; i < n; i += kBinBlock) c[i] = cr[base + i];
   __syncthreads();
-  // Pairs of entries: thread t takes pairs p = t + m * kBinBlock of the even-aligned range (16-byte stores); a lone
-  // first or last entry is written on its own.
+  // Pairs of entries: thread t takes pairs p = t + m * kBinBlock of the even-aligned range (16-byte stores, with the
+  // non-temporal hint: X is read back only by the next kernel, and keeping it out of L2 helps both); a lone first
+  // or last entry is written on its own.
   uint32_t k0 = it.y, k1 = it.z;
   if (k0 & 1u) {
     if (threadIdx.x == 0) P.X[k0] = c[ld_nt(P.src16 + k0)];
@@ -125,14 +126,14 @@ __global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const d
     const uint32_t a0 = __builtin_nontemporal_load(s32 + p), a1 = __builtin_nontemporal_load(s32 + p + kBinBlock);
     const uint32_t a2 = __builtin_nontemporal_load(s32 + p + 2 * kBinBlock);
     const uint32_t a3 = __builtin_nontemporal_load(s32 + p + 3 * kBinBlock);
-    x2[p] = d2{c[a0 & 0xFFFFu], c[a0 >> 16]};
-    x2[p + kBinBlock] = d2{c[a1 & 0xFFFFu], c[a1 >> 16]};
-    x2[p + 2 * kBinBlock] = d2{c[a2 & 0xFFFFu], c[a2 >> 16]};
-    x2[p + 3 * kBinBlock] = d2{c[a3 & 0xFFFFu], c[a3 >> 16]};
+    __builtin_nontemporal_store(d2{c[a0 & 0xFFFFu], c[a0 >> 16]}, x2 + p);
+    __builtin_nontemporal_store(d2{c[a1 & 0xFFFFu], c[a1 >> 16]}, x2 + p + kBinBlock);
+    __builtin_nontemporal_store(d2{c[a2 & 0xFFFFu], c[a2 >> 16]}, x2 + p + 2 * kBinBlock);
+    __builtin_nontemporal_store(d2{c[a3 & 0xFFFFu], c[a3 >> 16]}, x2 + p + 3 * kBinBlock);
   }
   for (; p < p1; p += kBinBlock) {
     const uint32_t a = __builtin_nontemporal_load(s32 + p);
-    x2[p] = d2{c[a & 0xFFFFu], c[a >> 16]};
+    __builtin_nontemporal_store(d2{c[a & 0xFFFFu], c[a >> 16]}, x2 + p);
   }
 }
 
