@@ -41,9 +41,12 @@ __device__ __forceinline__ uint32_t ld_nt(const uint32_t *p) { return __builtin_
 __device__ __forceinline__ uint32_t ld_nt(const uint16_t *p) { return __builtin_nontemporal_load(p); }
 
 // Vertex v of a bin from its in-sum: updated (update.hpp), or carried over when a dynamic variant left it
-// unaffected; its extension costs are added to `cost` (frontier variants with A.stats). Returns the L-inf term.
+// unaffected; `extend` (frontier variants: an in-neighbour was in the last frontier) marks it affected first. Its
+// extension costs are added to `cost` (frontier variants with A.stats). Returns the L-inf term.
 template <int VARIANT>
-__device__ __forceinline__ double bin_finish(const RankArgs &A, uint32_t v, double cv, CostAcc &cost) {
+__device__ __forceinline__ double bin_finish(const RankArgs &A, uint32_t v, double cv, CostAcc &cost, bool extend) {
+  if constexpr (Traits<VARIANT>::frontier)
+    if (extend) A.vaff[v] = 1;
   const uint32_t d = A.deg[v];
   const double rv = A.r[v];
   const double crv = Traits<VARIANT>::dynamic ? A.cr[v] : 0.0;
@@ -97,14 +100,17 @@ __device__ __forceinline__ double bin_block_max(double x, double *lds) {
   return m;
 }
 
+// naff (frontier variants after their first iteration): the previous iteration's frontier flags, carried as the
+// sign of the contribution (contributions are positive), so that accumulate can extend the frontier.
 template <int B>
-__global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const double *cr, LoopCtl ctl) {
+__global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const double *cr, const uint8_t *naff,
+                                                           LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;
   __shared__ double c[B];
   const uint4 it = P.ex_item[blockIdx.x];  // block, first entry, end entry
   const uint32_t base = it.x * uint32_t(B);
   const uint32_t n = min(uint32_t(B), P.N - base);
-  for (uint32_t i = threadIdx.x; i < n; i += kBinBlock) c[i] = cr[base + i];
+  for (uint32_t i = threadIdx.x; i < n; i += kBinBlock) c[i] = naff && naff[base + i] ? -cr[base + i] : cr[base + i];
   __syncthreads();
   // Pairs of entries: thread t takes pairs p = t + m * kBinBlock of the even-aligned range (16-byte stores, with the
   // non-temporal hint: X is read back only by the next kernel, and keeping it out of L2 helps both); a lone first
@@ -216,8 +222,18 @@ __device__ __forceinline__ double pull_f64(double x, uint32_t src_lane) {
 // Adds one wave's 64 slots into the LDS sums. All lanes must be active; lanes past the item carry ok = false.
 // Lanes with the same destination as their left neighbour (hot destinations repeat within a destination-sorted
 // run) are summed first, so that each such segment costs one LDS addition instead of colliding on one address.
-__device__ __forceinline__ void add_slots(double *sum, uint32_t d, double x, bool ok) {
+// EXT (frontier variants): a negative slot comes from a frontier source; its destination's flag in ext is set and
+// its magnitude added.
+template <bool EXT>
+__device__ __forceinline__ void add_slots(double *sum, uint8_t *ext, uint32_t d, double x, bool ok) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
+  if constexpr (EXT) {
+    const bool neg = ok && x < 0.0;
+    if (__ballot(neg)) {
+      if (neg) ext[d] = 1;
+      x = fabs(x);
+    }
+  }
   const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
   if (__ballot(d == d0 || !ok) == ~0ull) {  // one destination for the whole wave
     const double t = group_sum<kWave>(ok ? x : 0.0);
@@ -238,14 +254,22 @@ __device__ __forceinline__ void add_slots(double *sum, uint32_t d, double x, boo
   if (tail) atomicAdd(&sum[d], h == 0 ? s : s - before);
 }
 
+// Frontier variants: a vertex with a frontier in-neighbour (flag set in ext) is marked affected before its update,
+// which is the reference's extension step (pagerankAffectedExtend*CukU, pagerankCuda.hxx:313-385) done by pull over
+// the slots already being read; a split bin's partial rows carry the flag as their sign (a flagged sum is positive).
 template <int B, int VARIANT>
 __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, BinnedPlan P, LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;
+  constexpr bool EXT = Traits<VARIANT>::frontier;
   __shared__ double sum[B];
+  __shared__ uint8_t ext[EXT ? B : 1];
   __shared__ double lds_max[kBinWaves];
   __shared__ double lds_cost[2 * kBinWaves];
   const uint4 it = P.ac_item[blockIdx.x];  // bin, first slot, end slot, partial row (kWholeBin: none)
-  for (uint32_t i = threadIdx.x; i < uint32_t(B); i += kBinBlock) sum[i] = 0.0;
+  for (uint32_t i = threadIdx.x; i < uint32_t(B); i += kBinBlock) {
+    sum[i] = 0.0;
+    if constexpr (EXT) ext[i] = 0;
+  }
   __syncthreads();
   const int w = threadIdx.x / kWave;
   const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -261,15 +285,15 @@ __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, Binne
     const double x0 = ok0 ? __builtin_nontemporal_load(P.X + k0) : 0.0;
     const double x1 = __builtin_nontemporal_load(P.X + k1), x2 = __builtin_nontemporal_load(P.X + k2);
     const double x3 = ok3 ? __builtin_nontemporal_load(P.X + k3) : 0.0;
-    add_slots(sum, d0, x0, ok0);
-    add_slots(sum, d1, x1, true);
-    add_slots(sum, d2, x2, true);
-    add_slots(sum, d3, x3, ok3);
+    add_slots<EXT>(sum, ext, d0, x0, ok0);
+    add_slots<EXT>(sum, ext, d1, x1, true);
+    add_slots<EXT>(sum, ext, d2, x2, true);
+    add_slots<EXT>(sum, ext, d3, x3, ok3);
   }
   for (; g < it.z; g += kBinBlock) {
     const uint32_t k = slot_entry(P, g, lane);
     const bool ok = g + lane >= it.y && g + lane < it.z;
-    add_slots(sum, ok ? ld_nt(P.dst16 + k) : 0u, ok ? __builtin_nontemporal_load(P.X + k) : 0.0, ok);
+    add_slots<EXT>(sum, ext, ok ? ld_nt(P.dst16 + k) : 0u, ok ? __builtin_nontemporal_load(P.X + k) : 0.0, ok);
   }
   __syncthreads();
   const uint32_t b0 = it.x * uint32_t(B);
@@ -278,10 +302,10 @@ __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, Binne
   double emax = 0.0;
   if (it.w == kWholeBin) {
     for (uint32_t i = threadIdx.x; i < nb; i += kBinBlock)
-      emax = fmax(emax, bin_finish<VARIANT>(A, b0 + i, sum[i], cost));
+      emax = fmax(emax, bin_finish<VARIANT>(A, b0 + i, sum[i], cost, EXT && ext[EXT ? i : 0]));
   } else {
     double *p = P.partial + size_t(it.w) * B;
-    for (uint32_t i = threadIdx.x; i < nb; i += kBinBlock) p[i] = sum[i];
+    for (uint32_t i = threadIdx.x; i < nb; i += kBinBlock) p[i] = EXT && ext[EXT ? i : 0] ? -sum[i] : sum[i];
   }
   bin_store_costs<VARIANT>(A, cost, blockIdx.x, lds_cost);
   emax = bin_block_max(emax, lds_max);
@@ -308,33 +332,40 @@ __global__ __launch_bounds__(kBinBlock) void finish_bins_kernel(RankArgs A, Binn
   const bool ok = i < uint32_t(P.B) && v < P.N;
   const double *p = P.partial + size_t(m.y) * P.B + i;
   double emax = 0.0;
+  // Partial rows carry the frontier flag as their sign (frontier variants): add magnitudes, OR the signs.
+  bool flag = false;
+  auto take = [&](double x) {
+    flag |= x < 0.0;
+    return fabs(x);
+  };
   if (!split) {
     if (ok) {
       double s0 = 0.0, s1 = 0.0;
       uint32_t k = 0;
       for (; k + 1 < m.z; k += 2) {
-        s0 += p[size_t(k) * P.B];
-        s1 += p[size_t(k + 1) * P.B];
+        s0 += take(p[size_t(k) * P.B]);
+        s1 += take(p[size_t(k + 1) * P.B]);
       }
-      if (k < m.z) s0 += p[size_t(k) * P.B];
-      emax = bin_finish<VARIANT>(A, v, s0 + s1, cost);
+      if (k < m.z) s0 += take(p[size_t(k) * P.B]);
+      emax = bin_finish<VARIANT>(A, v, s0 + s1, cost, flag);
     }
   } else {
     double s0 = 0.0, s1 = 0.0;
     if (ok) {
       uint32_t k = w;
       for (; k + kBinWaves < m.z; k += 2 * kBinWaves) {
-        s0 += p[size_t(k) * P.B];
-        s1 += p[size_t(k + kBinWaves) * P.B];
+        s0 += take(p[size_t(k) * P.B]);
+        s1 += take(p[size_t(k + kBinWaves) * P.B]);
       }
-      if (k < m.z) s0 += p[size_t(k) * P.B];
+      if (k < m.z) s0 += take(p[size_t(k) * P.B]);
     }
-    part[w][lane] = s0 + s1;
+    part[w][lane] = flag ? -(s0 + s1) : s0 + s1;  // a flagged sum is positive
     __syncthreads();
     if (w == 0 && ok) {
       double t = 0.0;
-      for (int q = 0; q < kBinWaves; ++q) t += part[q][lane];
-      emax = bin_finish<VARIANT>(A, v, t, cost);
+      flag = false;
+      for (int q = 0; q < kBinWaves; ++q) t += take(part[q][lane]);
+      emax = bin_finish<VARIANT>(A, v, t, cost, flag);
     }
   }
   bin_store_costs<VARIANT>(A, cost, P.n_ac + blockIdx.x, lds_cost);
@@ -343,8 +374,10 @@ __global__ __launch_bounds__(kBinBlock) void finish_bins_kernel(RankArgs A, Binn
 }
 
 template <int B, int VARIANT>
-hipError_t binned_update(const RankArgs &A, const BinnedPlan &P, const LoopCtl &ctl, hipStream_t s, hipEvent_t mark) {
-  if (P.n_ex) expand_kernel<B><<<P.n_ex, kBinBlock, 0, s>>>(P, A.cr, ctl);
+hipError_t binned_update(const RankArgs &A, const BinnedPlan &P, const LoopCtl &ctl, bool extend, hipStream_t s,
+                         hipEvent_t mark) {
+  const uint8_t *naff = Traits<VARIANT>::frontier && extend ? A.naff : nullptr;
+  if (P.n_ex) expand_kernel<B><<<P.n_ex, kBinBlock, 0, s>>>(P, A.cr, naff, ctl);
   accumulate_kernel<B, VARIANT><<<P.n_ac, kBinBlock, 0, s>>>(A, P, ctl);
   if (P.n_multi) finish_bins_kernel<VARIANT><<<P.n_multi, kBinBlock, 0, s>>>(A, P, ctl);
   if (mark) {
@@ -435,12 +468,12 @@ __global__ __launch_bounds__(kBlock) void lower_bounds_kernel(const uint32_t *ke
 
 template <int B>
 hipError_t binned_update_variant(int variant, const RankArgs &A, const BinnedPlan &P, const LoopCtl &ctl,
-                                 hipStream_t s, hipEvent_t mark) {
+                                 bool extend, hipStream_t s, hipEvent_t mark) {
   switch (variant) {
-    case kPlain: return binned_update<B, kPlain>(A, P, ctl, s, mark);
-    case kTraversal: return binned_update<B, kTraversal>(A, P, ctl, s, mark);
-    case kFrontier: return binned_update<B, kFrontier>(A, P, ctl, s, mark);
-    case kPrune: return binned_update<B, kPrune>(A, P, ctl, s, mark);
+    case kPlain: return binned_update<B, kPlain>(A, P, ctl, extend, s, mark);
+    case kTraversal: return binned_update<B, kTraversal>(A, P, ctl, extend, s, mark);
+    case kFrontier: return binned_update<B, kFrontier>(A, P, ctl, extend, s, mark);
+    case kPrune: return binned_update<B, kPrune>(A, P, ctl, extend, s, mark);
     default: return hipErrorInvalidValue;
   }
 }
@@ -448,14 +481,14 @@ hipError_t binned_update_variant(int variant, const RankArgs &A, const BinnedPla
 }  // namespace
 
 hipError_t launch_binned_update(int variant, const RankArgs &A, const BinnedPlan &P, unsigned long long *err,
-                                int *done, int l, double tol, hipStream_t s, hipEvent_t mark) {
+                                int *done, int l, double tol, bool extend, hipStream_t s, hipEvent_t mark) {
   const LoopCtl ctl{err, done, l, tol};
   switch (P.B) {
-    case 1024: return binned_update_variant<1024>(variant, A, P, ctl, s, mark);
-    case 2048: return binned_update_variant<2048>(variant, A, P, ctl, s, mark);
-    case 4096: return binned_update_variant<4096>(variant, A, P, ctl, s, mark);
-    case 8192: return binned_update_variant<8192>(variant, A, P, ctl, s, mark);
-    case 16384: return binned_update_variant<16384>(variant, A, P, ctl, s, mark);
+    case 1024: return binned_update_variant<1024>(variant, A, P, ctl, extend, s, mark);
+    case 2048: return binned_update_variant<2048>(variant, A, P, ctl, extend, s, mark);
+    case 4096: return binned_update_variant<4096>(variant, A, P, ctl, extend, s, mark);
+    case 8192: return binned_update_variant<8192>(variant, A, P, ctl, extend, s, mark);
+    case 16384: return binned_update_variant<16384>(variant, A, P, ctl, extend, s, mark);
     default: return hipErrorInvalidValue;
   }
 }

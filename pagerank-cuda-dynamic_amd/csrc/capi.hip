@@ -810,7 +810,7 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   // Extension direction (frontier variants): 0 push only, 1 chosen per iteration from the rank launch's costs,
   // 2 pull only. The pull form splits rows at in-degree 64 (a wave each) and 4 (8 lanes each).
   const int dir_mode = frontier ? c->tuning.extension_direction : 0;
-  const bool dir_auto = dir_mode == 1;
+  const bool dir_auto = dir_mode == 1 && !binned;
   // Per-workgroup extension costs: one slot per workgroup of the launches that finish vertices.
   const uint32_t stat_slots = binned ? g->binned.n_ac + g->binned.n_multi
                                      : plan.blocks + (plan.n_chunked ? prh::finish_chunked_grid(plan.n_chunked) : 0);
@@ -842,10 +842,12 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
                       o->prune_tolerance, dir_auto ? c->ext_stats.p : nullptr};
       if (c->ktiming) PRH_HIP(c, hipEventRecord(c->kev[2 * l], s));
       hipEvent_t mark = c->ktiming ? c->kev[2 * l + 1] : nullptr;
-      if (binned) PRH_HIP(c, prh::launch_binned_update(variant, A, g->binned, err, done, l, o->tolerance, s, mark));
+      if (binned)
+        PRH_HIP(c, prh::launch_binned_update(variant, A, g->binned, err, done, l, o->tolerance, frontier && l > 0, s,
+                                             mark));
       else if (sliced) PRH_HIP(c, prh::launch_sliced_update(A, g->sliced, err, done, l, o->tolerance, s, mark));
       else PRH_HIP(c, prh::launch_rank_update(variant, A, plan, err, done, l, o->tolerance, s, mark));
-      if (frontier) {
+      if (frontier && !binned) {  // the blocked plan extends inside the next iteration's update
         if (dir_auto) PRH_HIP(c, prh::launch_direction(c->ext_stats.p, stat_slots, dir, err, done, l, o->tolerance, s));
         prh::ExtendArgs E{c->vaff.p, c->naff.p, g->l_out_off, g->l_out_dst, out_s, nullptr, nullptr, dir};
         PRH_HIP(c, prh::launch_extend(E, err, done, l, o->tolerance, s));

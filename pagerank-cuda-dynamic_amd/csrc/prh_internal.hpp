@@ -200,10 +200,12 @@ hipError_t launch_gather_ranges(const uint32_t *off, const uint32_t *ids, size_t
                                 hipStream_t s);
 
 // ---- binned.hip ----
-// One iteration through the propagation-blocked plan: expand, accumulate, and (with split bins) finish. `mark`,
-// when non-null, is recorded after the last of them.
+// One iteration through the propagation-blocked plan: expand, accumulate, and (with split bins) finish. `extend`
+// (frontier variants, every iteration after the first): the extension of the previous iteration's frontier
+// (A.naff) happens inside this update, so no separate extension launch follows it. `mark`, when non-null, is
+// recorded after the last kernel.
 hipError_t launch_binned_update(int variant, const RankArgs &A, const BinnedPlan &P, unsigned long long *err,
-                                int *done, int l, double tol, hipStream_t s, hipEvent_t mark = nullptr);
+                                int *done, int l, double tol, bool extend, hipStream_t s, hipEvent_t mark = nullptr);
 // Build steps. key[e] = bin(row) * nblocks + block(src[e]), row[e] = row of in-CSR slot e.
 hipError_t launch_bin_keys(const uint32_t *off, const uint32_t *src, uint32_t N, uint32_t B, uint32_t nblocks,
                            uint32_t *key, uint32_t *row, hipStream_t s);
