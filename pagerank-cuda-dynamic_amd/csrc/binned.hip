@@ -45,6 +45,17 @@ __device__ __forceinline__ uint32_t ld_nt(const uint16_t *p) { return __builtin_
 // extension costs are added to `cost` (frontier variants with A.stats). Returns the L-inf term.
 template <int VARIANT>
 __device__ __forceinline__ double bin_finish(const RankArgs &A, uint32_t v, double cv, CostAcc &cost, bool extend) {
+  if constexpr (VARIANT == kWeightedForward) {  // r = (1-P) s + P sum, gathered value r / W
+    const double av = fma(A.P, cv, (1 - A.P) * A.term[v]);
+    A.a[v] = av;
+    A.ca[v] = av / A.W[v];
+    return fabs(av - A.r[v]);
+  } else if constexpr (VARIANT == kWeightedAdjoint) {  // lam = g + (P / W) sum, gathered value lam
+    const double av = fma(A.P / A.W[v], cv, A.term[v]);
+    A.a[v] = av;
+    A.ca[v] = av;
+    return fabs(av - A.r[v]);
+  }
   if constexpr (Traits<VARIANT>::frontier)
     if (extend) A.vaff[v] = 1;
   const uint32_t d = A.deg[v];
@@ -117,17 +128,26 @@ __global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const d
   // or last entry is written on its own.
   uint32_t k0 = it.y, k1 = it.z;
   if (k0 & 1u) {
-    if (threadIdx.x == 0) P.X[k0] = c[ld_nt(P.src16 + k0)];
+    if (threadIdx.x == 0) P.X[k0] = c[ld_nt(P.src16 + k0)] * (P.wx ? double(P.wx[k0]) : 1.0);
     ++k0;
   }
   if (k1 > k0 && (k1 & 1u)) {
-    if (threadIdx.x == 0) P.X[k1 - 1] = c[ld_nt(P.src16 + k1 - 1)];
+    if (threadIdx.x == 0) P.X[k1 - 1] = c[ld_nt(P.src16 + k1 - 1)] * (P.wx ? double(P.wx[k1 - 1]) : 1.0);
     --k1;
   }
   const uint32_t *s32 = reinterpret_cast<const uint32_t *>(P.src16);
   d2 *x2 = reinterpret_cast<d2 *>(P.X);
   const uint32_t p0 = k0 / 2, p1 = k1 / 2;
   uint32_t p = p0 + threadIdx.x;
+  if (P.wx) {  // weighted solves: contribution times the edge's weight
+    const float2 *w2 = reinterpret_cast<const float2 *>(P.wx);
+    for (; p < p1; p += kBinBlock) {
+      const uint32_t a = __builtin_nontemporal_load(s32 + p);
+      const float2 w = w2[p];
+      __builtin_nontemporal_store(d2{c[a & 0xFFFFu] * double(w.x), c[a >> 16] * double(w.y)}, x2 + p);
+    }
+    return;
+  }
   for (; p + 3 * kBinBlock < p1; p += 4 * kBinBlock) {
     const uint32_t a0 = __builtin_nontemporal_load(s32 + p), a1 = __builtin_nontemporal_load(s32 + p + kBinBlock);
     const uint32_t a2 = __builtin_nontemporal_load(s32 + p + 2 * kBinBlock);
@@ -452,6 +472,12 @@ __global__ __launch_bounds__(kBlock) void bin_runs_kernel(const uint32_t *flag, 
   }
 }
 
+__global__ __launch_bounds__(kBlock) void binned_weights_kernel(const float *w, const uint32_t *eid,
+                                                                const uint32_t *slot, size_t M, float *wx) {
+  for (size_t k = size_t(blockIdx.x) * kBlock + threadIdx.x; k < M; k += size_t(gridDim.x) * kBlock)
+    wx[k] = w[eid[slot[k]]];
+}
+
 // start[s] = first k with key[k] >= s, for s = 0 .. n (key sorted ascending).
 __global__ __launch_bounds__(kBlock) void lower_bounds_kernel(const uint32_t *key, size_t M, uint32_t n,
                                                               uint32_t *start) {
@@ -474,6 +500,8 @@ hipError_t binned_update_variant(int variant, const RankArgs &A, const BinnedPla
     case kTraversal: return binned_update<B, kTraversal>(A, P, ctl, extend, s, mark);
     case kFrontier: return binned_update<B, kFrontier>(A, P, ctl, extend, s, mark);
     case kPrune: return binned_update<B, kPrune>(A, P, ctl, extend, s, mark);
+    case kWeightedForward: return binned_update<B, kWeightedForward>(A, P, ctl, false, s, mark);
+    case kWeightedAdjoint: return binned_update<B, kWeightedAdjoint>(A, P, ctl, false, s, mark);
     default: return hipErrorInvalidValue;
   }
 }
@@ -524,6 +552,13 @@ hipError_t launch_bin_runs(const uint32_t *flag, const uint32_t *rid, const uint
                            uint32_t *run_x, uint32_t *grp_run, hipStream_t s) {
   if (!M) return hipSuccess;
   bin_runs_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(flag, rid, inv, M, run_j, run_x, grp_run);
+  return hipGetLastError();
+}
+
+hipError_t launch_binned_weights(const float *w, const uint32_t *eid, const uint32_t *slot, size_t M, float *wx,
+                                 hipStream_t s) {
+  if (!M) return hipSuccess;
+  binned_weights_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(w, eid, slot, M, wx);
   return hipGetLastError();
 }
 

@@ -97,6 +97,7 @@ struct prh_graph {
   // Propagation-blocked plan of the plain update, built on first use for the current block width.
   int binned_B = 0;
   bool binned_valid = false;
+  bool binned_slot = false;  // the plan keeps the X-entry -> CSR-slot map (weighted solves)
   prh::BinnedPlan binned{};
   std::vector<std::pair<void *, size_t>> binned_bufs;
 };
@@ -392,9 +393,9 @@ void free_binned(prh_context *c, prh_graph *g) {
 //   3. src16 / dst16 in X order, the run tables (first slot, first X entry; the run of every 64th slot);
 //   4. block starts in X order (lower bounds of the sorted key1) and bin starts from the in-degrees cut the two
 //      orders into items; split bins get partial rows.
-int build_binned_plan(prh_context *c, prh_graph *g) {
+int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot = false) {
   const int B = c->tuning.blocking;
-  if (g->binned_valid && g->binned_B == B) return PRH_SUCCESS;
+  if (g->binned_valid && g->binned_B == B && (g->binned_slot || !keep_slot)) return PRH_SUCCESS;
   hipStream_t s = c->stream;
   PRH_HIP(c, hipStreamSynchronize(s));
   free_binned(c, g);
@@ -465,6 +466,12 @@ int build_binned_plan(prh_context *c, prh_graph *g) {
     PRH_HIP(c, prh::scan_u32(work, wb, row, rid, M, s));
     wb = work_bytes;
     PRH_HIP(c, prh::sort_pairs_u32(work, wb, key, k1s, nullptr, perm1, M, key1_bits, s));
+    if (keep_slot) {  // CSR slot of X entry k = perm2[perm1[k]]
+      uint32_t *slot;
+      PRH_CHECK(keep(&slot, M * 4));
+      PRH_HIP(c, prh::launch_gather_u32(perm2, perm1, M, slot, s));
+      P.slot = slot;
+    }
     // perm2 is free again: it takes inv (slot -> X entry).
     PRH_HIP(c, prh::launch_bin_expand(perm1, u2, dst, M, uint32_t(B), src16, dst16, perm2, s));
     PRH_HIP(c, prh::launch_lower_bounds(k1s, M, uint32_t(nb), starts, s));
@@ -542,6 +549,7 @@ int build_binned_plan(prh_context *c, prh_graph *g) {
   g->binned = P;
   g->binned_B = B;
   g->binned_valid = true;
+  g->binned_slot = keep_slot;
   return PRH_SUCCESS;
 }
 
@@ -1372,6 +1380,33 @@ int weighted_forward_impl(prh_context *c, const prh_graph *g, const float *w, co
   const double *term = nullptr;
   PRH_CHECK(stage(c, s, N, device, c->wt_term, &term));
   PRH_CHECK(to_layout(c, g, term, c->wt_terml));
+  if (c->tuning.blocking > 0) {  // through the propagation-blocked plan of the layout in-CSR
+    prh_graph *gm = const_cast<prh_graph *>(g);
+    PRH_CHECK(build_binned_plan(c, gm, true));
+    PRH_HIP(c, prh::launch_binned_weights(w_out, g->l_in_eid, g->binned.slot, M, c->wt_wl.p, hs));
+    prh::BinnedPlan plan = g->binned;
+    plan.wx = c->wt_wl.p;
+    PRH_HIP(c, hipEventRecord(c->ev[0], hs));
+    PRH_HIP(c, prh::launch_weighted_init(c->wt_terml.p, c->wt_Wl.p, N, false, c->wt_x[0].p, c->wt_z[0].p, hs));
+    PRH_HIP(c, hipEventRecord(c->ev[1], hs));
+    int n = 0;
+    PRH_CHECK(run_iterations(c, o->max_iterations, o->tolerance, [&](int l, unsigned long long *err, int *done) -> int {
+      prh::RankArgs A{c->wt_x[(l + 1) & 1].p, c->wt_x[l & 1].p, c->wt_z[(l + 1) & 1].p, c->wt_z[l & 1].p, nullptr,
+                      nullptr, g->l_in_off, g->l_in_src, nullptr, 0.0, o->damping, 0.0, 0.0, nullptr};
+      A.term = c->wt_terml.p;
+      A.W = c->wt_Wl.p;
+      PRH_HIP(c, prh::launch_binned_update(prh::kWeightedForward, A, plan, err, done, l, o->tolerance, false, hs));
+      return PRH_SUCCESS;
+    }, &n));
+    PRH_HIP(c, hipEventRecord(c->ev[2], hs));
+    PRH_CHECK(from_layout(c, c->wt_x[n & 1].p, g->perm, N, ranks, device, c->wt_outN));
+    PRH_HIP(c, hipStreamSynchronize(hs));
+    float t01 = 0, t12 = 0;
+    PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
+    PRH_HIP(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+    if (st) *st = prh_stats{n, t01 + t12, t01, 0.0f, t12};
+    return PRH_SUCCESS;
+  }
   PRH_HIP(c, prh::launch_permute_weights(w_out, g->l_in_eid, M, c->wt_wl.p, hs));
   prh::WeightedArgs A0{};
   PRH_CHECK(weighted_schedule(c, g->l_in_off, N, c->sched_in, A0));

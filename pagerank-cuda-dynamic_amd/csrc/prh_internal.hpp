@@ -24,6 +24,10 @@ enum Variant : int {
   kTraversal = 1,   // DYNAMIC: unaffected vertices copied through (Dynamic Traversal)
   kFrontier = 2,    // DYNAMIC + FRONTIER (Dynamic Frontier)
   kPrune = 3,       // DYNAMIC + FRONTIER + PRUNE (Dynamic Frontier with Pruning)
+  // Weighted solves through the propagation-blocked plan (no reference counterpart): the forward
+  // r_v = (1-P) s_v + P sum_u w_uv r_u / W_u and the adjoint lam_u = g_u + (P / W_u) sum_v w_uv lam_v.
+  kWeightedForward = 4,
+  kWeightedAdjoint = 5,
 };
 
 // Schedule of one CSR: vertex ids stably partitioned by degree into [low | high | huge] (see partition kernels).
@@ -76,6 +80,9 @@ struct RankArgs {
   // Frontier variants, optional: per workgroup, [2 b] = in-edges of the rows left unaffected (the cost of pulling
   // the next extension) and [2 b + 1] = out-edges of the rows put in the frontier (the cost of pushing it).
   double *stats;
+  // Weighted variants: the teleport share s (forward) or the seed g (adjoint), and the out-weight sums W; there a
+  // holds the new values, ca the new gathered values (a / W forward, a adjoint), cr the old ones.
+  const double *term = nullptr, *W = nullptr;
 };
 
 struct ExtendArgs {
@@ -160,6 +167,8 @@ struct BinnedPlan {
   const uint4 *multi;      // [n_multi] finish units
   double *partial;         // [partial rows][B]
   uint32_t n_ex, n_ac, n_multi;
+  const uint32_t *slot;    // [M] CSR slot of X entry k (kept for the weighted solves), else null
+  const float *wx;         // [M] weighted solves: the edge weight of X entry k, applied in expand; else null
 };
 
 // ---- partition.hip ----
@@ -206,6 +215,9 @@ hipError_t launch_gather_ranges(const uint32_t *off, const uint32_t *ids, size_t
 // recorded after the last kernel.
 hipError_t launch_binned_update(int variant, const RankArgs &A, const BinnedPlan &P, unsigned long long *err,
                                 int *done, int l, double tol, bool extend, hipStream_t s, hipEvent_t mark = nullptr);
+// wx[k] = w[eid[slot[k]]]: edge weights (raw out-CSR order) carried to X order through the layout edge map.
+hipError_t launch_binned_weights(const float *w, const uint32_t *eid, const uint32_t *slot, size_t M, float *wx,
+                                 hipStream_t s);
 // Build steps. key[e] = bin(row) * nblocks + block(src[e]), row[e] = row of in-CSR slot e.
 hipError_t launch_bin_keys(const uint32_t *off, const uint32_t *src, uint32_t N, uint32_t B, uint32_t nblocks,
                            uint32_t *key, uint32_t *row, hipStream_t s);

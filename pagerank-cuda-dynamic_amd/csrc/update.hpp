@@ -13,8 +13,9 @@ namespace prh {
 
 template <int VARIANT>
 struct Traits {
-  static constexpr bool dynamic = VARIANT >= kTraversal;
-  static constexpr bool frontier = VARIANT >= kFrontier;
+  static constexpr bool weighted = VARIANT >= kWeightedForward;
+  static constexpr bool dynamic = !weighted && VARIANT >= kTraversal;
+  static constexpr bool frontier = !weighted && VARIANT >= kFrontier;
   static constexpr bool prune = VARIANT == kPrune;
 };
 

@@ -40,13 +40,25 @@ def _star_case(n=12000, seed=6):
     return y, rng.uniform(0.5, 1.5, y.M).astype(np.float32), rng.standard_normal(y.N), rng.dirichlet(np.ones(y.N))
 
 
+@pytest.mark.parametrize("blocking", [0, 1024], ids=["rows", "blocked"])
 @pytest.mark.parametrize("case", ["rmat11", "rmat14", "star12000"])
-def test_forward_and_adjoint_match_oracle(ctx, oracle_mod, case):
+def test_forward_and_adjoint_match_oracle(ctx, oracle_mod, case, blocking):
     """rmat11/rmat14: rows of every degree class; star12000: a hub whose in-row (forward) and out-row (adjoint)
-    hold 12000 edges, summed in chunks of 4096 by separate workgroups and combined by a second launch."""
-    from pagerank_amd import weighted_adjoint, weighted_pagerank
+    hold 12000 edges, summed in chunks of 4096 by separate workgroups and combined by a second launch. blocked: the
+    solves run through the propagation-blocked plan (B = 1024), the edge weight applied in expand."""
+    from pagerank_amd import Tuning
     y, w, c, s = _star_case() if case.startswith("star") else _case(int(case[4:]))
-    print(f"[adjoint] case={case} N={y.N} M={y.M}", flush=True)
+    print(f"[adjoint] case={case} blocking={blocking} N={y.N} M={y.M}", flush=True)
+    old = ctx.get_tuning()
+    ctx.set_tuning(Tuning(**{**old.__dict__, "blocking": blocking}))
+    try:
+        _forward_and_adjoint(ctx, oracle_mod, y, w, c, s)
+    finally:
+        ctx.set_tuning(old)
+
+
+def _forward_and_adjoint(ctx, oracle_mod, y, w, c, s):
+    from pagerank_amd import weighted_adjoint, weighted_pagerank
     yg = ctx.upload(y)
     fw = weighted_pagerank(yg, w, s)
     r, it, W = oracle_mod.weighted_forward(y, w, s)
