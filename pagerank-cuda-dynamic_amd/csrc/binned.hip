@@ -352,11 +352,16 @@ __global__ __launch_bounds__(kBinBlock) void finish_bins_kernel(RankArgs A, Binn
   const bool ok = i < uint32_t(P.B) && v < P.N;
   const double *p = P.partial + size_t(m.y) * P.B + i;
   double emax = 0.0;
-  // Partial rows carry the frontier flag as their sign (frontier variants): add magnitudes, OR the signs.
+  // Partial rows carry the frontier flag as their sign (frontier variants): add magnitudes, OR the signs. Other
+  // variants add them as they are (the weighted adjoint's sums take either sign).
   bool flag = false;
   auto take = [&](double x) {
-    flag |= x < 0.0;
-    return fabs(x);
+    if constexpr (Traits<VARIANT>::frontier) {
+      flag |= x < 0.0;
+      return fabs(x);
+    } else {
+      return x;
+    }
   };
   if (!split) {
     if (ok) {

@@ -64,6 +64,14 @@ struct prh_context {
   double k_ms = 0.0;
 };
 
+struct BinnedCache {
+  int B = 0;
+  bool valid = false;
+  bool slot = false;  // the plan keeps the X-entry -> CSR-slot map (weighted solves)
+  prh::BinnedPlan plan{};
+  std::vector<std::pair<void *, size_t>> bufs;  // its device allocations
+};
+
 struct prh_graph {
   prh_context *ctx = nullptr;
   uint32_t N = 0;
@@ -94,12 +102,9 @@ struct prh_graph {
   bool sliced_valid = false;
   prh::SlicedPlan sliced{};
   std::vector<std::pair<void *, size_t>> sliced_bufs;  // its device allocations
-  // Propagation-blocked plan of the plain update, built on first use for the current block width.
-  int binned_B = 0;
-  bool binned_valid = false;
-  bool binned_slot = false;  // the plan keeps the X-entry -> CSR-slot map (weighted solves)
-  prh::BinnedPlan binned{};
-  std::vector<std::pair<void *, size_t>> binned_bufs;
+  // Propagation-blocked plans, built on first use for the current block width: of the layout in-CSR (the rank
+  // update, the weighted forward) and of its out-CSR (the weighted adjoint).
+  BinnedCache bin, bin_t;
 };
 
 namespace {
@@ -379,10 +384,10 @@ int build_sliced_plan(prh_context *c, prh_graph *g) {
   return PRH_SUCCESS;
 }
 
-void free_binned(prh_context *c, prh_graph *g) {
-  for (auto &b : g->binned_bufs) dev_free(c, b.first, b.second);
-  g->binned_bufs.clear();
-  g->binned_valid = false;
+void free_binned(prh_context *c, BinnedCache &bc) {
+  for (auto &b : bc.bufs) dev_free(c, b.first, b.second);
+  bc.bufs.clear();
+  bc.valid = false;
 }
 
 // Propagation-blocked plan (prh::BinnedPlan) for blocks and bins of B = tuning.blocking layout rows:
@@ -393,14 +398,14 @@ void free_binned(prh_context *c, prh_graph *g) {
 //   3. src16 / dst16 in X order, the run tables (first slot, first X entry; the run of every 64th slot);
 //   4. block starts in X order (lower bounds of the sorted key1) and bin starts from the in-degrees cut the two
 //      orders into items; split bins get partial rows.
-int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot = false) {
+// Builds the plan of the pull CSR (off, idx) with rows of degree rowdeg (host), N rows, M slots, into bc.
+int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, const uint32_t *idx,
+                 const std::vector<uint32_t> &rowdeg, bool keep_slot, BinnedCache &bc) {
   const int B = c->tuning.blocking;
-  if (g->binned_valid && g->binned_B == B && (g->binned_slot || !keep_slot)) return PRH_SUCCESS;
+  if (bc.valid && bc.B == B && (bc.slot || !keep_slot)) return PRH_SUCCESS;
   hipStream_t s = c->stream;
   PRH_HIP(c, hipStreamSynchronize(s));
-  free_binned(c, g);
-  const uint32_t N = g->N;
-  const uint64_t M = g->M;
+  free_binned(c, bc);
   const uint64_t nb = (uint64_t(N) + B - 1) / B;
   if (nb * nb > (uint64_t(1) << 32))
     return fail(c, PRH_ERROR_UNSUPPORTED, "blocking %d: %llu blocks per side exceed 32-bit tile keys", B,
@@ -427,7 +432,7 @@ int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot = false) {
   auto keep = [&](auto **p, size_t bytes) -> int {
     void *q = nullptr;
     PRH_CHECK(dev_alloc(c, &q, bytes ? bytes : 16));
-    g->binned_bufs.emplace_back(q, bytes ? bytes : 16);
+    bc.bufs.emplace_back(q, bytes ? bytes : 16);
     *p = static_cast<std::remove_pointer_t<decltype(p)>>(q);
     return PRH_SUCCESS;
   };
@@ -457,9 +462,9 @@ int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot = false) {
     uint8_t *work;
     PRH_CHECK(temp(&work, work_bytes));
     size_t wb = work_bytes;
-    PRH_HIP(c, prh::launch_bin_keys(g->l_in_off, g->l_in_src, N, uint32_t(B), uint32_t(nb), key, row, s));
+    PRH_HIP(c, prh::launch_bin_keys(off, idx, N, uint32_t(B), uint32_t(nb), key, row, s));
     PRH_HIP(c, prh::sort_pairs_u32(work, wb, key, k2s, nullptr, perm2, M, key2_bits, s));
-    PRH_HIP(c, prh::launch_bin_slots(perm2, g->l_in_src, row, M, uint32_t(B), dst, u2, key, s));
+    PRH_HIP(c, prh::launch_bin_slots(perm2, idx, row, M, uint32_t(B), dst, u2, key, s));
     // row now holds the run flags, rid their exclusive scan; key holds key1.
     PRH_HIP(c, prh::launch_run_flags(k2s, M, row, s));
     wb = work_bytes;
@@ -502,7 +507,7 @@ int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot = false) {
   for (uint64_t b = 0; b < nb; ++b) {
     const uint64_t r1 = std::min<uint64_t>(N, (b + 1) * uint64_t(B));
     uint64_t e = 0;
-    for (uint64_t r = b * uint64_t(B); r < r1; ++r) e += g->sorted_deg[r];
+    for (uint64_t r = b * uint64_t(B); r < r1; ++r) e += rowdeg[r];
     const uint64_t n = std::max<uint64_t>(1, (e + span - 1) / span);
     if (n == 1) {
       ac.push_back(make_uint4(uint32_t(b), uint32_t(j), uint32_t(j + e), prh::kWholeBin));
@@ -546,11 +551,24 @@ int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot = false) {
   P.n_ac = uint32_t(ac.size());
   P.n_multi = uint32_t(multi.size());
   PRH_HIP(c, hipStreamSynchronize(s));
-  g->binned = P;
-  g->binned_B = B;
-  g->binned_valid = true;
-  g->binned_slot = keep_slot;
+  bc.plan = P;
+  bc.B = B;
+  bc.valid = true;
+  bc.slot = keep_slot;
   return PRH_SUCCESS;
+}
+
+// The plan of the layout in-CSR (rows in in-degree order).
+int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot = false) {
+  return build_binned(c, g->N, g->M, g->l_in_off, g->l_in_src, g->sorted_deg, keep_slot, g->bin);
+}
+
+// The plan of the layout out-CSR (the transpose: rows are sources), for the weighted adjoint.
+int build_binned_plan_t(prh_context *c, prh_graph *g) {
+  if (g->bin_t.valid && g->bin_t.B == c->tuning.blocking) return PRH_SUCCESS;
+  std::vector<uint32_t> outdeg(g->N);
+  if (g->N) PRH_HIP(c, hipMemcpy(outdeg.data(), g->l_outdeg, size_t(g->N) * 4, hipMemcpyDeviceToHost));
+  return build_binned(c, g->N, g->M, g->l_out_off, g->l_out_dst, outdeg, true, g->bin_t);
 }
 
 // Rank-kernel plan over the degree-ordered layout for the context's tuning (cached in the graph):
@@ -820,7 +838,7 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   const int dir_mode = frontier ? c->tuning.extension_direction : 0;
   const bool dir_auto = dir_mode == 1 && !binned;
   // Per-workgroup extension costs: one slot per workgroup of the launches that finish vertices.
-  const uint32_t stat_slots = binned ? g->binned.n_ac + g->binned.n_multi
+  const uint32_t stat_slots = binned ? g->bin.plan.n_ac + g->bin.plan.n_multi
                                      : plan.blocks + (plan.n_chunked ? prh::finish_chunked_grid(plan.n_chunked) : 0);
   if (dir_auto) PRH_CHECK(ensure(c, c->ext_stats, 2 * size_t(std::max<uint32_t>(stat_slots, 1))));
   int *dir = frontier ? c->flags.p + 2 : nullptr;
@@ -851,7 +869,7 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
       if (c->ktiming) PRH_HIP(c, hipEventRecord(c->kev[2 * l], s));
       hipEvent_t mark = c->ktiming ? c->kev[2 * l + 1] : nullptr;
       if (binned)
-        PRH_HIP(c, prh::launch_binned_update(variant, A, g->binned, err, done, l, o->tolerance, frontier && l > 0, s,
+        PRH_HIP(c, prh::launch_binned_update(variant, A, g->bin.plan, err, done, l, o->tolerance, frontier && l > 0, s,
                                              mark));
       else if (sliced) PRH_HIP(c, prh::launch_sliced_update(A, g->sliced, err, done, l, o->tolerance, s, mark));
       else PRH_HIP(c, prh::launch_rank_update(variant, A, plan, err, done, l, o->tolerance, s, mark));
@@ -1383,8 +1401,8 @@ int weighted_forward_impl(prh_context *c, const prh_graph *g, const float *w, co
   if (c->tuning.blocking > 0) {  // through the propagation-blocked plan of the layout in-CSR
     prh_graph *gm = const_cast<prh_graph *>(g);
     PRH_CHECK(build_binned_plan(c, gm, true));
-    PRH_HIP(c, prh::launch_binned_weights(w_out, g->l_in_eid, g->binned.slot, M, c->wt_wl.p, hs));
-    prh::BinnedPlan plan = g->binned;
+    PRH_HIP(c, prh::launch_binned_weights(w_out, g->l_in_eid, g->bin.plan.slot, M, c->wt_wl.p, hs));
+    prh::BinnedPlan plan = g->bin.plan;
     plan.wx = c->wt_wl.p;
     PRH_HIP(c, hipEventRecord(c->ev[0], hs));
     PRH_HIP(c, prh::launch_weighted_init(c->wt_terml.p, c->wt_Wl.p, N, false, c->wt_x[0].p, c->wt_z[0].p, hs));
@@ -1454,14 +1472,33 @@ int weighted_adjoint_impl(prh_context *c, const prh_graph *g, const float *w, co
   PRH_CHECK(stage(c, r, N, device, c->wt_r, &rr));
   PRH_CHECK(to_layout(c, g, term, c->wt_terml));
   PRH_CHECK(to_layout(c, g, rr, c->wt_rl));
-  PRH_HIP(c, prh::launch_permute_weights(w_out, g->l_out_eid, M, c->wt_wl.p, hs));
+  const bool blocked = c->tuning.blocking > 0;  // through the propagation-blocked plan of the layout out-CSR
+  prh::BinnedPlan plan{};
   prh::WeightedArgs A0{};
-  PRH_CHECK(weighted_schedule(c, g->l_out_off, N, c->sched_out, A0));
+  if (blocked) {
+    prh_graph *gm = const_cast<prh_graph *>(g);
+    PRH_CHECK(build_binned_plan_t(c, gm));
+    PRH_HIP(c, prh::launch_binned_weights(w_out, g->l_out_eid, g->bin_t.plan.slot, M, c->wt_wl.p, hs));
+    plan = g->bin_t.plan;
+    plan.wx = c->wt_wl.p;
+  } else {
+    PRH_HIP(c, prh::launch_permute_weights(w_out, g->l_out_eid, M, c->wt_wl.p, hs));
+    PRH_CHECK(weighted_schedule(c, g->l_out_off, N, c->sched_out, A0));
+  }
   PRH_HIP(c, hipEventRecord(c->ev[0], hs));
   PRH_HIP(c, prh::launch_weighted_init(c->wt_terml.p, c->wt_Wl.p, N, true, c->wt_x[0].p, c->wt_z[0].p, hs));
   PRH_HIP(c, hipEventRecord(c->ev[1], hs));
   int n = 0;
   PRH_CHECK(run_iterations(c, o->max_iterations, o->tolerance, [&](int l, unsigned long long *err, int *done) -> int {
+    if (blocked) {  // lam_new = g + (P / W) * sum over out-edges of w * lam_old, gathered straight from lam_old
+      double *x_new = c->wt_x[(l + 1) & 1].p, *x_old = c->wt_x[l & 1].p;
+      prh::RankArgs A{x_new, x_old, x_new, x_old, nullptr, nullptr, g->l_out_off, g->l_out_dst, nullptr,
+                      0.0, o->damping, 0.0, 0.0, nullptr};
+      A.term = c->wt_terml.p;
+      A.W = c->wt_Wl.p;
+      PRH_HIP(c, prh::launch_binned_update(prh::kWeightedAdjoint, A, plan, err, done, l, o->tolerance, false, hs));
+      return PRH_SUCCESS;
+    }
     prh::WeightedArgs A = A0;
     A.off = g->l_out_off;
     A.idx = g->l_out_dst;
@@ -1735,7 +1772,8 @@ void prh_graph_destroy(prh_graph *g) {
   dev_free(c, g->chunk_tables, g->chunk_tables_bytes);
   dev_free(c, g->chunk_partial, g->chunk_partial_bytes);
   free_sliced(c, g);
-  free_binned(c, g);
+  free_binned(c, g->bin);
+  free_binned(c, g->bin_t);
   delete g;
 }
 
