@@ -3,24 +3,25 @@
 // The pull kernel gathers cr[u] = r[u]/deg[u] at random; every gather from a source outside the L2-resident hot
 // set costs a 128-byte line fill for 8 useful bytes. Propagation blocking (Beamer, Asanovic and Patterson,
 // "Reducing PageRank Communication via Propagation Blocking", IPDPS 2017) moves that randomness into LDS. Vertices
-// are cut into runs of B consecutive layout rows: source blocks on one side, destination bins on the other. Each
-// iteration makes two streaming passes through X, one FP64 slot per edge:
+// are cut into runs of B consecutive layout rows: source blocks on one side, destination bins on the other. Edges
+// from one source into one bin share a single entry of X, the pair's FP64 contribution (on a power-law graph a
+// source's edges crowd into the bins of the high in-degree vertices: on RMAT-24 with B = 8192 there are 0.38
+// pairs per edge). Each iteration makes two streaming passes through X:
 //   expand      one workgroup per item of a source block: the block's B contributions are loaded into LDS, and
-//               every edge leaving the block copies its source's contribution into its slot of X;
-//   accumulate  one workgroup per item of a destination bin: the bin's slots are read in order and added into B
-//               LDS sums (ds_add_f64; neighbouring slots with the same destination are summed first, by a wave
-//               prefix sum, so that they do not collide on one address); a bin covered by one item is finished
-//               right there, a split bin leaves one row of B partial sums per item;
+//               every pair leaving the block copies its source's contribution into its entry of X;
+//   accumulate  one workgroup per item of a destination bin: the bin's slots (one per edge) are read in order, each
+//               takes its pair's entry of X and adds it into one of B LDS sums (ds_add_f64); a bin covered by one
+//               item is finished right there, a split bin leaves one row of B partial sums per item;
 //   finish      split bins only: a vertex's partials are added in item order, then the update.
-// Expand stores pairs of entries (16 bytes a lane); accumulate takes 64-slot groups, most of which lie inside one
-// run and need only scalar loads of the run table.
-// Slot order is bin-major: bin b, then source block s, then the in-CSR order (destination, then source) inside the
-// (s, b) tile; accumulate items are ranges of it. Expand runs block-major (block s, then bin b, same order inside
-// a tile) and writes X in that order, sequentially, with a 16-bit local source id per entry. Each nonempty tile is
-// one run of consecutive slots and of consecutive X entries; accumulate maps slot j to its X entry through the run
-// table (run_j = first slot, run_x = first X entry; grp_run = the run holding slot 64 q), and reads X and the 16-bit
-// local destination ids (stored in X's order) run by run.
-// Per edge and iteration: expand reads 2 bytes and writes 8, accumulate reads 8 + 2 (plus a few bytes per run).
+// Slot order is bin-major: bin b, then source block s (the (s, b) tile), then source, then destination. A tile's
+// pairs are one run of consecutive X entries; X is stored block-major (block s, then bin b, then source), so that
+// expand writes it sequentially (pairs of entries, 16 bytes a lane) with a 16-bit local source id per entry.
+// Accumulate finds a slot's entry without a per-slot index: each slot's 16-bit local destination id carries two
+// flags, "starts a pair" and "starts a run", and a table gives, per 64 slots, the entry and run of the first slot's
+// pair; a wave counts the flags with ballots (the entry advances by one per pair start, and jumps to the run
+// table's entry at a run start).
+// Per edge and iteration: accumulate reads the 2-byte slot; per pair, expand reads 2 bytes and writes 8 and
+// accumulate reads the 8 back (repeated loads of one entry by neighbouring lanes share a cache line).
 // LDS additions are made in arrival order, so the sums repeat only to rounding, not bit for bit. Every vertex gets
 // its full in-sum; the update from it is the other plans' (update.hpp), so the dynamic variants carry unaffected
 // vertices over and set their prune and frontier flags as the row kernel does.
@@ -128,26 +129,17 @@ __global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const d
   // or last entry is written on its own.
   uint32_t k0 = it.y, k1 = it.z;
   if (k0 & 1u) {
-    if (threadIdx.x == 0) P.X[k0] = c[ld_nt(P.src16 + k0)] * (P.wx ? double(P.wx[k0]) : 1.0);
+    if (threadIdx.x == 0) P.X[k0] = c[ld_nt(P.src16 + k0)];
     ++k0;
   }
   if (k1 > k0 && (k1 & 1u)) {
-    if (threadIdx.x == 0) P.X[k1 - 1] = c[ld_nt(P.src16 + k1 - 1)] * (P.wx ? double(P.wx[k1 - 1]) : 1.0);
+    if (threadIdx.x == 0) P.X[k1 - 1] = c[ld_nt(P.src16 + k1 - 1)];
     --k1;
   }
   const uint32_t *s32 = reinterpret_cast<const uint32_t *>(P.src16);
   d2 *x2 = reinterpret_cast<d2 *>(P.X);
   const uint32_t p0 = k0 / 2, p1 = k1 / 2;
   uint32_t p = p0 + threadIdx.x;
-  if (P.wx) {  // weighted solves: contribution times the edge's weight
-    const float2 *w2 = reinterpret_cast<const float2 *>(P.wx);
-    for (; p < p1; p += kBinBlock) {
-      const uint32_t a = __builtin_nontemporal_load(s32 + p);
-      const float2 w = w2[p];
-      __builtin_nontemporal_store(d2{c[a & 0xFFFFu] * double(w.x), c[a >> 16] * double(w.y)}, x2 + p);
-    }
-    return;
-  }
   for (; p + 3 * kBinBlock < p1; p += 4 * kBinBlock) {
     const uint32_t a0 = __builtin_nontemporal_load(s32 + p), a1 = __builtin_nontemporal_load(s32 + p + kBinBlock);
     const uint32_t a2 = __builtin_nontemporal_load(s32 + p + 2 * kBinBlock);
@@ -163,90 +155,30 @@ __global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const d
   }
 }
 
-__device__ __forceinline__ uint32_t or_dpp_step(uint32_t x, int ctrl_sel) {
-  switch (ctrl_sel) {
-    case 0: return x | uint32_t(__builtin_amdgcn_update_dpp(0, int(x), kDppQuadSwap1, 0xF, 0xF, false));
-    case 1: return x | uint32_t(__builtin_amdgcn_update_dpp(0, int(x), kDppQuadSwap2, 0xF, 0xF, false));
-    case 2: return x | uint32_t(__builtin_amdgcn_update_dpp(0, int(x), kDppRowHalfMirror, 0xF, 0xF, false));
-    case 3: return x | uint32_t(__builtin_amdgcn_update_dpp(0, int(x), kDppRowMirror, 0xF, 0xF, false));
-    default: return x | uint32_t(__builtin_amdgcn_ds_swizzle(int(x), 0x401F));
-  }
+constexpr uint32_t kPairStart = 0x8000u, kRunStart = 0x4000u, kLocalMask = 0x3FFFu;
+
+// X entry of the pair of slot g + lane, for a wave-uniform g that is a multiple of 64, from d (the slot's 16-bit
+// local destination id with its flags): lane 0's entry and run come from the group table (scalar loads); a pair
+// start at a lane in 1..lane advances the entry by one, and the latest run start at or before the lane resets it to
+// that run's first entry. All lanes must be active (slots past M read as 0: no flags).
+__device__ __forceinline__ uint32_t pair_entry(const BinnedPlan &P, uint32_t g, uint32_t d, uint32_t lane) {
+  const uint2 t = P.grp[__builtin_amdgcn_readfirstlane(g) >> 6];
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // lanes 0 .. lane
+  const unsigned long long pairs = __ballot((d & kPairStart) != 0) & ~1ull;
+  const unsigned long long runs = __ballot((d & kRunStart) != 0) & ~1ull & upto;
+  const uint32_t x0 = __builtin_amdgcn_readfirstlane(t.x), r0 = __builtin_amdgcn_readfirstlane(t.y);
+  if (!runs) return x0 + uint32_t(__popcll(pairs & upto));
+  const uint32_t last = 63u - uint32_t(__clzll(runs));  // the latest run start, in 1 .. lane
+  const unsigned long long after = upto & ~(last == 63 ? ~0ull : ((2ull << last) - 1));  // lanes last+1 .. lane
+  return P.run_x[r0 + uint32_t(__popcll(runs))] + uint32_t(__popcll(pairs & after));
 }
 
-// OR of a 32-bit value over the wave (the pairing of group_sum), wave-uniform result. All lanes must be active.
-__device__ __forceinline__ uint32_t wave_or(uint32_t x) {
-#pragma unroll
-  for (int k = 0; k < 5; ++k) x = or_dpp_step(x, k);
-  return uint32_t(__builtin_amdgcn_readlane(int(x), 0)) | uint32_t(__builtin_amdgcn_readlane(int(x), 32));
-}
-
-__device__ __forceinline__ uint32_t lane_pull(uint32_t x, uint32_t src_lane) {
-  return uint32_t(__builtin_amdgcn_ds_bpermute(int(src_lane << 2), int(x)));
-}
-
-// X entry of slot g + lane, for a wave-uniform g that is a multiple of 64. r0 = the run holding slot g (scalar
-// loads: g is made uniform). If the next run starts at or after g + 64, the group lies in r0. Otherwise lane L
-// loads run r0 + L; the runs that start inside (g, g + 63] mark their offsets in a 64-bit mask, and slot g + i lies
-// in run r0 + (number of marks at offsets <= i). All lanes must be active; slots past the last run map to garbage
-// that the caller masks.
-__device__ __forceinline__ uint32_t slot_entry(const BinnedPlan &P, uint32_t g, uint32_t lane) {
-  g = __builtin_amdgcn_readfirstlane(g);
-  const uint32_t r0 = __builtin_amdgcn_readfirstlane(P.grp_run[g >> 6]);
-  const uint32_t nxt = r0 + 1 < P.n_runs ? __builtin_amdgcn_readfirstlane(P.run_j[r0 + 1]) : 0xFFFFFFFFu;
-  if (nxt - g >= 64u) {  // the whole group lies in run r0 (nxt > g always)
-    const uint32_t x0 = __builtin_amdgcn_readfirstlane(P.run_x[r0]);
-    const uint32_t j0 = __builtin_amdgcn_readfirstlane(P.run_j[r0]);
-    return x0 + (g + lane - j0);
-  }
-  const uint32_t rl = r0 + lane;
-  const bool valid = rl < P.n_runs;
-  const uint32_t rj = valid ? P.run_j[rl] : 0xFFFFFFFFu;
-  const uint32_t rx = valid ? P.run_x[rl] : 0u;
-  const uint32_t o = rj - g;  // >= 1 for lanes >= 1 (runs after r0 start after g)
-  const bool mark = lane >= 1 && valid && o < 64;
-  const uint32_t lo = wave_or(mark && o < 32 ? 1u << o : 0u);
-  const uint32_t hi = wave_or(mark && o >= 32 ? 1u << (o - 32) : 0u);
-  const unsigned long long m = (static_cast<unsigned long long>(hi) << 32) | lo;
-  const uint32_t c = uint32_t(__popcll(m & ((2ull << lane) - 1)));  // lane 63: 2 << 63 wraps to 0, mask all
-  const uint32_t sj = lane_pull(rj, c), sx = lane_pull(rx, c);
-  return sx + (g + lane - sj);
-}
-
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_rows(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, ROWS, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, ROWS, 0xF, false);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
-// Inclusive prefix sum over the 64 lanes (row shifts within 16-lane rows, then the row broadcasts of lanes 15 and
-// 31). All lanes must be active.
-__device__ __forceinline__ double wave_prefix_sum(double x) {
-  x += dpp_rows<0x111, 0xF>(x);  // row_shr:1
-  x += dpp_rows<0x112, 0xF>(x);  // row_shr:2
-  x += dpp_rows<0x114, 0xF>(x);  // row_shr:4
-  x += dpp_rows<0x118, 0xF>(x);  // row_shr:8
-  x += dpp_rows<0x142, 0xA>(x);  // row_bcast:15 into rows 1 and 3
-  x += dpp_rows<0x143, 0xC>(x);  // row_bcast:31 into rows 2 and 3
-  return x;
-}
-
-__device__ __forceinline__ double pull_f64(double x, uint32_t src_lane) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_ds_bpermute(int(src_lane << 2), int(b));
-  const int hi = __builtin_amdgcn_ds_bpermute(int(src_lane << 2), int(b >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
-// Adds one wave's 64 slots into the LDS sums. All lanes must be active; lanes past the item carry ok = false.
-// Lanes with the same destination as their left neighbour (hot destinations repeat within a destination-sorted
-// run) are summed first, so that each such segment costs one LDS addition instead of colliding on one address.
-// EXT (frontier variants): a negative slot comes from a frontier source; its destination's flag in ext is set and
-// its magnitude added.
+// Adds one wave's 64 slots into the LDS sums. All lanes must be active; lanes outside the item carry ok = false.
+// A wave whose slots all go to one destination (a hub bin: many sources, one destination) is summed first and
+// added once. EXT (frontier variants): a negative value comes from a frontier source; its destination's flag in ext
+// is set and its magnitude added.
 template <bool EXT>
 __device__ __forceinline__ void add_slots(double *sum, uint8_t *ext, uint32_t d, double x, bool ok) {
-  const uint32_t lane = threadIdx.x & (kWave - 1);
   if constexpr (EXT) {
     const bool neg = ok && x < 0.0;
     if (__ballot(neg)) {
@@ -255,32 +187,23 @@ __device__ __forceinline__ void add_slots(double *sum, uint8_t *ext, uint32_t d,
     }
   }
   const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-  if (__ballot(d == d0 || !ok) == ~0ull) {  // one destination for the whole wave
+  if (__ballot(d == d0 || !ok) == ~0ull) {
     const double t = group_sum<kWave>(ok ? x : 0.0);
-    if (lane == 0) atomicAdd(&sum[d0], t);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(&sum[d0], t);
     return;
   }
-  const uint32_t dp = uint32_t(__builtin_amdgcn_update_dpp(0, int(d), 0x138, 0xF, 0xF, false));  // wave_shr:1
-  const unsigned long long heads = __ballot(lane == 0 || d != dp || !ok);
-  if (heads == ~0ull || __ballot(ok) != ~0ull) {  // no repeats, or a partial group: one addition per lane
-    if (ok) atomicAdd(&sum[d], x);
-    return;
-  }
-  const double s = wave_prefix_sum(x);
-  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-  const uint32_t h = 63u - uint32_t(__clzll(heads & upto));  // head of this lane's segment
-  const double before = pull_f64(s, h == 0 ? 0u : h - 1);
-  const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
-  if (tail) atomicAdd(&sum[d], h == 0 ? s : s - before);
+  if (ok) atomicAdd(&sum[d], x);
 }
 
 // Frontier variants: a vertex with a frontier in-neighbour (flag set in ext) is marked affected before its update,
 // which is the reference's extension step (pagerankAffectedExtend*CukU, pagerankCuda.hxx:313-385) done by pull over
 // the slots already being read; a split bin's partial rows carry the flag as their sign (a flagged sum is positive).
+// Weighted variants multiply each slot's value by its edge's weight (P.ws, slot order).
 template <int B, int VARIANT>
 __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, BinnedPlan P, LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;
   constexpr bool EXT = Traits<VARIANT>::frontier;
+  constexpr bool WEIGHTED = Traits<VARIANT>::weighted;
   __shared__ double sum[B];
   __shared__ uint8_t ext[EXT ? B : 1];
   __shared__ double lds_max[kBinWaves];
@@ -293,27 +216,31 @@ __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, Binne
   __syncthreads();
   const int w = threadIdx.x / kWave;
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  // Groups of 64 slots aligned to multiples of 64, dealt to the waves in turn; slots outside the item are masked.
+  // Groups of 64 slots aligned to multiples of 64 (the slot array is padded to a multiple of 64), dealt to the
+  // waves in turn; slots outside the item are masked.
+  auto value = [&](uint32_t g, uint32_t d, bool ok) {
+    double x = __builtin_nontemporal_load(P.X + pair_entry(P, g, d, lane));
+    if constexpr (WEIGHTED) x *= ok ? double(P.ws[g + lane]) : 0.0;
+    return x;
+  };
   uint32_t g = __builtin_amdgcn_readfirstlane((it.y & ~63u) + uint32_t(w) * kWave);  // wave-uniform
   for (; g + 3 * kBinBlock < it.z; g += 4 * kBinBlock) {
-    const uint32_t k0 = slot_entry(P, g, lane), k1 = slot_entry(P, g + kBinBlock, lane);
-    const uint32_t k2 = slot_entry(P, g + 2 * kBinBlock, lane), k3 = slot_entry(P, g + 3 * kBinBlock, lane);
     const bool ok0 = g + lane >= it.y;  // groups 1..3 start past it.y; all four end before it.z except possibly 3
     const bool ok3 = g + 3 * kBinBlock + lane < it.z;
-    const uint32_t d0 = ok0 ? ld_nt(P.dst16 + k0) : 0u, d1 = ld_nt(P.dst16 + k1), d2 = ld_nt(P.dst16 + k2);
-    const uint32_t d3 = ok3 ? ld_nt(P.dst16 + k3) : 0u;
-    const double x0 = ok0 ? __builtin_nontemporal_load(P.X + k0) : 0.0;
-    const double x1 = __builtin_nontemporal_load(P.X + k1), x2 = __builtin_nontemporal_load(P.X + k2);
-    const double x3 = ok3 ? __builtin_nontemporal_load(P.X + k3) : 0.0;
-    add_slots<EXT>(sum, ext, d0, x0, ok0);
-    add_slots<EXT>(sum, ext, d1, x1, true);
-    add_slots<EXT>(sum, ext, d2, x2, true);
-    add_slots<EXT>(sum, ext, d3, x3, ok3);
+    const uint32_t d0 = ld_nt(P.dst16 + g + lane), d1 = ld_nt(P.dst16 + g + kBinBlock + lane);
+    const uint32_t d2 = ld_nt(P.dst16 + g + 2 * kBinBlock + lane), d3 = ld_nt(P.dst16 + g + 3 * kBinBlock + lane);
+    const double x0 = value(g, d0, ok0), x1 = value(g + kBinBlock, d1, true);
+    const double x2 = value(g + 2 * kBinBlock, d2, true), x3 = value(g + 3 * kBinBlock, d3, ok3);
+    add_slots<EXT>(sum, ext, d0 & kLocalMask, x0, ok0);
+    add_slots<EXT>(sum, ext, d1 & kLocalMask, x1, true);
+    add_slots<EXT>(sum, ext, d2 & kLocalMask, x2, true);
+    add_slots<EXT>(sum, ext, d3 & kLocalMask, x3, ok3);
   }
   for (; g < it.z; g += kBinBlock) {
-    const uint32_t k = slot_entry(P, g, lane);
     const bool ok = g + lane >= it.y && g + lane < it.z;
-    add_slots<EXT>(sum, ext, ok ? ld_nt(P.dst16 + k) : 0u, ok ? __builtin_nontemporal_load(P.X + k) : 0.0, ok);
+    const uint32_t d = ld_nt(P.dst16 + g + lane);
+    const double x = value(g, d, ok);
+    add_slots<EXT>(sum, ext, d & kLocalMask, x, ok);
   }
   __syncthreads();
   const uint32_t b0 = it.x * uint32_t(B);
@@ -432,50 +359,70 @@ __global__ __launch_bounds__(kBlock) void bin_keys_kernel(const uint32_t *off, c
   }
 }
 
-// Slot j holds in-CSR slot e = perm2[j]: dst[j] = row - bin start, u2[j] = source, key1[j] = source block.
-__global__ __launch_bounds__(kBlock) void bin_slots_kernel(const uint32_t *perm2, const uint32_t *src,
-                                                           const uint32_t *row, size_t M, uint32_t B, uint16_t *dst,
-                                                           uint32_t *u2, uint32_t *key1) {
+// Slot j holds in-CSR slot e = perm2[j] (slots sorted by tile, then source, then destination): its local
+// destination id with the pair and run start flags, and pf[j] = 1 where it starts a pair.
+__global__ __launch_bounds__(kBlock) void pair_slots_kernel(const uint32_t *perm2, const uint32_t *src,
+                                                            const uint32_t *row, const uint32_t *key2s, size_t M,
+                                                            uint32_t B, uint16_t *dst16, uint32_t *pf) {
   for (size_t j = size_t(blockIdx.x) * kBlock + threadIdx.x; j < M; j += size_t(gridDim.x) * kBlock) {
-    const uint32_t e = perm2[j], u = src[e], v = row[e];
-    dst[j] = uint16_t(v % B);
-    u2[j] = u;
-    key1[j] = u / B;
+    const uint32_t e = perm2[j], u = src[e];
+    const bool run = j == 0 || key2s[j] != key2s[j - 1];
+    const bool pair = run || src[perm2[j - 1]] != u;
+    dst16[j] = uint16_t((row[e] % B) | (pair ? kPairStart : 0u) | (run ? kRunStart : 0u));
+    pf[j] = pair ? 1u : 0u;
   }
 }
 
-// flag[j] = 1 where slot j starts a run (a new tile in the sorted tile keys).
+// Pair q = pid[j] of every pair-starting slot j (pid: exclusive scan of pf): its source psrc[q], tile key pkey2[q]
+// and source block key1[q].
+__global__ __launch_bounds__(kBlock) void pair_fill_kernel(const uint32_t *pf, const uint32_t *pid,
+                                                           const uint32_t *perm2, const uint32_t *src,
+                                                           const uint32_t *key2s, size_t M, uint32_t B,
+                                                           uint32_t *psrc, uint32_t *pkey2, uint32_t *key1) {
+  for (size_t j = size_t(blockIdx.x) * kBlock + threadIdx.x; j < M; j += size_t(gridDim.x) * kBlock) {
+    if (!pf[j]) continue;
+    const uint32_t q = pid[j], u = src[perm2[j]];
+    psrc[q] = u;
+    pkey2[q] = key2s[j];
+    key1[q] = u / B;
+  }
+}
+
+// flag[j] = 1 where entry j starts a run (a new tile in the sorted tile keys).
 __global__ __launch_bounds__(kBlock) void run_flags_kernel(const uint32_t *key2s, size_t M, uint32_t *flag) {
   for (size_t j = size_t(blockIdx.x) * kBlock + threadIdx.x; j < M; j += size_t(gridDim.x) * kBlock)
     flag[j] = (j == 0 || key2s[j] != key2s[j - 1]) ? 1u : 0u;
 }
 
-// X entry k is slot perm1[k]: src16[k], dst16[k] (its destination), inv[perm1[k]] = k.
-__global__ __launch_bounds__(kBlock) void bin_expand_kernel(const uint32_t *perm1, const uint32_t *u2,
-                                                            const uint16_t *dst, size_t M, uint32_t B,
-                                                            uint16_t *src16, uint16_t *dst16, uint32_t *inv) {
-  for (size_t k = size_t(blockIdx.x) * kBlock + threadIdx.x; k < M; k += size_t(gridDim.x) * kBlock) {
-    const uint32_t j = perm1[k];
-    src16[k] = uint16_t(u2[j] % B);
-    dst16[k] = dst[j];
-    inv[j] = uint32_t(k);
+// X entry k holds pair perm1[k]: src16[k] = its local source id, inv[perm1[k]] = k.
+__global__ __launch_bounds__(kBlock) void pair_expand_kernel(const uint32_t *perm1, const uint32_t *psrc, size_t Q,
+                                                             uint32_t B, uint16_t *src16, uint32_t *inv) {
+  for (size_t k = size_t(blockIdx.x) * kBlock + threadIdx.x; k < Q; k += size_t(gridDim.x) * kBlock) {
+    const uint32_t q = perm1[k];
+    src16[k] = uint16_t(psrc[q] % B);
+    inv[q] = uint32_t(k);
   }
 }
 
-// Run tables from the run flags and their exclusive scan rid: run rid[j] starts at slot j (flagged) and X entry
-// inv[j]; grp_run[j / 64] = run of slot j for j a multiple of 64.
-__global__ __launch_bounds__(kBlock) void bin_runs_kernel(const uint32_t *flag, const uint32_t *rid,
-                                                          const uint32_t *inv, size_t M, uint32_t *run_j,
-                                                          uint32_t *run_x, uint32_t *grp_run) {
-  for (size_t j = size_t(blockIdx.x) * kBlock + threadIdx.x; j < M; j += size_t(gridDim.x) * kBlock) {
-    const uint32_t f = flag[j], r = rid[j];
-    if (f) {
-      run_j[r] = uint32_t(j);
-      run_x[r] = inv[j];
-    }
-    if ((j & 63) == 0) grp_run[j >> 6] = r + f - 1;
+// run_x[rid[q]] = inv[q] for every run-starting pair q (rid: exclusive scan of the run flags rf).
+__global__ __launch_bounds__(kBlock) void pair_runs_kernel(const uint32_t *rf, const uint32_t *rid,
+                                                           const uint32_t *inv, size_t Q, uint32_t *run_x) {
+  for (size_t q = size_t(blockIdx.x) * kBlock + threadIdx.x; q < Q; q += size_t(gridDim.x) * kBlock)
+    if (rf[q]) run_x[rid[q]] = inv[q];
+}
+
+// grp[i] = (X entry, run) of the pair of slot 64 i.
+__global__ __launch_bounds__(kBlock) void group_table_kernel(const uint32_t *pf, const uint32_t *pid,
+                                                             const uint32_t *rf, const uint32_t *rid,
+                                                             const uint32_t *inv, size_t M, uint2 *grp) {
+  const size_t G = (M + 63) / 64;
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < G; i += size_t(gridDim.x) * kBlock) {
+    const size_t j = i * 64;
+    const uint32_t q = pid[j] + pf[j] - 1;  // slot 0 starts pair 0, so q >= 0
+    grp[i] = make_uint2(inv[q], rid[q] + rf[q] - 1);
   }
 }
+
 
 __global__ __launch_bounds__(kBlock) void binned_weights_kernel(const float *w, const uint32_t *eid,
                                                                 const uint32_t *slot, size_t M, float *wx) {
@@ -533,10 +480,18 @@ hipError_t launch_bin_keys(const uint32_t *off, const uint32_t *src, uint32_t N,
   return hipGetLastError();
 }
 
-hipError_t launch_bin_slots(const uint32_t *perm2, const uint32_t *src, const uint32_t *row, size_t M, uint32_t B,
-                            uint16_t *dst, uint32_t *u2, uint32_t *key1, hipStream_t s) {
+hipError_t launch_pair_slots(const uint32_t *perm2, const uint32_t *src, const uint32_t *row, const uint32_t *key2s,
+                             size_t M, uint32_t B, uint16_t *dst16, uint32_t *pf, hipStream_t s) {
   if (!M) return hipSuccess;
-  bin_slots_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(perm2, src, row, M, B, dst, u2, key1);
+  pair_slots_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(perm2, src, row, key2s, M, B, dst16, pf);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_fill(const uint32_t *pf, const uint32_t *pid, const uint32_t *perm2, const uint32_t *src,
+                            const uint32_t *key2s, size_t M, uint32_t B, uint32_t *psrc, uint32_t *pkey2,
+                            uint32_t *key1, hipStream_t s) {
+  if (!M) return hipSuccess;
+  pair_fill_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(pf, pid, perm2, src, key2s, M, B, psrc, pkey2, key1);
   return hipGetLastError();
 }
 
@@ -546,17 +501,24 @@ hipError_t launch_run_flags(const uint32_t *key2s, size_t M, uint32_t *flag, hip
   return hipGetLastError();
 }
 
-hipError_t launch_bin_expand(const uint32_t *perm1, const uint32_t *u2, const uint16_t *dst, size_t M, uint32_t B,
-                             uint16_t *src16, uint16_t *dst16, uint32_t *inv, hipStream_t s) {
-  if (!M) return hipSuccess;
-  bin_expand_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(perm1, u2, dst, M, B, src16, dst16, inv);
+hipError_t launch_pair_expand(const uint32_t *perm1, const uint32_t *psrc, size_t Q, uint32_t B, uint16_t *src16,
+                              uint32_t *inv, hipStream_t s) {
+  if (!Q) return hipSuccess;
+  pair_expand_kernel<<<grid(Q, kBlock, 16384), kBlock, 0, s>>>(perm1, psrc, Q, B, src16, inv);
   return hipGetLastError();
 }
 
-hipError_t launch_bin_runs(const uint32_t *flag, const uint32_t *rid, const uint32_t *inv, size_t M, uint32_t *run_j,
-                           uint32_t *run_x, uint32_t *grp_run, hipStream_t s) {
+hipError_t launch_pair_runs(const uint32_t *rf, const uint32_t *rid, const uint32_t *inv, size_t Q, uint32_t *run_x,
+                            hipStream_t s) {
+  if (!Q) return hipSuccess;
+  pair_runs_kernel<<<grid(Q, kBlock, 16384), kBlock, 0, s>>>(rf, rid, inv, Q, run_x);
+  return hipGetLastError();
+}
+
+hipError_t launch_group_table(const uint32_t *pf, const uint32_t *pid, const uint32_t *rf, const uint32_t *rid,
+                              const uint32_t *inv, size_t M, uint2 *grp, hipStream_t s) {
   if (!M) return hipSuccess;
-  bin_runs_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(flag, rid, inv, M, run_j, run_x, grp_run);
+  group_table_kernel<<<grid((M + 63) / 64, kBlock, 16384), kBlock, 0, s>>>(pf, pid, rf, rid, inv, M, grp);
   return hipGetLastError();
 }
 

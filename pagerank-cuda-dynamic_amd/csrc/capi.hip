@@ -391,12 +391,14 @@ void free_binned(prh_context *c, BinnedCache &bc) {
 }
 
 // Propagation-blocked plan (prh::BinnedPlan) for blocks and bins of B = tuning.blocking layout rows:
-//   1. key2[e] = bin(row) * nblocks + block(source) per in-CSR slot; a stable sort by key2 gives the slot order
-//      (bin, block, in-CSR order), perm2[j] = in-CSR slot of slot j; runs start where the sorted key2 changes;
-//   2. per slot: its destination, its source, and key1 = its block; a stable sort of the slots by key1 gives the
-//      X order (block, bin, in-CSR order), perm1[k] = slot of X entry k;
-//   3. src16 / dst16 in X order, the run tables (first slot, first X entry; the run of every 64th slot);
-//   4. block starts in X order (lower bounds of the sorted key1) and bin starts from the in-degrees cut the two
+//   1. a stable sort of the pull-CSR slots by source, then a stable sort of that order by key2 = bin(row) *
+//      nblocks + block(source), give the slot order (tile, source, row), perm2[j] = CSR slot of slot j;
+//   2. per slot: its local destination with the pair and run start flags; an exclusive scan of the pair starts
+//      numbers the pairs in slot order, and each pair records its source, tile key and key1 = its source block;
+//   3. a stable sort of the pairs by key1 gives the X order (block, bin, source), perm1[k] = pair of X entry k;
+//      src16 in X order; runs (tiles) start where the pairs' tile key changes: their first X entries, and per 64
+//      slots the X entry and run of the first slot's pair;
+//   4. block starts in X order (lower bounds of the sorted key1) and bin starts from the row degrees cut the two
 //      orders into items; split bins get partial rows.
 // Builds the plan of the pull CSR (off, idx) with rows of degree rowdeg (host), N rows, M slots, into bc.
 int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, const uint32_t *idx,
@@ -410,11 +412,13 @@ int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, co
   if (nb * nb > (uint64_t(1) << 32))
     return fail(c, PRH_ERROR_UNSUPPORTED, "blocking %d: %llu blocks per side exceed 32-bit tile keys", B,
                 (unsigned long long)nb);
-  int key2_bits = 0, key1_bits = 0;
+  int key2_bits = 0, key1_bits = 0, src_bits = 0;
   while ((uint64_t(1) << key2_bits) < nb * nb) ++key2_bits;
   while ((uint64_t(1) << key1_bits) < nb) ++key1_bits;
+  while ((uint64_t(1) << src_bits) < N) ++src_bits;
   key2_bits = std::max(key2_bits, 1);
   key1_bits = std::max(key1_bits, 1);
+  src_bits = std::max(src_bits, 1);
   struct Temps {
     prh_context *c;
     std::vector<std::pair<void *, size_t>> v;
@@ -439,64 +443,82 @@ int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, co
   prh::BinnedPlan P{};
   P.B = B;
   P.N = N;
-  uint16_t *src16, *dst16;
-  uint32_t *grp_run, *run_j = nullptr, *run_x = nullptr;
-  double *X;
-  PRH_CHECK(keep(&src16, M * 2));
-  PRH_CHECK(keep(&dst16, M * 2));
-  PRH_CHECK(keep(&X, M * 8));
-  PRH_CHECK(keep(&grp_run, (M + 63) / 64 * 4));
+  const uint64_t Mp = (M + 63) / 64 * 64;  // slots padded to whole groups of 64
+  uint16_t *dst16, *src16 = nullptr;
+  uint2 *grp;
+  uint32_t *run_x = nullptr;
+  double *X = nullptr;
+  PRH_CHECK(keep(&dst16, Mp * 2));
+  PRH_CHECK(keep(&grp, (M + 63) / 64 * sizeof(uint2)));
   std::vector<uint32_t> bstart(nb + 1, 0);
   uint32_t n_runs = 0;
+  uint64_t Q = 0;
   if (M) {
-    uint32_t *key, *row, *k2s, *perm2, *u2, *rid, *k1s, *perm1, *starts;
-    uint16_t *dst;
-    for (uint32_t **p : {&key, &row, &k2s, &perm2, &u2, &rid, &k1s, &perm1}) PRH_CHECK(temp(p, M * 4));
-    PRH_CHECK(temp(&dst, M * 2));
+    uint32_t *key, *row, *tk, *pa, *perm2, *starts;
+    for (uint32_t **p : {&key, &row, &tk, &pa}) PRH_CHECK(temp(p, M * 4));
+    if (keep_slot) PRH_CHECK(keep(&perm2, M * 4));
+    else PRH_CHECK(temp(&perm2, M * 4));
     PRH_CHECK(temp(&starts, (nb + 1) * 4));
-    size_t sort_bytes = 0, sort1_bytes = 0, scan_bytes = 0;
-    PRH_HIP(c, prh::sort_pairs_u32(nullptr, sort_bytes, key, k2s, nullptr, perm2, M, key2_bits, s));
-    PRH_HIP(c, prh::sort_pairs_u32(nullptr, sort1_bytes, key, k1s, nullptr, perm1, M, key1_bits, s));
-    PRH_HIP(c, prh::scan_u32(nullptr, scan_bytes, row, rid, M, s));
-    const size_t work_bytes = std::max({sort_bytes, sort1_bytes, scan_bytes});
+    size_t b_src = 0, b_key2 = 0, b_key1 = 0, b_scan = 0;
+    PRH_HIP(c, prh::sort_pairs_u32(nullptr, b_src, idx, tk, nullptr, pa, M, src_bits, s));
+    PRH_HIP(c, prh::sort_pairs_u32(nullptr, b_key2, tk, key, pa, perm2, M, key2_bits, s));
+    PRH_HIP(c, prh::sort_pairs_u32(nullptr, b_key1, key, tk, nullptr, pa, M, key1_bits, s));
+    PRH_HIP(c, prh::scan_u32(nullptr, b_scan, row, tk, M, s));
+    const size_t work_bytes = std::max({b_src, b_key2, b_key1, b_scan});
     uint8_t *work;
     PRH_CHECK(temp(&work, work_bytes));
     size_t wb = work_bytes;
+    // 1. slot order: by source (pa), then by tile, stably (perm2); key then holds the sorted tile keys.
+    PRH_HIP(c, prh::sort_pairs_u32(work, wb, idx, tk, nullptr, pa, M, src_bits, s));
     PRH_HIP(c, prh::launch_bin_keys(off, idx, N, uint32_t(B), uint32_t(nb), key, row, s));
-    PRH_HIP(c, prh::sort_pairs_u32(work, wb, key, k2s, nullptr, perm2, M, key2_bits, s));
-    PRH_HIP(c, prh::launch_bin_slots(perm2, idx, row, M, uint32_t(B), dst, u2, key, s));
-    // row now holds the run flags, rid their exclusive scan; key holds key1.
-    PRH_HIP(c, prh::launch_run_flags(k2s, M, row, s));
+    PRH_HIP(c, prh::launch_gather_u32(key, pa, M, tk, s));
     wb = work_bytes;
-    PRH_HIP(c, prh::scan_u32(work, wb, row, rid, M, s));
+    PRH_HIP(c, prh::sort_pairs_u32(work, wb, tk, key, pa, perm2, M, key2_bits, s));
+    // 2. slot flags (pf in pa) and pair numbers (pid in tk).
+    PRH_HIP(c, prh::launch_pair_slots(perm2, idx, row, key, M, uint32_t(B), dst16, pa, s));
+    if (Mp > M) PRH_HIP(c, hipMemsetAsync(dst16 + M, 0, (Mp - M) * 2, s));
     wb = work_bytes;
-    PRH_HIP(c, prh::sort_pairs_u32(work, wb, key, k1s, nullptr, perm1, M, key1_bits, s));
-    if (keep_slot) {  // CSR slot of X entry k = perm2[perm1[k]]
-      uint32_t *slot;
-      PRH_CHECK(keep(&slot, M * 4));
-      PRH_HIP(c, prh::launch_gather_u32(perm2, perm1, M, slot, s));
-      P.slot = slot;
-    }
-    // perm2 is free again: it takes inv (slot -> X entry).
-    PRH_HIP(c, prh::launch_bin_expand(perm1, u2, dst, M, uint32_t(B), src16, dst16, perm2, s));
-    PRH_HIP(c, prh::launch_lower_bounds(k1s, M, uint32_t(nb), starts, s));
-    PRH_HIP(c, hipMemcpyAsync(bstart.data(), starts, (nb + 1) * 4, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, prh::scan_u32(work, wb, pa, tk, M, s));
     uint32_t last[2];
-    PRH_HIP(c, hipMemcpyAsync(&last[0], rid + (M - 1), 4, hipMemcpyDeviceToHost, s));
-    PRH_HIP(c, hipMemcpyAsync(&last[1], row + (M - 1), 4, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipMemcpyAsync(&last[0], tk + (M - 1), 4, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipMemcpyAsync(&last[1], pa + (M - 1), 4, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipStreamSynchronize(s));
+    Q = uint64_t(last[0]) + last[1];
+    uint32_t *pkey2, *key1, *k1s, *perm1, *inv, *rf, *rid;
+    for (uint32_t **p : {&pkey2, &key1, &k1s, &perm1, &inv, &rf, &rid}) PRH_CHECK(temp(p, Q * 4));
+    uint32_t *psrc = row;  // row is free once the flags are written
+    PRH_HIP(c, prh::launch_pair_fill(pa, tk, perm2, idx, key, M, uint32_t(B), psrc, pkey2, key1, s));
+    // 3. X order of the pairs, src16, runs and the group table.
+    wb = work_bytes;
+    PRH_HIP(c, prh::sort_pairs_u32(work, wb, key1, k1s, nullptr, perm1, Q, key1_bits, s));
+    PRH_CHECK(keep(&src16, Q * 2));
+    PRH_CHECK(keep(&X, Q * 8));
+    PRH_HIP(c, prh::launch_pair_expand(perm1, psrc, Q, uint32_t(B), src16, inv, s));
+    PRH_HIP(c, prh::launch_run_flags(pkey2, Q, rf, s));
+    wb = work_bytes;
+    PRH_HIP(c, prh::scan_u32(work, wb, rf, rid, Q, s));
+    PRH_HIP(c, hipMemcpyAsync(&last[0], rid + (Q - 1), 4, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipMemcpyAsync(&last[1], rf + (Q - 1), 4, hipMemcpyDeviceToHost, s));
     PRH_HIP(c, hipStreamSynchronize(s));
     n_runs = last[0] + last[1];
-    PRH_CHECK(keep(&run_j, size_t(n_runs) * 4));
     PRH_CHECK(keep(&run_x, size_t(n_runs) * 4));
-    PRH_HIP(c, prh::launch_bin_runs(row, rid, perm2, M, run_j, run_x, grp_run, s));
+    PRH_HIP(c, prh::launch_pair_runs(rf, rid, inv, Q, run_x, s));
+    PRH_HIP(c, prh::launch_group_table(pa, tk, rf, rid, inv, M, grp, s));
+    // 4. block starts in X order.
+    PRH_HIP(c, prh::launch_lower_bounds(k1s, Q, uint32_t(nb), starts, s));
+    PRH_HIP(c, hipMemcpyAsync(bstart.data(), starts, (nb + 1) * 4, hipMemcpyDeviceToHost, s));
     PRH_HIP(c, hipStreamSynchronize(s));
+    if (keep_slot) P.slot = perm2;
+  } else {
+    PRH_CHECK(keep(&src16, 2));
+    PRH_CHECK(keep(&X, 16));
   }
-  // Items: expand items of at most 8 per CU over the whole graph, accumulate items of at most half that many
+  // Items: expand items of at most 8 per CU over all pairs, accumulate items of at most 4 per CU over all slots
   // (fewer split bins, hence fewer partial rows); both at least 16K entries.
   hipDeviceProp_t prop{};
   PRH_HIP(c, hipGetDeviceProperties(&prop, c->device));
   const uint64_t cus = uint64_t(std::max(prop.multiProcessorCount, 1));
-  const uint64_t span_ex = std::max<uint64_t>(M / (cus * 8), 16384);
+  const uint64_t span_ex = std::max<uint64_t>(Q / (cus * 8), 16384);
   const uint64_t span = std::max<uint64_t>(M / (cus * 4), 16384);
   std::vector<uint4> ex, ac, multi;
   for (uint64_t b = 0; b < nb; ++b)
@@ -539,10 +561,10 @@ int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, co
   P.src16 = src16;
   P.dst16 = dst16;
   P.X = X;
-  P.run_j = run_j;
   P.run_x = run_x;
-  P.grp_run = grp_run;
+  P.grp = grp;
   P.n_runs = n_runs;
+  P.Q = Q;
   P.ex_item = tables;
   P.ac_item = tables + ex.size();
   P.multi = tables + ex.size() + ac.size();
@@ -1403,7 +1425,7 @@ int weighted_forward_impl(prh_context *c, const prh_graph *g, const float *w, co
     PRH_CHECK(build_binned_plan(c, gm, true));
     PRH_HIP(c, prh::launch_binned_weights(w_out, g->l_in_eid, g->bin.plan.slot, M, c->wt_wl.p, hs));
     prh::BinnedPlan plan = g->bin.plan;
-    plan.wx = c->wt_wl.p;
+    plan.ws = c->wt_wl.p;
     PRH_HIP(c, hipEventRecord(c->ev[0], hs));
     PRH_HIP(c, prh::launch_weighted_init(c->wt_terml.p, c->wt_Wl.p, N, false, c->wt_x[0].p, c->wt_z[0].p, hs));
     PRH_HIP(c, hipEventRecord(c->ev[1], hs));
@@ -1480,7 +1502,7 @@ int weighted_adjoint_impl(prh_context *c, const prh_graph *g, const float *w, co
     PRH_CHECK(build_binned_plan_t(c, gm));
     PRH_HIP(c, prh::launch_binned_weights(w_out, g->l_out_eid, g->bin_t.plan.slot, M, c->wt_wl.p, hs));
     plan = g->bin_t.plan;
-    plan.wx = c->wt_wl.p;
+    plan.ws = c->wt_wl.p;
   } else {
     PRH_HIP(c, prh::launch_permute_weights(w_out, g->l_out_eid, M, c->wt_wl.p, hs));
     PRH_CHECK(weighted_schedule(c, g->l_out_off, N, c->sched_out, A0));

@@ -155,20 +155,20 @@ constexpr uint32_t kWholeBin = 0xFFFFFFFFu;
 struct BinnedPlan {
   int B;                   // vertices per block and bin: 1024, 2048, 4096, 8192 or 16384
   uint32_t N;
-  const uint16_t *src16;   // [M] X order: source - s * B
-  const uint16_t *dst16;   // [M] X order: destination - b * B
-  double *X;               // [M]
-  const uint32_t *run_j;   // [n_runs] first slot of each run (ascending)
-  const uint32_t *run_x;   // [n_runs] its X entry
-  const uint32_t *grp_run; // [ceil(M / 64)] run holding slot 64 q
+  const uint16_t *src16;   // [Q] X order: source - s * B
+  const uint16_t *dst16;   // [M padded to 64] slot order: destination - b * B, | 0x8000 pair start, | 0x4000 run start
+  double *X;               // [Q] one entry per (source, destination bin) pair, block-major
+  const uint32_t *run_x;   // [n_runs] first X entry of each run (a tile's pairs), runs in slot order
+  const uint2 *grp;        // [ceil(M / 64)] X entry and run of the pair of slot 64 i
   uint32_t n_runs;
   const uint4 *ex_item;    // [n_ex]
   const uint4 *ac_item;    // [n_ac]
   const uint4 *multi;      // [n_multi] finish units
   double *partial;         // [partial rows][B]
   uint32_t n_ex, n_ac, n_multi;
-  const uint32_t *slot;    // [M] CSR slot of X entry k (kept for the weighted solves), else null
-  const float *wx;         // [M] weighted solves: the edge weight of X entry k, applied in expand; else null
+  uint64_t Q;              // pairs
+  const uint32_t *slot;    // [M] CSR slot of slot j (kept for the weighted solves), else null
+  const float *ws;         // [M] weighted solves: the edge weight of slot j, applied in accumulate; else null
 };
 
 // ---- partition.hip ----
@@ -215,23 +215,31 @@ hipError_t launch_gather_ranges(const uint32_t *off, const uint32_t *ids, size_t
 // recorded after the last kernel.
 hipError_t launch_binned_update(int variant, const RankArgs &A, const BinnedPlan &P, unsigned long long *err,
                                 int *done, int l, double tol, bool extend, hipStream_t s, hipEvent_t mark = nullptr);
-// wx[k] = w[eid[slot[k]]]: edge weights (raw out-CSR order) carried to X order through the layout edge map.
-hipError_t launch_binned_weights(const float *w, const uint32_t *eid, const uint32_t *slot, size_t M, float *wx,
+// ws[j] = w[eid[slot[j]]]: edge weights (raw out-CSR order) carried to slot order through the layout edge map.
+hipError_t launch_binned_weights(const float *w, const uint32_t *eid, const uint32_t *slot, size_t M, float *ws,
                                  hipStream_t s);
 // Build steps. key[e] = bin(row) * nblocks + block(src[e]), row[e] = row of in-CSR slot e.
 hipError_t launch_bin_keys(const uint32_t *off, const uint32_t *src, uint32_t N, uint32_t B, uint32_t nblocks,
                            uint32_t *key, uint32_t *row, hipStream_t s);
-// For slot j (in-CSR slot perm2[j]): dst[j], u2[j] = source, key1[j] = source block.
-hipError_t launch_bin_slots(const uint32_t *perm2, const uint32_t *src, const uint32_t *row, size_t M, uint32_t B,
-                            uint16_t *dst, uint32_t *u2, uint32_t *key1, hipStream_t s);
-// flag[j] = slot j starts a run (its sorted tile key differs from slot j-1's).
+// For slot j (in-CSR slot perm2[j], slots sorted by tile key key2s, then source): dst16[j] with its flags, pf[j] =
+// it starts a pair.
+hipError_t launch_pair_slots(const uint32_t *perm2, const uint32_t *src, const uint32_t *row, const uint32_t *key2s,
+                             size_t M, uint32_t B, uint16_t *dst16, uint32_t *pf, hipStream_t s);
+// For the pair q = pid[j] of each pair-starting slot j: psrc[q], pkey2[q] (tile key), key1[q] (source block).
+hipError_t launch_pair_fill(const uint32_t *pf, const uint32_t *pid, const uint32_t *perm2, const uint32_t *src,
+                            const uint32_t *key2s, size_t M, uint32_t B, uint32_t *psrc, uint32_t *pkey2,
+                            uint32_t *key1, hipStream_t s);
+// flag[j] = entry j starts a run (its sorted tile key differs from entry j-1's).
 hipError_t launch_run_flags(const uint32_t *key2s, size_t M, uint32_t *flag, hipStream_t s);
-// For X entry k (slot perm1[k]): src16[k], dst16[k], inv[perm1[k]] = k.
-hipError_t launch_bin_expand(const uint32_t *perm1, const uint32_t *u2, const uint16_t *dst, size_t M, uint32_t B,
-                             uint16_t *src16, uint16_t *dst16, uint32_t *inv, hipStream_t s);
-// Run tables from the flags and their exclusive scan rid.
-hipError_t launch_bin_runs(const uint32_t *flag, const uint32_t *rid, const uint32_t *inv, size_t M, uint32_t *run_j,
-                           uint32_t *run_x, uint32_t *grp_run, hipStream_t s);
+// For X entry k (pair perm1[k]): src16[k], inv[perm1[k]] = k.
+hipError_t launch_pair_expand(const uint32_t *perm1, const uint32_t *psrc, size_t Q, uint32_t B, uint16_t *src16,
+                              uint32_t *inv, hipStream_t s);
+// run_x from the pairs' run flags rf and their exclusive scan rid.
+hipError_t launch_pair_runs(const uint32_t *rf, const uint32_t *rid, const uint32_t *inv, size_t Q, uint32_t *run_x,
+                            hipStream_t s);
+// grp[i] = (X entry, run) of the pair of slot 64 i.
+hipError_t launch_group_table(const uint32_t *pf, const uint32_t *pid, const uint32_t *rf, const uint32_t *rid,
+                              const uint32_t *inv, size_t M, uint2 *grp, hipStream_t s);
 // start[s] = lower bound of s in the sorted key[0..M), s = 0..n.
 hipError_t launch_lower_bounds(const uint32_t *key, size_t M, uint32_t n, uint32_t *start, hipStream_t s);
 // Stable radix sort of (key, value) on key bits [0, end_bit); val_in null means values 0, 1, 2, ...
