@@ -17,9 +17,9 @@
 // pairs are one run of consecutive X entries; X is stored block-major (block s, then bin b, then source), so that
 // expand writes it sequentially (pairs of entries, 16 bytes a lane) with a 16-bit local source id per entry.
 // Accumulate finds a slot's entry without a per-slot index: each slot's 16-bit local destination id carries two
-// flags, "starts a pair" and "starts a run", and a table gives, per 64 slots, the entry and run of the first slot's
-// pair; a wave counts the flags with ballots (the entry advances by one per pair start, and jumps to the run
-// table's entry at a run start).
+// flags, "starts a pair" and "starts a run", and a table gives, per 64 slots, the pair and run of the first slot
+// (with the X offsets of that run and the next); a wave counts the flags with ballots and mbcnt, and the run
+// offset turns a pair number into its X entry.
 // Per edge and iteration: accumulate reads the 2-byte slot; per pair, expand reads 2 bytes and writes 8 and
 // accumulate reads the 8 back (repeated loads of one entry by neighbouring lanes share a cache line).
 // LDS additions are made in arrival order, so the sums repeat only to rounding, not bit for bit. Every vertex gets
@@ -156,21 +156,24 @@ __global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const d
 }
 
 constexpr uint32_t kPairStart = 0x8000u, kRunStart = 0x4000u, kLocalMask = 0x3FFFu;
+constexpr int kAccGroups = 2;  // groups of 64 slots a wave of accumulate has in flight (2 measured faster than 4 or 8)
 
-// X entry of the pair of slot g + lane, for a wave-uniform g that is a multiple of 64, from d (the slot's 16-bit
-// local destination id with its flags): lane 0's entry and run come from the group table (scalar loads); a pair
-// start at a lane in 1..lane advances the entry by one, and the latest run start at or before the lane resets it to
-// that run's first entry. All lanes must be active (slots past M read as 0: no flags).
-__device__ __forceinline__ uint32_t pair_entry(const BinnedPlan &P, uint32_t g, uint32_t d, uint32_t lane) {
-  const uint2 t = P.grp[__builtin_amdgcn_readfirstlane(g) >> 6];
-  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // lanes 0 .. lane
-  const unsigned long long pairs = __ballot((d & kPairStart) != 0) & ~1ull;
-  const unsigned long long runs = __ballot((d & kRunStart) != 0) & ~1ull & upto;
-  const uint32_t x0 = __builtin_amdgcn_readfirstlane(t.x), r0 = __builtin_amdgcn_readfirstlane(t.y);
-  if (!runs) return x0 + uint32_t(__popcll(pairs & upto));
-  const uint32_t last = 63u - uint32_t(__clzll(runs));  // the latest run start, in 1 .. lane
-  const unsigned long long after = upto & ~(last == 63 ? ~0ull : ((2ull << last) - 1));  // lanes last+1 .. lane
-  return P.run_x[r0 + uint32_t(__popcll(runs))] + uint32_t(__popcll(pairs & after));
+// X entry of the pair of slot g + lane, for a group of 64 slots starting at a multiple of 64, from t (the group
+// table: run r0 of slot g, run_off of runs r0 and r0 + 1, pair q0 of slot g) and d (the slot's 16-bit local
+// destination id with its flags). Pairs are numbered in slot order; a run's pairs are consecutive in both orders,
+// so pair q of run r sits at X entry q + run_off[r]. Lane L's pair is q0 plus the pair starts among lanes 1..L, its
+// run r0 plus the run starts among them (mbcnt of the ballots shifted down by one lane); only lanes past a second
+// run start in the group load run_off. All lanes must be active (slots past M read as 0: no flags).
+__device__ __forceinline__ uint32_t pair_entry(const BinnedPlan &P, uint4 t, uint32_t d) {
+  const unsigned long long pairs = __ballot((d & kPairStart) != 0) >> 1;
+  const unsigned long long runs = __ballot((d & kRunStart) != 0) >> 1;
+  const uint32_t r0 = __builtin_amdgcn_readfirstlane(t.x), off0 = __builtin_amdgcn_readfirstlane(t.y);
+  const uint32_t off1 = __builtin_amdgcn_readfirstlane(t.z), q0 = __builtin_amdgcn_readfirstlane(t.w);
+  const uint32_t q = __builtin_amdgcn_mbcnt_hi(uint32_t(pairs >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(pairs), q0));
+  const uint32_t c = __builtin_amdgcn_mbcnt_hi(uint32_t(runs >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(runs), 0u));
+  uint32_t off = c == 0 ? off0 : off1;
+  if (c > 1) off = P.run_off[r0 + c];
+  return q + off;
 }
 
 // Adds one wave's 64 slots into the LDS sums. All lanes must be active; lanes outside the item carry ok = false.
@@ -217,30 +220,42 @@ __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, Binne
   const int w = threadIdx.x / kWave;
   const uint32_t lane = threadIdx.x & (kWave - 1);
   // Groups of 64 slots aligned to multiples of 64 (the slot array is padded to a multiple of 64), dealt to the
-  // waves in turn; slots outside the item are masked.
-  auto value = [&](uint32_t g, uint32_t d, bool ok) {
-    double x = __builtin_nontemporal_load(P.X + pair_entry(P, g, d, lane));
-    if constexpr (WEIGHTED) x *= ok ? double(P.ws[g + lane]) : 0.0;
-    return x;
-  };
+  // waves in turn, kAccGroups groups a batch; slots outside the item are masked. Software-pipelined: a batch's
+  // X loads are issued, then the next batch's table and slot loads, so that the wait for X overlaps them. Loads of
+  // groups past the item's end read the batch's first group again (always in bounds) and are not added.
+  auto weight = [&](uint32_t j, bool ok) { return WEIGHTED && ok ? double(P.ws[j]) : 1.0; };
   uint32_t g = __builtin_amdgcn_readfirstlane((it.y & ~63u) + uint32_t(w) * kWave);  // wave-uniform
-  for (; g + 3 * kBinBlock < it.z; g += 4 * kBinBlock) {
-    const bool ok0 = g + lane >= it.y;  // groups 1..3 start past it.y; all four end before it.z except possibly 3
-    const bool ok3 = g + 3 * kBinBlock + lane < it.z;
-    const uint32_t d0 = ld_nt(P.dst16 + g + lane), d1 = ld_nt(P.dst16 + g + kBinBlock + lane);
-    const uint32_t d2 = ld_nt(P.dst16 + g + 2 * kBinBlock + lane), d3 = ld_nt(P.dst16 + g + 3 * kBinBlock + lane);
-    const double x0 = value(g, d0, ok0), x1 = value(g + kBinBlock, d1, true);
-    const double x2 = value(g + 2 * kBinBlock, d2, true), x3 = value(g + 3 * kBinBlock, d3, ok3);
-    add_slots<EXT>(sum, ext, d0 & kLocalMask, x0, ok0);
-    add_slots<EXT>(sum, ext, d1 & kLocalMask, x1, true);
-    add_slots<EXT>(sum, ext, d2 & kLocalMask, x2, true);
-    add_slots<EXT>(sum, ext, d3 & kLocalMask, x3, ok3);
-  }
-  for (; g < it.z; g += kBinBlock) {
-    const bool ok = g + lane >= it.y && g + lane < it.z;
-    const uint32_t d = ld_nt(P.dst16 + g + lane);
-    const double x = value(g, d, ok);
-    add_slots<EXT>(sum, ext, d & kLocalMask, x, ok);
+  if (g < it.z) {
+    uint4 t[kAccGroups];
+    uint32_t d[kAccGroups];
+    auto fetch = [&](uint32_t gb, uint32_t gv) {  // batch at gb; gv: a group start below it.z
+#pragma unroll
+      for (int u = 0; u < kAccGroups; ++u) {
+        const uint32_t gu = gb + u * kBinBlock < it.z ? gb + u * kBinBlock : gv;
+        t[u] = P.grp[gu >> 6];
+        d[u] = ld_nt(P.dst16 + gu + lane);
+      }
+    };
+    fetch(g, g);
+    for (; g < it.z; g += kAccGroups * kBinBlock) {
+      uint32_t dc[kAccGroups], k[kAccGroups];
+      double x[kAccGroups];
+#pragma unroll
+      for (int u = 0; u < kAccGroups; ++u) {
+        dc[u] = d[u];
+        k[u] = pair_entry(P, t[u], d[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < kAccGroups; ++u) x[u] = __builtin_nontemporal_load(P.X + k[u]);
+      fetch(g + kAccGroups * kBinBlock, g);
+#pragma unroll
+      for (int u = 0; u < kAccGroups; ++u) {
+        const uint32_t gu = g + u * kBinBlock;
+        if (gu >= it.z) break;
+        const bool ok = gu + lane >= it.y && gu + lane < it.z;
+        add_slots<EXT>(sum, ext, dc[u] & kLocalMask, x[u] * weight(gu + lane, ok), ok);
+      }
+    }
   }
   __syncthreads();
   const uint32_t b0 = it.x * uint32_t(B);
@@ -404,22 +419,25 @@ __global__ __launch_bounds__(kBlock) void pair_expand_kernel(const uint32_t *per
   }
 }
 
-// run_x[rid[q]] = inv[q] for every run-starting pair q (rid: exclusive scan of the run flags rf).
+// run_off[rid[q]] = inv[q] - q for every run-starting pair q (rid: exclusive scan of the run flags rf): pair p of
+// that run is X entry p + run_off (modulo 2^32).
 __global__ __launch_bounds__(kBlock) void pair_runs_kernel(const uint32_t *rf, const uint32_t *rid,
-                                                           const uint32_t *inv, size_t Q, uint32_t *run_x) {
+                                                           const uint32_t *inv, size_t Q, uint32_t *run_off) {
   for (size_t q = size_t(blockIdx.x) * kBlock + threadIdx.x; q < Q; q += size_t(gridDim.x) * kBlock)
-    if (rf[q]) run_x[rid[q]] = inv[q];
+    if (rf[q]) run_off[rid[q]] = inv[q] - uint32_t(q);
 }
 
-// grp[i] = (X entry, run) of the pair of slot 64 i.
+// grp[i] = (run r, run_off[r], run_off[r + 1] (0 past the last run), pair) of slot 64 i.
 __global__ __launch_bounds__(kBlock) void group_table_kernel(const uint32_t *pf, const uint32_t *pid,
                                                              const uint32_t *rf, const uint32_t *rid,
-                                                             const uint32_t *inv, size_t M, uint2 *grp) {
+                                                             const uint32_t *run_off, uint32_t n_runs, size_t M,
+                                                             uint4 *grp) {
   const size_t G = (M + 63) / 64;
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < G; i += size_t(gridDim.x) * kBlock) {
     const size_t j = i * 64;
     const uint32_t q = pid[j] + pf[j] - 1;  // slot 0 starts pair 0, so q >= 0
-    grp[i] = make_uint2(inv[q], rid[q] + rf[q] - 1);
+    const uint32_t r = rid[q] + rf[q] - 1;
+    grp[i] = make_uint4(r, run_off[r], r + 1 < n_runs ? run_off[r + 1] : 0u, q);
   }
 }
 
@@ -508,17 +526,18 @@ hipError_t launch_pair_expand(const uint32_t *perm1, const uint32_t *psrc, size_
   return hipGetLastError();
 }
 
-hipError_t launch_pair_runs(const uint32_t *rf, const uint32_t *rid, const uint32_t *inv, size_t Q, uint32_t *run_x,
-                            hipStream_t s) {
+hipError_t launch_pair_runs(const uint32_t *rf, const uint32_t *rid, const uint32_t *inv, size_t Q,
+                            uint32_t *run_off, hipStream_t s) {
   if (!Q) return hipSuccess;
-  pair_runs_kernel<<<grid(Q, kBlock, 16384), kBlock, 0, s>>>(rf, rid, inv, Q, run_x);
+  pair_runs_kernel<<<grid(Q, kBlock, 16384), kBlock, 0, s>>>(rf, rid, inv, Q, run_off);
   return hipGetLastError();
 }
 
 hipError_t launch_group_table(const uint32_t *pf, const uint32_t *pid, const uint32_t *rf, const uint32_t *rid,
-                              const uint32_t *inv, size_t M, uint2 *grp, hipStream_t s) {
+                              const uint32_t *run_off, uint32_t n_runs, size_t M, uint4 *grp, hipStream_t s) {
   if (!M) return hipSuccess;
-  group_table_kernel<<<grid((M + 63) / 64, kBlock, 16384), kBlock, 0, s>>>(pf, pid, rf, rid, inv, M, grp);
+  group_table_kernel<<<grid((M + 63) / 64, kBlock, 16384), kBlock, 0, s>>>(pf, pid, rf, rid, run_off, n_runs, M,
+                                                                            grp);
   return hipGetLastError();
 }
 

@@ -396,8 +396,9 @@ void free_binned(prh_context *c, BinnedCache &bc) {
 //   2. per slot: its local destination with the pair and run start flags; an exclusive scan of the pair starts
 //      numbers the pairs in slot order, and each pair records its source, tile key and key1 = its source block;
 //   3. a stable sort of the pairs by key1 gives the X order (block, bin, source), perm1[k] = pair of X entry k;
-//      src16 in X order; runs (tiles) start where the pairs' tile key changes: their first X entries, and per 64
-//      slots the X entry and run of the first slot's pair;
+//      src16 in X order; runs (tiles) start where the pairs' tile key changes: per run, its first X entry minus
+//      its first pair number, and per 64 slots the pair and run of the first slot (with the offsets of that run
+//      and the next);
 //   4. block starts in X order (lower bounds of the sorted key1) and bin starts from the row degrees cut the two
 //      orders into items; split bins get partial rows.
 // Builds the plan of the pull CSR (off, idx) with rows of degree rowdeg (host), N rows, M slots, into bc.
@@ -443,13 +444,13 @@ int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, co
   prh::BinnedPlan P{};
   P.B = B;
   P.N = N;
-  const uint64_t Mp = (M + 63) / 64 * 64;  // slots padded to whole groups of 64
+  const uint64_t Mp = (M + 127) / 128 * 128;  // slots padded to whole pairs of 64-slot groups
   uint16_t *dst16, *src16 = nullptr;
-  uint2 *grp;
-  uint32_t *run_x = nullptr;
+  uint4 *grp;
+  uint32_t *run_off = nullptr;
   double *X = nullptr;
   PRH_CHECK(keep(&dst16, Mp * 2));
-  PRH_CHECK(keep(&grp, (M + 63) / 64 * sizeof(uint2)));
+  PRH_CHECK(keep(&grp, (M + 63) / 64 * sizeof(uint4)));
   std::vector<uint32_t> bstart(nb + 1, 0);
   uint32_t n_runs = 0;
   uint64_t Q = 0;
@@ -501,9 +502,9 @@ int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, co
     PRH_HIP(c, hipMemcpyAsync(&last[1], rf + (Q - 1), 4, hipMemcpyDeviceToHost, s));
     PRH_HIP(c, hipStreamSynchronize(s));
     n_runs = last[0] + last[1];
-    PRH_CHECK(keep(&run_x, size_t(n_runs) * 4));
-    PRH_HIP(c, prh::launch_pair_runs(rf, rid, inv, Q, run_x, s));
-    PRH_HIP(c, prh::launch_group_table(pa, tk, rf, rid, inv, M, grp, s));
+    PRH_CHECK(keep(&run_off, size_t(n_runs) * 4));
+    PRH_HIP(c, prh::launch_pair_runs(rf, rid, inv, Q, run_off, s));
+    PRH_HIP(c, prh::launch_group_table(pa, tk, rf, rid, run_off, n_runs, M, grp, s));
     // 4. block starts in X order.
     PRH_HIP(c, prh::launch_lower_bounds(k1s, Q, uint32_t(nb), starts, s));
     PRH_HIP(c, hipMemcpyAsync(bstart.data(), starts, (nb + 1) * 4, hipMemcpyDeviceToHost, s));
@@ -561,7 +562,7 @@ int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, co
   P.src16 = src16;
   P.dst16 = dst16;
   P.X = X;
-  P.run_x = run_x;
+  P.run_off = run_off;
   P.grp = grp;
   P.n_runs = n_runs;
   P.Q = Q;

@@ -158,8 +158,8 @@ struct BinnedPlan {
   const uint16_t *src16;   // [Q] X order: source - s * B
   const uint16_t *dst16;   // [M padded to 64] slot order: destination - b * B, | 0x8000 pair start, | 0x4000 run start
   double *X;               // [Q] one entry per (source, destination bin) pair, block-major
-  const uint32_t *run_x;   // [n_runs] first X entry of each run (a tile's pairs), runs in slot order
-  const uint2 *grp;        // [ceil(M / 64)] X entry and run of the pair of slot 64 i
+  const uint32_t *run_off; // [n_runs] per run (a tile's pairs, runs in slot order): X entry - pair number, mod 2^32
+  const uint4 *grp;        // [ceil(M / 64)] of slot 64 i: run r, run_off[r], run_off[r + 1], pair number
   uint32_t n_runs;
   const uint4 *ex_item;    // [n_ex]
   const uint4 *ac_item;    // [n_ac]
@@ -234,12 +234,12 @@ hipError_t launch_run_flags(const uint32_t *key2s, size_t M, uint32_t *flag, hip
 // For X entry k (pair perm1[k]): src16[k], inv[perm1[k]] = k.
 hipError_t launch_pair_expand(const uint32_t *perm1, const uint32_t *psrc, size_t Q, uint32_t B, uint16_t *src16,
                               uint32_t *inv, hipStream_t s);
-// run_x from the pairs' run flags rf and their exclusive scan rid.
-hipError_t launch_pair_runs(const uint32_t *rf, const uint32_t *rid, const uint32_t *inv, size_t Q, uint32_t *run_x,
-                            hipStream_t s);
-// grp[i] = (X entry, run) of the pair of slot 64 i.
+// run_off from the pairs' run flags rf, their exclusive scan rid and the pairs' X entries inv.
+hipError_t launch_pair_runs(const uint32_t *rf, const uint32_t *rid, const uint32_t *inv, size_t Q,
+                            uint32_t *run_off, hipStream_t s);
+// grp[i] = (run r, run_off[r], run_off[r + 1], pair) of slot 64 i.
 hipError_t launch_group_table(const uint32_t *pf, const uint32_t *pid, const uint32_t *rf, const uint32_t *rid,
-                              const uint32_t *inv, size_t M, uint2 *grp, hipStream_t s);
+                              const uint32_t *run_off, uint32_t n_runs, size_t M, uint4 *grp, hipStream_t s);
 // start[s] = lower bound of s in the sorted key[0..M), s = 0..n.
 hipError_t launch_lower_bounds(const uint32_t *key, size_t M, uint32_t n, uint32_t *start, hipStream_t s);
 // Stable radix sort of (key, value) on key bits [0, end_bit); val_in null means values 0, 1, 2, ...
