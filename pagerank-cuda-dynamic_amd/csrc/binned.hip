@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, Binne
         k[u] = pair_entry(P, t[u], d[u]);
       }
 #pragma unroll
-      for (int u = 0; u < kAccGroups; ++u) x[u] = __builtin_nontemporal_load(P.X + k[u]);
+      for (int u = 0; u < kAccGroups; ++u) x[u] = P.X[k[u]];
       fetch(g + kAccGroups * kBinBlock, g);
 #pragma unroll
       for (int u = 0; u < kAccGroups; ++u) {

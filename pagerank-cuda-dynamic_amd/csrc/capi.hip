@@ -514,12 +514,12 @@ int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, co
     PRH_CHECK(keep(&src16, 2));
     PRH_CHECK(keep(&X, 16));
   }
-  // Items: expand items of at most 8 per CU over all pairs, accumulate items of at most 4 per CU over all slots
+  // Items: expand items of at most 32 per CU over all pairs, accumulate items of at most 4 per CU over all slots
   // (fewer split bins, hence fewer partial rows); both at least 16K entries.
   hipDeviceProp_t prop{};
   PRH_HIP(c, hipGetDeviceProperties(&prop, c->device));
   const uint64_t cus = uint64_t(std::max(prop.multiProcessorCount, 1));
-  const uint64_t span_ex = std::max<uint64_t>(Q / (cus * 8), 16384);
+  const uint64_t span_ex = std::max<uint64_t>(Q / (cus * 32), 16384);
   const uint64_t span = std::max<uint64_t>(M / (cus * 4), 16384);
   std::vector<uint4> ex, ac, multi;
   for (uint64_t b = 0; b < nb; ++b)

@@ -4,7 +4,10 @@ It changes only the summation order (LDS additions in arrival order), so the bar
 L-inf 1e-9 of the oracle's restatement of the reference GPU path, iterations equal or +-1. The graphs cover one
 bin smaller than B, several bins with a partial last one, bins split over several items (items hold at least 16K
 slots: RMAT-14's heaviest 1024 rows hold 148K in-edges, finished a thread per vertex; RMAT-15's first 16384 rows
-hold more than 16 items' worth, finished 16 waves per 64 vertices), and rows without in-edges.
+hold more than 16 items' worth, finished 16 waves per 64 vertices), rows without in-edges, one destination for
+all slots of a group (the star), and groups of 64 slots spanning several short runs (a sparse uniform graph: about
+ten edges per nonempty off-diagonal (block, bin) tile at B = 1024, so most groups hold more than two runs and their
+later lanes take their X offset from the run table rather than from the group table).
 """
 from __future__ import annotations
 
@@ -41,6 +44,11 @@ def _graph(name):
         return star_graph(5000)
     if name == "grid70x50":
         return HostGraph.grid(70, 50, 0.62, seed=5)
+    if name == "uniform200k":
+        n, m = 200_000, 400_000
+        rng = np.random.default_rng(8)
+        return HostGraph.from_edges(n, rng.integers(0, n, m).astype(np.uint32), rng.integers(0, n, m).astype(np.uint32),
+                                    self_loops=True)
     n = 3000  # rows without in-edges, no self-loops
     u = np.arange(n, dtype=np.uint32)
     src = np.concatenate([u[1:], [0], u[:100]])
@@ -49,7 +57,7 @@ def _graph(name):
 
 
 CASES = [("rmat14", 1024), ("rmat14", 4096), ("rmat12x24", 2048), ("star5000", 8192), ("grid70x50", 1024),
-         ("no_in_edges", 1024), ("rmat14", 16384), ("rmat15", 16384)]
+         ("no_in_edges", 1024), ("rmat14", 16384), ("rmat15", 16384), ("uniform200k", 1024)]
 
 
 @pytest.mark.parametrize("name,blocking", CASES, ids=[f"{a}_B{b}" for a, b in CASES])
