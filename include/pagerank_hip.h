@@ -88,9 +88,9 @@ typedef struct prh_tuning {
                               found cheaper; 2 pull. The flags, hence the results, are the same for all [1] */
   int blocking;            /* every method, the weighted solves too: 0 = pull kernels above; 1024, 2048, 4096,
                               8192 or 16384 = propagation blocking with source blocks and destination bins of this
-                              many vertices (each iteration expands contributions into a per-edge buffer through
-                              LDS, then adds them per bin in LDS; takes precedence over source_slices; sums repeat
-                              to rounding, not bit for bit) [0] */
+                              many vertices (each iteration expands contributions through LDS into a buffer with
+                              one entry per (source, destination bin) pair, then adds them per bin in LDS; takes
+                              precedence over source_slices; sums repeat to rounding, not bit for bit) [0] */
 } prh_tuning;
 
 typedef struct prh_context prh_context;
