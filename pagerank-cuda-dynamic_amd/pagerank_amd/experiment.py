@@ -295,6 +295,9 @@ def run_ppr(args, out: CsvLog) -> int:
 
 
 def run_adjoint(args, out: CsvLog) -> int:
+    import torch  # before the engine library loads: torch's HIP runtime must serve the process
+
+    from .autograd import pagerank as pagerank_autograd
     g, name = make_rmat(args.scale, args.edge_factor, args.seed)
     ctx = Context(0)
     autotune(ctx, g, args)  # the plain static solve compared against uses the best launch plan
@@ -306,18 +309,42 @@ def run_adjoint(args, out: CsvLog) -> int:
     o = PagerankOptions(repeat=args.repeat)
     engine.weighted_pagerank(yg, w, s, PagerankOptions())  # warm-up
     fw = engine.weighted_pagerank(yg, w, s, o)
+    engine.weighted_adjoint(yg, w, fw.ranks, 2 * c * fw.ranks, PagerankOptions())  # warm-up (builds its plan)
     t0 = time.perf_counter()
     reps = args.repeat
     for _ in range(reps):
         lam, gw, gs, it = engine.weighted_adjoint(yg, w, fw.ranks, 2 * c * fw.ranks, o)
     adj_ms = (time.perf_counter() - t0) / reps * 1e3  # host wall time including the host<->device copies
+    # The same pair through torch.autograd on device tensors (no host copies): forward, then backward timed alone.
+    dev = torch.device("cuda", 0)
+    wt = torch.tensor(w, device=dev, requires_grad=True)
+    st_ = torch.tensor(s, device=dev, requires_grad=True)
+    ct = torch.tensor(c, device=dev)
+    fwd_ms = bwd_ms = 0.0
+    for rep in range(reps + 1):  # the first pass warms up
+        wt.grad = st_.grad = None
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rt = pagerank_autograd(wt, st_, yg, o)
+        loss = (ct * rt * rt).sum()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        if rep:
+            fwd_ms += (t2 - t1) * 1e3 / reps
+            bwd_ms += (t3 - t2) * 1e3 / reps
     static = engine.pagerankStaticHip(yg, o)
     log(f"forward weighted solve: {fw.time:.3f} ms, {fw.iterations} iterations "
         f"({g.M * fw.iterations / (fw.computationTime / 1e3) / 1e9:.1f} GEdges/s/iter)")
     log(f"adjoint solve + gradient assembly: {adj_ms:.3f} ms wall incl. copies, {it} iterations")
+    log(f"torch.autograd on device tensors: forward {fwd_ms:.3f} ms, backward (adjoint + gradients) {bwd_ms:.3f} ms "
+        f"wall")
     log(f"plain static solve: {static.time:.3f} ms, {static.iterations} iterations; forward+adjoint / static "
         f"{(fw.time + adj_ms) / static.time:.2f}x")
     for tech, t, its in (("weighted_forward", fw.time, fw.iterations), ("weighted_adjoint_wall", adj_ms, it),
+                         ("autograd_forward_wall", fwd_ms, fw.iterations), ("autograd_backward_wall", bwd_ms, it),
                          ("static", static.time, static.iterations)):
         out.row(mode="adjoint", graph=name, vertices=g.N, edges=g.M, technique=tech, time_ms=t, iterations=its)
     return 0

@@ -60,3 +60,16 @@ def test_dynamic_mode(tmp_path):
     assert len(got) == 2 * 2 * 5
     for r in got:
         assert float(r["l1_error"]) < _bound(r), r
+
+
+def test_adjoint_mode(tmp_path):
+    """Adjoint mode on a small RMAT graph: the host-array pair and the torch.autograd pair on device tensors both
+    run and report their times."""
+    from pagerank_amd import experiment
+    out = tmp_path / "a.csv"
+    assert experiment.main(["adjoint", "--scale", "12", "--repeat", "1", "--csv", str(out)]) == 0
+    rows = {r["technique"]: r for r in _rows(out)}
+    assert set(rows) == {"weighted_forward", "weighted_adjoint_wall", "autograd_forward_wall",
+                         "autograd_backward_wall", "static"}
+    assert all(float(r["time_ms"]) > 0 for r in rows.values())
+    assert int(rows["autograd_backward_wall"]["iterations"]) == int(rows["weighted_adjoint_wall"]["iterations"])
