@@ -441,11 +441,11 @@ __global__ __launch_bounds__(kBlock) void group_table_kernel(const uint32_t *pf,
   }
 }
 
-
+// ws[j] = w[eid[slot[j]]] (weighted solves: the weight of each slot's edge, in slot order).
 __global__ __launch_bounds__(kBlock) void binned_weights_kernel(const float *w, const uint32_t *eid,
-                                                                const uint32_t *slot, size_t M, float *wx) {
+                                                                const uint32_t *slot, size_t M, float *ws) {
   for (size_t k = size_t(blockIdx.x) * kBlock + threadIdx.x; k < M; k += size_t(gridDim.x) * kBlock)
-    wx[k] = w[eid[slot[k]]];
+    ws[k] = w[eid[slot[k]]];
 }
 
 // start[s] = first k with key[k] >= s, for s = 0 .. n (key sorted ascending).
@@ -541,10 +541,10 @@ hipError_t launch_group_table(const uint32_t *pf, const uint32_t *pid, const uin
   return hipGetLastError();
 }
 
-hipError_t launch_binned_weights(const float *w, const uint32_t *eid, const uint32_t *slot, size_t M, float *wx,
+hipError_t launch_binned_weights(const float *w, const uint32_t *eid, const uint32_t *slot, size_t M, float *ws,
                                  hipStream_t s) {
   if (!M) return hipSuccess;
-  binned_weights_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(w, eid, slot, M, wx);
+  binned_weights_kernel<<<grid(M, kBlock, 16384), kBlock, 0, s>>>(w, eid, slot, M, ws);
   return hipGetLastError();
 }
 
