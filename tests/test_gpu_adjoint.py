@@ -40,7 +40,7 @@ def _star_case(n=12000, seed=6):
     return y, rng.uniform(0.5, 1.5, y.M).astype(np.float32), rng.standard_normal(y.N), rng.dirichlet(np.ones(y.N))
 
 
-@pytest.mark.parametrize("blocking", [0, 1024], ids=["rows", "blocked"])
+@pytest.mark.parametrize("blocking", [0, 1024, 4096], ids=["rows", "blocked", "blocked4096"])
 @pytest.mark.parametrize("case", ["rmat11", "rmat14", "star12000"])
 def test_forward_and_adjoint_match_oracle(ctx, oracle_mod, case, blocking):
     """rmat11/rmat14: rows of every degree class; star12000: a hub whose in-row (forward) and out-row (adjoint)

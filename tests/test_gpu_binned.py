@@ -85,7 +85,8 @@ def test_binned_naive_dynamic(ctx, oracle_mod):
     assert np.abs(res.ranks - ref).max() <= RANK_TOL
 
 
-DYN = [(2, 1024, 1e-3), (3, 2048, 1e-3), (4, 1024, 1e-3), (3, 4096, 1e-2), (4, 2048, 1e-4)]
+DYN = [(2, 1024, 1e-3), (3, 2048, 1e-3), (4, 1024, 1e-3), (3, 4096, 1e-2), (4, 2048, 1e-4), (4, 8192, 1e-4),
+       (2, 16384, 1e-3)]
 
 
 @pytest.mark.parametrize("method,blocking,frac", DYN, ids=[f"m{m}_B{b}_{f:g}" for m, b, f in DYN])
