@@ -1985,7 +1985,7 @@ int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int
       cand.push_back(t);
     }
   if (g->M >= (uint64_t(1) << 20))
-    for (int bw : {4096, 8192, 16384}) {
+    for (int bw : {2048, 4096, 8192, 16384}) {
       const uint64_t nb = (uint64_t(g->N) + bw - 1) / bw;
       if (nb * nb > (uint64_t(1) << 32)) continue;
       prh_tuning t = base;
