@@ -43,6 +43,8 @@ struct prh_context {
   DevBuf<unsigned long long> loop;  // [L] error bits, then one slot holding `done`
   DevBuf<int> flags;                // [0] validation bits, [1] traversal "changed", [2] extension direction
   DevBuf<double> ext_stats;         // per rank workgroup: pull and push costs of the next extension
+  DevBuf<uint32_t> ext_chunks;      // initial extension: chunk counts and their scan per batch source
+  DevBuf<uint8_t> ext_tmp;          // its scan's temporary storage
   // batched personalized PageRank
   DevBuf<double> ppr_rank[2], ppr_contrib[2], ppr_partial, ppr_mass, ppr_out;
   DevBuf<int32_t> ppr_row;
@@ -752,8 +754,15 @@ int mark_affected(prh_context *c, const prh_graph *g, int method, const DevBatch
   PRH_HIP(c, prh::launch_fill_u8(c->naff.p, 0, N, s));
   if (b.ND + b.NI == 0) return PRH_SUCCESS;
   PRH_HIP(c, prh::launch_mark_partial(c->naff.p, c->vaff.p, b.delu, b.delv, b.ND, b.insu, b.NI, s));
-  prh::ExtendArgs E{c->vaff.p, c->naff.p, g->l_out_off, g->l_out_dst, out_s, nullptr, nullptr};
-  PRH_HIP(c, prh::launch_extend(E, nullptr, nullptr, 0, 0.0, s));
+  // The flagged vertices are the batch's sources: extend from the batch list.
+  const size_t nb = b.ND + b.NI + 1;
+  PRH_CHECK(ensure(c, c->ext_chunks, 2 * nb));
+  size_t tmp_bytes = 0;
+  PRH_HIP(c, prh::launch_extend_batch(c->vaff.p, b.delu, b.ND, b.insu, b.NI, g->l_out_off, g->l_out_dst,
+                                      c->ext_chunks.p, c->ext_chunks.p + nb, nullptr, tmp_bytes, s));
+  PRH_CHECK(ensure(c, c->ext_tmp, std::max<size_t>(tmp_bytes, 16)));
+  PRH_HIP(c, prh::launch_extend_batch(c->vaff.p, b.delu, b.ND, b.insu, b.NI, g->l_out_off, g->l_out_dst,
+                                      c->ext_chunks.p, c->ext_chunks.p + nb, c->ext_tmp.p, tmp_bytes, s));
   if (method != PRH_DYNAMIC_TRAVERSAL) return PRH_SUCCESS;
   // Traversal closure: breadth-first levels from every marked vertex until no vertex is newly marked.
   PRH_CHECK(ensure(c, c->flags, 4));
@@ -872,8 +881,10 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
     PRH_HIP(c, hipEventRecord(c->ev[0], s));
     // Initialisation. The degree classes the reference partitions here (pagerankCuda.hxx:811-814) are the row
     // ranges of the layout, fixed when the graph was prepared; the extension kernels' out-degree split is per solve.
+    // The out-degree schedule serves the traversal levels and the row plan's frontier extension; the initial
+    // extension runs from the batch list and the blocked plan extends inside its update.
     prh::Schedule out_s{};
-    if (dynamic) PRH_CHECK(out_schedule(c, g, out_s));
+    if (dynamic && (method == PRH_DYNAMIC_TRAVERSAL || (frontier && !binned))) PRH_CHECK(out_schedule(c, g, out_s));
     PRH_HIP(c, prh::launch_init_ranks(c->rank[0].p, c->rank[1].p, c->contrib[0].p, c->contrib[1].p, g->l_outdeg, N,
                                       q != nullptr, s));
     PRH_HIP(c, hipEventRecord(c->ev[1], s));
@@ -1650,6 +1661,8 @@ void prh_context_destroy(prh_context *c) {
   release(c, c->tiles);
   release(c, c->counts);
   release(c, c->ext_stats);
+  release(c, c->ext_chunks);
+  release(c, c->ext_tmp);
   release(c, c->batch);
   release(c, c->scratch_u32);
   release(c, c->loop);

@@ -7,6 +7,9 @@
 //                 split by out-degree: 8 lanes per low-out-degree vertex, one wave per high-out-degree vertex.
 //                 With `fresh` set it is one breadth-first level: only newly marked vertices enter `fresh`
 //                 and `changed` records whether any vertex was newly marked (the DT closure, :438-456).
+//   extend_batch  the same marking for the initial extension, whose flagged vertices are exactly the batch's
+//                 sources: driven by the batch list instead of a scan of every vertex, with each source's edges
+//                 cut into chunks of kBatchChunk spread over all the waves (a hub's edges are not left to one).
 //   direction /   the same extension in pull form (the direction-optimizing idea of Beamer et al., SC 2012):
 //   extend_pull   every unaffected row w takes vaff[w] = 1 when an in-neighbour is in the frontier. The rank
 //                 launch leaves per-workgroup costs of both forms (in-edges of unaffected rows, out-edges of
@@ -79,6 +82,50 @@ __global__ __launch_bounds__(kBlock) void extend_kernel(ExtendArgs A, uint32_t n
       // Only reached if the schedule's low class was split above 64.
       for (uint32_t e = beg + kLowLanes * kLowPerLane + j; e < end; e += kLowLanes) mark(A, A.dst[e]);
     }
+  }
+}
+
+constexpr uint32_t kBatchChunk = 1024;  // edges of one source per wave-sized work unit of extend_batch
+
+__device__ __forceinline__ uint32_t batch_source(const uint32_t *delu, size_t ND, const uint32_t *insu, size_t i) {
+  return i < ND ? delu[i] : insu[i - ND];
+}
+
+// cnt[i] = the number of kBatchChunk-edge chunks of batch source i's out-edges, cnt[n] = 0.
+__global__ __launch_bounds__(kBlock) void batch_chunks_kernel(const uint32_t *delu, size_t ND, const uint32_t *insu,
+                                                              size_t NI, const uint32_t *off, uint32_t *cnt) {
+  const size_t n = ND + NI;
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i <= n; i += size_t(gridDim.x) * kBlock) {
+    if (i == n) {
+      cnt[n] = 0;
+    } else {
+      const uint32_t u = batch_source(delu, ND, insu, i);
+      cnt[i] = (off[u + 1] - off[u] + kBatchChunk - 1) / kBatchChunk;
+    }
+  }
+}
+
+// Chunk t (of start[n] in all, start = exclusive scan of cnt) belongs to the source i with the largest start[i] <= t
+// (sources without out-edges own no chunk and are skipped by that rule); one wave marks its edges.
+__global__ __launch_bounds__(kBlock) void extend_batch_kernel(const uint32_t *delu, size_t ND, const uint32_t *insu,
+                                                              size_t NI, const uint32_t *off, const uint32_t *dst,
+                                                              const uint32_t *start, uint8_t *vaff) {
+  const uint32_t n = uint32_t(ND + NI);
+  const uint32_t total = __builtin_amdgcn_readfirstlane(start[n]);
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t nw = gridDim.x * kWavesPerBlock;
+  for (uint32_t t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); t < total;
+       t += nw) {
+    uint32_t lo = 0, hi = n;  // start[n] = total > t, so the answer is below n
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (__builtin_amdgcn_readfirstlane(start[mid]) <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t u = __builtin_amdgcn_readfirstlane(batch_source(delu, ND, insu, lo));
+    const uint32_t e0 = off[u] + (t - start[lo]) * kBatchChunk;
+    const uint32_t e1 = min(off[u + 1], e0 + kBatchChunk);
+    for (uint32_t e = e0 + lane; e < e1; e += kWave) vaff[dst[e]] = 1;
   }
 }
 
@@ -183,6 +230,19 @@ hipError_t launch_extend(const ExtendArgs &A, unsigned long long *err, int *done
   if (nb_high + nb_low == 0) return hipSuccess;
   const LoopCtl ctl{err, done, l, tol};
   extend_kernel<<<nb_high + nb_low, kBlock, 0, s>>>(A, nb_high, nb_low, ctl, err ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_extend_batch(uint8_t *vaff, const uint32_t *delu, size_t ND, const uint32_t *insu, size_t NI,
+                               const uint32_t *off, const uint32_t *dst, uint32_t *cnt, uint32_t *start, void *tmp,
+                               size_t &tmp_bytes, hipStream_t s) {
+  const size_t n = ND + NI;
+  if (!tmp) return scan_u32(nullptr, tmp_bytes, cnt, start, n + 1, s);
+  if (!n) return hipSuccess;
+  batch_chunks_kernel<<<clamp_blocks(n + 1, kBlock, 1024), kBlock, 0, s>>>(delu, ND, insu, NI, off, cnt);
+  hipError_t e = scan_u32(tmp, tmp_bytes, cnt, start, n + 1, s);
+  if (e != hipSuccess) return e;
+  extend_batch_kernel<<<1024, kBlock, 0, s>>>(delu, ND, insu, NI, off, dst, start, vaff);
   return hipGetLastError();
 }
 

@@ -387,6 +387,12 @@ hipError_t launch_advance(int *l, int *done, const unsigned long long *err, doub
 hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *delu, const uint32_t *delv,
                                size_t ND, const uint32_t *insu, size_t NI, hipStream_t s);
 // Extension; when err/done are non-null the kernel honours the iteration control of iteration l.
+// Initial extension from the batch list: vaff[w] = 1 for every out-neighbour w of every batch source (deletion
+// sources delu, insertion sources insu). cnt and start hold ND + NI + 1 entries. Call first with tmp = null for
+// the scan's temporary size.
+hipError_t launch_extend_batch(uint8_t *vaff, const uint32_t *delu, size_t ND, const uint32_t *insu, size_t NI,
+                               const uint32_t *off, const uint32_t *dst, uint32_t *cnt, uint32_t *start, void *tmp,
+                               size_t &tmp_bytes, hipStream_t s);
 hipError_t launch_extend(const ExtendArgs &A, unsigned long long *err, int *done, int l, double tol,
                          hipStream_t s);
 // Sums the per-workgroup costs the rank launches left in stats[0 .. 2 n) and sets *direction to 1 (pull) when
