@@ -758,10 +758,10 @@ int mark_affected(prh_context *c, const prh_graph *g, int method, const DevBatch
   const size_t nb = b.ND + b.NI + 1;
   PRH_CHECK(ensure(c, c->ext_chunks, 2 * nb));
   size_t tmp_bytes = 0;
-  PRH_HIP(c, prh::launch_extend_batch(c->vaff.p, b.delu, b.ND, b.insu, b.NI, g->l_out_off, g->l_out_dst,
+  PRH_HIP(c, prh::launch_extend_batch(c->vaff.p, c->naff.p, b.delu, b.ND, b.insu, b.NI, g->l_out_off, g->l_out_dst,
                                       c->ext_chunks.p, c->ext_chunks.p + nb, nullptr, tmp_bytes, s));
   PRH_CHECK(ensure(c, c->ext_tmp, std::max<size_t>(tmp_bytes, 16)));
-  PRH_HIP(c, prh::launch_extend_batch(c->vaff.p, b.delu, b.ND, b.insu, b.NI, g->l_out_off, g->l_out_dst,
+  PRH_HIP(c, prh::launch_extend_batch(c->vaff.p, c->naff.p, b.delu, b.ND, b.insu, b.NI, g->l_out_off, g->l_out_dst,
                                       c->ext_chunks.p, c->ext_chunks.p + nb, c->ext_tmp.p, tmp_bytes, s));
   if (method != PRH_DYNAMIC_TRAVERSAL) return PRH_SUCCESS;
   // Traversal closure: breadth-first levels from every marked vertex until no vertex is newly marked.
@@ -789,7 +789,7 @@ int ensure_workspace(prh_context *c, uint32_t N, bool dynamic, int L) {
   if (dynamic) {
     PRH_CHECK(ensure(c, c->sched_out, N));
     PRH_CHECK(ensure(c, c->vaff, N));
-    PRH_CHECK(ensure(c, c->naff, N));
+    PRH_CHECK(ensure(c, c->naff, (size_t(N) + 3) / 4 * 4));  // whole 32-bit words (the batch extension's claims)
     PRH_CHECK(ensure(c, c->buff, N));
   }
   PRH_CHECK(ensure(c, c->loop, size_t(L) + 1));

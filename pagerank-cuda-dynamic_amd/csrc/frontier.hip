@@ -91,16 +91,33 @@ __device__ __forceinline__ uint32_t batch_source(const uint32_t *delu, size_t ND
   return i < ND ? delu[i] : insu[i - ND];
 }
 
-// cnt[i] = the number of kBatchChunk-edge chunks of batch source i's out-edges, cnt[n] = 0.
+// Claims source u for one batch entry: naff[u] goes from 1 (set by mark_partial) to 2 by a compare-and-swap on
+// the aligned 32-bit word holding it (naff holds a multiple of 4 bytes); true for the one entry that made the change.
+// Both values read as "in the frontier" wherever naff is used.
+__device__ __forceinline__ bool claim_source(uint8_t *naff, uint32_t u) {
+  uint32_t *word = reinterpret_cast<uint32_t *>(naff + (u & ~3u));
+  const uint32_t shift = (u & 3u) * 8;
+  uint32_t old = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (((old >> shift) & 0xFFu) == 1u) {
+    const uint32_t prev = atomicCAS(word, old, (old & ~(0xFFu << shift)) | (2u << shift));
+    if (prev == old) return true;
+    old = prev;
+  }
+  return false;
+}
+
+// cnt[i] = the number of kBatchChunk-edge chunks of batch source i's out-edges for the first entry of each source
+// (a source repeated in the batch is extended once), 0 for the others; cnt[n] = 0.
 __global__ __launch_bounds__(kBlock) void batch_chunks_kernel(const uint32_t *delu, size_t ND, const uint32_t *insu,
-                                                              size_t NI, const uint32_t *off, uint32_t *cnt) {
+                                                              size_t NI, const uint32_t *off, uint8_t *naff,
+                                                              uint32_t *cnt) {
   const size_t n = ND + NI;
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i <= n; i += size_t(gridDim.x) * kBlock) {
     if (i == n) {
       cnt[n] = 0;
     } else {
       const uint32_t u = batch_source(delu, ND, insu, i);
-      cnt[i] = (off[u + 1] - off[u] + kBatchChunk - 1) / kBatchChunk;
+      cnt[i] = claim_source(naff, u) ? (off[u + 1] - off[u] + kBatchChunk - 1) / kBatchChunk : 0u;
     }
   }
 }
@@ -233,13 +250,13 @@ hipError_t launch_extend(const ExtendArgs &A, unsigned long long *err, int *done
   return hipGetLastError();
 }
 
-hipError_t launch_extend_batch(uint8_t *vaff, const uint32_t *delu, size_t ND, const uint32_t *insu, size_t NI,
-                               const uint32_t *off, const uint32_t *dst, uint32_t *cnt, uint32_t *start, void *tmp,
-                               size_t &tmp_bytes, hipStream_t s) {
+hipError_t launch_extend_batch(uint8_t *vaff, uint8_t *naff, const uint32_t *delu, size_t ND, const uint32_t *insu,
+                               size_t NI, const uint32_t *off, const uint32_t *dst, uint32_t *cnt, uint32_t *start,
+                               void *tmp, size_t &tmp_bytes, hipStream_t s) {
   const size_t n = ND + NI;
   if (!tmp) return scan_u32(nullptr, tmp_bytes, cnt, start, n + 1, s);
   if (!n) return hipSuccess;
-  batch_chunks_kernel<<<clamp_blocks(n + 1, kBlock, 1024), kBlock, 0, s>>>(delu, ND, insu, NI, off, cnt);
+  batch_chunks_kernel<<<clamp_blocks(n + 1, kBlock, 1024), kBlock, 0, s>>>(delu, ND, insu, NI, off, naff, cnt);
   hipError_t e = scan_u32(tmp, tmp_bytes, cnt, start, n + 1, s);
   if (e != hipSuccess) return e;
   extend_batch_kernel<<<1024, kBlock, 0, s>>>(delu, ND, insu, NI, off, dst, start, vaff);

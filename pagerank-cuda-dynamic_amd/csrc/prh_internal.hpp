@@ -388,11 +388,12 @@ hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *del
                                size_t ND, const uint32_t *insu, size_t NI, hipStream_t s);
 // Extension; when err/done are non-null the kernel honours the iteration control of iteration l.
 // Initial extension from the batch list: vaff[w] = 1 for every out-neighbour w of every batch source (deletion
-// sources delu, insertion sources insu). cnt and start hold ND + NI + 1 entries. Call first with tmp = null for
-// the scan's temporary size.
-hipError_t launch_extend_batch(uint8_t *vaff, const uint32_t *delu, size_t ND, const uint32_t *insu, size_t NI,
-                               const uint32_t *off, const uint32_t *dst, uint32_t *cnt, uint32_t *start, void *tmp,
-                               size_t &tmp_bytes, hipStream_t s);
+// sources delu, insertion sources insu), each source once: naff (1 for the sources, from mark_partial; its
+// allocation a multiple of 4 bytes) is set to 2 for a source when an entry claims it. cnt and start hold ND + NI + 1
+// entries. Call first with tmp = null for the scan's temporary size.
+hipError_t launch_extend_batch(uint8_t *vaff, uint8_t *naff, const uint32_t *delu, size_t ND, const uint32_t *insu,
+                               size_t NI, const uint32_t *off, const uint32_t *dst, uint32_t *cnt, uint32_t *start,
+                               void *tmp, size_t &tmp_bytes, hipStream_t s);
 hipError_t launch_extend(const ExtendArgs &A, unsigned long long *err, int *done, int l, double tol,
                          hipStream_t s);
 // Sums the per-workgroup costs the rank launches left in stats[0 .. 2 n) and sets *direction to 1 (pull) when
