@@ -104,6 +104,11 @@ struct prh_graph {
   bool sliced_valid = false;
   prh::SlicedPlan sliced{};
   std::vector<std::pair<void *, size_t>> sliced_bufs;  // its device allocations
+  // Out-degree schedule of the layout's out-CSR for the extension kernels (built on first use): the stable
+  // partition [<= 64 | <= kExtendChunk | heavier] and the heavier rows cut into chunks (beg, end, row).
+  bool out_valid = false;
+  uint32_t *out_order = nullptr, *out_chunks = nullptr;
+  uint32_t out_n[3] = {0, 0, 0}, out_n_chunks = 0;
   // Propagation-blocked plans, built on first use for the current block width: of the layout in-CSR (the rank
   // update, the weighted forward) and of its out-CSR (the weighted adjoint).
   BinnedCache bin, bin_t;
@@ -729,25 +734,69 @@ int stage_batch(prh_context *c, const prh_graph *g, const uint32_t *du, const ui
   return PRH_SUCCESS;
 }
 
-// Out-degree schedule of the layout's out-CSR for the extension kernels; one readback of the part sizes, as the
-// reference reads its partition point back (pagerankCuda.hxx:633).
-int out_schedule(prh_context *c, const prh_graph *g, prh::Schedule &out_s) {
+// Out-degree schedule of the layout's out-CSR for the extension kernels, built once per graph (one readback of
+// the part sizes, as the reference reads its partition point back, pagerankCuda.hxx:633), with the chunk table of
+// the rows above kExtendChunk edges, so that a hub's edges are spread over many waves.
+int out_schedule(prh_context *c, prh_graph *g, prh::ExtendArgs &E) {
   const uint32_t N = g->N;
-  PRH_CHECK(ensure(c, c->sched_out, N));  // the partition below writes N ids into it
-  PRH_CHECK(ensure(c, c->tiles, size_t(3) * prh::partition_tiles(N)));
-  PRH_CHECK(ensure(c, c->counts, 8));
-  PRH_HIP(c, prh::launch_partition(g->l_out_off, N, 64u, 0xFFFFFFFFu, c->sched_out.p, c->tiles.p, c->counts.p,
-                                   c->stream));
-  PRH_HIP(c, hipMemcpyAsync(c->pinned, c->counts.p, 8, hipMemcpyDeviceToHost, c->stream));
-  PRH_HIP(c, hipStreamSynchronize(c->stream));
-  const uint32_t *cnt = reinterpret_cast<const uint32_t *>(c->pinned);
-  out_s = prh::Schedule{c->sched_out.p, cnt[0], cnt[1], 0};
+  if (!g->out_valid) {
+    hipStream_t s = c->stream;
+    if (!g->out_order) PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&g->out_order), std::max<size_t>(N, 1) * 4));
+    PRH_CHECK(ensure(c, c->tiles, size_t(3) * prh::partition_tiles(N)));
+    PRH_CHECK(ensure(c, c->counts, 8));
+    PRH_HIP(c, prh::launch_partition(g->l_out_off, N, 64u, prh::kExtendChunk, g->out_order, c->tiles.p, c->counts.p,
+                                     s));
+    PRH_HIP(c, hipMemcpyAsync(c->pinned, c->counts.p, 12, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipStreamSynchronize(s));
+    const uint32_t *cnt = reinterpret_cast<const uint32_t *>(c->pinned);
+    g->out_n[0] = cnt[0];
+    g->out_n[1] = cnt[1];
+    g->out_n[2] = cnt[2];
+    const uint32_t nh = g->out_n[2];
+    g->out_n_chunks = 0;
+    if (nh) {
+      uint32_t *ranges = nullptr;
+      PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&ranges), size_t(2) * nh * 4));
+      std::vector<uint32_t> be(size_t(2) * nh);
+      hipError_t e = prh::launch_gather_ranges(g->l_out_off, g->out_order + cnt[0] + cnt[1], nh, ranges, ranges + nh,
+                                               s);
+      if (e == hipSuccess) e = hipMemcpyAsync(be.data(), ranges, be.size() * 4, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      dev_free(c, ranges, size_t(2) * nh * 4);
+      if (e != hipSuccess) return fail(c, PRH_ERROR_DEVICE, "out schedule: %s", hipGetErrorString(e));
+      std::vector<uint32_t> cb, ce, cr;
+      std::vector<uint32_t> rows(nh);
+      PRH_HIP(c, hipMemcpy(rows.data(), g->out_order + cnt[0] + cnt[1], size_t(nh) * 4, hipMemcpyDeviceToHost));
+      for (uint32_t h = 0; h < nh; ++h)
+        for (uint64_t x = be[h]; x < be[nh + h]; x += prh::kExtendChunk) {
+          cb.push_back(uint32_t(x));
+          ce.push_back(uint32_t(std::min<uint64_t>(be[nh + h], x + prh::kExtendChunk)));
+          cr.push_back(rows[h]);
+        }
+      const size_t nc = cb.size();
+      PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&g->out_chunks), std::max<size_t>(3 * nc, 1) * 4));
+      PRH_HIP(c, hipMemcpy(g->out_chunks, cb.data(), nc * 4, hipMemcpyHostToDevice));
+      PRH_HIP(c, hipMemcpy(g->out_chunks + nc, ce.data(), nc * 4, hipMemcpyHostToDevice));
+      PRH_HIP(c, hipMemcpy(g->out_chunks + 2 * nc, cr.data(), nc * 4, hipMemcpyHostToDevice));
+      g->out_n_chunks = uint32_t(nc);
+    }
+    g->out_valid = true;
+  }
+  E.sched = prh::Schedule{g->out_order, g->out_n[0], g->out_n[1], g->out_n[2]};
+  if (g->out_n_chunks) {
+    const uint32_t nc = g->out_n_chunks;
+    E.chunk_beg = g->out_chunks;
+    E.chunk_end = g->out_chunks + nc;
+    E.chunk_row = g->out_chunks + 2 * nc;
+    E.n_chunks = nc;
+  }
   return PRH_SUCCESS;
 }
 
 // Marking phase of the dynamic methods (pagerankCuda.hxx:821 with the fm lambdas :926-928, :955-957, :984-986),
 // on the layout's out-CSR with layout ids.
-int mark_affected(prh_context *c, const prh_graph *g, int method, const DevBatch &b, const prh::Schedule &out_s) {
+// xo: the out-degree schedule and chunk table (out_schedule), needed by the traversal levels only.
+int mark_affected(prh_context *c, const prh_graph *g, int method, const DevBatch &b, const prh::ExtendArgs &xo) {
   const uint32_t N = g->N;
   hipStream_t s = c->stream;
   PRH_HIP(c, prh::launch_fill_u8(c->vaff.p, 0, N, s));
@@ -771,7 +820,14 @@ int mark_affected(prh_context *c, const prh_graph *g, int method, const DevBatch
   for (;;) {
     PRH_HIP(c, prh::launch_fill_u8(fresh, 0, N, s));
     PRH_HIP(c, prh::launch_fill_i32(c->flags.p + 1, 0, 1, s));
-    prh::ExtendArgs T{c->vaff.p, front, g->l_out_off, g->l_out_dst, out_s, fresh, c->flags.p + 1};
+    prh::ExtendArgs T = xo;
+    T.vaff = c->vaff.p;
+    T.naff = front;
+    T.off = g->l_out_off;
+    T.dst = g->l_out_dst;
+    T.fresh = fresh;
+    T.changed = c->flags.p + 1;
+    T.direction = nullptr;
     PRH_HIP(c, prh::launch_extend(T, nullptr, nullptr, 0, 0.0, s));
     PRH_HIP(c, hipMemcpyAsync(c->pinned, c->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     PRH_HIP(c, hipStreamSynchronize(s));
@@ -883,12 +939,12 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
     // ranges of the layout, fixed when the graph was prepared; the extension kernels' out-degree split is per solve.
     // The out-degree schedule serves the traversal levels and the row plan's frontier extension; the initial
     // extension runs from the batch list and the blocked plan extends inside its update.
-    prh::Schedule out_s{};
-    if (dynamic && (method == PRH_DYNAMIC_TRAVERSAL || (frontier && !binned))) PRH_CHECK(out_schedule(c, g, out_s));
+    prh::ExtendArgs xo{};
+    if (dynamic && (method == PRH_DYNAMIC_TRAVERSAL || (frontier && !binned))) PRH_CHECK(out_schedule(c, gm, xo));
     PRH_HIP(c, prh::launch_init_ranks(c->rank[0].p, c->rank[1].p, c->contrib[0].p, c->contrib[1].p, g->l_outdeg, N,
                                       q != nullptr, s));
     PRH_HIP(c, hipEventRecord(c->ev[1], s));
-    if (dynamic) PRH_CHECK(mark_affected(c, g, method, b, out_s));
+    if (dynamic) PRH_CHECK(mark_affected(c, g, method, b, xo));
     PRH_HIP(c, hipEventRecord(c->ev[2], s));
     // Iterations.
     PRH_HIP(c, prh::launch_fill_u64(err, 0ull, size_t(L) + 1, s));
@@ -909,7 +965,14 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
       else PRH_HIP(c, prh::launch_rank_update(variant, A, plan, err, done, l, o->tolerance, s, mark));
       if (frontier && !binned) {  // the blocked plan extends inside the next iteration's update
         if (dir_auto) PRH_HIP(c, prh::launch_direction(c->ext_stats.p, stat_slots, dir, err, done, l, o->tolerance, s));
-        prh::ExtendArgs E{c->vaff.p, c->naff.p, g->l_out_off, g->l_out_dst, out_s, nullptr, nullptr, dir};
+        prh::ExtendArgs E = xo;
+        E.vaff = c->vaff.p;
+        E.naff = c->naff.p;
+        E.off = g->l_out_off;
+        E.dst = g->l_out_dst;
+        E.fresh = nullptr;
+        E.changed = nullptr;
+        E.direction = dir;
         PRH_HIP(c, prh::launch_extend(E, err, done, l, o->tolerance, s));
         if (dir_mode != 0) PRH_HIP(c, prh::launch_extend_pull(pull, err, done, l, o->tolerance, s));
       }
@@ -1807,6 +1870,8 @@ void prh_graph_destroy(prh_graph *g) {
   dev_free(c, g->l_out_dst, (M ? M : 4) * 4);
   dev_free(c, g->chunk_tables, g->chunk_tables_bytes);
   dev_free(c, g->chunk_partial, g->chunk_partial_bytes);
+  dev_free(c, g->out_order, std::max<size_t>(N, 1) * 4);
+  dev_free(c, g->out_chunks, std::max<size_t>(3 * size_t(g->out_n_chunks), 1) * 4);
   free_sliced(c, g);
   free_binned(c, g->bin);
   free_binned(c, g->bin_t);
@@ -2092,9 +2157,9 @@ int prh_mark_affected(prh_context *c, const prh_graph *g, prh_method method, con
   PRH_CHECK(ensure_workspace(c, g->N, true, 1));
   DevBatch b;
   PRH_CHECK(stage_batch(c, g, du, dv, nd, iu, ni, true, b));
-  prh::Schedule out_s{};
-  PRH_CHECK(out_schedule(c, g, out_s));
-  PRH_CHECK(mark_affected(c, g, method, b, out_s));
+  prh::ExtendArgs xo{};
+  if (method == PRH_DYNAMIC_TRAVERSAL) PRH_CHECK(out_schedule(c, const_cast<prh_graph *>(g), xo));
+  PRH_CHECK(mark_affected(c, g, method, b, xo));
   // back to the caller's vertex order (naff is free after marking)
   PRH_HIP(c, prh::launch_scatter_u8(c->vaff.p, g->perm, g->N, c->naff.p, c->stream));
   PRH_HIP(c, hipMemcpyAsync(affected, c->naff.p, g->N, hipMemcpyDeviceToHost, c->stream));
@@ -2410,12 +2475,12 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
   }
   // Out-degree schedule of the extension (fixed for the session; rows that outgrow the low class stay correct).
   if (st == PRH_SUCCESS && S->variant >= prh::kFrontier) {
-    prh::Schedule os{};
-    st = out_schedule(c, g, os);
-    if (st == PRH_SUCCESS) {
-      PRH_HIP(c, hipMemcpyAsync(S->sched_order, os.order, size_t(N) * 4, hipMemcpyDeviceToDevice, s));
-      S->sched_low = os.n_low;
-      S->sched_high = os.n_high;
+    prh::ExtendArgs xo{};
+    st = out_schedule(c, const_cast<prh_graph *>(g), xo);
+    if (st == PRH_SUCCESS) {  // the session's kernels take the huge rows as high ones (a wave each)
+      PRH_HIP(c, hipMemcpyAsync(S->sched_order, xo.sched.order, size_t(N) * 4, hipMemcpyDeviceToDevice, s));
+      S->sched_low = xo.sched.n_low;
+      S->sched_high = xo.sched.n_high + xo.sched.n_huge;
     }
   }
   // Initial ranks (layout order): the caller's, or a static solve of the base graph.

@@ -47,14 +47,22 @@ constexpr int kLowLanes = 8;                     // lanes per low-out-degree ver
 constexpr int kLowPerLane = 64 / kLowLanes;      // edges per lane (out-degree <= 64 in the low class)
 
 // High out-degree vertices: one wave per vertex. Low ones (out-degree <= 64): kLowLanes lanes per vertex, each
-// lane loading all of its (at most kLowPerLane) targets before marking any, so that the loads overlap.
+// lane loading all of its (at most kLowPerLane) targets before marking any, so that the loads overlap. Huge ones,
+// when the chunk table is given: one wave per chunk of kExtendChunk edges (blocks past nb_high + nb_low).
 __global__ __launch_bounds__(kBlock) void extend_kernel(ExtendArgs A, uint32_t nb_high, uint32_t nb_low, LoopCtl ctl,
                                                         int use_ctl) {
   if (use_ctl && !iteration_continues(ctl)) return;
   if (A.direction && *A.direction != 0) return;
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  if (blockIdx.x < nb_high) {
-    const uint32_t n = A.sched.n_high + A.sched.n_huge;
+  if (blockIdx.x >= nb_high + nb_low) {
+    const uint32_t nw = (gridDim.x - nb_high - nb_low) * kWavesPerBlock;
+    for (uint32_t k = (blockIdx.x - nb_high - nb_low) * kWavesPerBlock + w; k < A.n_chunks; k += nw) {
+      const uint32_t u = A.chunk_row[k];
+      if (!A.naff[u]) continue;
+      for (uint32_t e = A.chunk_beg[k] + lane; e < A.chunk_end[k]; e += kWave) mark(A, A.dst[e]);
+    }
+  } else if (blockIdx.x < nb_high) {
+    const uint32_t n = A.sched.n_high + (A.chunk_row ? 0u : A.sched.n_huge);
     const uint32_t *order = A.sched.order + A.sched.n_low;
     for (uint32_t i = blockIdx.x * kWavesPerBlock + w; i < n; i += nb_high * kWavesPerBlock) {
       const uint32_t u = order[i];
@@ -242,11 +250,13 @@ hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *del
 }
 
 hipError_t launch_extend(const ExtendArgs &A, unsigned long long *err, int *done, int l, double tol, hipStream_t s) {
-  const uint32_t nb_high = clamp_blocks(size_t(A.sched.n_high) + A.sched.n_huge, kWavesPerBlock, 4096);
+  const uint32_t nb_high =
+      clamp_blocks(size_t(A.sched.n_high) + (A.chunk_row ? 0u : A.sched.n_huge), kWavesPerBlock, 4096);
   const uint32_t nb_low = clamp_blocks(A.sched.n_low, kBlock / kLowLanes, 8192);
-  if (nb_high + nb_low == 0) return hipSuccess;
+  const uint32_t nb_chunk = A.chunk_row ? clamp_blocks(A.n_chunks, kWavesPerBlock, 4096) : 0u;
+  if (nb_high + nb_low + nb_chunk == 0) return hipSuccess;
   const LoopCtl ctl{err, done, l, tol};
-  extend_kernel<<<nb_high + nb_low, kBlock, 0, s>>>(A, nb_high, nb_low, ctl, err ? 1 : 0);
+  extend_kernel<<<nb_high + nb_low + nb_chunk, kBlock, 0, s>>>(A, nb_high, nb_low, ctl, err ? 1 : 0);
   return hipGetLastError();
 }
 

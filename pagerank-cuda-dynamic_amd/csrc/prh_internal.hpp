@@ -94,7 +94,12 @@ struct ExtendArgs {
   uint8_t *fresh;        // optional: also set for newly marked vertices (traversal levels)
   int *changed;          // optional: set to 1 when a vertex is newly marked
   const int *direction;  // optional: the push runs only while *direction == 0 (see launch_direction)
+  // optional: the huge class (sched.n_huge rows after the high ones) as chunks of at most kExtendChunk edges,
+  // chunk k covering edges [chunk_beg[k], chunk_end[k]) of row chunk_row[k]; without it huge rows take a wave each
+  const uint32_t *chunk_beg = nullptr, *chunk_end = nullptr, *chunk_row = nullptr;
+  uint32_t n_chunks = 0;
 };
+constexpr uint32_t kExtendChunk = 4096;  // out-degree above which a row's extension is cut into chunks
 
 // The pull form of the extension: every unaffected row w gets vaff[w] = 1 if one of its in-neighbours is in the
 // frontier. Same flags as the push. Rows [0, n_wave) take a wave each, [n_wave, n_group) 8 lanes, the rest 1 lane.

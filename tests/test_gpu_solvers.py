@@ -252,3 +252,34 @@ def test_extension_directions_agree(ctx, oracle_mod, method, frac):
         assert np.array_equal(out[d].ranks, out[0].ranks), d
     assert _close_iters(out[0].iterations, it)
     assert np.abs(out[0].ranks - ref).max() <= RANK_TOL
+
+
+def test_hub_source_chunks_and_schedule_kept_across_tunings(ctx, oracle_mod):
+    """A source with 6000 out-edges is cut into chunks by the extension's cached out-degree schedule (rows above
+    4096 out-edges). DT marking and DF through the row plan match the oracle, and a tuning change between two
+    solves, which rebuilds the rank plan, leaves the graph's cached schedule intact."""
+    from pagerank_amd import Batch, HostGraph, apply_batch, solve
+    n = 9000
+    rng = np.random.default_rng(4)
+    src = np.concatenate([np.zeros(6000, np.uint32), rng.integers(0, n, 30000).astype(np.uint32)])
+    dst = np.concatenate([np.arange(1, 6001, dtype=np.uint32), rng.integers(0, n, 30000).astype(np.uint32)])
+    x = HostGraph.from_edges(n, src, dst, self_loops=True)
+    b = Batch.tidy(x, [[0, 1]], [[0, 7000], [5, 8000]])  # the hub is a batch source
+    y = apply_batch(x, b)
+    print(f"[hub] N={y.N} M={y.M} hub out-degree={int(y.out_off[1] - y.out_off[0])}", flush=True)
+    yg = ctx.upload(y)
+    assert np.array_equal(ctx.mark_affected(yg, 2, b), oracle_mod.mark(2, y, b))
+    q, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, x)
+    ref, it, _ = oracle_mod.pagerank_cuda(3, y, b, q)
+    old = ctx.get_tuning()
+    try:
+        first = solve(yg, 3, batch=b, q=q)
+        t = ctx.get_tuning()
+        t.switch_degree = 128
+        ctx.set_tuning(t)
+        second = solve(yg, 3, batch=b, q=q)
+    finally:
+        ctx.set_tuning(old)
+    assert _close_iters(first.iterations, it) and _close_iters(second.iterations, it)
+    assert np.abs(first.ranks - ref).max() <= RANK_TOL
+    assert np.abs(second.ranks - ref).max() <= RANK_TOL
