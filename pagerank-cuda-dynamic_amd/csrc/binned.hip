@@ -122,7 +122,24 @@ __global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const d
   const uint4 it = P.ex_item[blockIdx.x];  // block, first entry, end entry
   const uint32_t base = it.x * uint32_t(B);
   const uint32_t n = min(uint32_t(B), P.N - base);
-  for (uint32_t i = threadIdx.x; i < n; i += kBinBlock) c[i] = naff && naff[base + i] ? -cr[base + i] : cr[base + i];
+  // Pairs of contributions a lane (base is a multiple of B, so the pairs are 16-byte aligned); an odd n leaves one.
+  const d2 *cr2 = reinterpret_cast<const d2 *>(cr + base);
+  if (naff) {
+    const uint16_t *f2 = reinterpret_cast<const uint16_t *>(naff + base);
+    for (uint32_t i = threadIdx.x; 2 * i + 1 < n; i += kBinBlock) {
+      const d2 v = cr2[i];
+      const uint32_t f = f2[i];
+      c[2 * i] = (f & 0xFFu) ? -v.x : v.x;
+      c[2 * i + 1] = (f >> 8) ? -v.y : v.y;
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; 2 * i + 1 < n; i += kBinBlock) {
+      const d2 v = cr2[i];
+      c[2 * i] = v.x;
+      c[2 * i + 1] = v.y;
+    }
+  }
+  if ((n & 1u) && threadIdx.x == 0) c[n - 1] = naff && naff[base + n - 1] ? -cr[base + n - 1] : cr[base + n - 1];
   __syncthreads();
   // Pairs of entries: thread t takes pairs p = t + m * kBinBlock of the even-aligned range (16-byte stores, with the
   // non-temporal hint: X is read back only by the next kernel, and keeping it out of L2 helps both); a lone first
