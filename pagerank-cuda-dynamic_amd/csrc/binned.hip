@@ -59,19 +59,18 @@ __device__ __forceinline__ double bin_finish(const RankArgs &A, uint32_t v, doub
   }
   if constexpr (Traits<VARIANT>::frontier)
     if (extend) A.vaff[v] = 1;
-  const uint32_t d = A.deg[v];
   const double rv = A.r[v];
   const double crv = Traits<VARIANT>::dynamic ? A.cr[v] : 0.0;
   RowFlags f;
   double e = 0.0;
-  if (affected<VARIANT>(A, v)) {
-    e = finish_vertex<VARIANT>(A, v, cv, d, rv, crv, f);
+  if (affected<VARIANT>(A, v)) {  // the out-degree is read only here: unaffected rows are copied
+    e = finish_vertex<VARIANT>(A, v, cv, A.deg[v], rv, crv, f);
   } else {
     copy_vertex<VARIANT>(A, v, rv, crv);
     f.affected = false;
   }
   if constexpr (Traits<VARIANT>::frontier)
-    if (A.stats) cost.add(f, A.off[v + 1] - A.off[v], d);
+    if (A.stats) cost.add(f, A.off[v + 1] - A.off[v], A.deg[v]);
   return e;
 }
 
