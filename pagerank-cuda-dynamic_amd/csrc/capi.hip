@@ -521,13 +521,15 @@ int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, co
     PRH_CHECK(keep(&src16, 2));
     PRH_CHECK(keep(&X, 16));
   }
-  // Items: expand items of at most 32 per CU over all pairs, accumulate items of at most 4 per CU over all slots
-  // (fewer split bins, hence fewer partial rows); both at least 16K entries.
+  // Items: expand items of at most 32 per CU over all pairs, accumulate items of at most 2 per CU over all slots
+  // (fewer split bins, hence fewer partial rows); both at least 16K entries. Measured on RMAT-24 (B = 8192) per
+  // update: expand spans of 6.4K, 12.9K, 16K, 32K and 64K entries 763, 712, 709, 724 and 744 us; accumulate
+  // spans of M/(k CUs) for k = 8, 4, 2, 1 and 1/2 733, 709, 697, 807 and 1195 us.
   hipDeviceProp_t prop{};
   PRH_HIP(c, hipGetDeviceProperties(&prop, c->device));
   const uint64_t cus = uint64_t(std::max(prop.multiProcessorCount, 1));
   const uint64_t span_ex = std::max<uint64_t>(Q / (cus * 32), 16384);
-  const uint64_t span = std::max<uint64_t>(M / (cus * 4), 16384);
+  const uint64_t span = std::max<uint64_t>(M / (cus * 2), 16384);
   std::vector<uint4> ex, ac, multi;
   for (uint64_t b = 0; b < nb; ++b)
     for (uint64_t k = bstart[b]; k < bstart[b + 1]; k += span_ex)
