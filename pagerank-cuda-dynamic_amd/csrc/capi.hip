@@ -524,7 +524,8 @@ int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, co
   // Items: expand items of at most 32 per CU over all pairs, accumulate items of at most 2 per CU over all slots
   // (fewer split bins, hence fewer partial rows); both at least 16K entries. Measured on RMAT-24 (B = 8192) per
   // update: expand spans of 6.4K, 12.9K, 16K, 32K and 64K entries 763, 712, 709, 724 and 744 us; accumulate
-  // spans of M/(k CUs) for k = 8, 4, 2, 1 and 1/2 733, 709, 697, 807 and 1195 us.
+  // spans of M/(k CUs) for k = 8, 4, 3, 2, 3/2, 1 and 1/2 733, 709, 711, 696, 710, 807 and 1195 us (k = 2: as many
+  // items as the two resident workgroups of every CU).
   hipDeviceProp_t prop{};
   PRH_HIP(c, hipGetDeviceProperties(&prop, c->device));
   const uint64_t cus = uint64_t(std::max(prop.multiProcessorCount, 1));
