@@ -1,0 +1,210 @@
+// capi_internal.hpp -- host-side state of the C ABI (contexts, graphs) and the helpers its translation units share.
+//
+// The C ABI (include/pagerank_hip.h) is implemented by several units:
+//   capi.hip      options, tuning, contexts, graph upload, the solve entry points, partition and marking exports;
+//   plans.hip     the degree-ordered layout and the launch plans built from it (row, sliced, propagation-blocked),
+//                 and the out-degree schedule of the extension kernels;
+//   solve.hip     the solver driver (pagerankInvokeCuda + pagerankLoopCuU, pagerankCuda.hxx:675-692, :713-849);
+//   forms.hip     batched personalized PageRank and the weighted forward / adjoint solves;
+//   autotune.hip  the launch-plan autotuner and its cache;
+//   session.hip   the streaming dynamic session.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/pagerank_hip.h"
+#include "prh_internal.hpp"
+
+namespace prhc {
+
+constexpr size_t kPinnedBytes = 4096;
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t cap = 0;  // elements
+};
+
+}  // namespace prhc
+
+struct prh_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  size_t limit = 0, used = 0;
+  prh_tuning tuning{};
+  prhc::DevBuf<double> rank[2], contrib[2];
+  prhc::DevBuf<uint8_t> vaff, naff, buff;
+  prhc::DevBuf<uint32_t> sched_in, sched_out, tiles, counts, batch, scratch_u32;
+  prhc::DevBuf<unsigned long long> loop;  // [L] error bits, then one slot holding `done`
+  prhc::DevBuf<int> flags;                // [0] validation bits, [1] traversal "changed", [2] extension direction
+  prhc::DevBuf<double> ext_stats;         // per rank workgroup: pull and push costs of the next extension
+  prhc::DevBuf<uint32_t> ext_chunks;      // initial extension: chunk counts and their scan per batch source
+  prhc::DevBuf<uint8_t> ext_tmp;          // its scan's temporary storage
+  // batched personalized PageRank
+  prhc::DevBuf<double> ppr_rank[2], ppr_contrib[2], ppr_partial, ppr_mass, ppr_out;
+  prhc::DevBuf<int32_t> ppr_row;
+  prhc::DevBuf<uint32_t> ppr_tables;
+  prhc::DevBuf<unsigned long long> ppr_err;
+  prhc::DevBuf<prh::PprState> ppr_state;
+  // weighted forward / adjoint
+  // weighted forward / adjoint, on the degree-ordered layout ("l": layout order, else the caller's order)
+  prhc::DevBuf<double> wt_x[2], wt_z[2], wt_W, wt_Wl, wt_term, wt_terml, wt_r, wt_rl, wt_gwl, wt_gsl, wt_outM, wt_outN;
+  prhc::DevBuf<float> wt_wout, wt_wl;
+  prhc::DevBuf<uint32_t> wt_chunks;       // heavy-row chunk tables of the weighted pull
+  prhc::DevBuf<uint32_t> wt_sort, wt_sort_tmp;  // degree sort of its low rows
+  prhc::DevBuf<double> wt_partial;        // their sums
+  uint64_t *pinned = nullptr;       // small host-pinned readback area (kPinnedBytes)
+  hipEvent_t ev[4] = {};
+  bool ktiming = false;
+  std::vector<hipEvent_t> kev;  // start/stop pairs per iteration
+  uint64_t k_launches = 0;
+  double k_ms = 0.0;
+};
+
+struct BinnedCache {
+  int B = 0;
+  bool valid = false;
+  bool slot = false;  // the plan keeps the X-entry -> CSR-slot map (weighted solves)
+  prh::BinnedPlan plan{};
+  std::vector<std::pair<void *, size_t>> bufs;  // its device allocations
+};
+
+struct prh_graph {
+  prh_context *ctx = nullptr;
+  uint32_t N = 0;
+  uint64_t M = 0;
+  // As uploaded (caller's ids): used by the batched and weighted solvers.
+  uint32_t *in_off = nullptr, *in_src = nullptr, *outdeg = nullptr, *out_off = nullptr, *out_dst = nullptr;
+  // Raw out-CSR slot of every slot of the layout's in-CSR and out-CSR, built on first weighted use.
+  uint32_t *l_in_eid = nullptr, *l_out_eid = nullptr;
+  // Degree-ordered layout used by the five solvers: layout row i is vertex perm[i] (in-degree descending, stable),
+  // inv is the inverse; in-rows sorted by new source id.
+  uint32_t *perm = nullptr, *inv = nullptr;
+  uint32_t *l_in_off = nullptr, *l_in_src = nullptr, *l_outdeg = nullptr, *l_out_off = nullptr, *l_out_dst = nullptr;
+  std::vector<uint32_t> sorted_deg;  // host: in-degree of layout row i
+  float prep_ms = 0;
+  // Rank-kernel plan for the current tuning: section table and the chunk tables of the heaviest rows.
+  prh_tuning plan_tuning{};
+  bool plan_valid = false;
+  prh::RankPlan plan{};
+  uint32_t *chunk_tables = nullptr;  // item_row, item_beg, item_end, row_first
+  size_t chunk_tables_bytes = 0;
+  double *chunk_partial = nullptr;
+  size_t chunk_partial_bytes = 0;
+  uint32_t n_chunked = 0;
+  // Source-sliced plan of the plain update, built on first use for the current slices / chunk threshold.
+  int sliced_slices = 0;
+  uint32_t sliced_huge = 0;
+  uint32_t sliced_block_edges = 0;
+  bool sliced_valid = false;
+  prh::SlicedPlan sliced{};
+  std::vector<std::pair<void *, size_t>> sliced_bufs;  // its device allocations
+  // Out-degree schedule of the layout's out-CSR for the extension kernels (built on first use): the stable
+  // partition [<= 64 | <= kExtendChunk | heavier] and the heavier rows cut into chunks (beg, end, row).
+  bool out_valid = false;
+  uint32_t *out_order = nullptr, *out_chunks = nullptr;
+  uint32_t out_n[3] = {0, 0, 0}, out_n_chunks = 0;
+  // Propagation-blocked plans, built on first use for the current block width: of the layout in-CSR (the rank
+  // update, the weighted forward) and of its out-CSR (the weighted adjoint).
+  BinnedCache bin, bin_t;
+};
+
+namespace prhc {
+
+int fail(prh_context *c, int status, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+
+#define PRH_HIP(ctx, expr)                                                                              \
+  do {                                                                                                  \
+    hipError_t e_ = (expr);                                                                             \
+    if (e_ != hipSuccess) return prhc::fail(ctx, PRH_ERROR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define PRH_CHECK(expr)      \
+  do {                       \
+    int s_ = (expr);         \
+    if (s_ != PRH_SUCCESS) return s_; \
+  } while (0)
+
+int dev_alloc(prh_context *c, void **p, size_t bytes);
+void dev_free(prh_context *c, void *p, size_t bytes);
+
+template <class T>
+int ensure(prh_context *c, DevBuf<T> &b, size_t n) {
+  if (b.cap >= n && b.p) return PRH_SUCCESS;
+  if (b.p) {
+    PRH_HIP(c, hipStreamSynchronize(c->stream));
+    dev_free(c, b.p, b.cap * sizeof(T));
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  void *p = nullptr;
+  PRH_CHECK(dev_alloc(c, &p, n * sizeof(T)));
+  b.p = static_cast<T *>(p);
+  b.cap = n;
+  return PRH_SUCCESS;
+}
+
+template <class T>
+void release(prh_context *c, DevBuf<T> &b) {
+  dev_free(c, b.p, b.cap * sizeof(T));
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+inline bool valid_method(int m) { return m >= PRH_STATIC && m <= PRH_PRUNE_DYNAMIC_FRONTIER; }
+
+// Number of rows with in-degree > x in the descending degree list.
+inline uint32_t rows_above(const std::vector<uint32_t> &deg, uint32_t x) {
+  return uint32_t(std::lower_bound(deg.begin(), deg.end(), x, [](uint32_t a, uint32_t b) { return a > b; }) -
+                  deg.begin());
+}
+
+// Capture `body` (launches on the context stream only) into an executable graph.
+template <class F>
+int capture_graph(prh_context *c, F body, hipGraphExec_t *exec) {
+  hipGraph_t graph = nullptr;
+  PRH_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int st = body();
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (st != PRH_SUCCESS || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st != PRH_SUCCESS ? st : fail(c, PRH_ERROR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  }
+  const hipError_t ie = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) return fail(c, PRH_ERROR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+  return PRH_SUCCESS;
+}
+
+// Batch arrays staged in the context (host or device source), checked against N and mapped to layout ids.
+struct DevBatch {
+  const uint32_t *delu = nullptr, *delv = nullptr, *insu = nullptr;
+  size_t ND = 0, NI = 0;
+};
+
+// plans.hip
+int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uint32_t *outdeg, bool with_out);
+void free_sliced(prh_context *c, prh_graph *g);
+int build_sliced_plan(prh_context *c, prh_graph *g);
+void free_binned(prh_context *c, BinnedCache &bc);
+int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot = false);
+int build_binned_plan_t(prh_context *c, prh_graph *g);
+int build_rank_plan(prh_context *c, prh_graph *g);
+int out_schedule(prh_context *c, prh_graph *g, prh::ExtendArgs &E);
+
+// solve.hip
+int stage_batch(prh_context *c, const prh_graph *g, const uint32_t *du, const uint32_t *dv, size_t nd,
+                const uint32_t *iu, size_t ni, bool host, DevBatch &b);
+int mark_affected(prh_context *c, const prh_graph *g, int method, const DevBatch &b, const prh::ExtendArgs &xo);
+int ensure_workspace(prh_context *c, uint32_t N, bool dynamic, int L);
+int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options *o, const uint32_t *du,
+               const uint32_t *dv, size_t nd, const uint32_t *iu, size_t ni, const double *q, double *ranks,
+               prh_stats *st, bool host);
+
+}  // namespace prhc

@@ -1,0 +1,441 @@
+// session.hip -- the streaming dynamic session (prh_stream_*).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "capi_internal.hpp"
+
+using namespace prhc;
+
+
+// ---- streaming dynamic session -------------------------------------------------------------------------------
+//
+// A device-resident copy of a graph's degree-ordered layout that batches update in place (stream.hip), with
+// the ranks kept between batches and each batch's work replayed from captured HIP graphs:
+//   apply graph (one per buffer parity): batch -> keys -> sort -> tidy -> new in- and out-CSR (into the other
+//       buffers) -> out-degrees -> chunk tables -> solver initialisation and, for DF / DF-P, the initial marking;
+//   iteration graph (one per parity): kChunk iterations of the rank update (and, for DF / DF-P, the direction
+//       choice and both extension forms) followed each by the device-side convergence step; replayed until the
+//       device reports the loop done.
+// Only the batch upload, one flag read after the apply and one read per iteration chunk touch the host.
+
+struct prh_stream {
+  prh_context *ctx = nullptr;
+  const prh_graph *g = nullptr;
+  int method = 0, variant = 0;
+  prh_options opt{};
+  uint32_t N = 0;
+  uint64_t M = 0;          // current edge count
+  uint64_t cap_edges = 0;  // capacity of the edge arrays
+  uint32_t cap_batch = 0;  // capacity per batch side (deletions, insertions)
+  uint32_t *in_off[2] = {}, *in_src[2] = {}, *out_off[2] = {}, *out_dst[2] = {};
+  int cur = 0;             // buffers holding the current graph
+  uint32_t *outdeg = nullptr, *new_deg = nullptr, *sched_order = nullptr;
+  uint32_t sched_low = 0, sched_high = 0;
+  int *delta = nullptr;
+  uint8_t *changed = nullptr;
+  uint32_t *h_batch = nullptr;  // pinned: del_u, del_v, ins_u, ins_v [cap_batch each], nd, ni
+  uint32_t *d_batch = nullptr;
+  unsigned long long *keys[4] = {};     // raw keys: in-CSR del, in-CSR ins, out-CSR del, out-CSR ins
+  unsigned long long *compact[4] = {};  // sorted, tidied
+  unsigned long long *sorted = nullptr;
+  uint32_t *valid = nullptr, *pos = nullptr, *counts = nullptr;  // counts[4]: sizes of compact[]
+  void *sort_tmp = nullptr, *scan_tmp = nullptr;
+  size_t sort_bytes = 0, scan_bytes = 0;
+  double *rank[2] = {}, *contrib[2] = {}, *stats = nullptr;
+  uint8_t *vaff = nullptr, *naff = nullptr;
+  int *ctl = nullptr;  // [0] done, [1] iteration index (adjacent, as LoopCtl expects), [2] direction, [3] flags
+  unsigned long long *err = nullptr;
+  prh::RankPlan plan{};
+  uint32_t n_items = 0, chunk = 0, stat_slots = 0, pull_wave = 0, pull_group = 0;
+  hipGraphExec_t apply[2] = {}, iter[2] = {};
+  std::vector<std::pair<void *, size_t>> bufs;
+  bool broken = false;
+};
+
+namespace {
+
+constexpr int kStreamChunk = 8;  // iterations per replay of the iteration graph (even: ping-pong buffers realign)
+
+int stream_alloc(prh_stream *S, void **p, size_t bytes) {
+  PRH_CHECK(dev_alloc(S->ctx, p, bytes));
+  S->bufs.emplace_back(*p, bytes ? bytes : 16);
+  return PRH_SUCCESS;
+}
+
+template <class T>
+int stream_alloc(prh_stream *S, T **p, size_t n) {
+  void *v = nullptr;
+  PRH_CHECK(stream_alloc(S, &v, n * sizeof(T)));
+  *p = static_cast<T *>(v);
+  return PRH_SUCCESS;
+}
+
+void stream_free(prh_stream *S) {
+  for (int k = 0; k < 2; ++k) {
+    if (S->apply[k]) (void)hipGraphExecDestroy(S->apply[k]);
+    if (S->iter[k]) (void)hipGraphExecDestroy(S->iter[k]);
+  }
+  for (auto &b : S->bufs) dev_free(S->ctx, b.first, b.second);
+  S->bufs.clear();
+  if (S->h_batch) (void)hipHostFree(S->h_batch);
+  S->h_batch = nullptr;
+}
+
+// One CSR's tidy-and-apply: keys k_del / k_ins (raw) of the current rows (off, idx) into new rows (noff, nidx).
+int stream_apply_csr(prh_stream *S, hipStream_t s, int kd, int ki, const uint32_t *off, const uint32_t *idx,
+                     uint32_t *noff, uint32_t *nidx, int empty_bit) {
+  prh_context *c = S->ctx;
+  const uint32_t cap = S->cap_batch, N = S->N;
+  for (int k : {kd, ki}) {
+    PRH_HIP(c, prh::sort_keys_u64(S->sort_tmp, S->sort_bytes, S->keys[k], S->sorted, cap, s));
+    PRH_HIP(c, prh::launch_batch_valid(S->sorted, cap, off, idx, k == kd, S->valid, s));
+    PRH_HIP(c, prh::scan_u32(S->scan_tmp, S->scan_bytes, S->valid, S->pos, cap, s));
+    PRH_HIP(c, prh::launch_batch_compact(S->sorted, S->valid, S->pos, cap, S->compact[k], S->counts + k, s));
+  }
+  PRH_HIP(c, prh::launch_fill_i32(S->delta, 0, N, s));
+  PRH_HIP(c, prh::launch_fill_u8(S->changed, 0, N, s));
+  PRH_HIP(c, prh::launch_row_delta(S->compact[kd], S->counts + kd, S->compact[ki], S->counts + ki, 2 * cap, S->delta,
+                                   S->changed, s));
+  PRH_HIP(c, prh::launch_new_degrees(off, S->delta, N, S->new_deg, S->ctl + 3, empty_bit, s));
+  PRH_HIP(c, prh::scan_u32(S->scan_tmp, S->scan_bytes, S->new_deg, noff, size_t(N) + 1, s));
+  PRH_HIP(c, prh::launch_rewrite(off, idx, noff, S->changed, S->compact[kd], S->counts + kd, S->compact[ki],
+                                 S->counts + ki, N, S->cap_edges, nidx, s));
+  PRH_HIP(c, prh::launch_insert(off, idx, noff, S->compact[kd], S->counts + kd, S->compact[ki], S->counts + ki, cap,
+                                nidx, s));
+  return PRH_SUCCESS;
+}
+
+// Work of one batch from buffers p to buffers 1 - p, up to the first iteration (captured once per parity).
+int stream_apply_body(prh_stream *S, int p) {
+  prh_context *c = S->ctx;
+  hipStream_t s = c->stream;
+  const uint32_t N = S->N, cap = S->cap_batch;
+  const int q = 1 - p;
+  const uint32_t *b = S->d_batch;
+  PRH_HIP(c, prh::launch_fill_i32(S->ctl, 0, 4, s));
+  PRH_HIP(c, prh::launch_batch_keys(b, b + cap, b + 4 * size_t(cap), cap, S->g->inv, N, S->keys[0], S->keys[2],
+                                    S->ctl + 3, 1, s));
+  PRH_HIP(c, prh::launch_batch_keys(b + 2 * size_t(cap), b + 3 * size_t(cap), b + 4 * size_t(cap) + 1, cap,
+                                    S->g->inv, N, S->keys[1], S->keys[3], S->ctl + 3, 1, s));
+  PRH_CHECK(stream_apply_csr(S, s, 0, 1, S->in_off[p], S->in_src[p], S->in_off[q], S->in_src[q], 0));
+  PRH_CHECK(stream_apply_csr(S, s, 2, 3, S->out_off[p], S->out_dst[p], S->out_off[q], S->out_dst[q], 2));
+  PRH_HIP(c, prh::launch_degrees(S->out_off[q], N, S->outdeg, s));
+  PRH_HIP(c, prh::launch_refresh_chunks(S->plan.item_row, S->plan.row_first, S->n_items, S->in_off[q], S->chunk,
+                                        const_cast<uint32_t *>(S->plan.item_beg),
+                                        const_cast<uint32_t *>(S->plan.item_end), s));
+  // Solver initialisation (pagerankCuda.hxx:811-821): ranks from the previous batch's result (or 1/N for Static),
+  // contributions with the new out-degrees, loop state cleared, and the DF initial marking on the new graph.
+  PRH_HIP(c, prh::launch_fill_u64(S->err, 0ull, size_t(S->opt.max_iterations) + 1, s));
+  PRH_HIP(c, prh::launch_init_ranks(S->rank[0], S->rank[1], S->contrib[0], S->contrib[1], S->outdeg, N,
+                                    S->method != PRH_STATIC, s));
+  if (S->variant >= prh::kFrontier) {
+    PRH_HIP(c, prh::launch_fill_u8(S->vaff, 0, N, s));
+    PRH_HIP(c, prh::launch_fill_u8(S->naff, 0, N, s));
+    PRH_HIP(c, prh::launch_mark_batch(S->naff, S->vaff, S->compact[2], S->counts + 2, S->compact[3], S->counts + 3,
+                                      2 * cap, s));
+    const prh::Schedule os{S->sched_order, S->sched_low, S->sched_high, 0};
+    prh::ExtendArgs E{S->vaff, S->naff, S->out_off[q], S->out_dst[q], os, nullptr, nullptr, nullptr};
+    PRH_HIP(c, prh::launch_extend(E, nullptr, nullptr, 0, 0.0, s));
+    const int mode = c->tuning.extension_direction;
+    if (mode != 1) PRH_HIP(c, prh::launch_fill_i32(S->ctl + 2, mode == 2 ? 1 : 0, 1, s));
+  }
+  return PRH_SUCCESS;
+}
+
+// kStreamChunk iterations on buffers q (captured once per parity); the iteration index comes from ctl[1].
+int stream_iter_body(prh_stream *S, int q) {
+  prh_context *c = S->ctx;
+  hipStream_t s = c->stream;
+  const prh_options &o = S->opt;
+  const double P = o.damping, C0 = (1 - P) / S->N;
+  const bool frontier = S->variant >= prh::kFrontier;
+  const int mode = c->tuning.extension_direction;
+  unsigned long long *err = S->err;
+  int *done = S->ctl, *dir = S->ctl + 2;
+  const prh::Schedule os{S->sched_order, S->sched_low, S->sched_high, 0};
+  const prh::PullArgs pull{S->vaff, S->naff, S->in_off[q], S->in_src[q], S->pull_wave, S->pull_group, S->N, dir};
+  for (int i = 0; i < kStreamChunk; ++i) {
+    prh::RankArgs A{S->rank[(i + 1) & 1], S->rank[i & 1], S->contrib[(i + 1) & 1], S->contrib[i & 1], S->naff,
+                    S->vaff, S->in_off[q], S->in_src[q], S->outdeg, C0, P, o.frontier_tolerance, o.prune_tolerance,
+                    frontier && mode == 1 ? S->stats : nullptr};
+    PRH_HIP(c, prh::launch_rank_update(S->variant, A, S->plan, err, done, -1, o.tolerance, s));
+    if (frontier) {
+      if (mode == 1) PRH_HIP(c, prh::launch_direction(S->stats, S->stat_slots, dir, err, done, -1, o.tolerance, s));
+      prh::ExtendArgs E{S->vaff, S->naff, S->out_off[q], S->out_dst[q], os, nullptr, nullptr, dir};
+      PRH_HIP(c, prh::launch_extend(E, err, done, -1, o.tolerance, s));
+      if (mode != 0) PRH_HIP(c, prh::launch_extend_pull(pull, err, done, -1, o.tolerance, s));
+    }
+    PRH_HIP(c, prh::launch_advance(S->ctl + 1, done, err, o.tolerance, o.max_iterations, s));
+  }
+  return PRH_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, const prh_options *options,
+                      uint32_t max_batch, uint64_t max_edges, const double *initial_ranks, prh_stream **out) {
+  if (!c || !out) return PRH_ERROR_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (!g || !options || g->ctx != c) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null graph/options or foreign graph");
+  if (method == PRH_DYNAMIC_TRAVERSAL)
+    return fail(c, PRH_ERROR_UNSUPPORTED, "the traversal closure needs host-side level checks; not streamed");
+  if (!valid_method(method)) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad method");
+  if (!g->l_out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "the graph was uploaded without its out-CSR");
+  if (g->N == 0 || max_batch == 0) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "empty graph or zero batch capacity");
+  const prh_options &o = *options;
+  if (o.max_iterations < 1 || !(o.tolerance >= 0) || !(o.damping > 0 && o.damping < 1))
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad options");
+  const uint64_t cap_edges = max_edges ? max_edges : g->M + 16ull * max_batch;
+  if (cap_edges < g->M || cap_edges >= (uint64_t(1) << 32))
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "edge capacity must hold the graph and stay below 2^32");
+  PRH_HIP(c, hipSetDevice(c->device));
+  PRH_CHECK(build_rank_plan(c, const_cast<prh_graph *>(g)));
+  auto *S = new prh_stream;
+  S->ctx = c;
+  S->g = g;
+  S->method = method;
+  S->variant = method == PRH_PRUNE_DYNAMIC_FRONTIER ? prh::kPrune
+               : method == PRH_DYNAMIC_FRONTIER      ? prh::kFrontier
+                                                     : prh::kPlain;
+  S->opt = o;
+  S->opt.repeat = 1;
+  S->N = g->N;
+  S->M = g->M;
+  S->cap_edges = cap_edges;
+  S->cap_batch = max_batch;
+  const uint32_t N = g->N;
+  const size_t N1 = size_t(N) + 1;
+  hipStream_t s = c->stream;
+  int st = PRH_SUCCESS;
+  auto step = [&](int r) {
+    if (st == PRH_SUCCESS) st = r;
+  };
+  for (int k = 0; k < 2; ++k) {
+    step(stream_alloc(S, &S->in_off[k], N1));
+    step(stream_alloc(S, &S->in_src[k], cap_edges));
+    step(stream_alloc(S, &S->out_off[k], N1));
+    step(stream_alloc(S, &S->out_dst[k], cap_edges));
+    step(stream_alloc(S, &S->rank[k], N));
+    step(stream_alloc(S, &S->contrib[k], N));
+  }
+  step(stream_alloc(S, &S->outdeg, N));
+  step(stream_alloc(S, &S->new_deg, N1));
+  step(stream_alloc(S, &S->sched_order, N));
+  step(stream_alloc(S, &S->delta, N));
+  step(stream_alloc(S, &S->changed, N));
+  step(stream_alloc(S, &S->d_batch, 4 * size_t(max_batch) + 2));
+  for (int k = 0; k < 4; ++k) {
+    step(stream_alloc(S, &S->keys[k], max_batch));
+    step(stream_alloc(S, &S->compact[k], max_batch));
+  }
+  step(stream_alloc(S, &S->sorted, max_batch));
+  step(stream_alloc(S, &S->valid, max_batch));
+  step(stream_alloc(S, &S->pos, max_batch));
+  step(stream_alloc(S, &S->counts, 4));
+  step(stream_alloc(S, &S->vaff, N));
+  step(stream_alloc(S, &S->naff, N));
+  step(stream_alloc(S, &S->ctl, 4));
+  step(stream_alloc(S, &S->err, size_t(o.max_iterations) + 1));
+  if (st == PRH_SUCCESS && hipHostMalloc(reinterpret_cast<void **>(&S->h_batch), (4 * size_t(max_batch) + 2) * 4) !=
+                               hipSuccess) {
+    (void)hipGetLastError();
+    S->h_batch = nullptr;
+    st = fail(c, PRH_ERROR_OUT_OF_MEMORY, "pinned batch staging");
+  }
+  // Temporary storage of the sort and of the scans (the largest scan is the N + 1 offsets).
+  if (st == PRH_SUCCESS) {
+    hipError_t e = prh::sort_keys_u64(nullptr, S->sort_bytes, S->keys[0], S->sorted, max_batch, s);
+    size_t b1 = 0, b2 = 0;
+    if (e == hipSuccess) e = prh::scan_u32(nullptr, b1, S->valid, S->pos, max_batch, s);
+    if (e == hipSuccess) e = prh::scan_u32(nullptr, b2, S->new_deg, S->in_off[0], N1, s);
+    S->scan_bytes = std::max(b1, b2);
+    if (e != hipSuccess) st = fail(c, PRH_ERROR_DEVICE, "stream temporaries: %s", hipGetErrorString(e));
+  }
+  step(stream_alloc(S, &S->sort_tmp, S->sort_bytes));
+  step(stream_alloc(S, &S->scan_tmp, S->scan_bytes));
+  // Rank plan: the graph's sections, with the session's own chunk tables (refreshed after every batch).
+  if (st == PRH_SUCCESS) {
+    S->plan = g->plan;
+    S->chunk = std::max<uint32_t>(c->tuning.huge_degree, 64);
+    const uint32_t nc = g->plan.n_chunked;
+    S->n_items = nc ? uint32_t((g->chunk_tables_bytes / 4 - nc - 1) / 3) : 0;
+    uint32_t *t = nullptr;
+    step(stream_alloc(S, &t, 3 * size_t(S->n_items) + nc + 1));
+    double *part = nullptr;
+    step(stream_alloc(S, &part, std::max<uint32_t>(S->n_items, 1)));
+    if (st == PRH_SUCCESS) {
+      PRH_HIP(c, hipMemcpyAsync(t, g->plan.item_row, size_t(S->n_items) * 4, hipMemcpyDeviceToDevice, s));
+      PRH_HIP(c, hipMemcpyAsync(t + 3 * size_t(S->n_items), g->plan.row_first, (size_t(nc) + 1) * 4,
+                                hipMemcpyDeviceToDevice, s));
+      S->plan.item_row = t;
+      S->plan.item_beg = t + S->n_items;
+      S->plan.item_end = t + 2 * size_t(S->n_items);
+      S->plan.row_first = t + 3 * size_t(S->n_items);
+      S->plan.partial = part;
+    }
+    S->stat_slots = S->plan.blocks + (nc ? prh::finish_chunked_grid(nc) : 0);
+    step(stream_alloc(S, &S->stats, 2 * size_t(std::max<uint32_t>(S->stat_slots, 1))));
+    S->pull_wave = rows_above(g->sorted_deg, 64);
+    S->pull_group = rows_above(g->sorted_deg, 4);
+  }
+  // Current graph: the layout's in-CSR as is; its out-CSR with every row sorted (the batch tidy searches it).
+  if (st == PRH_SUCCESS) {
+    PRH_HIP(c, hipMemcpyAsync(S->in_off[0], g->l_in_off, N1 * 4, hipMemcpyDeviceToDevice, s));
+    PRH_HIP(c, hipMemcpyAsync(S->in_src[0], g->l_in_src, g->M * 4, hipMemcpyDeviceToDevice, s));
+    PRH_HIP(c, hipMemcpyAsync(S->out_off[0], g->l_out_off, N1 * 4, hipMemcpyDeviceToDevice, s));
+    PRH_HIP(c, hipMemcpyAsync(S->outdeg, g->l_outdeg, size_t(N) * 4, hipMemcpyDeviceToDevice, s));
+    size_t sb = 0;
+    hipError_t e = prh::sort_rows(nullptr, sb, g->l_out_dst, S->out_dst[0], g->M, N, g->l_out_off, s);
+    void *tmp = nullptr;
+    if (e == hipSuccess) st = dev_alloc(c, &tmp, sb);
+    if (st == PRH_SUCCESS && e == hipSuccess) e = prh::sort_rows(tmp, sb, g->l_out_dst, S->out_dst[0], g->M, N,
+                                                                 g->l_out_off, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dev_free(c, tmp, sb);
+    if (e != hipSuccess && st == PRH_SUCCESS) st = fail(c, PRH_ERROR_DEVICE, "stream setup: %s", hipGetErrorString(e));
+  }
+  // Out-degree schedule of the extension (fixed for the session; rows that outgrow the low class stay correct).
+  if (st == PRH_SUCCESS && S->variant >= prh::kFrontier) {
+    prh::ExtendArgs xo{};
+    st = out_schedule(c, const_cast<prh_graph *>(g), xo);
+    if (st == PRH_SUCCESS) {  // the session's kernels take the huge rows as high ones (a wave each)
+      PRH_HIP(c, hipMemcpyAsync(S->sched_order, xo.sched.order, size_t(N) * 4, hipMemcpyDeviceToDevice, s));
+      S->sched_low = xo.sched.n_low;
+      S->sched_high = xo.sched.n_high + xo.sched.n_huge;
+    }
+  }
+  // Initial ranks (layout order): the caller's, or a static solve of the base graph.
+  if (st == PRH_SUCCESS) {
+    double *tmp = nullptr;
+    st = dev_alloc(c, reinterpret_cast<void **>(&tmp), size_t(N) * 8);
+    if (st == PRH_SUCCESS) {
+      if (initial_ranks) {
+        for (uint32_t v = 0; v < N && st == PRH_SUCCESS; ++v)
+          if (!std::isfinite(initial_ranks[v])) st = fail(c, PRH_ERROR_INVALID_ARGUMENT, "initial ranks not finite");
+        if (st == PRH_SUCCESS && hipMemcpy(tmp, initial_ranks, size_t(N) * 8, hipMemcpyHostToDevice) != hipSuccess)
+          st = fail(c, PRH_ERROR_DEVICE, "initial ranks upload");
+      } else {
+        prh_stats stt{};
+        st = solve_impl(c, g, PRH_STATIC, &o, nullptr, nullptr, 0, nullptr, 0, nullptr, tmp, &stt, false);
+      }
+      if (st == PRH_SUCCESS) {
+        hipError_t e = prh::launch_gather_f64(tmp, g->perm, N, S->rank[0], s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = fail(c, PRH_ERROR_DEVICE, "initial ranks: %s", hipGetErrorString(e));
+      }
+      dev_free(c, tmp, size_t(N) * 8);
+    }
+  }
+  if (st != PRH_SUCCESS) {
+    stream_free(S);
+    delete S;
+    return st;
+  }
+  *out = S;
+  return PRH_SUCCESS;
+}
+
+int prh_stream_update(prh_stream *S, const uint32_t *del_u, const uint32_t *del_v, size_t nd, const uint32_t *ins_u,
+                      const uint32_t *ins_v, size_t ni, double *ranks, prh_stats *stats) {
+  if (!S) return PRH_ERROR_INVALID_ARGUMENT;
+  prh_context *c = S->ctx;
+  if (S->broken) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "stream is unusable after an earlier failed update");
+  if (nd > S->cap_batch || ni > S->cap_batch)
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "batch of %zu deletions / %zu insertions exceeds the capacity %u", nd,
+                ni, S->cap_batch);
+  if ((nd && (!del_u || !del_v)) || (ni && (!ins_u || !ins_v)))
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null batch array");
+  if (S->M + ni > S->cap_edges)
+    return fail(c, PRH_ERROR_OUT_OF_MEMORY, "the graph could outgrow the stream's edge capacity %llu",
+                (unsigned long long)S->cap_edges);
+  PRH_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const uint32_t cap = S->cap_batch;
+  uint32_t *h = S->h_batch;
+  if (nd) {
+    std::memcpy(h, del_u, nd * 4);
+    std::memcpy(h + cap, del_v, nd * 4);
+  }
+  if (ni) {
+    std::memcpy(h + 2 * size_t(cap), ins_u, ni * 4);
+    std::memcpy(h + 3 * size_t(cap), ins_v, ni * 4);
+  }
+  h[4 * size_t(cap)] = uint32_t(nd);
+  h[4 * size_t(cap) + 1] = uint32_t(ni);
+  PRH_HIP(c, hipEventRecord(c->ev[0], s));
+  PRH_HIP(c, hipMemcpyAsync(S->d_batch, h, (4 * size_t(cap) + 2) * 4, hipMemcpyHostToDevice, s));
+  const int p = S->cur, q = 1 - p;
+  if (!S->apply[p]) PRH_CHECK(capture_graph(c, [&]() { return stream_apply_body(S, p); }, &S->apply[p]));
+  PRH_HIP(c, hipGraphLaunch(S->apply[p], s));
+  PRH_HIP(c, hipEventRecord(c->ev[1], s));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, S->ctl + 3, 4, hipMemcpyDeviceToHost, s));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned + 1, S->in_off[q] + S->N, 4, hipMemcpyDeviceToHost, s));
+  PRH_HIP(c, hipStreamSynchronize(s));
+  const int flags = *reinterpret_cast<const int *>(c->pinned);
+  if (flags) {
+    S->broken = true;
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, flags & 1 ? "batch vertex id out of range"
+                                                          : "the batch leaves a vertex without out-edges (dead end)");
+  }
+  S->M = *reinterpret_cast<const uint32_t *>(c->pinned + 1);
+  S->cur = q;
+  if (!S->iter[q]) PRH_CHECK(capture_graph(c, [&]() { return stream_iter_body(S, q); }, &S->iter[q]));
+  int n = 0;
+  for (int launched = 0; launched < S->opt.max_iterations; launched += kStreamChunk) {
+    PRH_HIP(c, hipGraphLaunch(S->iter[q], s));
+    PRH_HIP(c, hipMemcpyAsync(c->pinned, S->ctl, 4, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipStreamSynchronize(s));
+    n = *reinterpret_cast<const int *>(c->pinned);
+    if (n) break;
+  }
+  if (n & 1) {  // the result is in buffer 1: bring it to buffer 0 for the next batch
+    PRH_HIP(c, prh::launch_copy_f64(S->rank[0], S->rank[1], S->N, s));
+    PRH_HIP(c, prh::launch_copy_f64(S->contrib[0], S->contrib[1], S->N, s));
+  }
+  PRH_HIP(c, hipEventRecord(c->ev[2], s));
+  PRH_HIP(c, hipEventSynchronize(c->ev[2]));
+  float t01 = 0, t12 = 0;
+  PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
+  PRH_HIP(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+  if (stats) {
+    stats->iterations = n;
+    stats->initialization_time = t01;  // batch upload, application and marking
+    stats->marking_time = 0;
+    stats->computation_time = t12;
+    stats->time = t01 + t12;
+  }
+  if (ranks) PRH_CHECK(prh_stream_ranks(S, ranks));
+  return PRH_SUCCESS;
+}
+
+int prh_stream_ranks(prh_stream *S, double *ranks) {
+  if (!S || !ranks) return S ? fail(S->ctx, PRH_ERROR_INVALID_ARGUMENT, "null ranks") : PRH_ERROR_INVALID_ARGUMENT;
+  prh_context *c = S->ctx;
+  PRH_CHECK(ensure(c, c->contrib[0], S->N));
+  PRH_HIP(c, prh::launch_scatter_f64(S->rank[0], S->g->perm, S->N, c->contrib[0].p, c->stream));
+  PRH_HIP(c, hipMemcpyAsync(ranks, c->contrib[0].p, size_t(S->N) * 8, hipMemcpyDeviceToHost, c->stream));
+  PRH_HIP(c, hipStreamSynchronize(c->stream));
+  return PRH_SUCCESS;
+}
+
+uint64_t prh_stream_size(const prh_stream *S) { return S ? S->M : 0; }
+
+void prh_stream_destroy(prh_stream *S) {
+  if (!S) return;
+  (void)hipSetDevice(S->ctx->device);
+  (void)hipStreamSynchronize(S->ctx->stream);
+  stream_free(S);
+  delete S;
+}
+
+}  // extern "C"
