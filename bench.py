@@ -3,8 +3,9 @@
 
 A step is one complete static solve (tolerance 1e-10 on the L-inf change, damping 0.85, FP64 ranks, at most 500
 iterations) of a synthetic graph that is already resident in HBM, writing its ranks to a device buffer
-(prh_solve_device). Default workload: directed RMAT scale 24 (16,777,216 vertices, 268,435,456 samples,
-a/b/c/d = 0.57/0.19/0.19/0.05, seed 42, duplicates removed, one self-loop per vertex). Before timing, the
+(prh_solve_device). Default workload: the web-scale graph of the dynamic configuration, directed RMAT scale 26 with
+edge factor 30 (67,108,864 vertices, 2,013,265,920 samples, a/b/c/d = 0.57/0.19/0.19/0.05, seed 42, duplicates
+removed, one self-loop per vertex: 2.04B edges); --scale 24 --edge-factor 16 gives RMAT-24. Before timing, the
 autotuner picks the launch plan (prh_autotune; --no-autotune keeps the defaults). `value` is GEdges/s per
 iteration over the whole timed region (|E| x iterations / wall time of the K solves, including initialisation
 and the ranks' return to the caller's vertex order). config.gedges_per_s_iter_host_buffers is the same solve with
@@ -58,13 +59,15 @@ def log(msg: str) -> None:
 def make_graph(args):
     from pagerank_amd import HostGraph
     if args.workload == "dfp":
-        g = HostGraph.rmat(args.scale, 16.0, 0.57, 0.19, 0.19, seed=42)
-        desc = (f"DF-P on directed RMAT scale {args.scale} (edge factor 16, a/b/c/d=.57/.19/.19/.05, seed 42, dedup, "
-                f"self-loops), batches of {args.batch_fraction:g}|E| (80% insertions)")
+        ef = args.edge_factor
+        g = HostGraph.rmat(args.scale, ef, 0.57, 0.19, 0.19, seed=42)
+        desc = (f"DF-P on directed RMAT scale {args.scale} (edge factor {ef:g}, a/b/c/d=.57/.19/.19/.05, seed 42, "
+                f"dedup, self-loops), batches of {args.batch_fraction:g}|E| (80% insertions) applied in place")
         return g, desc
     if args.workload == "rmat":
-        g = HostGraph.rmat(args.scale, 16.0, 0.57, 0.19, 0.19, seed=42)
-        desc = (f"static PageRank, directed RMAT scale {args.scale} (edge factor 16, a/b/c/d=.57/.19/.19/.05, "
+        ef = args.edge_factor
+        g = HostGraph.rmat(args.scale, ef, 0.57, 0.19, 0.19, seed=42)
+        desc = (f"static PageRank, directed RMAT scale {args.scale} (edge factor {ef:g}, a/b/c/d=.57/.19/.19/.05, "
                 f"seed 42, dedup, self-loops)")
     else:
         g = HostGraph.grid(args.rows, args.cols, 0.62, seed=42)
@@ -95,34 +98,42 @@ def cpu_baseline(g, budget_s: float):
 
 
 def bench_dfp(args, ctx, g, desc) -> int:
-    """GPU DF-P per update batch against GPU static on the same updated graph; CPU: the async DF-P port."""
+    """GPU DF-P per update batch against GPU static on the same updated graph; CPU: the async DF-P port.
+
+    Each step applies a fresh seeded batch to the device graph in place (prh_graph_update: both CSRs rewritten, the
+    propagation-blocked plan patched) and solves DF-P from the previous graph's static ranks, then Static on the same
+    graph. Both times include the update (DF-P's solve reports it as preparation; it is added to Static's)."""
     from pagerank_amd import PRUNE_DYNAMIC_FRONTIER, STATIC, Batch, PagerankOptions, apply_batch, solve
     o = PagerankOptions()
-    xg = ctx.upload(g, with_out=False)
+    xg = ctx.upload(g)
     autotuned = False
     if not args.no_autotune:
         t, ms, hit = ctx.autotune(xg, args.tuning_cache, iterations=10)
         autotuned = True
         log(f"autotune: {t} at {ms:.3f} ms/iteration")
     q = solve(xg, STATIC, o).ranks
-    xg.close()
     size = max(1, int(args.batch_fraction * g.M))
-    dfp_ms, st_ms, its, st_its, last = [], [], [], [], None
+    dfp_ms, st_ms, upd_ms, its, st_its, last = [], [], [], [], [], None
+    x = g
     for i in range(args.warmup + args.steps):
-        b = Batch.random(g, size, 0.8, seed=1000 + i)
-        y = apply_batch(g, b)
-        yg = ctx.upload(y)
-        r = solve(yg, PRUNE_DYNAMIC_FRONTIER, o, batch=b, q=q)
-        s = solve(yg, STATIC, o)
-        yg.close()
-        log(f"batch {i}: -{b.deletions}/+{b.insertions}: DF-P {r.time:.2f} ms ({r.iterations} it), static "
-            f"{s.time:.2f} ms ({s.iterations} it), L-inf DF-P vs static {abs(r.ranks - s.ranks).max():.2e}")
+        b = Batch.random(x, size, 0.8, seed=1000 + i)
+        y = apply_batch(x, b)  # host copy: the next batch's base and the CPU baseline's input
+        u = xg.update(b)
+        r = solve(xg, PRUNE_DYNAMIC_FRONTIER, o, batch=b, q=q)
+        s = solve(xg, STATIC, o)
+        log(f"batch {i}: -{b.deletions}/+{b.insertions}: update {u:.2f} ms, DF-P {r.time:.2f} ms with it "
+            f"({r.iterations} it), static {s.time + u:.2f} ms with it ({s.iterations} it), L-inf DF-P vs static "
+            f"{abs(r.ranks - s.ranks).max():.2e}")
         if i >= args.warmup:
             dfp_ms.append(r.time)
-            st_ms.append(s.time)
+            st_ms.append(s.time + u)
+            upd_ms.append(u)
             its.append(r.iterations)
             st_its.append(s.iterations)
-            last = (y, b)
+            last = (y, b, q)
+        q = s.ranks
+        x = y
+    xg.close()
     value = sum(dfp_ms) / len(dfp_ms)
     static = sum(st_ms) / len(st_ms)
     tuning = ctx.get_tuning()
@@ -141,6 +152,7 @@ def bench_dfp(args, ctx, g, desc) -> int:
         "data": "synthetic",
         "config": {"workload": desc, "vertices": g.N, "edges": g.M, "batch_edges": size,
                    "dfp_iterations": its, "static_ms": round(static, 3), "static_iterations": st_its,
+                   "update_ms": round(sum(upd_ms) / len(upd_ms), 3),
                    "speedup_over_gpu_static": round(static / value, 3), "plan": plan_name(tuning),
                    "autotuned": autotuned},
         "roofline": None,
@@ -148,7 +160,7 @@ def bench_dfp(args, ctx, g, desc) -> int:
     if not args.no_cpu:
         import oracle
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        y, b = last
+        y, b, q = last
         _, it, ms = oracle.pagerank_async(PRUNE_DYNAMIC_FRONTIER, y, b, q, threads=threads)
         out["cpu_baseline"] = {"value": round(ms, 3), "unit": "ms", "cores": threads, "kind": "port",
                                "sample": f"one asynchronous OpenMP DF-P solve (the reference's "
@@ -203,7 +215,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["rmat", "grid", "dfp"], default="rmat")
     ap.add_argument("--batch-fraction", type=float, default=1e-5, help="dfp workload: batch size as a fraction of |E|")
-    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--scale", type=int, default=26)
+    ap.add_argument("--edge-factor", type=float, default=30.0, help="rmat workload: edge samples per vertex")
     ap.add_argument("--rows", type=int, default=6000)
     ap.add_argument("--cols", type=int, default=4000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)

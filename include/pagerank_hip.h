@@ -14,7 +14,10 @@
  * (so M < 2^32); the graph handed to a solver is the updated graph y, given as its transpose's CSR
  * (in_offsets[N+1], in_sources[M], in-neighbours ascending), out_degrees[N] (the transpose's vertex value,
  * transpose.hxx:109-134) and, for the dynamic solvers, y's own CSR (out_offsets[N+1], out_targets[M]).
- * Every vertex needs out-degree >= 1 (the reference adds a self-loop per vertex, main.cu:168).
+ * Vertices with out-degree 0 (dead ends) are accepted by the Static, Naive-dynamic, Dynamic Traversal and Dynamic
+ * Frontier solvers and the batched solver, as in the reference (a dead end is never an in-neighbour, so nothing divides
+ * by its degree); DF-P (whose closed-form update divides by the out-degree) and the weighted solves reject a graph
+ * with dead ends (PRH_ERROR_INVALID_GRAPH). The reference's experiment adds a self-loop per vertex (main.cu:168).
  *
  * Errors: every call returns a prh_status; nothing aborts the process (the reference exits on any CUDA
  * error, TRY_CUDA, inc/_cuda.hxx:136-150). prh_context_last_error() describes the last failure.
@@ -34,7 +37,7 @@ extern "C" {
 typedef enum prh_status {
   PRH_SUCCESS = 0,
   PRH_ERROR_INVALID_ARGUMENT = 1, /* null pointer, bad method, bad option, id out of range, NaN rank */
-  PRH_ERROR_INVALID_GRAPH = 2,    /* offsets not 0..M or not monotone, id >= N, out-degree 0 (dead end) */
+  PRH_ERROR_INVALID_GRAPH = 2,    /* offsets not 0..M or not monotone, id >= N; dead ends for DF-P / weighted */
   PRH_ERROR_OUT_OF_MEMORY = 3,    /* device allocation failed or exceeded the context's memory limit */
   PRH_ERROR_DEVICE = 4,           /* HIP runtime error */
   PRH_ERROR_UNSUPPORTED = 5       /* e.g. a dynamic solver on a graph uploaded without its out-CSR */
@@ -61,13 +64,17 @@ typedef struct prh_options {
 } prh_options;
 
 /* PagerankResult<double> (pagerank.hxx:64-113) minus the rank vector: iterations of the last repeat and
- * phase times in milliseconds averaged over repeat (pagerankCuda.hxx:807-824, :848). */
+ * phase times in milliseconds averaged over repeat (pagerankCuda.hxx:807-824, :848). The first solve that runs on
+ * a graph's degree-ordered layout or on a launch plan not used before also reports their one-time build
+ * (preparation_time, wall clock) and adds it to initialization_time and time, as the reference times its per-solve
+ * partition in its initialization time (pagerankCuda.hxx:811-814). */
 typedef struct prh_stats {
   int iterations;
   float time;
   float initialization_time;
   float marking_time;
   float computation_time;
+  float preparation_time;
 } prh_stats;
 
 /* Launch tuning (no counterpart in the reference, whose values are fixed: switch degree 64, 256-thread blocks).
@@ -116,13 +123,31 @@ void *prh_context_stream(prh_context *context);
 
 /* Upload y's CSR arrays (host pointers; out_offsets/out_targets may be NULL for static-only use).
  * Replaces the host->device copies of pagerankInvokeCuda (pagerankCuda.hxx:798-802). The arrays are checked
- * on the device: offsets 0..M monotone, ids < N, out-degree >= 1 and equal to the out-CSR degree. */
+ * on the device: offsets 0..M monotone, ids < N, out-degree equal to the out-CSR degree. */
 int prh_graph_create(prh_context *context, uint32_t n, uint64_t m, const uint32_t *in_offsets,
                      const uint32_t *in_sources, const uint32_t *out_degrees, const uint32_t *out_offsets,
                      const uint32_t *out_targets, prh_graph **graph);
 void prh_graph_destroy(prh_graph *graph);
 uint32_t prh_graph_order(const prh_graph *graph);
 uint64_t prh_graph_size(const prh_graph *graph);
+
+/* Apply an edge batch to an uploaded graph on the device (no counterpart in the reference, whose experiment builds
+ * the updated graph y on the host with tidyBatchUpdateU and applyBatchUpdateOmpU, batch.hxx:201-247, main.cu:80-86,
+ * and uploads it). The batch (host arrays, the caller's vertex ids) is tidied as tidyBatchUpdateU tidies it:
+ * deletions of existing edges only, insertions of absent edges only, each edge once. The graph must have been
+ * uploaded with its out-CSR and afterwards is y for every solver (prh_solve, prh_mark_affected, prh_stream_create);
+ * its propagation-blocked plan (blocks of at most 8192 vertices) is patched in place rather than rebuilt, and the
+ * update's time (also returned in *update_ms, may be NULL) is added to the initialization and preparation times of
+ * the next solve. The vertex set is fixed: an id >= N fails and leaves the graph unchanged. After an update the
+ * batched and weighted solvers refuse the graph (PRH_ERROR_UNSUPPORTED): upload y for them. */
+int prh_graph_update(prh_context *context, prh_graph *graph, const uint32_t *deletion_sources,
+                     const uint32_t *deletion_targets, size_t n_deletions, const uint32_t *insertion_sources,
+                     const uint32_t *insertion_targets, size_t n_insertions, float *update_ms);
+
+/* The graph's current CSR in the caller's vertex ids (host arrays): which = 0 the in-CSR (in_offsets[N+1], in-neighbour
+ * ids ascending), 1 the out-CSR (out_offsets[N+1], targets ascending). ids receives M values. No counterpart in the
+ * reference; it reads back what prh_graph_update built. */
+int prh_graph_csr(prh_context *context, const prh_graph *graph, int which, uint32_t *offsets, uint32_t *ids);
 
 /* Run a solver on y. Replaces pagerankInvokeCuda (pagerankCuda.hxx:713-849) as called by the five entry
  * points (:864-988). Host pointers. The batch is (deletion_sources[i], deletion_targets[i]) and
@@ -222,6 +247,9 @@ int prh_stream_update(prh_stream *stream, const uint32_t *deletion_sources, cons
 int prh_stream_ranks(prh_stream *stream, double *ranks);
 /* The current number of edges. */
 uint64_t prh_stream_size(const prh_stream *stream);
+/* The session's current CSR in the caller's vertex ids, as prh_graph_csr gives a graph's (which = 0 in-CSR, 1 out-CSR;
+ * offsets N + 1 values, ids prh_stream_size values). */
+int prh_stream_csr(prh_stream *stream, int which, uint32_t *offsets, uint32_t *ids);
 void prh_stream_destroy(prh_stream *stream);
 
 /* Per-launch timing of the rank-update kernel with HIP events on the context's stream (no counterpart in the

@@ -365,13 +365,17 @@ int orc_ppr(uint32_t N, const uint32_t *xtoff, const uint32_t *xtedg, const uint
   for (uint32_t v = 0; v < N; ++v) r[v] = s[v];
   int l = 0;
   while (l < L) {
-    for (uint32_t v = 0; v < N; ++v) {
-      double cv = orc_contrib(xtoff, xtdat, xtedg, r, v, 0, 1);
+    /* Vertices are independent (each sums its in-edges in ascending order), so the OpenMP loops give the same bits
+     * for any thread count; the max reduction is order-independent. */
+    #pragma omp parallel for schedule(dynamic, 2048)
+    for (int64_t v = 0; v < (int64_t)N; ++v) {
+      double cv = orc_contrib(xtoff, xtdat, xtedg, r, (uint32_t)v, 0, 1);
       a[v] = fma(P, cv, (1 - P) * s[v]);
     }
     ++l;
     double el = 0.0;
-    for (uint32_t v = 0; v < N; ++v) el = fmax(el, fabs(a[v] - r[v]));
+    #pragma omp parallel for schedule(static) reduction(max : el)
+    for (int64_t v = 0; v < (int64_t)N; ++v) el = fmax(el, fabs(a[v] - r[v]));
     double *t = a; a = r; r = t;
     if (el < E) break;
   }
@@ -392,15 +396,18 @@ int orc_weighted_forward(uint32_t N, const uint32_t *xtoff, const uint32_t *xted
   for (uint32_t v = 0; v < N; ++v) r[v] = s[v];
   int l = 0;
   while (l < L) {
-    for (uint32_t u = 0; u < N; ++u) y[u] = r[u] / W[u];
-    for (uint32_t v = 0; v < N; ++v) {
+    #pragma omp parallel for schedule(static)
+    for (int64_t u = 0; u < (int64_t)N; ++u) y[u] = r[u] / W[u];
+    #pragma omp parallel for schedule(dynamic, 2048)
+    for (int64_t v = 0; v < (int64_t)N; ++v) {
       double cv = 0.0;
       for (uint32_t i = xtoff[v]; i < xtoff[v + 1]; ++i) cv += (double)w_in[i] * y[xtedg[i]];
       a[v] = fma(P, cv, (1 - P) * s[v]);
     }
     ++l;
     double el = 0.0;
-    for (uint32_t v = 0; v < N; ++v) el = fmax(el, fabs(a[v] - r[v]));
+    #pragma omp parallel for schedule(static) reduction(max : el)
+    for (int64_t v = 0; v < (int64_t)N; ++v) el = fmax(el, fabs(a[v] - r[v]));
     double *t = a; a = r; r = t;
     if (el < E) break;
   }
@@ -420,14 +427,16 @@ int orc_weighted_adjoint(uint32_t N, const uint32_t *xoff, const uint32_t *xedg,
   for (uint32_t v = 0; v < N; ++v) r[v] = g[v];
   int l = 0;
   while (l < L) {
-    for (uint32_t u = 0; u < N; ++u) {
+    #pragma omp parallel for schedule(dynamic, 2048)
+    for (int64_t u = 0; u < (int64_t)N; ++u) {
       double cu = 0.0;
       for (uint32_t i = xoff[u]; i < xoff[u + 1]; ++i) cu += (double)w_out[i] * r[xedg[i]];
       a[u] = fma(P / W[u], cu, g[u]);
     }
     ++l;
     double el = 0.0;
-    for (uint32_t v = 0; v < N; ++v) el = fmax(el, fabs(a[v] - r[v]));
+    #pragma omp parallel for schedule(static) reduction(max : el)
+    for (int64_t v = 0; v < (int64_t)N; ++v) el = fmax(el, fabs(a[v] - r[v]));
     double *t = a; a = r; r = t;
     if (el < E) break;
   }
@@ -442,7 +451,8 @@ int orc_weighted_adjoint(uint32_t N, const uint32_t *xoff, const uint32_t *xedg,
  * fixed point as (lam_u - g_u)/P rather than stored. */
 void orc_weighted_grad(uint32_t N, const uint32_t *xoff, const uint32_t *xedg, const double *W, const double *r,
                        const double *lam, const double *g, double P, double *grad_w, double *grad_s) {
-  for (uint32_t u = 0; u < N; ++u) {
+  #pragma omp parallel for schedule(dynamic, 2048)
+  for (int64_t u = 0; u < (int64_t)N; ++u) {
     grad_s[u] = (1 - P) * lam[u];
     double su = (lam[u] - g[u]) / P;
     double fu = P * r[u] / W[u];

@@ -1,0 +1,389 @@
+// apply.hip -- prh_graph_update: an edge batch applied to an uploaded graph on the device, and the graph's
+// propagation-blocked plan patched in place instead of rebuilt.
+//
+// The reference builds every updated graph y on the host (tidyBatchUpdateU, applyBatchUpdateOmpU,
+// transposeWithDegreeOmp; batch.hxx:201-247, transpose.hxx:109-134, main.cu:80-86) and uploads it for each solve,
+// whose partition it then times (pagerankCuda.hxx:811-814). Uploading y here would mean a new degree-ordered layout
+// and a new blocked plan per batch: three radix sorts over the edges, about as long as a static solve. Instead:
+//   tidy        the batch's keys sorted per CSR and kept as tidyBatchUpdateU keeps them (deletions of existing edges,
+//               insertions of absent ones, each once), with the session's kernels (stream.hip);
+//   apply       both CSRs of the layout rewritten into new arrays (rows stay ascending), the out-degrees recounted;
+//   patch       the blocked plan keeps its slots: a deleted edge that is a slot gets the dead flag, an inserted one
+//               that is not becomes an entry of the plan's overlay (sorted by destination), which accumulate adds
+//               per bin. The vertex order (the layout's permutation) stays that of the upload.
+// Everything else that depends on the edges (the row plan's chunk tables, the sliced plan, the extension schedule,
+// the adjoint's plan) is rebuilt on its next use. The update's time is reported by the next solve, in its
+// initialization and preparation times.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "capi_internal.hpp"
+
+namespace {
+
+using prhc::fail;
+
+__global__ void count_zero_kernel(const uint32_t *deg, uint32_t N, uint32_t *out) {
+  uint32_t z = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) z += deg[v] == 0;
+  if (z) atomicAdd(out, z);
+}
+
+// Device temporaries of one update, freed when it ends.
+struct Temps {
+  prh_context *c;
+  std::vector<std::pair<void *, size_t>> v;
+  explicit Temps(prh_context *cc) : c(cc) {}
+  ~Temps() {
+    for (auto &b : v) prhc::dev_free(c, b.first, b.second);
+  }
+  template <class T>
+  int get(T **p, size_t n) {
+    void *q = nullptr;
+    const size_t bytes = std::max<size_t>(n * sizeof(T), 16);
+    PRH_CHECK(prhc::dev_alloc(c, &q, bytes));
+    v.emplace_back(q, bytes);
+    *p = static_cast<T *>(q);
+    return PRH_SUCCESS;
+  }
+};
+
+// The tidied keys of one CSR side: keys sorted, first of each key whose edge exists (deletions) or is absent
+// (insertions), compacted into out with their count in *count.
+int tidy(prh_context *c, const unsigned long long *keys, uint32_t cap, const uint32_t *off, const uint32_t *idx,
+         bool deletions, unsigned long long *sorted, uint32_t *valid, uint32_t *pos, void *sort_tmp, size_t sort_bytes,
+         void *scan_tmp, size_t scan_bytes, unsigned long long *out, uint32_t *count) {
+  hipStream_t s = c->stream;
+  if (!cap) return PRH_SUCCESS;
+  PRH_HIP(c, prh::sort_keys_u64(sort_tmp, sort_bytes, keys, sorted, cap, s));
+  PRH_HIP(c, prh::launch_batch_valid(sorted, cap, off, idx, deletions, valid, s));
+  PRH_HIP(c, prh::scan_u32(scan_tmp, scan_bytes, valid, pos, cap, s));
+  PRH_HIP(c, prh::launch_batch_compact(sorted, valid, pos, cap, out, count, s));
+  return PRH_SUCCESS;
+}
+
+}  // namespace
+
+namespace prhc {
+
+// The plan's in-place patch for one batch (tidied in-CSR keys dkeys / ikeys, counts on the device, nd / ni on the
+// host). On the first update the CSR the plan was built from becomes its base (ownership moves from the graph).
+int patch_plan(prh_context *c, prh_graph *g, const unsigned long long *dkeys, const uint32_t *dcount, uint32_t nd,
+               const unsigned long long *ikeys, const uint32_t *icount, uint32_t ni) {
+  BinnedCache &bc = g->bin;
+  hipStream_t s = c->stream;
+  const uint32_t N = g->N;
+  const uint32_t B = uint32_t(bc.B);
+  const uint32_t nb = (N + B - 1) / B;
+  if (!bc.slot_of) {  // slot map inverse: base in-CSR slot -> plan slot
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&bc.slot_of), std::max<size_t>(bc.base_M, 4) * 4));
+    if (bc.base_M) PRH_HIP(c, prh::launch_invert(bc.plan.slot, uint32_t(bc.base_M), bc.slot_of, s));
+  }
+  Temps tmp(c);
+  uint8_t *gone = nullptr;
+  unsigned long long *fresh = nullptr, *cand = nullptr, *cand_sorted = nullptr;
+  uint32_t *kept = nullptr;
+  const uint32_t ov_n = bc.plan.ov_n;
+  PRH_CHECK(tmp.get(&gone, std::max<uint32_t>(ov_n, 1)));
+  PRH_CHECK(tmp.get(&fresh, std::max<uint32_t>(ni, 1)));
+  PRH_CHECK(tmp.get(&kept, 1));
+  PRH_HIP(c, prh::launch_fill_u8(gone, 0, std::max<uint32_t>(ov_n, 1), s));
+  PRH_HIP(c, prh::launch_fill_u32(kept, 0, 1, s));
+  uint16_t *dst16 = const_cast<uint16_t *>(bc.plan.dst16);
+  PRH_HIP(c, prh::launch_patch_slots(dkeys, dcount, nd, true, bc.base_off, bc.base_src, bc.slot_of, dst16,
+                                     bc.plan.ov_key, ov_n, gone, nullptr, s));
+  PRH_HIP(c, prh::launch_patch_slots(ikeys, icount, ni, false, bc.base_off, bc.base_src, bc.slot_of, dst16,
+                                     bc.plan.ov_key, ov_n, gone, fresh, s));
+  const uint32_t total = ov_n + ni;
+  if (total == 0) return PRH_SUCCESS;
+  PRH_CHECK(tmp.get(&cand, total));
+  PRH_CHECK(tmp.get(&cand_sorted, total));
+  PRH_HIP(c, prh::launch_overlay_candidates(bc.plan.ov_key, ov_n, gone, fresh, ni, cand, kept, s));
+  size_t sort_bytes = 0;
+  PRH_HIP(c, prh::sort_keys_u64(nullptr, sort_bytes, cand, cand_sorted, total, s));
+  uint8_t *sort_tmp = nullptr;
+  PRH_CHECK(tmp.get(&sort_tmp, sort_bytes));
+  PRH_HIP(c, prh::sort_keys_u64(sort_tmp, sort_bytes, cand, cand_sorted, total, s));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, kept, 4, hipMemcpyDeviceToHost, s));
+  PRH_HIP(c, hipStreamSynchronize(s));
+  const uint32_t n = *reinterpret_cast<const uint32_t *>(c->pinned);
+  if (n > bc.ov_cap || !bc.ov_off) {
+    const size_t cap = std::max<size_t>(size_t(n) * 2, 4096);
+    free_overlay(c, bc);
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&bc.ov_key), cap * 8));
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&bc.ovX), cap * 8));
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&bc.ov_off), (size_t(nb) + 1) * 4));
+    bc.ov_cap = cap;
+  }
+  if (n) PRH_HIP(c, hipMemcpyAsync(bc.ov_key, cand_sorted, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+  PRH_HIP(c, prh::launch_overlay_bounds(bc.ov_key, n, B, nb, bc.ov_off, s));
+  PRH_HIP(c, hipStreamSynchronize(s));
+  bc.plan.ov_n = n;
+  bc.plan.ov_key = bc.ov_key;
+  bc.plan.ov_off = bc.ov_off;
+  bc.plan.ovX = bc.ovX;
+  return PRH_SUCCESS;
+}
+
+// A layout-order CSR on the device (rows = layout rows, ids = layout ids) as the caller's CSR on the host: row v is
+// layout row inv[v] with every id mapped through perm, ascending (relabelled and sorted per row on the device, then
+// copied back). off receives N + 1 offsets, idx M ids.
+int csr_to_caller(prh_context *c, const uint32_t *perm_d, const uint32_t *inv_d, uint32_t N, const uint32_t *loff_d,
+                  const uint32_t *lidx_d, uint64_t M, uint32_t *off, uint32_t *idx) {
+  hipStream_t s = c->stream;
+  Temps tmp(c);
+  uint32_t *ldeg = nullptr, *cdeg = nullptr, *coff = nullptr, *rows = nullptr, *sorted = nullptr;
+  PRH_CHECK(tmp.get(&ldeg, N));
+  PRH_CHECK(tmp.get(&cdeg, size_t(N) + 1));
+  PRH_CHECK(tmp.get(&coff, size_t(N) + 1));
+  PRH_CHECK(tmp.get(&rows, M));
+  PRH_CHECK(tmp.get(&sorted, M));
+  size_t scan_bytes = 0, sort_bytes = 0;
+  PRH_HIP(c, prh::scan_u32(nullptr, scan_bytes, cdeg, coff, size_t(N) + 1, s));
+  PRH_HIP(c, prh::sort_rows(nullptr, sort_bytes, rows, sorted, M, N, coff, s));
+  uint8_t *work = nullptr;
+  PRH_CHECK(tmp.get(&work, std::max(scan_bytes, sort_bytes)));
+  PRH_HIP(c, prh::launch_degrees(loff_d, N, ldeg, s));
+  PRH_HIP(c, prh::launch_gather_u32(ldeg, inv_d, N, cdeg, s));
+  PRH_HIP(c, prh::launch_fill_u32(cdeg + N, 0, 1, s));
+  PRH_HIP(c, prh::scan_u32(work, scan_bytes, cdeg, coff, size_t(N) + 1, s));
+  PRH_HIP(c, prh::launch_relabel_rows(loff_d, lidx_d, inv_d, perm_d, coff, N, rows, s));
+  PRH_HIP(c, prh::sort_rows(work, sort_bytes, rows, sorted, M, N, coff, s));
+  PRH_HIP(c, hipMemcpyAsync(off, coff, (size_t(N) + 1) * 4, hipMemcpyDeviceToHost, s));
+  if (M) PRH_HIP(c, hipMemcpyAsync(idx, sorted, M * 4, hipMemcpyDeviceToHost, s));
+  PRH_HIP(c, hipStreamSynchronize(s));
+  if (off[N] != M)
+    return fail(c, PRH_ERROR_DEVICE, "CSR read-back: %u edges in the offsets, %llu expected", off[N],
+                (unsigned long long)M);
+  return PRH_SUCCESS;
+}
+
+}  // namespace prhc
+
+using namespace prhc;
+
+extern "C" {
+
+int prh_graph_csr(prh_context *c, const prh_graph *g, int which, uint32_t *offsets, uint32_t *ids) {
+  if (!c) return PRH_ERROR_INVALID_ARGUMENT;
+  if (!g || g->ctx != c || !offsets || (g->M && !ids) || (which != 0 && which != 1))
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null or foreign graph, null array or bad CSR selector");
+  if (g->N == 0) {
+    offsets[0] = 0;
+    return PRH_SUCCESS;
+  }
+  if (which == 1 && !g->l_out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "the graph was uploaded without its out-CSR");
+  PRH_HIP(c, hipSetDevice(c->device));
+  PRH_HIP(c, hipStreamSynchronize(c->stream));
+  return csr_to_caller(c, g->perm, g->inv, g->N, which ? g->l_out_off : g->l_in_off,
+                       which ? g->l_out_dst : g->l_in_src, g->M, offsets, ids);
+}
+
+int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const uint32_t *del_v, size_t nd,
+                     const uint32_t *ins_u, const uint32_t *ins_v, size_t ni, float *update_ms) {
+  if (!c) return PRH_ERROR_INVALID_ARGUMENT;
+  if (update_ms) *update_ms = 0.0f;
+  if (!g || g->ctx != c) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null or foreign graph");
+  if ((nd && (!del_u || !del_v)) || (ni && (!ins_u || !ins_v))) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null batch array");
+  if (!g->l_out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "an update needs the graph's out-CSR (upload it with one)");
+  if (nd >= (size_t(1) << 31) || ni >= (size_t(1) << 31)) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "batch too large");
+  const uint32_t N = g->N;
+  if (N == 0 || nd + ni == 0) return PRH_SUCCESS;
+  PRH_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint64_t M = g->M;
+  const size_t N1 = size_t(N) + 1;
+  Temps tmp(c);
+  // The layout's out-rows ascending (the tidy searches them), once per graph.
+  if (!g->out_sorted) {
+    uint32_t *sorted = nullptr;
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&sorted), std::max<size_t>(M, 4) * 4));
+    size_t sb = 0;
+    hipError_t e = prh::sort_rows(nullptr, sb, g->l_out_dst, sorted, M, N, g->l_out_off, s);
+    uint8_t *work = nullptr;
+    int st = e == hipSuccess ? tmp.get(&work, sb) : PRH_ERROR_DEVICE;
+    if (st == PRH_SUCCESS) e = prh::sort_rows(work, sb, g->l_out_dst, sorted, M, N, g->l_out_off, s);
+    if (st == PRH_SUCCESS && e == hipSuccess) e = hipStreamSynchronize(s);
+    if (st != PRH_SUCCESS || e != hipSuccess) {
+      dev_free(c, sorted, std::max<size_t>(M, 4) * 4);
+      return st != PRH_SUCCESS ? st : fail(c, PRH_ERROR_DEVICE, "out-row sort: %s", hipGetErrorString(e));
+    }
+    dev_free(c, g->l_out_dst, std::max<size_t>(M, 4) * 4);
+    g->l_out_dst = sorted;
+    g->out_sorted = true;
+  }
+  // Batch upload: deletions (u, v), insertions (u, v), then the two sizes.
+  const uint32_t cd = uint32_t(nd), ci = uint32_t(ni);
+  uint32_t *d_batch = nullptr;
+  PRH_CHECK(tmp.get(&d_batch, 2 * nd + 2 * ni + 2));
+  if (nd) {
+    PRH_HIP(c, hipMemcpyAsync(d_batch, del_u, nd * 4, hipMemcpyHostToDevice, s));
+    PRH_HIP(c, hipMemcpyAsync(d_batch + nd, del_v, nd * 4, hipMemcpyHostToDevice, s));
+  }
+  if (ni) {
+    PRH_HIP(c, hipMemcpyAsync(d_batch + 2 * nd, ins_u, ni * 4, hipMemcpyHostToDevice, s));
+    PRH_HIP(c, hipMemcpyAsync(d_batch + 2 * nd + ni, ins_v, ni * 4, hipMemcpyHostToDevice, s));
+  }
+  uint32_t *sizes = d_batch + 2 * nd + 2 * ni;
+  reinterpret_cast<uint32_t *>(c->pinned)[0] = cd;
+  reinterpret_cast<uint32_t *>(c->pinned)[1] = ci;
+  PRH_HIP(c, hipMemcpyAsync(sizes, c->pinned, 8, hipMemcpyHostToDevice, s));
+  // Keys per side and CSR (0: in-CSR deletions, 1: in-CSR insertions, 2: out-CSR deletions, 3: out-CSR insertions),
+  // tidied into compact[k] with their counts in counts[k].
+  const uint32_t cap = std::max(cd, ci);
+  unsigned long long *keys[4], *compact[4], *sorted = nullptr;
+  for (int k = 0; k < 4; ++k) {
+    PRH_CHECK(tmp.get(&keys[k], std::max<uint32_t>(k & 1 ? ci : cd, 1)));
+    PRH_CHECK(tmp.get(&compact[k], std::max<uint32_t>(k & 1 ? ci : cd, 1)));
+  }
+  uint32_t *valid = nullptr, *pos = nullptr, *counts = nullptr;
+  int *flags = nullptr;
+  PRH_CHECK(tmp.get(&sorted, cap));
+  PRH_CHECK(tmp.get(&valid, cap));
+  PRH_CHECK(tmp.get(&pos, cap));
+  PRH_CHECK(tmp.get(&counts, 4));
+  PRH_CHECK(tmp.get(&flags, 1));
+  PRH_HIP(c, prh::launch_fill_u32(counts, 0, 4, s));
+  PRH_HIP(c, prh::launch_fill_i32(flags, 0, 1, s));
+  size_t sort_bytes = 0, scan_bytes = 0, b1 = 0;
+  PRH_HIP(c, prh::sort_keys_u64(nullptr, sort_bytes, keys[0], sorted, cap, s));
+  PRH_HIP(c, prh::scan_u32(nullptr, scan_bytes, valid, pos, cap, s));
+  PRH_HIP(c, prh::scan_u32(nullptr, b1, valid, pos, N1, s));
+  scan_bytes = std::max(scan_bytes, b1);
+  uint8_t *sort_tmp = nullptr, *scan_tmp = nullptr;
+  PRH_CHECK(tmp.get(&sort_tmp, sort_bytes));
+  PRH_CHECK(tmp.get(&scan_tmp, scan_bytes));
+  if (cd) PRH_HIP(c, prh::launch_batch_keys(d_batch, d_batch + nd, sizes, cd, g->inv, N, keys[0], keys[2], flags, 1, s));
+  if (ci)
+    PRH_HIP(c, prh::launch_batch_keys(d_batch + 2 * nd, d_batch + 2 * nd + ni, sizes + 1, ci, g->inv, N, keys[1],
+                                      keys[3], flags, 1, s));
+  PRH_CHECK(tidy(c, keys[0], cd, g->l_in_off, g->l_in_src, true, sorted, valid, pos, sort_tmp, sort_bytes, scan_tmp,
+                 scan_bytes, compact[0], counts + 0));
+  PRH_CHECK(tidy(c, keys[1], ci, g->l_in_off, g->l_in_src, false, sorted, valid, pos, sort_tmp, sort_bytes, scan_tmp,
+                 scan_bytes, compact[1], counts + 1));
+  PRH_CHECK(tidy(c, keys[2], cd, g->l_out_off, g->l_out_dst, true, sorted, valid, pos, sort_tmp, sort_bytes,
+                 scan_tmp, scan_bytes, compact[2], counts + 2));
+  PRH_CHECK(tidy(c, keys[3], ci, g->l_out_off, g->l_out_dst, false, sorted, valid, pos, sort_tmp, sort_bytes,
+                 scan_tmp, scan_bytes, compact[3], counts + 3));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, counts, 16, hipMemcpyDeviceToHost, s));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned + 2, flags, 4, hipMemcpyDeviceToHost, s));
+  PRH_HIP(c, hipStreamSynchronize(s));
+  uint32_t cnt[4];
+  std::memcpy(cnt, c->pinned, 16);
+  if (*reinterpret_cast<const int *>(c->pinned + 2))
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "batch vertex id >= N (%u); the graph is unchanged", N);
+  if (cnt[0] != cnt[2] || cnt[1] != cnt[3])
+    return fail(c, PRH_ERROR_INVALID_GRAPH, "in-CSR and out-CSR disagree on the batch (%u/%u deletions, %u/%u "
+                                            "insertions); the graph is unchanged", cnt[0], cnt[2], cnt[1], cnt[3]);
+  const uint64_t M2 = M - cnt[0] + cnt[1];
+  if (M2 >= (uint64_t(1) << 32)) return fail(c, PRH_ERROR_UNSUPPORTED, "the update would exceed 2^32 - 1 edges");
+  // New CSRs. Nothing of the graph changes before this point, so an error above leaves it as it was.
+  int *delta = nullptr;
+  uint8_t *changed = nullptr;
+  uint32_t *new_deg = nullptr;
+  PRH_CHECK(tmp.get(&delta, N));
+  PRH_CHECK(tmp.get(&changed, N));
+  PRH_CHECK(tmp.get(&new_deg, N1));
+  uint32_t *n_in_off = nullptr, *n_in_src = nullptr, *n_out_off = nullptr, *n_out_dst = nullptr;
+  auto alloc = [&](uint32_t **p, size_t n) { return dev_alloc(c, reinterpret_cast<void **>(p), std::max<size_t>(n, 4) * 4); };
+  int st = alloc(&n_in_off, N1);
+  if (st == PRH_SUCCESS) st = alloc(&n_in_src, M2);
+  if (st == PRH_SUCCESS) st = alloc(&n_out_off, N1);
+  if (st == PRH_SUCCESS) st = alloc(&n_out_dst, M2);
+  auto apply_csr = [&](int kd, int ki, const uint32_t *off, const uint32_t *idx, uint32_t *noff, uint32_t *nidx) -> int {
+    PRH_HIP(c, prh::launch_fill_i32(delta, 0, N, s));
+    PRH_HIP(c, prh::launch_fill_u8(changed, 0, N, s));
+    PRH_HIP(c, prh::launch_row_delta(compact[kd], counts + kd, compact[ki], counts + ki, cd + ci, delta, changed, s));
+    PRH_HIP(c, prh::launch_new_degrees(off, delta, N, new_deg, flags, 0, s));
+    PRH_HIP(c, prh::scan_u32(scan_tmp, scan_bytes, new_deg, noff, N1, s));
+    PRH_HIP(c, prh::launch_rewrite(off, idx, noff, changed, compact[kd], counts + kd, compact[ki], counts + ki, N, M,
+                                   nidx, s));
+    PRH_HIP(c, prh::launch_insert(off, idx, noff, compact[kd], counts + kd, compact[ki], counts + ki, ci, nidx, s));
+    return PRH_SUCCESS;
+  };
+  if (st == PRH_SUCCESS) st = apply_csr(0, 1, g->l_in_off, g->l_in_src, n_in_off, n_in_src);
+  if (st == PRH_SUCCESS) st = apply_csr(2, 3, g->l_out_off, g->l_out_dst, n_out_off, n_out_dst);
+  // Host copies of the tidied in-CSR keys: the layout's in-degrees kept on the host follow them.
+  std::vector<unsigned long long> hd(cnt[0]), hi(cnt[1]);
+  if (st == PRH_SUCCESS && cnt[0] &&
+      hipMemcpyAsync(hd.data(), compact[0], size_t(cnt[0]) * 8, hipMemcpyDeviceToHost, s) != hipSuccess)
+    st = fail(c, PRH_ERROR_DEVICE, "batch readback");
+  if (st == PRH_SUCCESS && cnt[1] &&
+      hipMemcpyAsync(hi.data(), compact[1], size_t(cnt[1]) * 8, hipMemcpyDeviceToHost, s) != hipSuccess)
+    st = fail(c, PRH_ERROR_DEVICE, "batch readback");
+  if (st == PRH_SUCCESS && hipStreamSynchronize(s) != hipSuccess) st = fail(c, PRH_ERROR_DEVICE, "update: sync");
+  if (st != PRH_SUCCESS) {
+    (void)hipStreamSynchronize(s);
+    dev_free(c, n_in_off, N1 * 4);
+    dev_free(c, n_in_src, std::max<size_t>(M2, 4) * 4);
+    dev_free(c, n_out_off, N1 * 4);
+    dev_free(c, n_out_dst, std::max<size_t>(M2, 4) * 4);
+    return st;
+  }
+  // Swap in the new CSRs. The blocked plan (blocks of at most 8192 vertices) is patched; its first update keeps the
+  // in-CSR it was built from as its base. A plan with wider blocks has no room for the dead flag: it is rebuilt on
+  // its next use.
+  BinnedCache &bc = g->bin;
+  const bool patch = bc.valid && bc.B <= 8192 && bc.plan.slot;
+  if (patch && !bc.base_off) {
+    bc.base_off = g->l_in_off;
+    bc.base_src = g->l_in_src;
+    bc.base_M = M;
+  } else {
+    dev_free(c, g->l_in_off, N1 * 4);
+    dev_free(c, g->l_in_src, std::max<size_t>(M, 4) * 4);
+  }
+  if (!patch && bc.valid) free_binned(c, bc);
+  dev_free(c, g->l_out_off, N1 * 4);
+  dev_free(c, g->l_out_dst, std::max<size_t>(M, 4) * 4);
+  g->l_in_off = n_in_off;
+  g->l_in_src = n_in_src;
+  g->l_out_off = n_out_off;
+  g->l_out_dst = n_out_dst;
+  g->M = M2;
+  PRH_HIP(c, prh::launch_degrees(g->l_out_off, N, g->l_outdeg, s));
+  uint32_t *zeros = nullptr;
+  PRH_CHECK(tmp.get(&zeros, 1));
+  PRH_HIP(c, prh::launch_fill_u32(zeros, 0, 1, s));
+  count_zero_kernel<<<256, 256, 0, s>>>(g->l_outdeg, N, zeros);
+  PRH_HIP(c, hipGetLastError());
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, zeros, 4, hipMemcpyDeviceToHost, s));
+  if (patch) PRH_CHECK(patch_plan(c, g, compact[0], counts + 0, cnt[0], compact[1], counts + 1, cnt[1]));
+  PRH_HIP(c, hipStreamSynchronize(s));
+  g->dead_ends = *reinterpret_cast<const uint32_t *>(c->pinned);
+  for (unsigned long long k : hd) --g->sorted_deg[uint32_t(k >> 32)];
+  for (unsigned long long k : hi) ++g->sorted_deg[uint32_t(k >> 32)];
+  // What depends on the edges is rebuilt on its next use; the caller-order arrays are dropped.
+  g->plan_valid = false;
+  free_sliced(c, g);
+  free_binned(c, g->bin_t);
+  if (g->out_valid) {
+    dev_free(c, g->out_chunks, std::max<size_t>(3 * size_t(g->out_n_chunks), 1) * 4);
+    g->out_chunks = nullptr;
+    g->out_n_chunks = 0;
+    g->out_valid = false;
+  }
+  dev_free(c, g->l_in_eid, (M ? M : 4) * 4);
+  dev_free(c, g->l_out_eid, (M ? M : 4) * 4);
+  g->l_in_eid = g->l_out_eid = nullptr;
+  if (!g->updated) {
+    dev_free(c, g->in_off, N1 * 4);
+    dev_free(c, g->in_src, (M ? M : 4) * 4);
+    dev_free(c, g->outdeg, size_t(N) * 4);
+    dev_free(c, g->out_off, N1 * 4);
+    dev_free(c, g->out_dst, (M ? M : 4) * 4);
+    g->in_off = g->in_src = g->outdeg = g->out_off = g->out_dst = nullptr;
+    g->updated = true;
+  }
+  const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  g->update_prep.ms = (g->update_prep.reported ? 0.0f : g->update_prep.ms) + ms;
+  g->update_prep.reported = false;
+  if (update_ms) *update_ms = ms;
+  return PRH_SUCCESS;
+}
+
+}  // extern "C"

@@ -206,10 +206,10 @@ int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int
     prh_stats st{};
     o.max_iterations = 2;
     if (status == PRH_SUCCESS) status = solve_impl(c, g, PRH_STATIC, &o, nullptr, nullptr, 0, nullptr, 0, nullptr,
-                                                   ranks, &st, false);
+                                                   ranks, &st, false, false);
     o.max_iterations = iterations;
     if (status == PRH_SUCCESS) status = solve_impl(c, g, PRH_STATIC, &o, nullptr, nullptr, 0, nullptr, 0, nullptr,
-                                                   ranks, &st, false);
+                                                   ranks, &st, false, false);
     if (status != PRH_SUCCESS) break;
     const double ms = double(st.computation_time) / std::max(st.iterations, 1);
     if (best < 0 || ms < best_ms) {

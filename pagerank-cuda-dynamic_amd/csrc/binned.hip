@@ -171,7 +171,10 @@ __global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const d
   }
 }
 
-constexpr uint32_t kPairStart = 0x8000u, kRunStart = 0x4000u, kLocalMask = 0x3FFFu;
+constexpr uint32_t kPairStart = 0x8000u, kRunStart = 0x4000u;
+// A slot whose edge an in-place update deleted (bins of at most 8192 vertices, whose local ids take 13 bits): it
+// keeps its place, and its pair and run flags, but adds nothing.
+constexpr uint32_t kDeadSlot = 0x2000u;
 constexpr int kAccGroups = 2;  // groups of 64 slots a wave of accumulate has in flight (2 measured faster than 4 or 8)
 
 // X entry of the pair of slot g + lane, for a group of 64 slots starting at a multiple of 64, from t (the group
@@ -223,6 +226,8 @@ __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, Binne
   if (!iteration_begin(ctl)) return;
   constexpr bool EXT = Traits<VARIANT>::frontier;
   constexpr bool WEIGHTED = Traits<VARIANT>::weighted;
+  constexpr uint32_t kMask = uint32_t(B) - 1;  // local destination id
+  constexpr bool kDeadBit = B <= 8192;
   __shared__ double sum[B];
   __shared__ uint8_t ext[EXT ? B : 1];
   __shared__ double lds_max[kBinWaves];
@@ -268,9 +273,26 @@ __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, Binne
       for (int u = 0; u < kAccGroups; ++u) {
         const uint32_t gu = g + u * kBinBlock;
         if (gu >= it.z) break;
-        const bool ok = gu + lane >= it.y && gu + lane < it.z;
-        add_slots<EXT>(sum, ext, dc[u] & kLocalMask, x[u] * weight(gu + lane, ok), ok);
+        const bool ok = gu + lane >= it.y && gu + lane < it.z && !(kDeadBit && (dc[u] & kDeadSlot));
+        add_slots<EXT>(sum, ext, dc[u] & kMask, x[u] * weight(gu + lane, ok), ok);
       }
+    }
+  }
+  // Edges an in-place update inserted outside the plan (the overlay, sorted by destination): the bin's first item
+  // adds them, each with its source's value that overlay_expand_kernel wrote this iteration (negative: a frontier
+  // source, as in X).
+  if (P.ov_n && (blockIdx.x == 0 || P.ac_item[blockIdx.x - 1].x != it.x)) {
+    const uint32_t o1 = P.ov_off[it.x + 1];
+    for (uint32_t i = P.ov_off[it.x] + threadIdx.x; i < o1; i += kBinBlock) {
+      double x = P.ovX[i];
+      const uint32_t d = uint32_t(P.ov_key[i] >> 32) & kMask;
+      if constexpr (EXT) {
+        if (x < 0.0) {
+          ext[d] = 1;
+          x = -x;
+        }
+      }
+      atomicAdd(&sum[d], x);
     }
   }
   __syncthreads();
@@ -356,11 +378,28 @@ __global__ __launch_bounds__(kBinBlock) void finish_bins_kernel(RankArgs A, Binn
   if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
 }
 
+// The overlay's values for this iteration: entry i takes its source's contribution, negated when the source was in
+// the last frontier (naff, frontier variants after their first iteration), as expand does for X. It runs between
+// expand and accumulate: accumulate overwrites naff with the new frontier.
+__global__ __launch_bounds__(kBlock) void overlay_expand_kernel(BinnedPlan P, const double *cr, const uint8_t *naff,
+                                                                LoopCtl ctl) {
+  if (!iteration_begin(ctl)) return;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < P.ov_n; i += gridDim.x * kBlock) {
+    const uint32_t u = uint32_t(P.ov_key[i]);
+    const double c = cr[u];
+    P.ovX[i] = naff && naff[u] ? -c : c;
+  }
+}
+
 template <int B, int VARIANT>
 hipError_t binned_update(const RankArgs &A, const BinnedPlan &P, const LoopCtl &ctl, bool extend, hipStream_t s,
                          hipEvent_t mark) {
   const uint8_t *naff = Traits<VARIANT>::frontier && extend ? A.naff : nullptr;
   if (P.n_ex) expand_kernel<B><<<P.n_ex, kBinBlock, 0, s>>>(P, A.cr, naff, ctl);
+  if (P.ov_n) {
+    const uint32_t g = (P.ov_n + kBlock - 1) / kBlock;
+    overlay_expand_kernel<<<g < 4096 ? g : 4096, kBlock, 0, s>>>(P, A.cr, naff, ctl);
+  }
   accumulate_kernel<B, VARIANT><<<P.n_ac, kBinBlock, 0, s>>>(A, P, ctl);
   if (P.n_multi) finish_bins_kernel<VARIANT><<<P.n_multi, kBinBlock, 0, s>>>(A, P, ctl);
   if (mark) {
@@ -478,6 +517,76 @@ __global__ __launch_bounds__(kBlock) void lower_bounds_kernel(const uint32_t *ke
   start[s] = uint32_t(lo);
 }
 
+// ---- in-place update of a plan (prh_graph_update) ----
+
+constexpr unsigned long long kNoKey = ~0ull;
+
+// First index in a[lo, hi) with a[i] >= x.
+template <class T>
+__device__ __forceinline__ uint64_t lower_index(const T *a, uint64_t lo, uint64_t hi, T x) {
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// One tidied batch edge per thread (keys: destination << 32 | source, layout ids; count: device word). The edge is
+// looked up in the base in-CSR (the CSR the plan was built from, rows ascending). A deletion (dead = 1) found there
+// sets its slot's dead flag, one not found there is an overlay entry, marked gone; an insertion (dead = 0) found
+// there (a base edge deleted earlier) clears the flag, one not found there becomes a new overlay entry (fresh[i]).
+__global__ __launch_bounds__(kBlock) void patch_slots_kernel(const unsigned long long *keys, const uint32_t *count,
+                                                             uint32_t cap, int dead, const uint32_t *base_off,
+                                                             const uint32_t *base_src, const uint32_t *slot_of,
+                                                             uint16_t *dst16, const unsigned long long *ov_key,
+                                                             uint32_t ov_n, uint8_t *ov_gone,
+                                                             unsigned long long *fresh) {
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cap; i += gridDim.x * kBlock) {
+    unsigned long long f = kNoKey;
+    if (i < n) {
+      const unsigned long long k = keys[i];
+      const uint32_t v = uint32_t(k >> 32), u = uint32_t(k);
+      const uint32_t b = base_off[v], e = base_off[v + 1];
+      const uint32_t at = uint32_t(lower_index(base_src, b, e, u));
+      if (at < e && base_src[at] == u) {
+        const uint32_t j = slot_of[at];
+        dst16[j] = uint16_t(dead ? (dst16[j] | kDeadSlot) : (dst16[j] & ~kDeadSlot));
+      } else if (dead) {
+        const uint64_t o = lower_index(ov_key, 0, ov_n, k);
+        if (o < ov_n && ov_key[o] == k) ov_gone[o] = 1;
+      } else {
+        f = k;
+      }
+    }
+    if (!dead) fresh[i] = f;
+  }
+}
+
+// The next overlay's candidates: the current entries not gone, then the fresh insertions (kNoKey for none); *kept
+// counts the live ones.
+__global__ __launch_bounds__(kBlock) void overlay_candidates_kernel(const unsigned long long *ov_key, uint32_t ov_n,
+                                                                    const uint8_t *ov_gone,
+                                                                    const unsigned long long *fresh, uint32_t n_fresh,
+                                                                    unsigned long long *cand, uint32_t *kept) {
+  uint32_t live = 0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < ov_n + n_fresh; i += gridDim.x * kBlock) {
+    const unsigned long long k = i < ov_n ? (ov_gone[i] ? kNoKey : ov_key[i]) : fresh[i - ov_n];
+    cand[i] = k;
+    live += k != kNoKey;
+  }
+  if (live) atomicAdd(kept, live);
+}
+
+// off[b] = first overlay entry with destination >= b B, for b = 0 .. nb (keys sorted).
+__global__ __launch_bounds__(kBlock) void overlay_bounds_kernel(const unsigned long long *keys, uint32_t n, uint32_t B,
+                                                                uint32_t nb, uint32_t *off) {
+  const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+  if (b > nb) return;
+  off[b] = uint32_t(lower_index(keys, 0, n, (unsigned long long)b * B << 32));
+}
+
 template <int B>
 hipError_t binned_update_variant(int variant, const RankArgs &A, const BinnedPlan &P, const LoopCtl &ctl,
                                  bool extend, hipStream_t s, hipEvent_t mark) {
@@ -505,6 +614,31 @@ hipError_t launch_binned_update(int variant, const RankArgs &A, const BinnedPlan
     case 16384: return binned_update_variant<16384>(variant, A, P, ctl, extend, s, mark);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_patch_slots(const unsigned long long *keys, const uint32_t *count, uint32_t cap, bool dead,
+                              const uint32_t *base_off, const uint32_t *base_src, const uint32_t *slot_of,
+                              uint16_t *dst16, const unsigned long long *ov_key, uint32_t ov_n, uint8_t *ov_gone,
+                              unsigned long long *fresh, hipStream_t s) {
+  if (!cap) return hipSuccess;
+  patch_slots_kernel<<<grid(cap, kBlock, 4096), kBlock, 0, s>>>(keys, count, cap, dead ? 1 : 0, base_off, base_src,
+                                                                 slot_of, dst16, ov_key, ov_n, ov_gone, fresh);
+  return hipGetLastError();
+}
+
+hipError_t launch_overlay_candidates(const unsigned long long *ov_key, uint32_t ov_n, const uint8_t *ov_gone,
+                                     const unsigned long long *fresh, uint32_t n_fresh, unsigned long long *cand,
+                                     uint32_t *kept, hipStream_t s) {
+  if (!(ov_n + n_fresh)) return hipSuccess;
+  overlay_candidates_kernel<<<grid(ov_n + n_fresh, kBlock, 4096), kBlock, 0, s>>>(ov_key, ov_n, ov_gone, fresh,
+                                                                                   n_fresh, cand, kept);
+  return hipGetLastError();
+}
+
+hipError_t launch_overlay_bounds(const unsigned long long *keys, uint32_t n, uint32_t B, uint32_t nb, uint32_t *off,
+                                 hipStream_t s) {
+  overlay_bounds_kernel<<<(nb + kBlock) / kBlock, kBlock, 0, s>>>(keys, n, B, nb, off);
+  return hipGetLastError();
 }
 
 hipError_t launch_bin_keys(const uint32_t *off, const uint32_t *src, uint32_t N, uint32_t B, uint32_t nblocks,

@@ -226,13 +226,15 @@ int prh_graph_create(prh_context *c, uint32_t N, uint64_t M, const uint32_t *in_
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) st = fail(c, PRH_ERROR_DEVICE, "graph validation: %s", hipGetErrorString(e));
     else {
-      const int bits = *reinterpret_cast<int *>(c->pinned);
+      const int bits = *reinterpret_cast<int *>(c->pinned) & ~2;  // bit 2 (a dead end) is not an error
       if (bits)
         st = fail(c, PRH_ERROR_INVALID_GRAPH, "invalid graph (bits 0x%x:%s%s%s%s%s%s)", bits,
                   bits & 1 ? " in-offsets" : "", bits & 2 ? " dead-end" : "", bits & 4 ? " out-degree-mismatch" : "",
                   bits & 8 ? " in-id-range" : "", bits & 16 ? " out-id-range" : "", bits & 32 ? " out-offsets" : "");
     }
   }
+  if (st == PRH_SUCCESS)
+    for (uint32_t v = 0; v < N; ++v) g->dead_ends += outdeg[v] == 0;
   if (st == PRH_SUCCESS) st = build_layout(c, g, in_off, outdeg, out_off != nullptr);
   if (st != PRH_SUCCESS) {
     prh_graph_destroy(g);
@@ -278,14 +280,14 @@ int prh_solve(prh_context *c, const prh_graph *g, prh_method method, int partiti
               const uint32_t *du, const uint32_t *dv, size_t nd, const uint32_t *iu, size_t ni, const double *q,
               double *ranks, prh_stats *st) {
   (void)partition;
-  return solve_impl(c, g, method, o, du, dv, nd, iu, ni, q, ranks, st, true);
+  return solve_impl(c, g, method, o, du, dv, nd, iu, ni, q, ranks, st, true, true);
 }
 
 int prh_solve_device(prh_context *c, const prh_graph *g, prh_method method, int partition, const prh_options *o,
                      const uint32_t *du, const uint32_t *dv, size_t nd, const uint32_t *iu, size_t ni,
                      const double *q, double *ranks, prh_stats *st) {
   (void)partition;
-  return solve_impl(c, g, method, o, du, dv, nd, iu, ni, q, ranks, st, false);
+  return solve_impl(c, g, method, o, du, dv, nd, iu, ni, q, ranks, st, false, true);
 }
 
 int prh_partition(prh_context *c, uint32_t N, const uint32_t *off, uint32_t d, uint32_t *order, uint32_t *np) {
@@ -315,7 +317,7 @@ int prh_mark_affected(prh_context *c, const prh_graph *g, prh_method method, con
   if (!c) return PRH_ERROR_INVALID_ARGUMENT;
   if (!g || !affected || method < PRH_DYNAMIC_TRAVERSAL || method > PRH_PRUNE_DYNAMIC_FRONTIER)
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null argument or non-dynamic method");
-  if (!g->out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "marking needs the graph's out-CSR");
+  if (!g->l_out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "marking needs the graph's out-CSR");
   if (g->N == 0) return PRH_SUCCESS;
   PRH_HIP(c, hipSetDevice(c->device));
   PRH_CHECK(ensure_workspace(c, g->N, true, 1));

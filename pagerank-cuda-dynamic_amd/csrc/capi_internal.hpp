@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <string>
 #include <utility>
@@ -23,6 +24,30 @@
 namespace prhc {
 
 constexpr size_t kPinnedBytes = 4096;
+
+// One-time preparation of a graph (its layout, a launch plan, the extension schedule): the wall time of its last
+// build, and whether a solve has reported it yet. The first solve that uses it adds it to its initialization time,
+// as the reference times its per-solve partition there (pagerankCuda.hxx:811-814).
+struct Prep {
+  float ms = 0.0f;
+  bool reported = true;
+  float take() {
+    if (reported) return 0.0f;
+    reported = true;
+    return ms;
+  }
+};
+
+// Sets p to the wall time from construction to destruction (a builder's scope), not yet reported.
+struct PrepTimer {
+  Prep &p;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit PrepTimer(Prep &q) : p(q) {}
+  ~PrepTimer() {
+    p.ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    p.reported = false;
+  }
+};
 
 template <class T>
 struct DevBuf {
@@ -70,7 +95,16 @@ struct prh_context {
 struct BinnedCache {
   int B = 0;
   bool valid = false;
-  bool slot = false;  // the plan keeps the X-entry -> CSR-slot map (weighted solves)
+  bool slot = false;  // the plan keeps the slot -> in-CSR-slot map (weighted solves, in-place updates)
+  prhc::Prep prep;
+  // In-place updates (prh_graph_update): the in-CSR the plan was built from (kept from the first update on), the
+  // inverse of its slot map, and the overlay of inserted edges that are not slots of the plan (capacity ov_cap).
+  uint32_t *base_off = nullptr, *base_src = nullptr, *slot_of = nullptr;
+  uint64_t base_M = 0;
+  unsigned long long *ov_key = nullptr;
+  uint32_t *ov_off = nullptr;
+  double *ovX = nullptr;
+  size_t ov_cap = 0;
   prh::BinnedPlan plan{};
   std::vector<std::pair<void *, size_t>> bufs;  // its device allocations
 };
@@ -88,7 +122,12 @@ struct prh_graph {
   uint32_t *perm = nullptr, *inv = nullptr;
   uint32_t *l_in_off = nullptr, *l_in_src = nullptr, *l_outdeg = nullptr, *l_out_off = nullptr, *l_out_dst = nullptr;
   std::vector<uint32_t> sorted_deg;  // host: in-degree of layout row i
-  float prep_ms = 0;
+  uint32_t dead_ends = 0;  // vertices with out-degree 0
+  // In-place updates (prh_graph_update): once one has run, the caller-order arrays above are freed (the batched and
+  // weighted solvers then refuse the graph); the layout's out-rows are kept ascending for the batch tidy.
+  bool updated = false, out_sorted = false;
+  prhc::Prep update_prep;  // update time not yet reported by a solve
+  prhc::Prep layout_prep, plan_prep, sliced_prep, out_prep;
   // Rank-kernel plan for the current tuning: section table and the chunk tables of the heaviest rows.
   prh_tuning plan_tuning{};
   bool plan_valid = false;
@@ -193,10 +232,15 @@ int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uin
 void free_sliced(prh_context *c, prh_graph *g);
 int build_sliced_plan(prh_context *c, prh_graph *g);
 void free_binned(prh_context *c, BinnedCache &bc);
+void free_overlay(prh_context *c, BinnedCache &bc);
 int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot = false);
 int build_binned_plan_t(prh_context *c, prh_graph *g);
 int build_rank_plan(prh_context *c, prh_graph *g);
 int out_schedule(prh_context *c, prh_graph *g, prh::ExtendArgs &E);
+
+// apply.hip
+int csr_to_caller(prh_context *c, const uint32_t *perm_d, const uint32_t *inv_d, uint32_t N, const uint32_t *loff_d,
+                  const uint32_t *lidx_d, uint64_t M, uint32_t *off, uint32_t *idx);
 
 // solve.hip
 int stage_batch(prh_context *c, const prh_graph *g, const uint32_t *du, const uint32_t *dv, size_t nd,
@@ -205,6 +249,6 @@ int mark_affected(prh_context *c, const prh_graph *g, int method, const DevBatch
 int ensure_workspace(prh_context *c, uint32_t N, bool dynamic, int L);
 int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options *o, const uint32_t *du,
                const uint32_t *dv, size_t nd, const uint32_t *iu, size_t ni, const double *q, double *ranks,
-               prh_stats *st, bool host);
+               prh_stats *st, bool host, bool account_prep);
 
 }  // namespace prhc

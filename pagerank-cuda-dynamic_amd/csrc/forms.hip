@@ -25,6 +25,8 @@ int ppr_impl(prh_context *c, const prh_graph *g, int k, const uint32_t *so, cons
   if (!g || !o || !so || !ranks || !iterations || k < 1 || k > prh::kPprCols)
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null argument or k outside 1..64");
   if (g->ctx != c) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "graph belongs to another context");
+  if (g->updated)
+    return fail(c, PRH_ERROR_UNSUPPORTED, "the batched solver needs a graph as uploaded (this one was updated in place)");
   if (o->max_iterations < 0 || !(o->tolerance >= 0) || !(o->damping > 0 && o->damping < 1))
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad options");
   const uint32_t N = g->N;
@@ -219,7 +221,12 @@ int weighted_common(prh_context *c, const prh_graph *g, const float *w, const pr
                     const float **w_out) {
   if (!g || !o || !w) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null argument");
   if (g->ctx != c) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "graph belongs to another context");
+  if (g->updated)
+    return fail(c, PRH_ERROR_UNSUPPORTED, "weighted solves need a graph as uploaded (this one was updated in place)");
   if (!g->out_off || !g->l_out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "weighted solves need the graph's out-CSR");
+  if (g->dead_ends)
+    return fail(c, PRH_ERROR_INVALID_GRAPH, "weighted solves need every out-weight sum positive: %u dead ends",
+                g->dead_ends);
   if (o->max_iterations < 0 || !(o->tolerance >= 0) || !(o->damping > 0 && o->damping < 1))
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad options");
   PRH_HIP(c, hipSetDevice(c->device));
@@ -378,6 +385,7 @@ int weighted_forward_impl(prh_context *c, const prh_graph *g, const float *w, co
   if (c->tuning.blocking > 0) {  // through the propagation-blocked plan of the layout in-CSR
     prh_graph *gm = const_cast<prh_graph *>(g);
     PRH_CHECK(build_binned_plan(c, gm, true));
+    const float prep = gm->layout_prep.take() + gm->bin.prep.take();  // one-time preparation, as solve_impl reports it
     PRH_HIP(c, prh::launch_binned_weights(w_out, g->l_in_eid, g->bin.plan.slot, M, c->wt_wl.p, hs));
     prh::BinnedPlan plan = g->bin.plan;
     plan.ws = c->wt_wl.p;
@@ -399,9 +407,10 @@ int weighted_forward_impl(prh_context *c, const prh_graph *g, const float *w, co
     float t01 = 0, t12 = 0;
     PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
     PRH_HIP(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
-    if (st) *st = prh_stats{n, t01 + t12, t01, 0.0f, t12};
+    if (st) *st = prh_stats{n, t01 + t12 + prep, t01 + prep, 0.0f, t12, prep};
     return PRH_SUCCESS;
   }
+  const float prep = const_cast<prh_graph *>(g)->layout_prep.take();
   PRH_HIP(c, prh::launch_permute_weights(w_out, g->l_in_eid, M, c->wt_wl.p, hs));
   prh::WeightedArgs A0{};
   PRH_CHECK(weighted_schedule(c, g->l_in_off, N, c->sched_in, A0));
@@ -430,7 +439,7 @@ int weighted_forward_impl(prh_context *c, const prh_graph *g, const float *w, co
   float t01 = 0, t12 = 0;
   PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
   PRH_HIP(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
-  if (st) *st = prh_stats{n, t01 + t12, t01, 0.0f, t12};
+  if (st) *st = prh_stats{n, t01 + t12 + prep, t01 + prep, 0.0f, t12, prep};
   return PRH_SUCCESS;
 }
 
@@ -452,9 +461,11 @@ int weighted_adjoint_impl(prh_context *c, const prh_graph *g, const float *w, co
   const bool blocked = c->tuning.blocking > 0;  // through the propagation-blocked plan of the layout out-CSR
   prh::BinnedPlan plan{};
   prh::WeightedArgs A0{};
+  float prep = const_cast<prh_graph *>(g)->layout_prep.take();
   if (blocked) {
     prh_graph *gm = const_cast<prh_graph *>(g);
     PRH_CHECK(build_binned_plan_t(c, gm));
+    prep += gm->bin_t.prep.take();
     PRH_HIP(c, prh::launch_binned_weights(w_out, g->l_out_eid, g->bin_t.plan.slot, M, c->wt_wl.p, hs));
     plan = g->bin_t.plan;
     plan.ws = c->wt_wl.p;
@@ -503,7 +514,7 @@ int weighted_adjoint_impl(prh_context *c, const prh_graph *g, const float *w, co
   float t01 = 0, t12 = 0;
   PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
   PRH_HIP(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
-  if (st) *st = prh_stats{n, t01 + t12, t01, 0.0f, t12};
+  if (st) *st = prh_stats{n, t01 + t12 + prep, t01 + prep, 0.0f, t12, prep};
   return PRH_SUCCESS;
 }
 

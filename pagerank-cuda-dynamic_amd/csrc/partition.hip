@@ -15,6 +15,7 @@ namespace {
 
 constexpr int kPartIpt = 8;                      // vertices per thread per tile
 constexpr uint32_t kTile = kBlock * kPartIpt;    // 2048 vertices per tile
+static_assert(kTile == kPartitionTile, "tile size is part of the scratch contract");
 constexpr int kScanThreads = 1024;
 
 __device__ __forceinline__ int degree_class(const uint32_t *off, uint32_t v, uint32_t t1, uint32_t t2) {

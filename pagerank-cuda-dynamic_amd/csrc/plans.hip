@@ -36,6 +36,7 @@ int build_sliced_plan(prh_context *c, prh_graph *g) {
   if (g->sliced_valid && g->sliced_slices == S && g->sliced_huge == t.huge_degree &&
       g->sliced_block_edges == t.block_edges)
     return PRH_SUCCESS;
+  PrepTimer timer(g->sliced_prep);
   hipStream_t s = c->stream;
   PRH_HIP(c, hipStreamSynchronize(s));
   free_sliced(c, g);
@@ -220,6 +221,28 @@ void free_binned(prh_context *c, BinnedCache &bc) {
   for (auto &b : bc.bufs) dev_free(c, b.first, b.second);
   bc.bufs.clear();
   bc.valid = false;
+  const uint32_t N = bc.plan.N;
+  dev_free(c, bc.base_off, bc.base_off ? (size_t(N) + 1) * 4 : 0);
+  dev_free(c, bc.base_src, bc.base_src ? std::max<size_t>(bc.base_M, 4) * 4 : 0);
+  dev_free(c, bc.slot_of, bc.slot_of ? std::max<size_t>(bc.base_M, 4) * 4 : 0);
+  free_overlay(c, bc);
+  bc.base_off = bc.base_src = bc.slot_of = nullptr;
+  bc.base_M = 0;
+}
+
+void free_overlay(prh_context *c, BinnedCache &bc) {
+  const size_t nb = (size_t(bc.plan.N) + std::max(bc.B, 1) - 1) / std::max(bc.B, 1);
+  dev_free(c, bc.ov_key, bc.ov_key ? bc.ov_cap * 8 : 0);
+  dev_free(c, bc.ovX, bc.ovX ? bc.ov_cap * 8 : 0);
+  dev_free(c, bc.ov_off, bc.ov_off ? (nb + 1) * 4 : 0);
+  bc.ov_key = nullptr;
+  bc.ovX = nullptr;
+  bc.ov_off = nullptr;
+  bc.ov_cap = 0;
+  bc.plan.ov_n = 0;
+  bc.plan.ov_key = nullptr;
+  bc.plan.ov_off = nullptr;
+  bc.plan.ovX = nullptr;
 }
 
 // Propagation-blocked plan (prh::BinnedPlan) for blocks and bins of B = tuning.blocking layout rows:
@@ -238,6 +261,7 @@ static int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *
                  const std::vector<uint32_t> &rowdeg, bool keep_slot, BinnedCache &bc) {
   const int B = c->tuning.blocking;
   if (bc.valid && bc.B == B && (bc.slot || !keep_slot)) return PRH_SUCCESS;
+  PrepTimer timer(bc.prep);
   hipStream_t s = c->stream;
   PRH_HIP(c, hipStreamSynchronize(s));
   free_binned(c, bc);
@@ -418,7 +442,9 @@ static int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *
 
 // The plan of the layout in-CSR (rows in in-degree order).
 int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot) {
-  return build_binned(c, g->N, g->M, g->l_in_off, g->l_in_src, g->sorted_deg, keep_slot, g->bin);
+  // A graph with its out-CSR can be updated in place, which patches the plan through its slot map: keep it.
+  return build_binned(c, g->N, g->M, g->l_in_off, g->l_in_src, g->sorted_deg, keep_slot || g->l_out_off != nullptr,
+                      g->bin);
 }
 
 // The plan of the layout out-CSR (the transpose: rows are sources), for the weighted adjoint.
@@ -437,6 +463,7 @@ int build_binned_plan_t(prh_context *c, prh_graph *g) {
 int build_rank_plan(prh_context *c, prh_graph *g) {
   const prh_tuning &t = c->tuning;
   if (g->plan_valid && std::memcmp(&g->plan_tuning, &t, sizeof(t)) == 0) return PRH_SUCCESS;
+  PrepTimer timer(g->plan_prep);
   const uint32_t N = g->N;
   const std::vector<uint32_t> &deg = g->sorted_deg;
   const uint64_t kEdgesPerBlock = t.block_edges;
@@ -534,6 +561,7 @@ int build_rank_plan(prh_context *c, prh_graph *g) {
 int out_schedule(prh_context *c, prh_graph *g, prh::ExtendArgs &E) {
   const uint32_t N = g->N;
   if (!g->out_valid) {
+    PrepTimer timer(g->out_prep);
     hipStream_t s = c->stream;
     if (!g->out_order) PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&g->out_order), std::max<size_t>(N, 1) * 4));
     PRH_CHECK(ensure(c, c->tiles, size_t(3) * prh::partition_tiles(N)));
@@ -593,7 +621,7 @@ int out_schedule(prh_context *c, prh_graph *g, prh::ExtendArgs &E) {
 constexpr uint32_t kDegreeBuckets = 1u << 20;
 
 int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uint32_t *outdeg, bool with_out) {
-  const auto t0 = std::chrono::steady_clock::now();
+  PrepTimer timer(g->layout_prep);
   const uint32_t N = g->N;
   const uint64_t M = g->M;
   hipStream_t s = c->stream;
@@ -674,7 +702,6 @@ int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uin
   dev_free(c, sort_tmp, sort_bytes);
   dev_free(c, tmp, (M ? M : 4) * 4);
   PRH_CHECK(st);
-  g->prep_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return PRH_SUCCESS;
 }
 

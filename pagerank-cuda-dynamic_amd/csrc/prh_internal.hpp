@@ -172,13 +172,22 @@ struct BinnedPlan {
   double *partial;         // [partial rows][B]
   uint32_t n_ex, n_ac, n_multi;
   uint64_t Q;              // pairs
-  const uint32_t *slot;    // [M] CSR slot of slot j (kept for the weighted solves), else null
+  const uint32_t *slot;    // [M] CSR slot of slot j (the in-CSR slot at build time), else null
   const float *ws;         // [M] weighted solves: the edge weight of slot j, applied in accumulate; else null
+  // Overlay of an in-place updated graph (prh_graph_update): edges inserted since the build that are not slots of
+  // the plan, keyed (destination << 32 | source) in layout ids, sorted; the entries of bin b are
+  // [ov_off[b], ov_off[b + 1]); ovX[i] holds entry i's value for the current iteration. Deleted edges that are
+  // slots of the plan carry the dead flag in dst16 instead (blocks of at most 8192 vertices).
+  uint32_t ov_n;
+  const unsigned long long *ov_key;
+  const uint32_t *ov_off;  // [bins + 1]
+  double *ovX;
 };
 
 // ---- partition.hip ----
 // Stable 3-way partition of [0,N) by degree d = off[v+1]-off[v]: d <= t1, t1 < d <= t2, d > t2.
 // tiles: scratch of 3 * partition_tiles(N) uint32; counts_dev: 3 uint32 outputs (device).
+constexpr uint32_t kPartitionTile = 2048;  // vertices per tile: partition_tiles(N) = ceil(N / kPartitionTile)
 uint32_t partition_tiles(uint32_t N);
 hipError_t launch_partition(const uint32_t *off, uint32_t N, uint32_t t1, uint32_t t2, uint32_t *order,
                             uint32_t *tiles, uint32_t *counts_dev, hipStream_t s);
@@ -220,6 +229,17 @@ hipError_t launch_gather_ranges(const uint32_t *off, const uint32_t *ids, size_t
 // recorded after the last kernel.
 hipError_t launch_binned_update(int variant, const RankArgs &A, const BinnedPlan &P, unsigned long long *err,
                                 int *done, int l, double tol, bool extend, hipStream_t s, hipEvent_t mark = nullptr);
+// In-place update of a plan (see binned.hip): dead flags and overlay candidates from a batch's tidied in-CSR keys,
+// the next overlay's candidates (live count added to *kept), and the overlay's per-bin offsets.
+hipError_t launch_patch_slots(const unsigned long long *keys, const uint32_t *count, uint32_t cap, bool dead,
+                              const uint32_t *base_off, const uint32_t *base_src, const uint32_t *slot_of,
+                              uint16_t *dst16, const unsigned long long *ov_key, uint32_t ov_n, uint8_t *ov_gone,
+                              unsigned long long *fresh, hipStream_t s);
+hipError_t launch_overlay_candidates(const unsigned long long *ov_key, uint32_t ov_n, const uint8_t *ov_gone,
+                                     const unsigned long long *fresh, uint32_t n_fresh, unsigned long long *cand,
+                                     uint32_t *kept, hipStream_t s);
+hipError_t launch_overlay_bounds(const unsigned long long *keys, uint32_t n, uint32_t B, uint32_t nb, uint32_t *off,
+                                 hipStream_t s);
 // ws[j] = w[eid[slot[j]]]: edge weights (raw out-CSR order) carried to slot order through the layout edge map.
 hipError_t launch_binned_weights(const float *w, const uint32_t *eid, const uint32_t *slot, size_t M, float *ws,
                                  hipStream_t s);

@@ -191,6 +191,7 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
     return fail(c, PRH_ERROR_UNSUPPORTED, "the traversal closure needs host-side level checks; not streamed");
   if (!valid_method(method)) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad method");
   if (!g->l_out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "the graph was uploaded without its out-CSR");
+  if (g->dead_ends) return fail(c, PRH_ERROR_INVALID_GRAPH, "the streamed graph has %u dead ends", g->dead_ends);
   if (g->N == 0 || max_batch == 0) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "empty graph or zero batch capacity");
   const prh_options &o = *options;
   if (o.max_iterations < 1 || !(o.tolerance >= 0) || !(o.damping > 0 && o.damping < 1))
@@ -326,7 +327,7 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
           st = fail(c, PRH_ERROR_DEVICE, "initial ranks upload");
       } else {
         prh_stats stt{};
-        st = solve_impl(c, g, PRH_STATIC, &o, nullptr, nullptr, 0, nullptr, 0, nullptr, tmp, &stt, false);
+        st = solve_impl(c, g, PRH_STATIC, &o, nullptr, nullptr, 0, nullptr, 0, nullptr, tmp, &stt, false, false);
       }
       if (st == PRH_SUCCESS) {
         hipError_t e = prh::launch_gather_f64(tmp, g->perm, N, S->rank[0], s);
@@ -429,6 +430,18 @@ int prh_stream_ranks(prh_stream *S, double *ranks) {
 }
 
 uint64_t prh_stream_size(const prh_stream *S) { return S ? S->M : 0; }
+
+int prh_stream_csr(prh_stream *S, int which, uint32_t *offsets, uint32_t *ids) {
+  if (!S) return PRH_ERROR_INVALID_ARGUMENT;
+  prh_context *c = S->ctx;
+  if (!offsets || (S->M && !ids) || (which != 0 && which != 1))
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null array or bad CSR selector");
+  PRH_HIP(c, hipSetDevice(c->device));
+  PRH_HIP(c, hipStreamSynchronize(c->stream));
+  const int p = S->cur;
+  return csr_to_caller(c, S->g->perm, S->g->inv, S->N, which ? S->out_off[p] : S->in_off[p],
+                       which ? S->out_dst[p] : S->in_src[p], S->M, offsets, ids);
+}
 
 void prh_stream_destroy(prh_stream *S) {
   if (!S) return;

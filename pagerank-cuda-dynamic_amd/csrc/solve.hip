@@ -122,13 +122,16 @@ static int load_initial_ranks(prh_context *c, const prh_graph *g, const double *
 
 int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options *o, const uint32_t *du,
                const uint32_t *dv, size_t nd, const uint32_t *iu, size_t ni, const double *q, double *ranks,
-               prh_stats *st, bool host) {
+               prh_stats *st, bool host, bool account_prep) {
   if (!c) return PRH_ERROR_INVALID_ARGUMENT;
   if (!g || !o || !valid_method(method)) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null graph/options or bad method");
   if (g->ctx != c) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "graph belongs to another context");
   if (o->repeat < 1 || o->max_iterations < 0 || !(o->tolerance >= 0) || !(o->damping > 0 && o->damping < 1))
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad options (repeat >= 1, max_iterations >= 0, tolerance >= 0, 0 < damping < 1)");
   if (method != PRH_STATIC && !q) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "initial ranks required for this method");
+  if (method == PRH_PRUNE_DYNAMIC_FRONTIER && g->dead_ends)
+    return fail(c, PRH_ERROR_INVALID_GRAPH,
+                "DF-P's closed-form update divides by the out-degree: %u vertices have none (dead ends)", g->dead_ends);
   const uint32_t N = g->N;
   prh_stats stats{};
   if (N == 0) {  // the reference returns an empty result for an empty graph (pagerankCuda.hxx:869)
@@ -138,7 +141,7 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   if (!ranks) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null ranks");
   const bool dynamic = method >= PRH_DYNAMIC_TRAVERSAL;
   const bool frontier = method >= PRH_DYNAMIC_FRONTIER;
-  if (dynamic && !g->out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "dynamic method needs the graph's out-CSR");
+  if (dynamic && !g->l_out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "dynamic method needs the graph's out-CSR");
   const int L = o->max_iterations;
   const int variant = method == PRH_PRUNE_DYNAMIC_FRONTIER ? prh::kPrune
                       : method == PRH_DYNAMIC_FRONTIER    ? prh::kFrontier
@@ -153,6 +156,12 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   if (binned) PRH_CHECK(build_binned_plan(c, gm));
   else if (sliced) PRH_CHECK(build_sliced_plan(c, gm));
   else PRH_CHECK(build_rank_plan(c, gm));
+  // One-time preparation of this graph not yet reported by a solve: its layout (built at upload) and the plan this
+  // solve runs (built on first use), plus the extension schedule below.
+  float prep = 0.0f;
+  if (account_prep)
+    prep = gm->layout_prep.take() + gm->update_prep.take() +
+           (binned ? gm->bin.prep.take() : sliced ? gm->sliced_prep.take() : gm->plan_prep.take());
   DevBatch b;
   if (dynamic) PRH_CHECK(stage_batch(c, g, du, dv, nd, iu, ni, host, b));
   if (q && !host) {  // device initial ranks: reject NaN/Inf (the reference would propagate them)
@@ -195,7 +204,10 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
     // The out-degree schedule serves the traversal levels and the row plan's frontier extension; the initial
     // extension runs from the batch list and the blocked plan extends inside its update.
     prh::ExtendArgs xo{};
-    if (dynamic && (method == PRH_DYNAMIC_TRAVERSAL || (frontier && !binned))) PRH_CHECK(out_schedule(c, gm, xo));
+    if (dynamic && (method == PRH_DYNAMIC_TRAVERSAL || (frontier && !binned))) {
+      PRH_CHECK(out_schedule(c, gm, xo));
+      if (account_prep) prep += gm->out_prep.take();
+    }
     PRH_HIP(c, prh::launch_init_ranks(c->rank[0].p, c->rank[1].p, c->contrib[0].p, c->contrib[1].p, g->l_outdeg, N,
                                       q != nullptr, s));
     PRH_HIP(c, hipEventRecord(c->ev[1], s));
@@ -275,10 +287,11 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   }
   PRH_HIP(c, hipStreamSynchronize(s));
   stats.iterations = iterations;
-  stats.initialization_time = float(ti / o->repeat);
+  stats.initialization_time = float(ti / o->repeat) + prep;
   stats.marking_time = float(tm / o->repeat);
   stats.computation_time = float(tc / o->repeat);
-  stats.time = float((ti + tm + tc) / o->repeat);
+  stats.time = float((ti + tm + tc) / o->repeat) + prep;
+  stats.preparation_time = prep;
   if (st) *st = stats;
   return PRH_SUCCESS;
 }

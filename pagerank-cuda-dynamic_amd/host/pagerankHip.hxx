@@ -34,21 +34,58 @@
 namespace prh_host {
 
 // One engine context per host thread (device 0, or PRH_DEVICE); it keeps its device workspace between calls.
+//
+// Launch plan. Graphs of at least kAutotuneEdges edges run the plan the engine's autotuner picks for them
+// (prh_autotune: on RMAT-20 and larger graphs, the propagation-blocked update), chosen once per graph size: a later
+// graph with the same vertex count and an edge count within 1% of the tuned one (an updated graph of a dynamic
+// series) keeps it. PRH_TUNING_CACHE names a JSON file that keeps the choices across processes. Smaller graphs, and
+// every graph when PRH_DETERMINISTIC=1, run the row plan: its sums are in a fixed order, so repeated solves are bit
+// for bit identical, where the blocked plan's LDS additions repeat to rounding only.
 class ContextHandle {
  public:
+  static constexpr uint64_t kAutotuneEdges = uint64_t(1) << 20;
+
   ContextHandle() {
     int dev = 0;
     if (const char* e = std::getenv("PRH_DEVICE")) dev = std::atoi(e);
     if (prh_context_create(dev, &ctx_) != PRH_SUCCESS) throw std::runtime_error("prh_context_create failed");
+    const char* d = std::getenv("PRH_DETERMINISTIC");
+    deterministic_ = d && std::atoi(d) != 0;
+    if (const char* c = std::getenv("PRH_TUNING_CACHE")) cache_ = c;
+    prh_default_tuning(&row_plan_);
   }
   ~ContextHandle() { prh_context_destroy(ctx_); }
   prh_context* get() const { return ctx_; }
   void check(int status) const {
     if (status != PRH_SUCCESS) throw std::runtime_error(std::string("pagerank_hip: ") + prh_context_last_error(ctx_));
   }
+  // Sets the context's launch plan for graph g (n vertices, m edges) as described above.
+  void choosePlan(prh_graph* g, uint32_t n, uint64_t m) {
+    if (deterministic_ || m < kAutotuneEdges) {
+      check(prh_context_set_tuning(ctx_, &row_plan_));
+      return;
+    }
+    if (tuned_ && n == tuned_n_ && (m > tuned_m_ ? m - tuned_m_ : tuned_m_ - m) * 100 <= tuned_m_) {
+      check(prh_context_set_tuning(ctx_, &tuned_plan_));
+      return;
+    }
+    double ms = 0;
+    int hit = 0;
+    check(prh_context_set_tuning(ctx_, &row_plan_));
+    check(prh_autotune(ctx_, g, cache_.empty() ? nullptr : cache_.c_str(), 10, &tuned_plan_, &ms, &hit));
+    tuned_ = true;
+    tuned_n_ = n;
+    tuned_m_ = m;
+  }
 
  private:
   prh_context* ctx_ = nullptr;
+  bool deterministic_ = false;
+  std::string cache_;
+  prh_tuning row_plan_{}, tuned_plan_{};
+  bool tuned_ = false;
+  uint32_t tuned_n_ = 0;
+  uint64_t tuned_m_ = 0;
 };
 
 inline ContextHandle& context() {
@@ -140,6 +177,7 @@ PagerankResult<V> invoke(prh_method method, bool partition, const G& y, const H&
     for (const auto& [u, v] : insertions) insu.push_back(index.id(u));
   }
   GraphHandle g(c.get(), in_off, in_src, outdeg, dynamic ? &out_off : nullptr, dynamic ? &out_dst : nullptr);
+  c.choosePlan(g.get(), uint32_t(N), in_src.size());
   prh_options opt{o.repeat, o.tolerance, o.frontierTolerance, o.pruneTolerance, o.damping, o.maxIterations};
   prh_stats st{};
   std::vector<double> rc(N);

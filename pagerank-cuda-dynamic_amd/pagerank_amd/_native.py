@@ -39,7 +39,7 @@ class prh_options(C.Structure):
 
 class prh_stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("time", C.c_float), ("initialization_time", C.c_float),
-                ("marking_time", C.c_float), ("computation_time", C.c_float)]
+                ("marking_time", C.c_float), ("computation_time", C.c_float), ("preparation_time", C.c_float)]
 
 
 class prh_tuning(C.Structure):
@@ -64,6 +64,8 @@ ENGINE_SIGNATURES = {
     "prh_graph_destroy": (None, [vp]),
     "prh_graph_order": (C.c_uint32, [vp]),
     "prh_graph_size": (C.c_uint64, [vp]),
+    "prh_graph_update": (C.c_int, [vp, vp, u32p, u32p, C.c_size_t, u32p, u32p, C.c_size_t, C.POINTER(C.c_float)]),
+    "prh_graph_csr": (C.c_int, [vp, vp, C.c_int, u32p, u32p]),
     "prh_solve": (C.c_int, [vp, vp, C.c_int, C.c_int, C.POINTER(prh_options), u32p, u32p, C.c_size_t, u32p,
                             C.c_size_t, f64p, f64p, C.POINTER(prh_stats)]),
     "prh_solve_device": (C.c_int, [vp, vp, C.c_int, C.c_int, C.POINTER(prh_options), vp, vp, C.c_size_t, vp,
@@ -82,6 +84,7 @@ ENGINE_SIGNATURES = {
     "prh_stream_update": (C.c_int, [vp, u32p, u32p, C.c_size_t, u32p, u32p, C.c_size_t, f64p, C.POINTER(prh_stats)]),
     "prh_stream_ranks": (C.c_int, [vp, f64p]),
     "prh_stream_size": (C.c_uint64, [vp]),
+    "prh_stream_csr": (C.c_int, [vp, C.c_int, u32p, u32p]),
     "prh_stream_destroy": (None, [vp]),
     "prh_kernel_timing_enable": (C.c_int, [vp, C.c_int]),
     "prh_kernel_timing_read": (C.c_int, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),

@@ -51,13 +51,16 @@ class PagerankOptions:
 
 @dataclass
 class PagerankResult:
-    """PagerankResult<double> (pagerank.hxx:64-113); times in milliseconds averaged over repeat."""
+    """PagerankResult<double> (pagerank.hxx:64-113); times in milliseconds averaged over repeat. preparationTime is
+    the one-time build of the graph's layout and launch plan that the first solve using them reports (included in
+    initializationTime and time)."""
     ranks: np.ndarray = field(default_factory=lambda: np.zeros(0))
     iterations: int = 0
     time: float = 0.0
     initializationTime: float = 0.0
     markingTime: float = 0.0
     computationTime: float = 0.0
+    preparationTime: float = 0.0
 
 
 @dataclass
@@ -196,6 +199,24 @@ class DeviceGraph:
     def handle(self):
         return self._h
 
+    def update(self, batch: Batch) -> float:
+        """Apply the batch to this graph on the device (prh_graph_update): tidied as tidyBatchUpdateU tidies it, the
+        propagation-blocked plan patched in place. Returns the update's milliseconds."""
+        ms = C.c_float()
+        self.ctx.check(self.ctx._lib.prh_graph_update(self.ctx.handle, self._h, _p(batch.del_u), _p(batch.del_v),
+                                                      batch.deletions, _p(batch.ins_u), _p(batch.ins_v),
+                                                      batch.insertions, C.byref(ms)))
+        self.M = int(self.ctx._lib.prh_graph_size(self._h))
+        return float(ms.value)
+
+    def csr(self, which: str = "out") -> tuple[np.ndarray, np.ndarray]:
+        """The graph's current CSR in vertex ids (prh_graph_csr): "in" (in-neighbours) or "out" (targets)."""
+        off = np.zeros(self.N + 1, dtype=np.uint32)
+        idx = np.zeros(max(self.M, 1), dtype=np.uint32)
+        self.ctx.check(self.ctx._lib.prh_graph_csr(self.ctx.handle, self._h, 1 if which == "out" else 0,
+                                                   off.ctypes.data_as(u32p), idx.ctypes.data_as(u32p)))
+        return off, idx[:self.M]
+
     def close(self):
         if getattr(self, "_h", None) and self.ctx._h:
             self.ctx._lib.prh_graph_destroy(self._h)
@@ -223,7 +244,7 @@ def solve(yg: DeviceGraph, method: int, o: PagerankOptions | None = None, batch:
                                  qq.ctypes.data_as(f64p) if qq is not None else None,
                                  ranks.ctypes.data_as(f64p), C.byref(st)))
     return PagerankResult(ranks, st.iterations, st.time, st.initialization_time, st.marking_time,
-                          st.computation_time)
+                          st.computation_time, st.preparation_time)
 
 
 def solve_device(yg: DeviceGraph, method: int, o: PagerankOptions, q_ptr: int | None, ranks_ptr: int,
@@ -284,7 +305,8 @@ def weighted_pagerank(yg: DeviceGraph, weights: np.ndarray, teleport: np.ndarray
     st = prh_stats()
     ctx.check(ctx._lib.prh_weighted_pagerank(ctx.handle, yg.handle, _ptr(w), _ptr(s), C.byref(o.native()), _ptr(r),
                                              C.byref(st), 0))
-    return PagerankResult(r, st.iterations, st.time, st.initialization_time, st.marking_time, st.computation_time)
+    return PagerankResult(r, st.iterations, st.time, st.initialization_time, st.marking_time, st.computation_time,
+                          st.preparation_time)
 
 
 def weighted_adjoint(yg: DeviceGraph, weights: np.ndarray, ranks: np.ndarray, grad_ranks: np.ndarray,
@@ -343,6 +365,15 @@ class Stream:
     @property
     def size(self) -> int:
         return int(self.ctx._lib.prh_stream_size(self._h))
+
+    def csr(self, which: str = "out") -> tuple[np.ndarray, np.ndarray]:
+        """The session's current CSR in vertex ids (prh_stream_csr): "in" or "out"."""
+        m = self.size
+        off = np.zeros(self.N + 1, dtype=np.uint32)
+        idx = np.zeros(max(m, 1), dtype=np.uint32)
+        self.ctx.check(self.ctx._lib.prh_stream_csr(self._h, 1 if which == "out" else 0, off.ctypes.data_as(u32p),
+                                                    idx.ctypes.data_as(u32p)))
+        return off, idx[:m]
 
     def close(self):
         if getattr(self, "_h", None):

@@ -10,11 +10,15 @@
 
 dynamic: as runExperiment (main.cu:47-148). A stream of update batches is applied to the graph one after another
 (80% insertions of uniform random vertex pairs, 20% deletions of uniform random existing non-self-loop edges,
-tidied as tidyBatchUpdateU does). After each batch every GPU method runs on the updated graph: Static, and ND /
-DT / DF / DF-P, each warm-started from its own result of the previous batch (main.cu:102-143). Each result is
-compared with a static solve of the same graph at tolerance 1e-100, which runs the full 500 iterations
-(main.cu:88). The reference computes that solve on the CPU; here it runs on the GPU (same Jacobi iteration, FP64).
-One line per method is printed in glog's format (main.cu:56-60). A summary of the speedups over Static follows.
+tidied as tidyBatchUpdateU does). Each batch is applied to the device graph in place (prh_graph_update: both CSRs
+rewritten, the propagation-blocked plan patched), or with --upload the host-built updated graph is uploaded (a new
+layout and plan). After each batch every GPU method runs on the updated graph: Static, and ND / DT / DF / DF-P, each
+warm-started from its own result of the previous batch (main.cu:102-143). Every method's time includes the batch's
+preparation (the update, or the upload's layout and plan build), which each of them needs. Each result is compared
+with a static solve of the same graph at tolerance 1e-100, which runs the full 500 iterations (main.cu:88). The
+reference computes that solve on the CPU; here it runs on the GPU (same Jacobi iteration, FP64). One line per method
+is printed in glog's format (main.cu:56-60), with the batch's deletion and insertion fractions of |E|. A summary of
+the speedups over Static follows.
 
 temporal: the same loop on the reference's input (main.cu:157-174): the first 90% of a SNAP temporal file's lines
 make the base graph (self-loop on every vertex), and each batch is the next fraction*size lines as insertions,
@@ -106,16 +110,29 @@ def autotune(ctx: Context, g: HostGraph, args) -> None:
         f"{' (cache)' if hit else ''}")
 
 
+def _with_prep(a, prep: float):
+    """A copy of result a that also carries the batch's preparation (the update of the graph, or the layout and plan
+    of a fresh upload) in its initialization time and time."""
+    return engine.PagerankResult(a.ranks, a.iterations, a.time + prep, a.initializationTime + prep, a.markingTime,
+                                 a.computationTime, prep)
+
+
 def run_batches(ctx: Context, x: HostGraph, name: str, frac: float, next_batch, args, out: CsvLog,
                 summary: list) -> None:
     """runExperiment's loop (main.cu:79-146) on one batch source: next_batch(x, bi) gives the tidied batch bi
     against the current graph x (None ends the stream early). Every method runs on each updated graph and is
-    compared with a static solve of it at tolerance 1e-100."""
+    compared with a static solve of it at tolerance 1e-100 (on the GPU; the reference runs that solve on the CPU).
+
+    The updated graph is made on the device by prh_graph_update (the blocked plan patched in place), or, with
+    --upload, built on the host and uploaded (a new layout and plan). Either way that preparation is paid once per
+    batch, by whichever solve runs first on the new graph; every method's reported time includes it (each needs the
+    updated graph), so the speedups over Static count it on both sides."""
     o = PagerankOptions(repeat=args.repeat)
     exact = PagerankOptions(repeat=1, tolerance=1e-100)
-    xg = ctx.upload(x, with_out=False)
+    xg = ctx.upload(x, with_out=not args.upload)
     r0 = solve(xg, engine.STATIC, exact).ranks
-    xg.close()
+    if args.upload:
+        xg.close()
     prev = {m: r0 for m, _ in METHODS}
     for bi in range(args.batches):
         t0 = time.time()
@@ -124,33 +141,45 @@ def run_batches(ctx: Context, x: HostGraph, name: str, frac: float, next_batch, 
             log(f"input ended before batch {bi}")
             break
         y = apply_batch(x, b)
-        yg = ctx.upload(y)
+        if args.upload:
+            yg = ctx.upload(y)
+        else:
+            yg = xg
+            yg.update(b)
+        s0 = solve(yg, engine.STATIC, exact)
+        prep = s0.preparationTime
         log(f"batch {bi} of {frac:g}|E|: -{b.deletions}/+{b.insertions} edges, y has {y.M} edges "
-            f"({time.time() - t0:.1f} s to build and upload)")
-        s0 = solve(yg, engine.STATIC, exact).ranks
-        res = {engine.STATIC: solve(yg, engine.STATIC, o)}
+            f"({time.time() - t0:.1f} s on the host to make the batch and y); "
+            f"{'upload + layout + plan' if args.upload else 'device update'}: {prep:.3f} ms, added to every method")
+        res = {engine.STATIC: _with_prep(solve(yg, engine.STATIC, o), prep)}
         for m, _ in METHODS:
-            res[m] = solve(yg, m, o, batch=b, q=prev[m])
+            res[m] = _with_prep(solve(yg, m, o, batch=b, q=prev[m]), prep)
         for m, tech in [(engine.STATIC, "pagerankStaticHip")] + METHODS:
             a = res[m]
-            print(glog(a, s0, tech, 1, 0.0, frac, bi), flush=True)
+            print(glog(a, s0.ranks, tech, 1, b.deletions / x.M, b.insertions / x.M, bi), flush=True)
             out.row(mode=args.mode, graph=name, vertices=y.N, edges=y.M, batch_fraction=frac, batch_index=bi,
                     technique=tech, time_ms=a.time, init_ms=a.initializationTime, mark_ms=a.markingTime,
-                    comp_ms=a.computationTime, iterations=a.iterations, l1_error=l1_error(a.ranks, s0),
-                    extra=f"deletions={b.deletions};insertions={b.insertions}")
+                    comp_ms=a.computationTime, iterations=a.iterations, l1_error=l1_error(a.ranks, s0.ranks),
+                    extra=f"deletions={b.deletions};insertions={b.insertions};preparation_ms={prep:.3f};"
+                          f"{'upload' if args.upload else 'update'}")
         for m, _ in METHODS:
             prev[m] = res[m].ranks
-        summary.append((frac, bi, {m: res[m].time for m in res}))
-        yg.close()
+        summary.append((frac, bi, {m: res[m].time for m in res}, prep))
+        if args.upload:
+            yg.close()
         x = y
+    if not args.upload:
+        xg.close()
 
 
 def print_summary(summary: list) -> None:
-    log("speedup over GPU Static (time of Static / time of the method), mean over batches:")
+    log("speedup over GPU Static (time of Static / time of the method, each with the batch's preparation), mean over "
+        "batches:")
     for frac in sorted({s[0] for s in summary}):
         rows = [s[2] for s in summary if s[0] == frac]
+        prep = np.mean([s[3] for s in summary if s[0] == frac])
         st = np.mean([r[engine.STATIC] for r in rows])
-        parts = [f"Static {st:.3f} ms"]
+        parts = [f"preparation {prep:.3f} ms", f"Static {st:.3f} ms"]
         for m, tech in METHODS:
             t = np.mean([r[m] for r in rows])
             parts.append(f"{tech.replace('pagerank', '').replace('Hip', '')} {t:.3f} ms ({st / t:.2f}x)")
@@ -362,6 +391,9 @@ def main(argv=None) -> int:
     ap.add_argument("--fractions", default="1e-5,1e-4,1e-3")
     ap.add_argument("--batches", type=int, default=3)
     ap.add_argument("--repeat", type=int, default=5)
+    ap.add_argument("--upload", action="store_true",
+                    help="dynamic, temporal: build each updated graph on the host and upload it (a new layout and plan "
+                         "per batch) instead of updating the device graph in place")
     ap.add_argument("--columns", type=int, default=64)
     ap.add_argument("--csv", default=None)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch plan (dynamic, temporal, ppr, adjoint)")

@@ -23,6 +23,22 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP engine library")
 
 
+def emit(config, msg: str) -> None:
+    """A progress line that reaches the terminal while a long test or fixture runs (output capture is suspended for
+    the line), so that a run of several minutes shows it is alive."""
+    capman = config.pluginmanager.getplugin("capturemanager")
+    if capman is None:
+        print(msg, flush=True)
+        return
+    with capman.global_and_fixture_disabled():
+        print(f"\n[progress] {msg}", flush=True)
+
+
+@pytest.fixture
+def say(request):
+    return lambda msg: emit(request.config, f"{request.node.name}: {msg}")
+
+
 @pytest.fixture(scope="session")
 def oracle_mod():
     import oracle

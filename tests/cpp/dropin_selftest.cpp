@@ -4,8 +4,13 @@
 // usage: dropin_selftest <x_off.bin> <x_edg.bin> <y_off.bin> <y_edg.bin> <batch.bin> <q.bin> <out_prefix>
 //   *.bin are raw little-endian arrays (uint32 for graphs and the batch [nd, ni, du..., dv..., iu..., iv...],
 //   float64 for q); writes <out_prefix>.<method>.bin (float64 ranks) and prints one line per method.
+// usage: dropin_selftest --static <off.bin> <edg.bin> <repeats> <out_prefix>
+//   static solves of one graph, `repeats` times: one line each with the times and the launch plan the drop-in's
+//   context ran (its blocking width; 0 = the row plan), and the ranks of each solve in <out_prefix>.<i>.bin.
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
+#include <string>
 #include <fstream>
 #include <stdexcept>
 #include <tuple>
@@ -31,7 +36,32 @@ static void save(const std::string &path, const std::vector<double> &a) {
   f.write(reinterpret_cast<const char *>(a.data()), a.size() * sizeof(double));
 }
 
+static int static_only(int argc, char **argv) {
+  if (argc != 6) {
+    std::fprintf(stderr, "usage: %s --static off edg repeats out_prefix\n", argv[0]);
+    return 2;
+  }
+  using K = uint32_t;
+  using V = double;
+  CsrDiGraph<K> y(load<uint32_t>(argv[2]), load<K>(argv[3]));
+  auto yt = transposeWithDegree(y);
+  const int repeats = std::atoi(argv[4]);
+  const std::string out = argv[5];
+  PagerankOptions<V> o;
+  for (int i = 0; i < repeats; ++i) {
+    auto r = pagerankStaticCuda(y, yt, o);
+    prh_tuning t{};
+    prh_context_get_tuning(prh_host::context().get(), &t);
+    std::printf("static[%d] iterations=%d time=%.3f init=%.3f comp=%.3f ms_per_iteration=%.4f blocking=%d\n", i,
+                r.iterations, r.time, r.initializationTime, r.computationTime,
+                r.computationTime / (r.iterations > 0 ? r.iterations : 1), t.blocking);
+    save(out + "." + std::to_string(i) + ".bin", r.ranks);
+  }
+  return 0;
+}
+
 int main(int argc, char **argv) {
+  if (argc > 1 && std::string(argv[1]) == "--static") return static_only(argc, argv);
   if (argc != 8) {
     std::fprintf(stderr, "usage: %s x_off x_edg y_off y_edg batch q out_prefix\n", argv[0]);
     return 2;

@@ -82,3 +82,51 @@ def test_reference_caller_compiled_against_header(tmp_path, oracle_mod):
         # L1 error against the reference's converged CPU ranks (main.cu:55): tolerance-level for the exact methods
         assert err < (1e-4 if name == "prune" else 1e-6), line
     assert seen == set(methods)
+
+
+def _run_static(tmp_path, y, repeats: int, deterministic: bool):
+    import os
+    y.out_off.tofile(tmp_path / "y_off.bin")
+    y.out_edg.tofile(tmp_path / "y_edg.bin")
+    env = {k: v for k, v in os.environ.items() if k not in ("PRH_DETERMINISTIC", "PRH_TUNING_CACHE")}
+    if deterministic:
+        env["PRH_DETERMINISTIC"] = "1"
+    p = subprocess.run([str(SELFTEST), "--static", str(tmp_path / "y_off.bin"), str(tmp_path / "y_edg.bin"),
+                        str(repeats), str(tmp_path / "s")], capture_output=True, text=True, timeout=300, env=env)
+    print(p.stdout, p.stderr)
+    assert p.returncode == 0
+    rows = []
+    for line in p.stdout.splitlines():
+        f = dict(kv.split("=") for kv in line.split()[1:])
+        rows.append({k: float(v) for k, v in f.items()})
+    return rows, [np.fromfile(tmp_path / f"s.{i}.bin") for i in range(repeats)]
+
+
+def test_dropin_large_graph_takes_autotuned_plan(tmp_path, oracle_mod):
+    """A graph of 2^20 edges or more: the drop-in runs the autotuner's plan (the propagation-blocked update on
+    RMAT-20), reports its one-time build in the first solve's initialization time, and matches the oracle."""
+    if not SELFTEST.exists():
+        pytest.fail(f"{SELFTEST} not built (make -C pagerank-cuda-dynamic_amd)")
+    from pagerank_amd import HostGraph
+    y = HostGraph.rmat(20, 16, seed=5)
+    rows, ranks = _run_static(tmp_path, y, 2, deterministic=False)
+    assert all(r["blocking"] > 0 for r in rows), rows
+    ref, it, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, y)
+    for r, got in zip(rows, ranks):
+        assert abs(int(r["iterations"]) - it) <= 1
+        assert np.abs(got - ref).max() <= 1e-9
+        assert r["init"] >= r["time"] - r["comp"] - 1e-3  # time = init + comp (no marking for Static)
+
+
+def test_dropin_deterministic_keeps_row_plan(tmp_path, oracle_mod):
+    """PRH_DETERMINISTIC=1: the row plan, whose fixed summation order repeats bit for bit."""
+    if not SELFTEST.exists():
+        pytest.fail(f"{SELFTEST} not built (make -C pagerank-cuda-dynamic_amd)")
+    from pagerank_amd import HostGraph
+    y = HostGraph.rmat(20, 16, seed=5)
+    rows, ranks = _run_static(tmp_path, y, 2, deterministic=True)
+    assert all(r["blocking"] == 0 for r in rows), rows
+    assert np.array_equal(ranks[0], ranks[1])
+    ref, it, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, y)
+    assert abs(int(rows[0]["iterations"]) - it) <= 1
+    assert np.abs(ranks[0] - ref).max() <= 1e-9

@@ -16,12 +16,36 @@ def test_empty_graph_returns_empty_result(ctx):
     assert res.iterations == 0 and len(res.ranks) == 0
 
 
-def test_dead_end_rejected(ctx):
-    from pagerank_amd import HostGraph, PagerankError
-    y = HostGraph.from_edges(4, [0, 1, 2], [1, 2, 3], self_loops=False)  # vertex 3 has out-degree 0
+def _dead_end_graph():
+    """RMAT-11 edges without self-loops, where a quarter of the vertices keep no out-edge (dead ends)."""
+    from pagerank_amd import HostGraph
+    x = HostGraph.rmat(11, 8, seed=31, self_loops=False)
+    src = np.repeat(np.arange(x.N, dtype=np.uint32), np.diff(x.out_off.astype(np.int64)))
+    keep = (src % 4) != 3
+    return HostGraph.from_edges(x.N, src[keep], x.out_edg[keep], self_loops=False)
+
+
+def test_dead_ends_accepted_except_dfp(ctx, oracle_mod):
+    """Dead ends (out-degree 0) are accepted as the reference accepts them: a dead end is never an in-neighbour, so
+    nothing divides by its degree. Static, ND, DT and DF match the oracle; DF-P, whose closed form divides by the
+    out-degree, is rejected with a status."""
+    from pagerank_amd import (DYNAMIC_FRONTIER, DYNAMIC_TRAVERSAL, NAIVE_DYNAMIC, PRUNE_DYNAMIC_FRONTIER, STATIC,
+                              Batch, PagerankError, apply_batch, solve)
+    x = _dead_end_graph()
+    assert int((x.outdeg == 0).sum()) > 100
+    b = Batch.random(x, 40, 0.8, seed=5)
+    y = apply_batch(x, b)
+    yg = ctx.upload(y)
+    q, _, _ = oracle_mod.pagerank_cuda(STATIC, x)
+    for m in (STATIC, NAIVE_DYNAMIC, DYNAMIC_TRAVERSAL, DYNAMIC_FRONTIER):
+        res = solve(yg, m, batch=b, q=None if m == STATIC else q)
+        ref, it, _ = oracle_mod.pagerank_cuda(m, y, b, None if m == STATIC else q)
+        assert abs(res.iterations - it) <= 1, m
+        assert np.abs(res.ranks - ref).max() <= 1e-9, m
+        assert res.ranks.sum() < 1.0  # rank mass leaks through the dead ends
     with pytest.raises(PagerankError) as e:
-        ctx.upload(y)
-    assert e.value.status == 2 and "dead-end" in str(e.value)
+        solve(yg, PRUNE_DYNAMIC_FRONTIER, batch=b, q=q)
+    assert e.value.status == 2 and "dead ends" in str(e.value)
 
 
 def test_out_of_range_id_rejected(ctx):

@@ -51,15 +51,21 @@ def test_temporal_mode(tmp_path, capsys):
     assert "pagerankPruneDynamicFrontierHip" in capsys.readouterr().out
 
 
-def test_dynamic_mode(tmp_path):
+@pytest.mark.parametrize("path", ["update", "upload"])
+def test_dynamic_mode(tmp_path, path):
+    """Both ways of making each updated graph: the device update (default) and the host build + upload. Every
+    method's time includes the batch's preparation."""
     from pagerank_amd import experiment
     out = tmp_path / "r.csv"
+    extra = ["--upload"] if path == "upload" else []
     assert experiment.main(["dynamic", "--scale", "12", "--fractions", "1e-4,1e-3", "--batches", "2",
-                            "--repeat", "1", "--no-autotune", "--csv", str(out)]) == 0
+                            "--repeat", "1", "--csv", str(out)] + extra) == 0
     got = _rows(out)
     assert len(got) == 2 * 2 * 5
     for r in got:
         assert float(r["l1_error"]) < _bound(r), r
+        prep = float(r["extra"].split("preparation_ms=")[1].split(";")[0])
+        assert prep > 0 and float(r["init_ms"]) >= prep and r["extra"].endswith(path)
 
 
 def test_adjoint_mode(tmp_path):

@@ -164,6 +164,29 @@ def test_dense_solve(oracle_mod):
     assert np.abs(r - exact).max() <= 1e-14
 
 
+def test_dead_ends(oracle_mod):
+    """A graph with dead ends (out-degree 0, no self-loops): the restatement solves (I - a A) r = (1-a)/N 1 with the
+    dead ends' columns of A zero (numpy.linalg.solve), and, where oracle/_ref is built, the OpenMP port equals the
+    compiled pagerankStaticOmp bit for bit."""
+    from pagerank_amd import HostGraph
+    x = HostGraph.rmat(9, 4, seed=21, self_loops=False)
+    src = np.repeat(np.arange(x.N, dtype=np.uint32), np.diff(x.out_off.astype(np.int64)))
+    keep = (src % 3) != 0
+    y = HostGraph.from_edges(x.N, src[keep], x.out_edg[keep], self_loops=False)
+    assert int((y.outdeg == 0).sum()) >= x.N // 3
+    A = np.zeros((y.N, y.N))
+    for u in range(y.N):
+        for v in y.out_edg[y.out_off[u]:y.out_off[u + 1]]:
+            A[v, u] += 1.0 / y.outdeg[u]
+    exact = np.linalg.solve(np.eye(y.N) - 0.85 * A, np.full(y.N, 0.15 / y.N))
+    r, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, y, tolerance=1e-15)
+    assert np.abs(r - exact).max() <= 1e-14 and exact.sum() < 1.0
+    if oracle_mod.ref_available():
+        rr, itr, _ = oracle_mod.ref_pagerank(0, y)
+        ro, ito, _ = oracle_mod.static_omp(y)
+        assert itr == ito and np.array_equal(rr, ro)
+
+
 @pytest.mark.parametrize("n", [1, 5, 64, 3000])
 def test_partition_brute_force(oracle_mod, n):
     rng = np.random.default_rng(n)
