@@ -75,26 +75,84 @@ def make_graph(args):
     return g, desc
 
 
-def cpu_baseline(g, budget_s: float):
-    """Static solves (tolerance 1e-10) of the same graph by the OpenMP port, bounded to about budget_s seconds:
-    iterations of one solve are cut at the budget, and a solve that converges early is repeated."""
+def cpu_baseline(g, budget_s: float, kind: str = "port"):
+    """Static solves (tolerance 1e-10) of the same graph on this host's cores, bounded to about budget_s seconds:
+    iterations of one solve are cut at the budget, and a solve that converges early is repeated. kind "port": the
+    oracle's OpenMP restatement of pagerankStaticOmp (bit-identical to the compiled reference on the test graphs);
+    "reference": the reference's own pagerankStaticOmp compiled from its headers (oracle/_ref), its time being
+    PagerankResult.time (graph loading into its DiGraph excluded), where that build exists."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    _, it, ms = oracle.static_omp(g, max_iterations=2, threads=threads)
+    if kind == "reference" and not oracle.ref_available():
+        log("cpu baseline: oracle/_ref is not built here; timing the port instead")
+        kind = "port"
+
+    def run(k):
+        if kind == "reference":
+            _, it, ms = oracle.ref_pagerank(0, g, threads=threads, max_iterations=k)
+        else:
+            _, it, ms = oracle.static_omp(g, max_iterations=k, threads=threads)
+        return it, ms
+    it, ms = run(2)
     per_iter = ms / max(it, 1) / 1e3
     k = int(max(2, min(500, budget_s / max(per_iter, 1e-6))))
     iters, total_ms, solves = 0, 0.0, 0
     while total_ms / 1e3 < 0.8 * budget_s or solves == 0:
-        _, it, ms = oracle.static_omp(g, max_iterations=k, threads=threads)
+        it, ms = run(k)
         iters += it
         total_ms += ms
         solves += 1
         if it < 2:
             break
     value = g.M * iters / (total_ms / 1e3) / 1e9
-    return {"value": round(value, 4), "unit": "GEdges/s/iter", "cores": threads, "kind": "port",
-            "sample": f"{solves} static solve(s) of the OpenMP port on the same graph, {iters} synchronous iterations "
-                      f"in {total_ms / 1e3:.1f} s"}
+    what = ("the reference's pagerankStaticOmp (compiled from its headers)" if kind == "reference"
+            else "the OpenMP port of pagerankStaticOmp")
+    return {"value": round(value, 4), "unit": "GEdges/s/iter", "cores": threads, "kind": kind,
+            "sample": f"{solves} static solve(s) of {what} on the same graph, {iters} synchronous iterations in "
+                      f"{total_ms / 1e3:.1f} s"}
+
+
+def bench_c1(args) -> int:
+    """The CPU configuration: static PageRank on RMAT scale 16 (edge factor 16, seed 42, dedup, self-loops; fixed) by the
+    reference's own pagerankStaticOmp (compiled from its headers; PagerankResult.time averaged over 5 repeats) and by
+    the OpenMP port, on this host's cores, beside the GPU static solve of the same graph."""
+    import oracle
+    from pagerank_amd import STATIC, Context, HostGraph, PagerankOptions, solve
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    scale, ef = 16, 16.0
+    g = HostGraph.rmat(scale, ef, 0.57, 0.19, 0.19, seed=42)
+    rows = {}
+    if oracle.ref_available():
+        r_ref, it, ms = oracle.ref_pagerank(0, g, threads=threads, repeat=5)
+        rows["reference"] = (it, ms)
+    r_port, it, ms = oracle.static_omp(g, threads=threads)
+    rows["port"] = (it, ms)
+    with Context(0) as ctx:
+        yg = ctx.upload(g, with_out=False)
+        solve(yg, STATIC, PagerankOptions())  # the first solve builds the layout's plan
+        res = solve(yg, STATIC, PagerankOptions(repeat=5))
+        yg.close()
+    rows["gpu"] = (res.iterations, res.time)
+    for k, (it, ms) in rows.items():
+        log(f"{k}: {it} iterations, {ms:.3f} ms, {g.M * it / (ms / 1e3) / 1e9:.3f} GEdges/s/iter")
+    kind = "reference" if "reference" in rows else "port"
+    it, ms = rows[kind]
+    out = {
+        "metric": "static PageRank on the CPU path (GEdges/s/iter), with the GPU solve of the same graph",
+        "value": round(g.M * it / (ms / 1e3) / 1e9, 4),
+        "unit": "GEdges/s/iter",
+        "n_gpus": 1, "steps": 5, "warmup": 0, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"static PageRank, directed RMAT scale {scale} (edge factor {ef:g}, "
+                               f"a/b/c/d=.57/.19/.19/.05, seed 42, dedup, self-loops), on {threads} host threads",
+                   "vertices": g.N, "edges": g.M, "kind": kind,
+                   "iterations": {k: v[0] for k, v in rows.items()},
+                   "ms": {k: round(v[1], 3) for k, v in rows.items()},
+                   "gpu_speedup_over_cpu": round(rows[kind][1] / rows["gpu"][1], 2)},
+        "roofline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def bench_dfp(args, ctx, g, desc) -> int:
@@ -213,13 +271,15 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["rmat", "grid", "dfp"], default="rmat")
+    ap.add_argument("--workload", choices=["rmat", "grid", "dfp", "c1"], default="rmat")
     ap.add_argument("--batch-fraction", type=float, default=1e-5, help="dfp workload: batch size as a fraction of |E|")
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--edge-factor", type=float, default=30.0, help="rmat workload: edge samples per vertex")
     ap.add_argument("--rows", type=int, default=6000)
     ap.add_argument("--cols", type=int, default=4000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-kind", choices=["port", "reference"], default="port",
+                    help="cpu baseline: the oracle's OpenMP port, or the reference's compiled pagerankStaticOmp")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch plan instead of timing the autotuner's candidates first")
@@ -238,6 +298,9 @@ def main() -> int:
         return copy_calibration(args.copy_calibration)
 
     import torch  # device buffer for the ranks; imported before the engine library (shared HIP runtime)
+
+    if args.workload == "c1":
+        return bench_c1(args)
 
     from pagerank_amd import STATIC, Context, PagerankOptions, pagerankStaticHip, solve_device
 
@@ -324,7 +387,7 @@ def main() -> int:
     }
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds, args.cpu_kind)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -109,7 +109,9 @@ void prh_default_options(prh_options *options);
 void prh_default_tuning(prh_tuning *tuning);
 
 /* Context: device, stream and reusable device workspace. Replaces the per-call cudaSetDeviceFlags and
- * cudaMalloc/cudaFree of pagerankInvokeCuda (pagerankCuda.hxx:777-796, :829-847). */
+ * cudaMalloc/cudaFree of pagerankInvokeCuda (pagerankCuda.hxx:777-796, :829-847). The environment variable
+ * PRH_GRAPH_CAPTURE=0 at creation makes the context issue the launches it would replay from captured HIP graphs
+ * (batched PPR, the streaming session) one by one. */
 int prh_context_create(int device, prh_context **context);
 void prh_context_destroy(prh_context *context);
 const char *prh_context_last_error(const prh_context *context);
@@ -188,7 +190,9 @@ int prh_mark_affected(prh_context *context, const prh_graph *graph, prh_method m
  * iterates r_j <- (1-damping) s_j + damping * P^T r_j from r_j = s_j and stops at the first iteration whose L-inf
  * change is below options->tolerance, as a single solve would (pagerankLoopCuU's rule, pagerankCuda.hxx:680-690).
  * ranks receives N*k values, row-major by vertex (ranks[v*k + j]); iterations receives k counts. Iterations run as
- * a HIP graph of check_interval iterations replayed until every column has converged. */
+ * a HIP graph of check_interval iterations replayed until every column has converged (with PRH_GRAPH_CAPTURE=0 in the
+ * environment when the context was created: the same launches issued one by one, for tools that trace kernels; the
+ * streaming session below honours it too). */
 int prh_ppr_batched(prh_context *context, const prh_graph *graph, int k, const uint32_t *seed_offsets,
                     const uint32_t *seeds, const prh_options *options, double *ranks, int *iterations,
                     prh_stats *stats);
@@ -228,15 +232,19 @@ int prh_autotune(prh_context *context, const prh_graph *graph, const char *cache
  * batches modify on the device, with the ranks kept on the device between batches. Each update tidies the batch as
  * tidyBatchUpdateU does (batch.hxx:201-208: deletions of existing edges only, insertions of absent edges only,
  * duplicates dropped), applies it (applyBatchUpdateU, batch.hxx:219-247), and solves with `method` from the previous
- * ranks: PRH_STATIC (from 1/N), PRH_NAIVE_DYNAMIC, PRH_DYNAMIC_FRONTIER or PRH_PRUNE_DYNAMIC_FRONTIER (the traversal
- * closure's host-checked levels are not streamed: PRH_ERROR_UNSUPPORTED). The batch application and marking run as
- * one captured HIP graph and the iterations as a captured graph of 8 iterations replayed until the device reports
- * convergence. Batch ids are the caller's vertex ids (host arrays); each side holds at most max_batch edges;
- * max_edges bounds the graph's growth (0: |E| + 16 max_batch). initial_ranks (host, caller order) may be NULL: a
- * static solve of `graph` then provides them. A batch that names a vertex outside the graph or leaves a vertex
- * without out-edges fails and leaves the stream unusable. stats->initialization_time covers the batch upload,
- * application and marking; computation_time the iterations. The context's tuning at the first update of each
- * buffer parity is captured with the graphs. */
+ * ranks: PRH_STATIC (from 1/N), PRH_NAIVE_DYNAMIC, PRH_DYNAMIC_TRAVERSAL (the closure's levels run on the device,
+ * each only while the one before it marked a vertex), PRH_DYNAMIC_FRONTIER or PRH_PRUNE_DYNAMIC_FRONTIER. The
+ * batch application, the update plan's patch and the marking run as one captured HIP graph, the traversal closure as
+ * a captured graph of 8 levels and the iterations as a captured graph of 8 iterations, each replayed until the
+ * device reports it done. When the context's tuning asks for propagation blocking with blocks of at most 8192
+ * vertices, the session builds that plan of the base graph and patches it per batch (deleted slots flagged dead,
+ * other inserted edges kept in an overlay), as prh_graph_update does; otherwise it runs the row plan. Batch ids are
+ * the caller's vertex ids (host arrays); each side holds at most max_batch edges; max_edges bounds the graph's growth
+ * (0: |E| + 16 max_batch). initial_ranks (host, caller order) may be NULL: a static solve of `graph` then provides
+ * them. A batch that names a vertex outside the graph or leaves a vertex without out-edges fails and leaves the
+ * stream unusable. stats->initialization_time covers the batch upload, application, patch and initial marking,
+ * marking_time the traversal closure, computation_time the iterations. The context's tuning at the first update of
+ * each buffer parity is captured with the graphs. */
 typedef struct prh_stream prh_stream;
 int prh_stream_create(prh_context *context, const prh_graph *graph, prh_method method, const prh_options *options,
                       uint32_t max_batch, uint64_t max_edges, const double *initial_ranks, prh_stream **stream);

@@ -85,29 +85,30 @@ int patch_plan(prh_context *c, prh_graph *g, const unsigned long long *dkeys, co
   Temps tmp(c);
   uint8_t *gone = nullptr;
   unsigned long long *fresh = nullptr, *cand = nullptr, *cand_sorted = nullptr;
-  uint32_t *kept = nullptr;
-  const uint32_t ov_n = bc.plan.ov_n;
-  PRH_CHECK(tmp.get(&gone, std::max<uint32_t>(ov_n, 1)));
+  uint32_t *words = nullptr;  // [0] the current overlay's size when it has none yet, [1] the next one's
+  const uint32_t ov_cap = uint32_t(bc.ov_cap);
+  PRH_CHECK(tmp.get(&gone, std::max<uint32_t>(ov_cap, 1)));
   PRH_CHECK(tmp.get(&fresh, std::max<uint32_t>(ni, 1)));
-  PRH_CHECK(tmp.get(&kept, 1));
-  PRH_HIP(c, prh::launch_fill_u8(gone, 0, std::max<uint32_t>(ov_n, 1), s));
-  PRH_HIP(c, prh::launch_fill_u32(kept, 0, 1, s));
+  PRH_CHECK(tmp.get(&words, 2));
+  PRH_HIP(c, prh::launch_fill_u8(gone, 0, std::max<uint32_t>(ov_cap, 1), s));
+  PRH_HIP(c, prh::launch_fill_u32(words, 0, 2, s));
+  const uint32_t *ov_count = bc.ov_off ? bc.ov_off + nb : words;
   uint16_t *dst16 = const_cast<uint16_t *>(bc.plan.dst16);
   PRH_HIP(c, prh::launch_patch_slots(dkeys, dcount, nd, true, bc.base_off, bc.base_src, bc.slot_of, dst16,
-                                     bc.plan.ov_key, ov_n, gone, nullptr, s));
+                                     bc.ov_key, ov_count, gone, nullptr, s));
   PRH_HIP(c, prh::launch_patch_slots(ikeys, icount, ni, false, bc.base_off, bc.base_src, bc.slot_of, dst16,
-                                     bc.plan.ov_key, ov_n, gone, fresh, s));
-  const uint32_t total = ov_n + ni;
+                                     bc.ov_key, ov_count, gone, fresh, s));
+  const uint32_t total = ov_cap + ni;
   if (total == 0) return PRH_SUCCESS;
   PRH_CHECK(tmp.get(&cand, total));
   PRH_CHECK(tmp.get(&cand_sorted, total));
-  PRH_HIP(c, prh::launch_overlay_candidates(bc.plan.ov_key, ov_n, gone, fresh, ni, cand, kept, s));
+  PRH_HIP(c, prh::launch_overlay_candidates(bc.ov_key, ov_count, ov_cap, gone, fresh, ni, cand, words + 1, s));
   size_t sort_bytes = 0;
   PRH_HIP(c, prh::sort_keys_u64(nullptr, sort_bytes, cand, cand_sorted, total, s));
   uint8_t *sort_tmp = nullptr;
   PRH_CHECK(tmp.get(&sort_tmp, sort_bytes));
   PRH_HIP(c, prh::sort_keys_u64(sort_tmp, sort_bytes, cand, cand_sorted, total, s));
-  PRH_HIP(c, hipMemcpyAsync(c->pinned, kept, 4, hipMemcpyDeviceToHost, s));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, words + 1, 4, hipMemcpyDeviceToHost, s));
   PRH_HIP(c, hipStreamSynchronize(s));
   const uint32_t n = *reinterpret_cast<const uint32_t *>(c->pinned);
   if (n > bc.ov_cap || !bc.ov_off) {
@@ -118,10 +119,10 @@ int patch_plan(prh_context *c, prh_graph *g, const unsigned long long *dkeys, co
     PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&bc.ov_off), (size_t(nb) + 1) * 4));
     bc.ov_cap = cap;
   }
-  if (n) PRH_HIP(c, hipMemcpyAsync(bc.ov_key, cand_sorted, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
-  PRH_HIP(c, prh::launch_overlay_bounds(bc.ov_key, n, B, nb, bc.ov_off, s));
+  PRH_HIP(c, prh::launch_copy_u64(bc.ov_key, cand_sorted, n, s));
+  PRH_HIP(c, prh::launch_overlay_bounds(bc.ov_key, words + 1, uint32_t(bc.ov_cap), B, nb, bc.ov_off, nullptr, 0, s));
   PRH_HIP(c, hipStreamSynchronize(s));
-  bc.plan.ov_n = n;
+  bc.plan.ov_n = n ? uint32_t(bc.ov_cap) : 0;
   bc.plan.ov_key = bc.ov_key;
   bc.plan.ov_off = bc.ov_off;
   bc.plan.ovX = bc.ovX;

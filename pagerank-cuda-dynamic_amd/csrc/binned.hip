@@ -117,6 +117,7 @@ template <int B>
 __global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const double *cr, const uint8_t *naff,
                                                            LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;
+  if (iter_index(ctl) == 0) naff = nullptr;  // the first iteration extends nothing (the marking did)
   __shared__ double c[B];
   const uint4 it = P.ex_item[blockIdx.x];  // block, first entry, end entry
   const uint32_t base = it.x * uint32_t(B);
@@ -380,11 +381,13 @@ __global__ __launch_bounds__(kBinBlock) void finish_bins_kernel(RankArgs A, Binn
 
 // The overlay's values for this iteration: entry i takes its source's contribution, negated when the source was in
 // the last frontier (naff, frontier variants after their first iteration), as expand does for X. It runs between
-// expand and accumulate: accumulate overwrites naff with the new frontier.
+// expand and accumulate: accumulate overwrites naff with the new frontier. The entry count is ov_off[bins].
 __global__ __launch_bounds__(kBlock) void overlay_expand_kernel(BinnedPlan P, const double *cr, const uint8_t *naff,
                                                                 LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < P.ov_n; i += gridDim.x * kBlock) {
+  if (iter_index(ctl) == 0) naff = nullptr;
+  const uint32_t n = P.ov_off[(P.N + P.B - 1) / P.B];
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const uint32_t u = uint32_t(P.ov_key[i]);
     const double c = cr[u];
     P.ovX[i] = naff && naff[u] ? -c : c;
@@ -540,9 +543,9 @@ __global__ __launch_bounds__(kBlock) void patch_slots_kernel(const unsigned long
                                                              uint32_t cap, int dead, const uint32_t *base_off,
                                                              const uint32_t *base_src, const uint32_t *slot_of,
                                                              uint16_t *dst16, const unsigned long long *ov_key,
-                                                             uint32_t ov_n, uint8_t *ov_gone,
+                                                             const uint32_t *ov_count, uint8_t *ov_gone,
                                                              unsigned long long *fresh) {
-  const uint32_t n = *count;
+  const uint32_t n = *count, ov_n = *ov_count;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cap; i += gridDim.x * kBlock) {
     unsigned long long f = kNoKey;
     if (i < n) {
@@ -564,27 +567,38 @@ __global__ __launch_bounds__(kBlock) void patch_slots_kernel(const unsigned long
   }
 }
 
-// The next overlay's candidates: the current entries not gone, then the fresh insertions (kNoKey for none); *kept
-// counts the live ones.
-__global__ __launch_bounds__(kBlock) void overlay_candidates_kernel(const unsigned long long *ov_key, uint32_t ov_n,
+// The next overlay's candidates, cand[0 .. ov_cap + fresh_cap): the current entries (ov_count of the ov_cap slots)
+// not gone, then the fresh insertions (kNoKey for none); *kept counts the live ones.
+__global__ __launch_bounds__(kBlock) void overlay_candidates_kernel(const unsigned long long *ov_key,
+                                                                    const uint32_t *ov_count, uint32_t ov_cap,
                                                                     const uint8_t *ov_gone,
-                                                                    const unsigned long long *fresh, uint32_t n_fresh,
+                                                                    const unsigned long long *fresh, uint32_t fresh_cap,
                                                                     unsigned long long *cand, uint32_t *kept) {
+  const uint32_t ov_n = *ov_count;
   uint32_t live = 0;
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < ov_n + n_fresh; i += gridDim.x * kBlock) {
-    const unsigned long long k = i < ov_n ? (ov_gone[i] ? kNoKey : ov_key[i]) : fresh[i - ov_n];
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < ov_cap + fresh_cap; i += gridDim.x * kBlock) {
+    const unsigned long long k = i < ov_cap ? (i < ov_n && !ov_gone[i] ? ov_key[i] : kNoKey) : fresh[i - ov_cap];
     cand[i] = k;
     live += k != kNoKey;
   }
   if (live) atomicAdd(kept, live);
 }
 
-// off[b] = first overlay entry with destination >= b B, for b = 0 .. nb (keys sorted).
-__global__ __launch_bounds__(kBlock) void overlay_bounds_kernel(const unsigned long long *keys, uint32_t n, uint32_t B,
-                                                                uint32_t nb, uint32_t *off) {
+// off[b] = first overlay entry with destination >= b B, for b = 0 .. nb, over the first min(*count, cap) sorted keys
+// (off[nb] is the overlay's size); a count above cap sets bit in *flags.
+__global__ __launch_bounds__(kBlock) void overlay_bounds_kernel(const unsigned long long *keys, const uint32_t *count,
+                                                                uint32_t cap, uint32_t B, uint32_t nb, uint32_t *off,
+                                                                int *flags, int bit) {
   const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
   if (b > nb) return;
+  const uint32_t c = *count, n = c < cap ? c : cap;
+  if (b == 0 && c > cap && flags) atomicOr(flags, bit);
   off[b] = uint32_t(lower_index(keys, 0, n, (unsigned long long)b * B << 32));
+}
+
+__global__ __launch_bounds__(kBlock) void copy_u64_kernel(unsigned long long *dst, const unsigned long long *src,
+                                                          size_t n) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock) dst[i] = src[i];
 }
 
 template <int B>
@@ -618,26 +632,32 @@ hipError_t launch_binned_update(int variant, const RankArgs &A, const BinnedPlan
 
 hipError_t launch_patch_slots(const unsigned long long *keys, const uint32_t *count, uint32_t cap, bool dead,
                               const uint32_t *base_off, const uint32_t *base_src, const uint32_t *slot_of,
-                              uint16_t *dst16, const unsigned long long *ov_key, uint32_t ov_n, uint8_t *ov_gone,
-                              unsigned long long *fresh, hipStream_t s) {
+                              uint16_t *dst16, const unsigned long long *ov_key, const uint32_t *ov_count,
+                              uint8_t *ov_gone, unsigned long long *fresh, hipStream_t s) {
   if (!cap) return hipSuccess;
   patch_slots_kernel<<<grid(cap, kBlock, 4096), kBlock, 0, s>>>(keys, count, cap, dead ? 1 : 0, base_off, base_src,
-                                                                 slot_of, dst16, ov_key, ov_n, ov_gone, fresh);
+                                                                 slot_of, dst16, ov_key, ov_count, ov_gone, fresh);
   return hipGetLastError();
 }
 
-hipError_t launch_overlay_candidates(const unsigned long long *ov_key, uint32_t ov_n, const uint8_t *ov_gone,
-                                     const unsigned long long *fresh, uint32_t n_fresh, unsigned long long *cand,
-                                     uint32_t *kept, hipStream_t s) {
-  if (!(ov_n + n_fresh)) return hipSuccess;
-  overlay_candidates_kernel<<<grid(ov_n + n_fresh, kBlock, 4096), kBlock, 0, s>>>(ov_key, ov_n, ov_gone, fresh,
-                                                                                   n_fresh, cand, kept);
+hipError_t launch_overlay_candidates(const unsigned long long *ov_key, const uint32_t *ov_count, uint32_t ov_cap,
+                                     const uint8_t *ov_gone, const unsigned long long *fresh, uint32_t fresh_cap,
+                                     unsigned long long *cand, uint32_t *kept, hipStream_t s) {
+  if (!(ov_cap + fresh_cap)) return hipSuccess;
+  overlay_candidates_kernel<<<grid(size_t(ov_cap) + fresh_cap, kBlock, 4096), kBlock, 0, s>>>(
+      ov_key, ov_count, ov_cap, ov_gone, fresh, fresh_cap, cand, kept);
   return hipGetLastError();
 }
 
-hipError_t launch_overlay_bounds(const unsigned long long *keys, uint32_t n, uint32_t B, uint32_t nb, uint32_t *off,
-                                 hipStream_t s) {
-  overlay_bounds_kernel<<<(nb + kBlock) / kBlock, kBlock, 0, s>>>(keys, n, B, nb, off);
+hipError_t launch_overlay_bounds(const unsigned long long *keys, const uint32_t *count, uint32_t cap, uint32_t B,
+                                 uint32_t nb, uint32_t *off, int *flags, int bit, hipStream_t s) {
+  overlay_bounds_kernel<<<(nb + kBlock) / kBlock, kBlock, 0, s>>>(keys, count, cap, B, nb, off, flags, bit);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_u64(unsigned long long *dst, const unsigned long long *src, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  copy_u64_kernel<<<grid(n, kBlock, 4096), kBlock, 0, s>>>(dst, src, n);
   return hipGetLastError();
 }
 

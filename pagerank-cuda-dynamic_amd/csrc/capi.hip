@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -46,6 +47,28 @@ void dev_free(prh_context *c, void *p, size_t bytes) {
   if (!p) return;
   (void)hipFree(p);
   c->used -= std::min(c->used, bytes ? bytes : 16);
+}
+
+int copy_to_host(prh_context *c, void *dst, const void *src, size_t n) {
+  hipStream_t s = c->stream;
+  if (n <= kStagingBytes) {
+    if (n) PRH_HIP(c, hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipStreamSynchronize(s));
+    return PRH_SUCCESS;
+  }
+  if (!c->staging && hipHostMalloc(reinterpret_cast<void **>(&c->staging), kStagingBytes, hipHostMallocDefault) !=
+                         hipSuccess) {
+    (void)hipGetLastError();
+    c->staging = nullptr;
+    return fail(c, PRH_ERROR_OUT_OF_MEMORY, "pinned staging area (%zu bytes)", kStagingBytes);
+  }
+  for (size_t o = 0; o < n; o += kStagingBytes) {
+    const size_t m = std::min(kStagingBytes, n - o);
+    PRH_HIP(c, hipMemcpyAsync(c->staging, static_cast<const uint8_t *>(src) + o, m, hipMemcpyDeviceToHost, s));
+    PRH_HIP(c, hipStreamSynchronize(s));
+    std::memcpy(static_cast<uint8_t *>(dst) + o, c->staging, m);
+  }
+  return PRH_SUCCESS;
 }
 
 }  // namespace prhc
@@ -87,6 +110,7 @@ int prh_context_create(int device, prh_context **out) {
   prh_context *c = new prh_context();
   c->device = device;
   prh_default_tuning(&c->tuning);
+  if (const char *e = std::getenv("PRH_GRAPH_CAPTURE")) c->capture = std::strcmp(e, "0") != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void **>(&c->pinned), kPinnedBytes, hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
@@ -152,6 +176,7 @@ void prh_context_destroy(prh_context *c) {
   for (auto &e : c->kev)
     if (e) (void)hipEventDestroy(e);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->staging) (void)hipHostFree(c->staging);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -25,6 +25,8 @@ namespace prhc {
 
 constexpr size_t kPinnedBytes = 4096;
 
+constexpr size_t kStagingBytes = size_t(64) << 20;
+
 // One-time preparation of a graph (its layout, a launch plan, the extension schedule): the wall time of its last
 // build, and whether a solve has reported it yet. The first solve that uses it adds it to its initialization time,
 // as the reference times its per-solve partition there (pagerankCuda.hxx:811-814).
@@ -85,6 +87,11 @@ struct prh_context {
   prhc::DevBuf<uint32_t> wt_sort, wt_sort_tmp;  // degree sort of its low rows
   prhc::DevBuf<double> wt_partial;        // their sums
   uint64_t *pinned = nullptr;       // small host-pinned readback area (kPinnedBytes)
+  // Launch mode of the iteration loops that replay captured HIP graphs (batched PPR, the streaming session): true =
+  // captured graphs; false (PRH_GRAPH_CAPTURE=0 in the environment at context creation) = the same launches issued
+  // one by one, for tools that trace kernels.
+  bool capture = true;
+  uint8_t *staging = nullptr;  // host-pinned staging area of large device-to-host copies (kStagingBytes), on demand
   hipEvent_t ev[4] = {};
   bool ktiming = false;
   std::vector<hipEvent_t> kev;  // start/stop pairs per iteration
@@ -227,10 +234,18 @@ struct DevBatch {
   size_t ND = 0, NI = 0;
 };
 
+// Device-to-host copy of n bytes into caller (pageable) memory on the context's stream, synchronously: in chunks of
+// kStagingBytes through the context's pinned staging area when n exceeds one chunk, else directly (capi.hip).
+int copy_to_host(prh_context *c, void *dst, const void *src, size_t n);
+
 // plans.hip
 int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uint32_t *outdeg, bool with_out);
 void free_sliced(prh_context *c, prh_graph *g);
 int build_sliced_plan(prh_context *c, prh_graph *g);
+// The propagation-blocked plan of a pull CSR (off, idx; N rows of degrees rowdeg, M slots) into bc, for the
+// context's blocking width (kept: the slot -> CSR-slot map).
+int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, const uint32_t *idx,
+                 const std::vector<uint32_t> &rowdeg, bool keep_slot, BinnedCache &bc);
 void free_binned(prh_context *c, BinnedCache &bc);
 void free_overlay(prh_context *c, BinnedCache &bc);
 int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot = false);

@@ -134,14 +134,17 @@ int ppr_impl(prh_context *c, const prh_graph *g, int k, const uint32_t *so, cons
   PRH_HIP(c, prh::launch_ppr_init(A, s));
   PRH_HIP(c, hipEventRecord(c->ev[1], s));
   const int chunk = std::max(1, c->tuning.check_interval);
-  hipGraphExec_t exec = nullptr;
-  int status = L > 0 ? capture_graph(c, [&]() -> int {
+  auto iterations_chunk = [&]() -> int {
     for (int i = 0; i < chunk; ++i) PRH_HIP(c, prh::launch_ppr_iteration(A, s));
     return PRH_SUCCESS;
-  }, &exec) : PRH_SUCCESS;
+  };
+  hipGraphExec_t exec = nullptr;
+  int status = L > 0 && c->capture ? capture_graph(c, iterations_chunk, &exec) : PRH_SUCCESS;
   prh::PprState hs{};
   for (int launched = 0; status == PRH_SUCCESS && launched < L; launched += chunk) {
-    hipError_t e = hipGraphLaunch(exec, s);
+    hipError_t e = hipSuccess;
+    if (exec) e = hipGraphLaunch(exec, s);
+    else if ((status = iterations_chunk()) != PRH_SUCCESS) break;
     if (e == hipSuccess) e = hipMemcpyAsync(c->pinned, c->ppr_state.p, sizeof(prh::PprState), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
@@ -157,9 +160,8 @@ int ppr_impl(prh_context *c, const prh_graph *g, int k, const uint32_t *so, cons
   PRH_CHECK(status);
   PRH_HIP(c, hipEventRecord(c->ev[2], s));
   PRH_HIP(c, prh::launch_ppr_collect(A, c->ppr_out.p, s));
-  PRH_HIP(c, hipMemcpyAsync(ranks, c->ppr_out.p, size_t(N) * k * 8, hipMemcpyDeviceToHost, s));
   PRH_HIP(c, hipMemcpyAsync(c->pinned, c->ppr_state.p, sizeof(prh::PprState), hipMemcpyDeviceToHost, s));
-  PRH_HIP(c, hipStreamSynchronize(s));
+  PRH_CHECK(copy_to_host(c, ranks, c->ppr_out.p, size_t(N) * k * 8));  // synchronises the stream
   std::memcpy(&hs, c->pinned, sizeof(hs));
   int most = 0;
   for (int j = 0; j < k; ++j) {

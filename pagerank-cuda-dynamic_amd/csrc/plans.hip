@@ -257,7 +257,7 @@ void free_overlay(prh_context *c, BinnedCache &bc) {
 //   4. block starts in X order (lower bounds of the sorted key1) and bin starts from the row degrees cut the two
 //      orders into items; split bins get partial rows.
 // Builds the plan of the pull CSR (off, idx) with rows of degree rowdeg (host), N rows, M slots, into bc.
-static int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, const uint32_t *idx,
+int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, const uint32_t *idx,
                  const std::vector<uint32_t> &rowdeg, bool keep_slot, BinnedCache &bc) {
   const int B = c->tuning.blocking;
   if (bc.valid && bc.B == B && (bc.slot || !keep_slot)) return PRH_SUCCESS;

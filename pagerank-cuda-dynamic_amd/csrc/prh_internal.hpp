@@ -174,10 +174,11 @@ struct BinnedPlan {
   uint64_t Q;              // pairs
   const uint32_t *slot;    // [M] CSR slot of slot j (the in-CSR slot at build time), else null
   const float *ws;         // [M] weighted solves: the edge weight of slot j, applied in accumulate; else null
-  // Overlay of an in-place updated graph (prh_graph_update): edges inserted since the build that are not slots of
-  // the plan, keyed (destination << 32 | source) in layout ids, sorted; the entries of bin b are
-  // [ov_off[b], ov_off[b + 1]); ovX[i] holds entry i's value for the current iteration. Deleted edges that are
-  // slots of the plan carry the dead flag in dst16 instead (blocks of at most 8192 vertices).
+  // Overlay of an in-place updated graph (prh_graph_update, the streaming session): edges inserted since the build
+  // that are not slots of the plan, keyed (destination << 32 | source) in layout ids, sorted; the entries of bin b
+  // are [ov_off[b], ov_off[b + 1]), ov_off[bins] of them in all; ovX[i] holds entry i's value for the current
+  // iteration. ov_n: 0 = no overlay, else its capacity (the kernels' grid). Deleted edges that are slots of the plan
+  // carry the dead flag in dst16 instead (blocks of at most 8192 vertices).
   uint32_t ov_n;
   const unsigned long long *ov_key;
   const uint32_t *ov_off;  // [bins + 1]
@@ -229,17 +230,20 @@ hipError_t launch_gather_ranges(const uint32_t *off, const uint32_t *ids, size_t
 // recorded after the last kernel.
 hipError_t launch_binned_update(int variant, const RankArgs &A, const BinnedPlan &P, unsigned long long *err,
                                 int *done, int l, double tol, bool extend, hipStream_t s, hipEvent_t mark = nullptr);
-// In-place update of a plan (see binned.hip): dead flags and overlay candidates from a batch's tidied in-CSR keys,
-// the next overlay's candidates (live count added to *kept), and the overlay's per-bin offsets.
+// In-place update of a plan (see binned.hip): dead flags and fresh overlay entries from a batch's tidied in-CSR keys
+// (the current overlay: *ov_count sorted keys), the next overlay's candidates (ov_cap + fresh_cap of them, the live
+// count added to *kept), and the overlay's per-bin offsets over the first min(*count, cap) sorted keys (a count above
+// cap sets bit in *flags).
 hipError_t launch_patch_slots(const unsigned long long *keys, const uint32_t *count, uint32_t cap, bool dead,
                               const uint32_t *base_off, const uint32_t *base_src, const uint32_t *slot_of,
-                              uint16_t *dst16, const unsigned long long *ov_key, uint32_t ov_n, uint8_t *ov_gone,
-                              unsigned long long *fresh, hipStream_t s);
-hipError_t launch_overlay_candidates(const unsigned long long *ov_key, uint32_t ov_n, const uint8_t *ov_gone,
-                                     const unsigned long long *fresh, uint32_t n_fresh, unsigned long long *cand,
-                                     uint32_t *kept, hipStream_t s);
-hipError_t launch_overlay_bounds(const unsigned long long *keys, uint32_t n, uint32_t B, uint32_t nb, uint32_t *off,
-                                 hipStream_t s);
+                              uint16_t *dst16, const unsigned long long *ov_key, const uint32_t *ov_count,
+                              uint8_t *ov_gone, unsigned long long *fresh, hipStream_t s);
+hipError_t launch_overlay_candidates(const unsigned long long *ov_key, const uint32_t *ov_count, uint32_t ov_cap,
+                                     const uint8_t *ov_gone, const unsigned long long *fresh, uint32_t fresh_cap,
+                                     unsigned long long *cand, uint32_t *kept, hipStream_t s);
+hipError_t launch_overlay_bounds(const unsigned long long *keys, const uint32_t *count, uint32_t cap, uint32_t B,
+                                 uint32_t nb, uint32_t *off, int *flags, int bit, hipStream_t s);
+hipError_t launch_copy_u64(unsigned long long *dst, const unsigned long long *src, size_t n, hipStream_t s);
 // ws[j] = w[eid[slot[j]]]: edge weights (raw out-CSR order) carried to slot order through the layout edge map.
 hipError_t launch_binned_weights(const float *w, const uint32_t *eid, const uint32_t *slot, size_t M, float *ws,
                                  hipStream_t s);
@@ -407,6 +411,12 @@ hipError_t launch_refresh_chunks(const uint32_t *item_row, const uint32_t *row_f
 hipError_t launch_mark_batch(uint8_t *naff, uint8_t *vaff, const unsigned long long *dkeys, const uint32_t *nd,
                              const unsigned long long *ikeys, const uint32_t *ni, uint32_t cap_total, hipStream_t s);
 hipError_t launch_advance(int *l, int *done, const unsigned long long *err, double tol, int max_iter, hipStream_t s);
+// Device-driven traversal closure (lvl: [0] level, [1] stop, [2], [3] changed flags of even and odd levels): the
+// step before a level stops the closure when the previous level marked nothing, else clears this level's flag; the
+// advance after it moves to the next level unless stopped.
+hipError_t launch_closure_step(int *lvl, int parity, hipStream_t s);
+hipError_t launch_closure_advance(int *lvl, hipStream_t s);
+hipError_t launch_copy_u8(uint8_t *dst, const uint8_t *src, size_t n, hipStream_t s);
 
 // ---- frontier.hip ----
 hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *delu, const uint32_t *delv,

@@ -20,11 +20,17 @@ using namespace prhc;
 // A device-resident copy of a graph's degree-ordered layout that batches update in place (stream.hip), with
 // the ranks kept between batches and each batch's work replayed from captured HIP graphs:
 //   apply graph (one per buffer parity): batch -> keys -> sort -> tidy -> new in- and out-CSR (into the other
-//       buffers) -> out-degrees -> chunk tables -> solver initialisation and, for DF / DF-P, the initial marking;
-//   iteration graph (one per parity): kChunk iterations of the rank update (and, for DF / DF-P, the direction
-//       choice and both extension forms) followed each by the device-side convergence step; replayed until the
-//       device reports the loop done.
-// Only the batch upload, one flag read after the apply and one read per iteration chunk touch the host.
+//       buffers) -> out-degrees -> the update plan's patch (blocked: dead slots and the overlay; rows: the chunk
+//       tables) -> solver initialisation and, for DT / DF / DF-P, the initial marking;
+//   closure graph (DT, one per parity): levels of the traversal closure, each run only while the level before it
+//       marked a vertex; replayed until the device reports the closure done;
+//   iteration graph (one per parity): kChunk iterations of the rank update (the blocked plan extends the frontier
+//       inside it; the row plan adds the direction choice and both extension forms) followed each by the
+//       device-side convergence step; replayed until the device reports the loop done.
+// The rank update runs through the session's own propagation-blocked plan when the context's tuning asks for
+// blocks of at most 8192 vertices (the plan of the base graph, patched per batch as prh_graph_update patches a
+// graph's), else through the row plan.
+// Only the batch upload, one flag read after the apply and one read per closure or iteration chunk touch the host.
 
 struct prh_stream {
   prh_context *ctx = nullptr;
@@ -55,14 +61,32 @@ struct prh_stream {
   unsigned long long *err = nullptr;
   prh::RankPlan plan{};
   uint32_t n_items = 0, chunk = 0, stat_slots = 0, pull_wave = 0, pull_group = 0;
-  hipGraphExec_t apply[2] = {}, iter[2] = {};
+  hipGraphExec_t apply[2] = {}, iter[2] = {}, closure[2] = {};
   std::vector<std::pair<void *, size_t>> bufs;
   bool broken = false;
+  // Propagation-blocked update (blocked): the session's plan of the base graph, its own copy of the slot words (the
+  // dead flags), the base in-CSR with the slot map's inverse, and the overlay at a fixed capacity.
+  bool blocked = false;
+  BinnedCache bc;
+  prh::BinnedPlan bplan{};
+  uint32_t *base_off = nullptr, *base_src = nullptr, *slot_of = nullptr;
+  unsigned long long *ov_key = nullptr, *ov_cand = nullptr, *ov_sorted = nullptr, *fresh = nullptr;
+  uint32_t *ov_off = nullptr, *ov_words = nullptr;  // ov_words[0]: the next overlay's size
+  uint8_t *ov_gone = nullptr;
+  double *ovX = nullptr;
+  void *ov_sort_tmp = nullptr;
+  size_t ov_sort_bytes = 0;
+  uint32_t ov_cap = 0, nb = 0;
+  // Traversal closure (DT): two frontier buffers and the level words: [0] level, [1] stop, [2], [3] changed flags of
+  // even and odd levels.
+  uint8_t *front[2] = {};
+  int *lvl = nullptr;
 };
 
 namespace {
 
 constexpr int kStreamChunk = 8;  // iterations per replay of the iteration graph (even: ping-pong buffers realign)
+constexpr int kClosureLevels = 8;  // traversal levels per replay of the closure graph (even: its buffers realign)
 
 int stream_alloc(prh_stream *S, void **p, size_t bytes) {
   PRH_CHECK(dev_alloc(S->ctx, p, bytes));
@@ -82,7 +106,9 @@ void stream_free(prh_stream *S) {
   for (int k = 0; k < 2; ++k) {
     if (S->apply[k]) (void)hipGraphExecDestroy(S->apply[k]);
     if (S->iter[k]) (void)hipGraphExecDestroy(S->iter[k]);
+    if (S->closure[k]) (void)hipGraphExecDestroy(S->closure[k]);
   }
+  free_binned(S->ctx, S->bc);
   for (auto &b : S->bufs) dev_free(S->ctx, b.first, b.second);
   S->bufs.clear();
   if (S->h_batch) (void)hipHostFree(S->h_batch);
@@ -128,15 +154,34 @@ int stream_apply_body(prh_stream *S, int p) {
   PRH_CHECK(stream_apply_csr(S, s, 0, 1, S->in_off[p], S->in_src[p], S->in_off[q], S->in_src[q], 0));
   PRH_CHECK(stream_apply_csr(S, s, 2, 3, S->out_off[p], S->out_dst[p], S->out_off[q], S->out_dst[q], 2));
   PRH_HIP(c, prh::launch_degrees(S->out_off[q], N, S->outdeg, s));
-  PRH_HIP(c, prh::launch_refresh_chunks(S->plan.item_row, S->plan.row_first, S->n_items, S->in_off[q], S->chunk,
-                                        const_cast<uint32_t *>(S->plan.item_beg),
-                                        const_cast<uint32_t *>(S->plan.item_end), s));
+  if (S->blocked) {
+    // The plan's patch (see prh_graph_update): dead flags, then the next overlay from the live entries and the
+    // fresh insertions, sorted at the fixed capacity; an overlay past it sets flag bit 4.
+    const uint32_t nbin = S->nb;
+    uint16_t *dst16 = const_cast<uint16_t *>(S->bplan.dst16);
+    PRH_HIP(c, prh::launch_fill_u8(S->ov_gone, 0, S->ov_cap, s));
+    PRH_HIP(c, prh::launch_fill_u32(S->ov_words, 0, 1, s));
+    PRH_HIP(c, prh::launch_patch_slots(S->compact[0], S->counts + 0, cap, true, S->base_off, S->base_src, S->slot_of,
+                                       dst16, S->ov_key, S->ov_off + nbin, S->ov_gone, nullptr, s));
+    PRH_HIP(c, prh::launch_patch_slots(S->compact[1], S->counts + 1, cap, false, S->base_off, S->base_src,
+                                       S->slot_of, dst16, S->ov_key, S->ov_off + nbin, S->ov_gone, S->fresh, s));
+    PRH_HIP(c, prh::launch_overlay_candidates(S->ov_key, S->ov_off + nbin, S->ov_cap, S->ov_gone, S->fresh, cap,
+                                              S->ov_cand, S->ov_words, s));
+    PRH_HIP(c, prh::sort_keys_u64(S->ov_sort_tmp, S->ov_sort_bytes, S->ov_cand, S->ov_sorted, S->ov_cap + cap, s));
+    PRH_HIP(c, prh::launch_copy_u64(S->ov_key, S->ov_sorted, S->ov_cap, s));
+    PRH_HIP(c, prh::launch_overlay_bounds(S->ov_key, S->ov_words, S->ov_cap, uint32_t(S->bc.B), nbin, S->ov_off,
+                                          S->ctl + 3, 4, s));
+  } else {
+    PRH_HIP(c, prh::launch_refresh_chunks(S->plan.item_row, S->plan.row_first, S->n_items, S->in_off[q], S->chunk,
+                                          const_cast<uint32_t *>(S->plan.item_beg),
+                                          const_cast<uint32_t *>(S->plan.item_end), s));
+  }
   // Solver initialisation (pagerankCuda.hxx:811-821): ranks from the previous batch's result (or 1/N for Static),
   // contributions with the new out-degrees, loop state cleared, and the DF initial marking on the new graph.
   PRH_HIP(c, prh::launch_fill_u64(S->err, 0ull, size_t(S->opt.max_iterations) + 1, s));
   PRH_HIP(c, prh::launch_init_ranks(S->rank[0], S->rank[1], S->contrib[0], S->contrib[1], S->outdeg, N,
                                     S->method != PRH_STATIC, s));
-  if (S->variant >= prh::kFrontier) {
+  if (S->variant >= prh::kTraversal) {
     PRH_HIP(c, prh::launch_fill_u8(S->vaff, 0, N, s));
     PRH_HIP(c, prh::launch_fill_u8(S->naff, 0, N, s));
     PRH_HIP(c, prh::launch_mark_batch(S->naff, S->vaff, S->compact[2], S->counts + 2, S->compact[3], S->counts + 3,
@@ -144,8 +189,34 @@ int stream_apply_body(prh_stream *S, int p) {
     const prh::Schedule os{S->sched_order, S->sched_low, S->sched_high, 0};
     prh::ExtendArgs E{S->vaff, S->naff, S->out_off[q], S->out_dst[q], os, nullptr, nullptr, nullptr};
     PRH_HIP(c, prh::launch_extend(E, nullptr, nullptr, 0, 0.0, s));
-    const int mode = c->tuning.extension_direction;
-    if (mode != 1) PRH_HIP(c, prh::launch_fill_i32(S->ctl + 2, mode == 2 ? 1 : 0, 1, s));
+    if (S->variant == prh::kTraversal) {  // the closure starts from every vertex marked so far
+      PRH_HIP(c, prh::launch_copy_u8(S->front[0], S->vaff, N, s));
+      PRH_HIP(c, prh::launch_fill_i32(S->lvl, 0, 4, s));
+    } else if (!S->blocked) {
+      const int mode = c->tuning.extension_direction;
+      if (mode != 1) PRH_HIP(c, prh::launch_fill_i32(S->ctl + 2, mode == 2 ? 1 : 0, 1, s));
+    }
+  }
+  return PRH_SUCCESS;
+}
+
+// kClosureLevels levels of the traversal closure on buffers q (captured once per parity), after
+// pagerankAffectedTraversalCuW (pagerankCuda.hxx:438-456): level l extends from the vertices level l - 1 marked
+// (front[l & 1]; level 0 from every vertex the initial marking set), marks their out-neighbours affected, and
+// records the newly marked ones in front[(l + 1) & 1]. A level runs only while the one before it marked a vertex;
+// the first level that finds nothing new sets lvl[1] (stop). Levels replay from an even index, so level l's
+// buffers are those of capture slot l mod 2.
+int stream_closure_body(prh_stream *S, int q) {
+  prh_context *c = S->ctx;
+  hipStream_t s = c->stream;
+  const prh::Schedule os{S->sched_order, S->sched_low, S->sched_high, 0};
+  for (int k = 0; k < kClosureLevels; ++k) {
+    PRH_HIP(c, prh::launch_closure_step(S->lvl, k & 1, s));
+    PRH_HIP(c, prh::launch_fill_u8(S->front[(k + 1) & 1], 0, S->N, s));
+    prh::ExtendArgs E{S->vaff, S->front[k & 1], S->out_off[q], S->out_dst[q], os, S->front[(k + 1) & 1],
+                      S->lvl + 2 + (k & 1), S->lvl + 1};
+    PRH_HIP(c, prh::launch_extend(E, nullptr, nullptr, 0, 0.0, s));
+    PRH_HIP(c, prh::launch_closure_advance(S->lvl, s));
   }
   return PRH_SUCCESS;
 }
@@ -165,7 +236,12 @@ int stream_iter_body(prh_stream *S, int q) {
   for (int i = 0; i < kStreamChunk; ++i) {
     prh::RankArgs A{S->rank[(i + 1) & 1], S->rank[i & 1], S->contrib[(i + 1) & 1], S->contrib[i & 1], S->naff,
                     S->vaff, S->in_off[q], S->in_src[q], S->outdeg, C0, P, o.frontier_tolerance, o.prune_tolerance,
-                    frontier && mode == 1 ? S->stats : nullptr};
+                    frontier && mode == 1 && !S->blocked ? S->stats : nullptr};
+    if (S->blocked) {  // the frontier's extension happens inside the update (from the second iteration on)
+      PRH_HIP(c, prh::launch_binned_update(S->variant, A, S->bplan, err, done, -1, o.tolerance, frontier, s));
+      PRH_HIP(c, prh::launch_advance(S->ctl + 1, done, err, o.tolerance, o.max_iterations, s));
+      continue;
+    }
     PRH_HIP(c, prh::launch_rank_update(S->variant, A, S->plan, err, done, -1, o.tolerance, s));
     if (frontier) {
       if (mode == 1) PRH_HIP(c, prh::launch_direction(S->stats, S->stat_slots, dir, err, done, -1, o.tolerance, s));
@@ -187,8 +263,6 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
   if (!c || !out) return PRH_ERROR_INVALID_ARGUMENT;
   *out = nullptr;
   if (!g || !options || g->ctx != c) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "null graph/options or foreign graph");
-  if (method == PRH_DYNAMIC_TRAVERSAL)
-    return fail(c, PRH_ERROR_UNSUPPORTED, "the traversal closure needs host-side level checks; not streamed");
   if (!valid_method(method)) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad method");
   if (!g->l_out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "the graph was uploaded without its out-CSR");
   if (g->dead_ends) return fail(c, PRH_ERROR_INVALID_GRAPH, "the streamed graph has %u dead ends", g->dead_ends);
@@ -207,7 +281,9 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
   S->method = method;
   S->variant = method == PRH_PRUNE_DYNAMIC_FRONTIER ? prh::kPrune
                : method == PRH_DYNAMIC_FRONTIER      ? prh::kFrontier
+               : method == PRH_DYNAMIC_TRAVERSAL     ? prh::kTraversal
                                                      : prh::kPlain;
+  S->blocked = c->tuning.blocking > 0 && c->tuning.blocking <= 8192;
   S->opt = o;
   S->opt.repeat = 1;
   S->N = g->N;
@@ -305,8 +381,60 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
     dev_free(c, tmp, sb);
     if (e != hipSuccess && st == PRH_SUCCESS) st = fail(c, PRH_ERROR_DEVICE, "stream setup: %s", hipGetErrorString(e));
   }
+  // The session's propagation-blocked plan: built from the base graph's layout in-CSR, with its own slot words (the
+  // dead flags), the base in-CSR and the slot map's inverse, and an overlay of fixed capacity: twice the room the
+  // edge capacity leaves above the base graph, plus two batches (every edge beyond the base that the graph can hold,
+  // with room for base edges deleted and inserted edges kept).
+  if (st == PRH_SUCCESS && S->blocked) {
+    const uint64_t M = g->M;
+    st = build_binned(c, N, M, g->l_in_off, g->l_in_src, g->sorted_deg, true, S->bc);
+    if (st == PRH_SUCCESS) {
+      const uint32_t B = uint32_t(S->bc.B);
+      S->nb = (N + B - 1) / B;
+      const uint64_t room = 2 * (cap_edges - M) + 2ull * max_batch;
+      if (room >= (uint64_t(1) << 31)) st = fail(c, PRH_ERROR_INVALID_ARGUMENT, "edge capacity too far above the graph");
+      S->ov_cap = uint32_t(room);
+    }
+    const uint64_t Mp = (M + 127) / 128 * 128;
+    uint16_t *dst16 = nullptr;
+    step(stream_alloc(S, &dst16, Mp));
+    step(stream_alloc(S, &S->base_off, N1));
+    step(stream_alloc(S, &S->base_src, std::max<uint64_t>(M, 1)));
+    step(stream_alloc(S, &S->slot_of, std::max<uint64_t>(M, 1)));
+    step(stream_alloc(S, &S->ov_key, S->ov_cap));
+    step(stream_alloc(S, &S->ov_cand, size_t(S->ov_cap) + max_batch));
+    step(stream_alloc(S, &S->ov_sorted, size_t(S->ov_cap) + max_batch));
+    step(stream_alloc(S, &S->fresh, max_batch));
+    step(stream_alloc(S, &S->ov_off, size_t(S->nb) + 1));
+    step(stream_alloc(S, &S->ov_words, 1));
+    step(stream_alloc(S, &S->ov_gone, S->ov_cap));
+    step(stream_alloc(S, &S->ovX, S->ov_cap));
+    if (st == PRH_SUCCESS) {
+      hipError_t e = prh::sort_keys_u64(nullptr, S->ov_sort_bytes, S->ov_cand, S->ov_sorted, S->ov_cap + max_batch, s);
+      if (e != hipSuccess) st = fail(c, PRH_ERROR_DEVICE, "overlay sort: %s", hipGetErrorString(e));
+    }
+    step(stream_alloc(S, &S->ov_sort_tmp, S->ov_sort_bytes));
+    if (st == PRH_SUCCESS) {
+      PRH_HIP(c, hipMemcpyAsync(dst16, S->bc.plan.dst16, Mp * 2, hipMemcpyDeviceToDevice, s));
+      PRH_HIP(c, hipMemcpyAsync(S->base_off, g->l_in_off, N1 * 4, hipMemcpyDeviceToDevice, s));
+      if (M) PRH_HIP(c, hipMemcpyAsync(S->base_src, g->l_in_src, M * 4, hipMemcpyDeviceToDevice, s));
+      if (M) PRH_HIP(c, prh::launch_invert(S->bc.plan.slot, uint32_t(M), S->slot_of, s));
+      PRH_HIP(c, prh::launch_fill_u32(S->ov_off, 0, size_t(S->nb) + 1, s));
+      S->bplan = S->bc.plan;
+      S->bplan.dst16 = dst16;
+      S->bplan.ov_n = S->ov_cap;
+      S->bplan.ov_key = S->ov_key;
+      S->bplan.ov_off = S->ov_off;
+      S->bplan.ovX = S->ovX;
+    }
+  }
+  if (S->variant == prh::kTraversal) {
+    step(stream_alloc(S, &S->front[0], N));
+    step(stream_alloc(S, &S->front[1], N));
+    step(stream_alloc(S, &S->lvl, 4));
+  }
   // Out-degree schedule of the extension (fixed for the session; rows that outgrow the low class stay correct).
-  if (st == PRH_SUCCESS && S->variant >= prh::kFrontier) {
+  if (st == PRH_SUCCESS && S->variant >= prh::kTraversal) {
     prh::ExtendArgs xo{};
     st = out_schedule(c, const_cast<prh_graph *>(g), xo);
     if (st == PRH_SUCCESS) {  // the session's kernels take the huge rows as high ones (a wave each)
@@ -376,8 +504,12 @@ int prh_stream_update(prh_stream *S, const uint32_t *del_u, const uint32_t *del_
   PRH_HIP(c, hipEventRecord(c->ev[0], s));
   PRH_HIP(c, hipMemcpyAsync(S->d_batch, h, (4 * size_t(cap) + 2) * 4, hipMemcpyHostToDevice, s));
   const int p = S->cur, q = 1 - p;
-  if (!S->apply[p]) PRH_CHECK(capture_graph(c, [&]() { return stream_apply_body(S, p); }, &S->apply[p]));
-  PRH_HIP(c, hipGraphLaunch(S->apply[p], s));
+  if (c->capture) {
+    if (!S->apply[p]) PRH_CHECK(capture_graph(c, [&]() { return stream_apply_body(S, p); }, &S->apply[p]));
+    PRH_HIP(c, hipGraphLaunch(S->apply[p], s));
+  } else {
+    PRH_CHECK(stream_apply_body(S, p));
+  }
   PRH_HIP(c, hipEventRecord(c->ev[1], s));
   PRH_HIP(c, hipMemcpyAsync(c->pinned, S->ctl + 3, 4, hipMemcpyDeviceToHost, s));
   PRH_HIP(c, hipMemcpyAsync(c->pinned + 1, S->in_off[q] + S->N, 4, hipMemcpyDeviceToHost, s));
@@ -385,15 +517,31 @@ int prh_stream_update(prh_stream *S, const uint32_t *del_u, const uint32_t *del_
   const int flags = *reinterpret_cast<const int *>(c->pinned);
   if (flags) {
     S->broken = true;
-    return fail(c, PRH_ERROR_INVALID_ARGUMENT, flags & 1 ? "batch vertex id out of range"
-                                                          : "the batch leaves a vertex without out-edges (dead end)");
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, flags & 1   ? "batch vertex id out of range"
+                                               : flags & 2 ? "the batch leaves a vertex without out-edges (dead end)"
+                                                           : "the update plan's overlay outgrew its capacity");
   }
   S->M = *reinterpret_cast<const uint32_t *>(c->pinned + 1);
   S->cur = q;
-  if (!S->iter[q]) PRH_CHECK(capture_graph(c, [&]() { return stream_iter_body(S, q); }, &S->iter[q]));
+  if (S->variant == prh::kTraversal) {  // the closure's levels, replayed until one marks nothing new
+    if (c->capture && !S->closure[q])
+      PRH_CHECK(capture_graph(c, [&]() { return stream_closure_body(S, q); }, &S->closure[q]));
+    int stop = 0;
+    for (uint64_t lv = 0; !stop && lv <= uint64_t(S->N) + kClosureLevels; lv += kClosureLevels) {
+      if (c->capture) PRH_HIP(c, hipGraphLaunch(S->closure[q], s));
+      else PRH_CHECK(stream_closure_body(S, q));
+      PRH_HIP(c, hipMemcpyAsync(c->pinned, S->lvl + 1, 4, hipMemcpyDeviceToHost, s));
+      PRH_HIP(c, hipStreamSynchronize(s));
+      stop = *reinterpret_cast<const int *>(c->pinned);
+    }
+  }
+  PRH_HIP(c, hipEventRecord(c->ev[3], s));
+  if (c->capture && !S->iter[q])
+    PRH_CHECK(capture_graph(c, [&]() { return stream_iter_body(S, q); }, &S->iter[q]));
   int n = 0;
   for (int launched = 0; launched < S->opt.max_iterations; launched += kStreamChunk) {
-    PRH_HIP(c, hipGraphLaunch(S->iter[q], s));
+    if (c->capture) PRH_HIP(c, hipGraphLaunch(S->iter[q], s));
+    else PRH_CHECK(stream_iter_body(S, q));
     PRH_HIP(c, hipMemcpyAsync(c->pinned, S->ctl, 4, hipMemcpyDeviceToHost, s));
     PRH_HIP(c, hipStreamSynchronize(s));
     n = *reinterpret_cast<const int *>(c->pinned);
@@ -405,15 +553,17 @@ int prh_stream_update(prh_stream *S, const uint32_t *del_u, const uint32_t *del_
   }
   PRH_HIP(c, hipEventRecord(c->ev[2], s));
   PRH_HIP(c, hipEventSynchronize(c->ev[2]));
-  float t01 = 0, t12 = 0;
+  float t01 = 0, t13 = 0, t32 = 0;
   PRH_HIP(c, hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
-  PRH_HIP(c, hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+  PRH_HIP(c, hipEventElapsedTime(&t13, c->ev[1], c->ev[3]));
+  PRH_HIP(c, hipEventElapsedTime(&t32, c->ev[3], c->ev[2]));
   if (stats) {
     stats->iterations = n;
-    stats->initialization_time = t01;  // batch upload, application and marking
-    stats->marking_time = 0;
-    stats->computation_time = t12;
-    stats->time = t01 + t12;
+    stats->initialization_time = t01;  // batch upload, application, plan patch and the initial marking
+    stats->marking_time = t13;         // the flag read and, for DT, the traversal closure
+    stats->computation_time = t32;
+    stats->time = t01 + t13 + t32;
+    stats->preparation_time = 0;
   }
   if (ranks) PRH_CHECK(prh_stream_ranks(S, ranks));
   return PRH_SUCCESS;

@@ -198,6 +198,26 @@ __global__ __launch_bounds__(kBlock) void mark_batch_kernel(uint8_t *naff, uint8
   }
 }
 
+// Traversal closure levels (lvl: [0] level, [1] stop, [2] and [3] changed flags of even and odd levels). Before
+// level l: stop if level l - 1 marked nothing new, else clear level l's flag.
+__global__ void closure_step_kernel(int *lvl) {
+  if (lvl[1]) return;
+  const int l = lvl[0];
+  if (l > 0 && lvl[2 + ((l - 1) & 1)] == 0) {
+    lvl[1] = 1;
+    return;
+  }
+  lvl[2 + (l & 1)] = 0;
+}
+
+__global__ void closure_advance_kernel(int *lvl) {
+  if (!lvl[1]) ++lvl[0];
+}
+
+__global__ __launch_bounds__(kBlock) void copy_u8_kernel(uint8_t *dst, const uint8_t *src, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock) dst[i] = src[i];
+}
+
 // After iteration l's launches: stop when its L-inf change is below tolerance (done = l + 1 iterations) or when
 // the iteration cap is reached; otherwise move to iteration l + 1.
 __global__ void advance_kernel(int *l, int *done, const unsigned long long *err, double tol, int max_iter) {
@@ -284,6 +304,23 @@ hipError_t launch_refresh_chunks(const uint32_t *item_row, const uint32_t *row_f
 hipError_t launch_mark_batch(uint8_t *naff, uint8_t *vaff, const unsigned long long *dkeys, const uint32_t *nd,
                              const unsigned long long *ikeys, const uint32_t *ni, uint32_t cap_total, hipStream_t s) {
   mark_batch_kernel<<<grid_for(cap_total, kBlock, 1024), kBlock, 0, s>>>(naff, vaff, dkeys, nd, ikeys, ni);
+  return hipGetLastError();
+}
+
+hipError_t launch_closure_step(int *lvl, int parity, hipStream_t s) {
+  (void)parity;  // level l uses flag l & 1; replays start at even levels, so this is the capture slot's parity
+  closure_step_kernel<<<1, 1, 0, s>>>(lvl);
+  return hipGetLastError();
+}
+
+hipError_t launch_closure_advance(int *lvl, hipStream_t s) {
+  closure_advance_kernel<<<1, 1, 0, s>>>(lvl);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_u8(uint8_t *dst, const uint8_t *src, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  copy_u8_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(dst, src, n);
   return hipGetLastError();
 }
 

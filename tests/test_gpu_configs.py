@@ -97,13 +97,15 @@ def test_c4_batched_ppr_64_columns(rmat22, oracle_mod, say):
 
 
 def test_c4_stream_16_dfp_batches(rmat22, oracle_mod, say):
-    """16 consecutive DF-P batches of 1e-4 |E| (80% insertions) through the streaming session; the last ranks
-    against the port's converged solve of the host-applied graph."""
+    """16 consecutive DF-P batches of 1e-4 |E| (80% insertions) through the streaming session on the autotuned plan
+    (the propagation-blocked update, patched per batch); after every batch the session's out-CSR equals the
+    host-applied graph's bit for bit; the last ranks against the port's converged solve of that graph."""
     from pagerank_amd import PRUNE_DYNAMIC_FRONTIER, Batch, Context, Stream, apply_batch
     x = rmat22
     size = int(1e-4 * x.M)
     with Context(0) as c:
         xg = c.upload(x)
+        t, _, _ = c.autotune(xg, None, iterations=3)
         st = Stream(xg, PRUNE_DYNAMIC_FRONTIER, max_batch=size + 16)
         y = x
         for i in range(16):
@@ -111,8 +113,11 @@ def test_c4_stream_16_dfp_batches(rmat22, oracle_mod, say):
             got = st.update(b, want_ranks=False)
             y = apply_batch(y, b)
             assert st.size == y.M, i
+            off, idx = st.csr("out")
+            assert np.array_equal(off, y.out_off) and np.array_equal(idx, y.out_edg), i
             if i % 4 == 3:
-                say(f"C4b batch {i}: -{b.deletions}/+{b.insertions}, {got.iterations} iterations, {got.time:.2f} ms")
+                say(f"C4b batch {i} (blocking {t.blocking}): -{b.deletions}/+{b.insertions}, {got.iterations} "
+                    f"iterations, {got.time:.2f} ms")
         last = st.ranks()
         st.close()
         xg.close()

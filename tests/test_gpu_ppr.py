@@ -93,3 +93,27 @@ def test_bad_seeds_rejected(ctx):
         ppr_batched(yg, [np.zeros(0, np.uint32)])
     with pytest.raises(PagerankError):
         ppr_batched(yg, [np.array([1], np.uint32)] * 65)
+
+
+def test_uncaptured_mode_matches_captured(monkeypatch):
+    """PRH_GRAPH_CAPTURE=0 at context creation: the batched solve and the streaming session issue their launches
+    one by one instead of replaying captured graphs; the results are the same bits (no order-dependent sums here:
+    the row plan and the batched solver sum in a fixed order)."""
+    from pagerank_amd import Batch, Context, HostGraph, Stream, engine
+    y = HostGraph.rmat(12, 16, seed=8)
+    rng = np.random.default_rng(2)
+    sets = [rng.choice(y.N, size=int(rng.integers(1, 50)), replace=False).astype(np.uint32) for _ in range(5)]
+    b = Batch.random(y, 60, 0.8, seed=3)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("PRH_GRAPH_CAPTURE", mode)
+        with Context(0) as c:
+            yg = c.upload(y)
+            res = engine.ppr_batched(yg, sets)
+            st = Stream(yg, 4, max_batch=128)
+            got = st.update(b)
+            st.close()
+            yg.close()
+        out[mode] = (res.ranks, res.iterations, got.ranks, got.iterations)
+    assert np.array_equal(out["0"][0], out["1"][0]) and np.array_equal(out["0"][1], out["1"][1])
+    assert np.array_equal(out["0"][2], out["1"][2]) and out["0"][3] == out["1"][3]

@@ -24,27 +24,53 @@ def _sequence(x, n, size, seed):
     return out
 
 
-@pytest.mark.parametrize("method", [4, 3, 1, 0], ids=["DFP", "DF", "ND", "static"])
-def test_stream_matches_sequential_calls(ctx, oracle_mod, method):
-    """16 consecutive batches through the stream equal 16 sequential calls of the same method on host-built graphs."""
-    from pagerank_amd import HostGraph, Stream, solve
+@pytest.mark.parametrize("blocking", [0, 1024], ids=["rows", "blocked"])
+@pytest.mark.parametrize("method", [4, 3, 2, 1, 0], ids=["DFP", "DF", "DT", "ND", "static"])
+def test_stream_matches_sequential_calls(ctx, oracle_mod, method, blocking):
+    """16 consecutive batches through the stream equal 16 sequential calls of the same method on host-built graphs;
+    after every batch the session's in-CSR and out-CSR equal the host-applied graph's bit for bit. blocked: the
+    session's update runs through its own propagation-blocked plan (B = 1024), patched per batch."""
+    from pagerank_amd import HostGraph, Stream, Tuning, solve
     x = HostGraph.rmat(12, 16, seed=31)
     q0, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, x)
     seq = _sequence(x, 16, max(1, int(1e-3 * x.M)), 100)
-    xg = ctx.upload(x)
-    st = Stream(xg, method, max_batch=4096, initial_ranks=q0)
-    q = q0
-    for i, (b, y) in enumerate(seq):
-        print(f"[stream] method={method} batch={i} -{b.deletions}/+{b.insertions}", flush=True)
+    old = ctx.get_tuning()
+    ctx.set_tuning(Tuning(**{**old.__dict__, "blocking": blocking}))
+    try:
+        xg = ctx.upload(x)
+        st = Stream(xg, method, max_batch=4096, initial_ranks=q0)
+        q = q0
+        for i, (b, y) in enumerate(seq):
+            print(f"[stream] method={method} blocking={blocking} batch={i} -{b.deletions}/+{b.insertions}", flush=True)
+            got = st.update(b)
+            assert st.size == y.M
+            for which, off, idx in (("out", y.out_off, y.out_edg), ("in", y.in_off, y.in_edg)):
+                go, gi = st.csr(which)
+                assert np.array_equal(go, off) and np.array_equal(gi, idx), (i, which)
+            ref = solve(ctx.upload(y), method, batch=b, q=q if method else None)
+            assert abs(got.iterations - ref.iterations) <= 1, (i, got.iterations, ref.iterations)
+            assert np.abs(got.ranks - ref.ranks).max() <= RANK_TOL, i
+            q = ref.ranks
+        if method in (1, 2, 3):  # exact methods: the last ranks are the fixed point of the last graph
+            exact, _ = oracle_mod.power_iteration(seq[-1][1])
+            assert np.abs(st.ranks() - exact).max() <= RANK_TOL
+        st.close()
+    finally:
+        ctx.set_tuning(old)
+
+
+def test_stream_traversal_sets_match_oracle(ctx, oracle_mod):
+    """DT through the stream: after each batch the ranks equal the oracle's DT solve on the host-applied graph, whose
+    marking is the traversal closure (bit-exact with the reference's BFS closure, tests/test_oracle.py)."""
+    from pagerank_amd import HostGraph, Stream
+    x = HostGraph.rmat(11, 8, seed=17)
+    q, _, _ = oracle_mod.pagerank_cuda(oracle_mod.STATIC, x)
+    st = Stream(ctx.upload(x), 2, max_batch=512, initial_ranks=q)
+    for i, (b, y) in enumerate(_sequence(x, 6, 40, 900)):
         got = st.update(b)
-        assert st.size == y.M
-        ref = solve(ctx.upload(y), method, batch=b, q=q if method else None)
-        assert abs(got.iterations - ref.iterations) <= 1, (i, got.iterations, ref.iterations)
-        assert np.abs(got.ranks - ref.ranks).max() <= RANK_TOL, i
-        q = ref.ranks
-    if method in (1, 3):  # exact methods: the last ranks are the fixed point of the last graph
-        exact, _ = oracle_mod.power_iteration(seq[-1][1])
-        assert np.abs(st.ranks() - exact).max() <= RANK_TOL
+        ref, it, _ = oracle_mod.pagerank_cuda(2, y, b, q)
+        assert abs(got.iterations - it) <= 1 and np.abs(got.ranks - ref).max() <= RANK_TOL, i
+        q = got.ranks
     st.close()
 
 
@@ -93,8 +119,6 @@ def test_stream_rejects_bad_batches(ctx):
     with pytest.raises(PagerankError, match="dead end"):
         st.update(Batch(np.array([3], np.uint32), np.array([3], np.uint32), z, z))
     st.close()
-    with pytest.raises(PagerankError, match="traversal"):
-        Stream(ctx.upload(x), 2, max_batch=8)
 
 
 def test_stream_on_fresh_context(oracle_mod):
