@@ -256,9 +256,12 @@ def measured_traffic(desc: str, plan: str):
 
 def plan_name(t) -> str:
     """The rank-update plan a static solve runs with tuning t, and the kernels of one iteration."""
+    order = ", rows in the caller's vertex order" if t.layout == 1 else ""
     if t.blocking > 0:
-        return (f"propagation blocking {t.blocking}: expand_kernel<{t.blocking}> + accumulate_kernel<{t.blocking}> "
-                f"+ finish_bins_kernel")
+        return (f"propagation blocking {t.blocking}{order}: expand_kernel<{t.blocking}> + "
+                f"accumulate_kernel<{t.blocking}> + finish_bins_kernel")
+    if t.layout == 1:
+        return f"row plan{order} (block_edges {t.block_edges}): rank_update_kernel<0>"
     if t.source_slices > 0:
         return (f"source slices {t.source_slices} (huge_degree {t.huge_degree}, block_edges {t.block_edges}): "
                 f"sliced_gather_kernel + sliced_finish_kernel")
@@ -287,6 +290,7 @@ def main() -> int:
     ap.add_argument("--source-slices", type=int, default=None, help="override the tuning's source_slices")
     ap.add_argument("--huge-degree", type=int, default=None, help="override the tuning's huge_degree")
     ap.add_argument("--blocking", type=int, default=None, help="override the tuning's blocking (propagation blocking)")
+    ap.add_argument("--layout", type=int, default=None, help="override the tuning's layout (1: the caller's order)")
     ap.add_argument("--copy-calibration", type=float, metavar="GIB", default=0.0,
                     help="only run a plain device copy of GIB GiB (for scaling the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -310,7 +314,7 @@ def main() -> int:
     ctx = Context(0)
     if args.workload == "dfp":
         return bench_dfp(args, ctx, g, desc)
-    overridden = any(v is not None for v in (args.source_slices, args.huge_degree, args.blocking))
+    overridden = any(v is not None for v in (args.source_slices, args.huge_degree, args.blocking, args.layout))
     if overridden:
         t = ctx.get_tuning()
         if args.source_slices is not None:
@@ -319,6 +323,8 @@ def main() -> int:
             t.huge_degree = args.huge_degree
         if args.blocking is not None:
             t.blocking = args.blocking
+        if args.layout is not None:
+            t.layout = args.layout
         ctx.set_tuning(t)
     yg = ctx.upload(g, with_out=False)
     autotuned = False
@@ -376,7 +382,7 @@ def main() -> int:
         "config": {"workload": desc, "vertices": g.N, "edges": g.M, "tolerance": 1e-10, "damping": 0.85,
                    "tuning": {"switch_degree": tuning.switch_degree, "huge_degree": tuning.huge_degree,
                               "source_slices": tuning.source_slices, "block_edges": tuning.block_edges,
-                              "blocking": tuning.blocking, "autotuned": autotuned},
+                              "blocking": tuning.blocking, "layout": tuning.layout, "autotuned": autotuned},
                    "iterations_per_solve": iters / args.steps,
                    "gedges_per_s_iter_host_buffers": round(host_rate, 4),
                    "gedges_per_s_iter_compute_only": round(g.M * iters / (tcomp / 1e3) / 1e9, 4)},

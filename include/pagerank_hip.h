@@ -98,6 +98,12 @@ typedef struct prh_tuning {
                               many vertices (each iteration expands contributions through LDS into a buffer with
                               one entry per (source, destination bin) pair, then adds them per bin in LDS; takes
                               precedence over source_slices; sums repeat to rounding, not bit for bit) [0] */
+  int layout;              /* vertex order of the rows every plan runs over: 0 = by in-degree, descending (each
+                              degree class a contiguous row range); 1 = the caller's order, which keeps the
+                              neighbourhoods of a graph with locality (a road network, a grid) close together; the
+                              row plan then runs one lane width for all rows and the sliced plan is not used. A graph
+                              changes layout at its next solve (counted as preparation); one updated in place keeps
+                              its layout [0] */
 } prh_tuning;
 
 typedef struct prh_context prh_context;

@@ -17,7 +17,7 @@ namespace prhc {
 namespace {
 
 // Key of a (device, graph) pair: the device's architecture and compute-unit count, N, M and an FNV-1a hash of
-// the graph's in-degree sequence in layout order (which fixes every launch plan).
+// the graph's in-degree histogram (which, with the layout, fixes every launch plan).
 std::string tuning_key(prh_context *c, const prh_graph *g) {
   hipDeviceProp_t prop{};
   if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) (void)hipGetLastError();
@@ -30,7 +30,17 @@ std::string tuning_key(prh_context *c, const prh_graph *g) {
   };
   mix(g->N);
   mix(g->M);
-  for (uint32_t d : g->sorted_deg) mix(d);
+  // The in-degree histogram (the same in every vertex order, so the key survives a change of layout).
+  std::vector<uint64_t> hist;
+  for (uint32_t d : g->sorted_deg) {
+    if (d >= hist.size()) hist.resize(size_t(d) + 1, 0);
+    ++hist[d];
+  }
+  for (size_t d = 0; d < hist.size(); ++d)
+    if (hist[d]) {
+      mix(d);
+      mix(hist[d]);
+    }
   char buf[160];
   std::string arch(prop.gcnArchName);
   arch = arch.substr(0, arch.find(':'));
@@ -39,9 +49,10 @@ std::string tuning_key(prh_context *c, const prh_graph *g) {
   return buf;
 }
 
-constexpr int kTuningCount = 8;
+constexpr int kTuningCount = 9;
 const char *const kTuningFields[kTuningCount] = {"switch_degree", "huge_degree", "low_width", "check_interval",
-                                                 "source_slices", "block_edges", "extension_direction", "blocking"};
+                                                 "source_slices", "block_edges", "extension_direction", "blocking",
+                                                 "layout"};
 
 void tuning_to_values(const prh_tuning &t, long long v[kTuningCount]) {
   v[0] = t.switch_degree;
@@ -52,6 +63,7 @@ void tuning_to_values(const prh_tuning &t, long long v[kTuningCount]) {
   v[5] = t.block_edges;
   v[6] = t.extension_direction;
   v[7] = t.blocking;
+  v[8] = t.layout;
 }
 
 void values_to_tuning(const long long v[kTuningCount], prh_tuning &t) {
@@ -63,6 +75,7 @@ void values_to_tuning(const long long v[kTuningCount], prh_tuning &t) {
   t.block_edges = uint32_t(v[5]);
   t.extension_direction = int(v[6]);
   t.blocking = int(v[7]);
+  t.layout = int(v[8]);
 }
 
 // The cache is a JSON object with one entry per line:
@@ -152,10 +165,15 @@ int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int
         return PRH_SUCCESS;
       }
   }
-  // Candidates: the row plan over switch degree x chunk threshold x block size, the sliced plan over slice
-  // count x block size, and the propagation-blocked plan over its block width (graphs of 2^20 edges and more,
-  // where its streaming passes pay). Each is timed on `iterations` static iterations (tolerance 0) after one
-  // warm-up solve.
+  // Candidates on the degree-ordered layout: the row plan over switch degree x chunk threshold x block size, the
+  // sliced plan over slice count x block size, and the propagation-blocked plan over its block width (graphs of 2^20
+  // edges and more, where its streaming passes pay). On graphs whose in-degrees all stay at or below 64 (a road
+  // network, a grid), also the row plan and the blocked plan on the caller's vertex order, which keeps their
+  // neighbourhoods together (tried last: each layout change rebuilds the layout). A graph updated in place keeps its
+  // layout, so its candidates all take that one. Each is timed on `iterations` static iterations (tolerance 0) after
+  // one warm-up solve.
+  const int fixed_layout = g->updated ? g->layout : -1;
+  const uint32_t max_deg = *std::max_element(g->sorted_deg.begin(), g->sorted_deg.end());
   std::vector<prh_tuning> cand;
   for (uint32_t sw : {64u, 128u, 256u})
     for (uint32_t hu : {4096u, 16384u})
@@ -193,6 +211,34 @@ int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int
       t.blocking = bw;
       cand.push_back(t);
     }
+  for (prh_tuning &t : cand) t.layout = fixed_layout < 0 ? 0 : fixed_layout;
+  if (max_deg <= 64 && fixed_layout < 0) {
+    for (uint32_t be : {2048u, 4096u}) {
+      prh_tuning t = base;
+      t.switch_degree = 64;
+      t.huge_degree = 16384;
+      t.low_width = 0;
+      t.source_slices = 0;
+      t.block_edges = be;
+      t.blocking = 0;
+      t.layout = 1;
+      cand.push_back(t);
+    }
+    if (g->M >= (uint64_t(1) << 20))
+      for (int bw : {2048, 4096, 8192, 16384}) {
+        const uint64_t nb = (uint64_t(g->N) + bw - 1) / bw;
+        if (nb * nb > (uint64_t(1) << 32)) continue;
+        prh_tuning t = base;
+        t.switch_degree = 64;
+        t.huge_degree = 16384;
+        t.low_width = 0;
+        t.source_slices = 0;
+        t.block_edges = 2048;
+        t.blocking = bw;
+        t.layout = 1;
+        cand.push_back(t);
+      }
+  }
   double *ranks = nullptr;
   PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&ranks), size_t(g->N) * 8));
   prh_options o;

@@ -97,6 +97,7 @@ void prh_default_tuning(prh_tuning *t) {
   t->block_edges = 2048;
   t->extension_direction = 1;
   t->blocking = 0;
+  t->layout = 0;
 }
 
 int prh_context_create(int device, prh_context **out) {
@@ -199,7 +200,8 @@ int prh_context_set_tuning(prh_context *c, const prh_tuning *t) {
       t->huge_degree < t->switch_degree || !(sl == 0 || sl == 1 || sl == 2 || sl == 4 || sl == 8) ||
       t->block_edges < 256 || t->extension_direction < 0 || t->extension_direction > 2 ||
       !(t->blocking == 0 || t->blocking == 1024 || t->blocking == 2048 || t->blocking == 4096 ||
-        t->blocking == 8192 || t->blocking == 16384))
+        t->blocking == 8192 || t->blocking == 16384) ||
+      !(t->layout == 0 || t->layout == 1))
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad tuning");
   c->tuning = *t;
   return PRH_SUCCESS;

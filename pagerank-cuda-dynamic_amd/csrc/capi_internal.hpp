@@ -128,7 +128,8 @@ struct prh_graph {
   // inv is the inverse; in-rows sorted by new source id.
   uint32_t *perm = nullptr, *inv = nullptr;
   uint32_t *l_in_off = nullptr, *l_in_src = nullptr, *l_outdeg = nullptr, *l_out_off = nullptr, *l_out_dst = nullptr;
-  std::vector<uint32_t> sorted_deg;  // host: in-degree of layout row i
+  std::vector<uint32_t> sorted_deg;  // host: in-degree of layout row i (descending when layout == 0)
+  int layout = 0;                    // 0: rows by in-degree, descending; 1: the caller's vertex order
   uint32_t dead_ends = 0;  // vertices with out-degree 0
   // In-place updates (prh_graph_update): once one has run, the caller-order arrays above are freed (the batched and
   // weighted solvers then refuse the graph); the layout's out-rows are kept ascending for the batch tidy.
@@ -211,6 +212,12 @@ inline uint32_t rows_above(const std::vector<uint32_t> &deg, uint32_t x) {
                   deg.begin());
 }
 
+// Rows [0, n) of a degree-ordered layout are those with in-degree > x; a layout in the caller's order has no such
+// prefix (0).
+inline uint32_t layout_rows_above(const prh_graph *g, uint32_t x) {
+  return g->layout == 0 ? rows_above(g->sorted_deg, x) : 0u;
+}
+
 // Capture `body` (launches on the context stream only) into an executable graph.
 template <class F>
 int capture_graph(prh_context *c, F body, hipGraphExec_t *exec) {
@@ -240,6 +247,9 @@ int copy_to_host(prh_context *c, void *dst, const void *src, size_t n);
 
 // plans.hip
 int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uint32_t *outdeg, bool with_out);
+// Rebuilds the graph's layout in `mode` (0 degree-ordered, 1 the caller's order) from its uploaded arrays and drops
+// every plan built on the old one.
+int relayout(prh_context *c, prh_graph *g, int mode);
 void free_sliced(prh_context *c, prh_graph *g);
 int build_sliced_plan(prh_context *c, prh_graph *g);
 // The propagation-blocked plan of a pull CSR (off, idx; N rows of degrees rowdeg, M slots) into bc, for the

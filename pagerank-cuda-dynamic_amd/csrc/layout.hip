@@ -37,6 +37,10 @@ __global__ __launch_bounds__(kBlock) void relabel_rows_kernel(const uint32_t *of
   }
 }
 
+__global__ __launch_bounds__(kBlock) void iota_kernel(uint32_t *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = i;
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void gather_kernel(const T *src, const uint32_t *idx, size_t n, T *dst) {
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock)
@@ -162,6 +166,12 @@ hipError_t launch_relabel_rows(const uint32_t *off, const uint32_t *idx, const u
 hipError_t sort_rows(void *tmp, size_t &tmp_bytes, const uint32_t *in, uint32_t *out, size_t M, uint32_t N,
                      const uint32_t *off, hipStream_t s) {
   return rocprim::segmented_radix_sort_keys(tmp, tmp_bytes, in, out, M, N, off, off + 1, 0, 32, s);
+}
+
+hipError_t launch_iota_u32(uint32_t *p, uint32_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  iota_kernel<<<grid(n, kBlock, 8192), kBlock, 0, s>>>(p, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_gather_u32(const uint32_t *src, const uint32_t *idx, size_t n, uint32_t *dst, hipStream_t s) {

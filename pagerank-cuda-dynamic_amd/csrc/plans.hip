@@ -468,7 +468,7 @@ int build_rank_plan(prh_context *c, prh_graph *g) {
   const std::vector<uint32_t> &deg = g->sorted_deg;
   const uint64_t kEdgesPerBlock = t.block_edges;
   PRH_HIP(c, hipStreamSynchronize(c->stream));  // the old tables may still be in use
-  auto above = [&](uint32_t x) { return rows_above(deg, x); };
+  auto above = [&](uint32_t x) { return layout_rows_above(g, x); };
   auto edges = [&](uint32_t b, uint32_t e) {
     uint64_t s = 0;
     for (uint32_t i = b; i < e; ++i) s += deg[i];
@@ -538,6 +538,11 @@ int build_rank_plan(prh_context *c, prh_graph *g) {
   const uint32_t lb = std::max(n_chunked, w_end);
   if (t.low_width) {
     row_section(prh::kGroupRows, t.low_width, lb, N);
+  } else if (g->layout == 1) {  // rows in the caller's order: one width for all, half the largest in-degree
+    const uint32_t dmax = N ? *std::max_element(deg.begin(), deg.end()) : 1;
+    int width = 1;
+    while (width < 32 && uint32_t(2 * width) < dmax) width *= 2;
+    row_section(prh::kGroupRows, width, 0, N);
   } else {
     uint32_t b = lb;
     for (int width = 32; width >= 1; width /= 2) {
@@ -552,6 +557,68 @@ int build_rank_plan(prh_context *c, prh_graph *g) {
   g->plan_tuning = t;
   g->plan_valid = true;
   g->n_chunked = n_chunked;
+  return PRH_SUCCESS;
+}
+
+// The layout in the caller's order (mode 1: row i is vertex i; the layout arrays are copies of the uploaded ones) or
+// by in-degree (mode 0, from the uploaded arrays read back to the host). Plans built on the old layout are dropped.
+int relayout(prh_context *c, prh_graph *g, int mode) {
+  if (g->layout == mode) return PRH_SUCCESS;
+  if (g->updated) return fail(c, PRH_ERROR_UNSUPPORTED, "a graph updated in place keeps its layout");
+  hipStream_t s = c->stream;
+  PRH_HIP(c, hipStreamSynchronize(s));
+  const uint32_t N = g->N;
+  const uint64_t M = g->M;
+  const size_t N1 = size_t(N) + 1;
+  const bool with_out = g->out_off != nullptr;
+  // Everything built on the old layout.
+  g->plan_valid = false;
+  free_sliced(c, g);
+  free_binned(c, g->bin);
+  free_binned(c, g->bin_t);
+  if (g->out_valid) {
+    dev_free(c, g->out_chunks, std::max<size_t>(3 * size_t(g->out_n_chunks), 1) * 4);
+    g->out_chunks = nullptr;
+    g->out_n_chunks = 0;
+    g->out_valid = false;
+  }
+  dev_free(c, g->l_in_eid, (M ? M : 4) * 4);
+  dev_free(c, g->l_out_eid, (M ? M : 4) * 4);
+  g->l_in_eid = g->l_out_eid = nullptr;
+  dev_free(c, g->perm, size_t(N) * 4);
+  dev_free(c, g->inv, size_t(N) * 4);
+  dev_free(c, g->l_in_off, N1 * 4);
+  dev_free(c, g->l_in_src, (M ? M : 4) * 4);
+  dev_free(c, g->l_outdeg, size_t(N) * 4);
+  dev_free(c, g->l_out_off, N1 * 4);
+  dev_free(c, g->l_out_dst, (M ? M : 4) * 4);
+  g->perm = g->inv = g->l_in_off = g->l_in_src = g->l_outdeg = g->l_out_off = g->l_out_dst = nullptr;
+  g->out_sorted = false;
+  std::vector<uint32_t> in_off(N1), outdeg(N);
+  PRH_HIP(c, hipMemcpy(in_off.data(), g->in_off, N1 * 4, hipMemcpyDeviceToHost));
+  PRH_HIP(c, hipMemcpy(outdeg.data(), g->outdeg, size_t(N) * 4, hipMemcpyDeviceToHost));
+  if (mode == 0) return build_layout(c, g, in_off.data(), outdeg.data(), with_out);
+  PrepTimer timer(g->layout_prep);
+  auto copy = [&](uint32_t **dst, const uint32_t *src, size_t n) -> int {
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(dst), std::max<size_t>(n, 1) * 4));
+    if (n) PRH_HIP(c, hipMemcpyAsync(*dst, src, n * 4, hipMemcpyDeviceToDevice, s));
+    return PRH_SUCCESS;
+  };
+  PRH_CHECK(copy(&g->l_in_off, g->in_off, N1));
+  PRH_CHECK(copy(&g->l_in_src, g->in_src, M));
+  PRH_CHECK(copy(&g->l_outdeg, g->outdeg, N));
+  if (with_out) {
+    PRH_CHECK(copy(&g->l_out_off, g->out_off, N1));
+    PRH_CHECK(copy(&g->l_out_dst, g->out_dst, M));
+  }
+  PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&g->perm), std::max<size_t>(N, 1) * 4));
+  PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&g->inv), std::max<size_t>(N, 1) * 4));
+  PRH_HIP(c, prh::launch_iota_u32(g->perm, N, s));
+  PRH_HIP(c, prh::launch_iota_u32(g->inv, N, s));
+  g->sorted_deg.resize(N);
+  for (uint32_t v = 0; v < N; ++v) g->sorted_deg[v] = in_off[v + 1] - in_off[v];
+  PRH_HIP(c, hipStreamSynchronize(s));
+  g->layout = 1;
   return PRH_SUCCESS;
 }
 
@@ -622,6 +689,7 @@ constexpr uint32_t kDegreeBuckets = 1u << 20;
 
 int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uint32_t *outdeg, bool with_out) {
   PrepTimer timer(g->layout_prep);
+  g->layout = 0;
   const uint32_t N = g->N;
   const uint64_t M = g->M;
   hipStream_t s = c->stream;

@@ -362,6 +362,7 @@ hipError_t launch_relabel_rows(const uint32_t *off, const uint32_t *idx, const u
 hipError_t sort_rows(void *tmp, size_t &tmp_bytes, const uint32_t *in, uint32_t *out, size_t M, uint32_t N,
                      const uint32_t *off, hipStream_t s);
 hipError_t launch_gather_u32(const uint32_t *src, const uint32_t *idx, size_t n, uint32_t *dst, hipStream_t s);
+hipError_t launch_iota_u32(uint32_t *p, uint32_t n, hipStream_t s);  // p[i] = i
 hipError_t launch_gather_f64(const double *src, const uint32_t *idx, size_t n, double *dst, hipStream_t s);
 hipError_t launch_scatter_f64(const double *src, const uint32_t *idx, size_t n, double *dst, hipStream_t s);
 hipError_t launch_scatter_u8(const uint8_t *src, const uint32_t *idx, size_t n, uint8_t *dst, hipStream_t s);

@@ -44,6 +44,7 @@ struct prh_stream {
   uint32_t *in_off[2] = {}, *in_src[2] = {}, *out_off[2] = {}, *out_dst[2] = {};
   int cur = 0;             // buffers holding the current graph
   uint32_t *outdeg = nullptr, *new_deg = nullptr, *sched_order = nullptr;
+  uint32_t *perm = nullptr, *inv = nullptr;  // the vertex order (layout row -> vertex, vertex -> layout row)
   uint32_t sched_low = 0, sched_high = 0;
   int *delta = nullptr;
   uint8_t *changed = nullptr;
@@ -147,10 +148,10 @@ int stream_apply_body(prh_stream *S, int p) {
   const int q = 1 - p;
   const uint32_t *b = S->d_batch;
   PRH_HIP(c, prh::launch_fill_i32(S->ctl, 0, 4, s));
-  PRH_HIP(c, prh::launch_batch_keys(b, b + cap, b + 4 * size_t(cap), cap, S->g->inv, N, S->keys[0], S->keys[2],
+  PRH_HIP(c, prh::launch_batch_keys(b, b + cap, b + 4 * size_t(cap), cap, S->inv, N, S->keys[0], S->keys[2],
                                     S->ctl + 3, 1, s));
   PRH_HIP(c, prh::launch_batch_keys(b + 2 * size_t(cap), b + 3 * size_t(cap), b + 4 * size_t(cap) + 1, cap,
-                                    S->g->inv, N, S->keys[1], S->keys[3], S->ctl + 3, 1, s));
+                                    S->inv, N, S->keys[1], S->keys[3], S->ctl + 3, 1, s));
   PRH_CHECK(stream_apply_csr(S, s, 0, 1, S->in_off[p], S->in_src[p], S->in_off[q], S->in_src[q], 0));
   PRH_CHECK(stream_apply_csr(S, s, 2, 3, S->out_off[p], S->out_dst[p], S->out_off[q], S->out_dst[q], 2));
   PRH_HIP(c, prh::launch_degrees(S->out_off[q], N, S->outdeg, s));
@@ -274,6 +275,7 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
   if (cap_edges < g->M || cap_edges >= (uint64_t(1) << 32))
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "edge capacity must hold the graph and stay below 2^32");
   PRH_HIP(c, hipSetDevice(c->device));
+  if (c->tuning.layout != g->layout && !g->updated) PRH_CHECK(relayout(c, const_cast<prh_graph *>(g), c->tuning.layout));
   PRH_CHECK(build_rank_plan(c, const_cast<prh_graph *>(g)));
   auto *S = new prh_stream;
   S->ctx = c;
@@ -306,6 +308,12 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
     step(stream_alloc(S, &S->contrib[k], N));
   }
   step(stream_alloc(S, &S->outdeg, N));
+  step(stream_alloc(S, &S->perm, N));
+  step(stream_alloc(S, &S->inv, N));
+  if (st == PRH_SUCCESS) {  // the session's own vertex order: the graph may change layout later
+    PRH_HIP(c, hipMemcpyAsync(S->perm, g->perm, size_t(N) * 4, hipMemcpyDeviceToDevice, s));
+    PRH_HIP(c, hipMemcpyAsync(S->inv, g->inv, size_t(N) * 4, hipMemcpyDeviceToDevice, s));
+  }
   step(stream_alloc(S, &S->new_deg, N1));
   step(stream_alloc(S, &S->sched_order, N));
   step(stream_alloc(S, &S->delta, N));
@@ -362,8 +370,8 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
     }
     S->stat_slots = S->plan.blocks + (nc ? prh::finish_chunked_grid(nc) : 0);
     step(stream_alloc(S, &S->stats, 2 * size_t(std::max<uint32_t>(S->stat_slots, 1))));
-    S->pull_wave = rows_above(g->sorted_deg, 64);
-    S->pull_group = rows_above(g->sorted_deg, 4);
+    S->pull_wave = layout_rows_above(g, 64);
+    S->pull_group = layout_rows_above(g, 4);
   }
   // Current graph: the layout's in-CSR as is; its out-CSR with every row sorted (the batch tidy searches it).
   if (st == PRH_SUCCESS) {
@@ -458,7 +466,7 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
         st = solve_impl(c, g, PRH_STATIC, &o, nullptr, nullptr, 0, nullptr, 0, nullptr, tmp, &stt, false, false);
       }
       if (st == PRH_SUCCESS) {
-        hipError_t e = prh::launch_gather_f64(tmp, g->perm, N, S->rank[0], s);
+        hipError_t e = prh::launch_gather_f64(tmp, S->perm, N, S->rank[0], s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) st = fail(c, PRH_ERROR_DEVICE, "initial ranks: %s", hipGetErrorString(e));
       }
@@ -573,7 +581,7 @@ int prh_stream_ranks(prh_stream *S, double *ranks) {
   if (!S || !ranks) return S ? fail(S->ctx, PRH_ERROR_INVALID_ARGUMENT, "null ranks") : PRH_ERROR_INVALID_ARGUMENT;
   prh_context *c = S->ctx;
   PRH_CHECK(ensure(c, c->contrib[0], S->N));
-  PRH_HIP(c, prh::launch_scatter_f64(S->rank[0], S->g->perm, S->N, c->contrib[0].p, c->stream));
+  PRH_HIP(c, prh::launch_scatter_f64(S->rank[0], S->perm, S->N, c->contrib[0].p, c->stream));
   PRH_HIP(c, hipMemcpyAsync(ranks, c->contrib[0].p, size_t(S->N) * 8, hipMemcpyDeviceToHost, c->stream));
   PRH_HIP(c, hipStreamSynchronize(c->stream));
   return PRH_SUCCESS;
@@ -589,7 +597,7 @@ int prh_stream_csr(prh_stream *S, int which, uint32_t *offsets, uint32_t *ids) {
   PRH_HIP(c, hipSetDevice(c->device));
   PRH_HIP(c, hipStreamSynchronize(c->stream));
   const int p = S->cur;
-  return csr_to_caller(c, S->g->perm, S->g->inv, S->N, which ? S->out_off[p] : S->in_off[p],
+  return csr_to_caller(c, S->perm, S->inv, S->N, which ? S->out_off[p] : S->in_off[p],
                        which ? S->out_dst[p] : S->in_src[p], S->M, offsets, ids);
 }
 

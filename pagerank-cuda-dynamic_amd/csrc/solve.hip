@@ -151,8 +151,9 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   PRH_HIP(c, hipSetDevice(c->device));
   PRH_CHECK(ensure_workspace(c, N, dynamic, L));
   prh_graph *gm = const_cast<prh_graph *>(g);
+  if (c->tuning.layout != g->layout && !g->updated) PRH_CHECK(relayout(c, gm, c->tuning.layout));
   const bool binned = c->tuning.blocking > 0;
-  const bool sliced = !binned && variant == prh::kPlain && c->tuning.source_slices > 0;
+  const bool sliced = !binned && variant == prh::kPlain && c->tuning.source_slices > 0 && g->layout == 0;
   if (binned) PRH_CHECK(build_binned_plan(c, gm));
   else if (sliced) PRH_CHECK(build_sliced_plan(c, gm));
   else PRH_CHECK(build_rank_plan(c, gm));
@@ -194,8 +195,8 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
                                      : plan.blocks + (plan.n_chunked ? prh::finish_chunked_grid(plan.n_chunked) : 0);
   if (dir_auto) PRH_CHECK(ensure(c, c->ext_stats, 2 * size_t(std::max<uint32_t>(stat_slots, 1))));
   int *dir = frontier ? c->flags.p + 2 : nullptr;
-  const prh::PullArgs pull{c->vaff.p, c->naff.p, g->l_in_off, g->l_in_src, rows_above(g->sorted_deg, 64),
-                           rows_above(g->sorted_deg, 4), N, dir};
+  const prh::PullArgs pull{c->vaff.p, c->naff.p, g->l_in_off, g->l_in_src, layout_rows_above(g, 64),
+                           layout_rows_above(g, 4), N, dir};
   for (int rep = 0; rep < o->repeat; ++rep) {
     if (q) PRH_CHECK(load_initial_ranks(c, g, q, host));  // untimed, as the reference's q copy (pagerankCuda.hxx:810)
     PRH_HIP(c, hipEventRecord(c->ev[0], s));

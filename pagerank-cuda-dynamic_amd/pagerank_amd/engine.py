@@ -73,11 +73,12 @@ class Tuning:
     block_edges: int = 2048
     extension_direction: int = 1
     blocking: int = 0
+    layout: int = 0
 
 
 def _tuning_of(n) -> Tuning:
     return Tuning(n.switch_degree, n.huge_degree, n.low_width, n.check_interval, n.source_slices, n.block_edges,
-                  n.extension_direction, n.blocking)
+                  n.extension_direction, n.blocking, n.layout)
 
 
 class Context:
@@ -128,7 +129,8 @@ class Context:
     def set_tuning(self, t: Tuning):
         self.check(self._lib.prh_context_set_tuning(
             self._h, C.byref(prh_tuning(t.switch_degree, t.huge_degree, t.low_width, t.check_interval,
-                                        t.source_slices, t.block_edges, t.extension_direction, t.blocking))))
+                                        t.source_slices, t.block_edges, t.extension_direction, t.blocking,
+                                        t.layout))))
 
     def get_tuning(self) -> Tuning:
         n = prh_tuning()
