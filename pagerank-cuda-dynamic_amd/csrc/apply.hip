@@ -202,7 +202,7 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   // The layout's out-rows ascending (the tidy searches them), once per graph.
   if (!g->out_sorted) {
     uint32_t *sorted = nullptr;
-    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&sorted), std::max<size_t>(M, 4) * 4));
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&sorted), std::max<size_t>(g->l_cap, 4) * 4));
     size_t sb = 0;
     hipError_t e = prh::sort_rows(nullptr, sb, g->l_out_dst, sorted, M, N, g->l_out_off, s);
     uint8_t *work = nullptr;
@@ -210,10 +210,10 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
     if (st == PRH_SUCCESS) e = prh::sort_rows(work, sb, g->l_out_dst, sorted, M, N, g->l_out_off, s);
     if (st == PRH_SUCCESS && e == hipSuccess) e = hipStreamSynchronize(s);
     if (st != PRH_SUCCESS || e != hipSuccess) {
-      dev_free(c, sorted, std::max<size_t>(M, 4) * 4);
+      dev_free(c, sorted, std::max<size_t>(g->l_cap, 4) * 4);
       return st != PRH_SUCCESS ? st : fail(c, PRH_ERROR_DEVICE, "out-row sort: %s", hipGetErrorString(e));
     }
-    dev_free(c, g->l_out_dst, std::max<size_t>(M, 4) * 4);
+    dev_free(c, g->l_out_dst, std::max<size_t>(g->l_cap, 4) * 4);
     g->l_out_dst = sorted;
     g->out_sorted = true;
   }
@@ -283,18 +283,39 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   const uint64_t M2 = M - cnt[0] + cnt[1];
   if (M2 >= (uint64_t(1) << 32)) return fail(c, PRH_ERROR_UNSUPPORTED, "the update would exceed 2^32 - 1 edges");
   // New CSRs. Nothing of the graph changes before this point, so an error above leaves it as it was.
-  int *delta = nullptr;
-  uint8_t *changed = nullptr;
-  uint32_t *new_deg = nullptr;
-  PRH_CHECK(tmp.get(&delta, N));
-  PRH_CHECK(tmp.get(&changed, N));
-  PRH_CHECK(tmp.get(&new_deg, N1));
+  PRH_CHECK(ensure(c, c->upd_delta, N));
+  PRH_CHECK(ensure(c, c->upd_changed, N));
+  PRH_CHECK(ensure(c, c->upd_deg, N1));
+  int *delta = c->upd_delta.p;
+  uint8_t *changed = c->upd_changed.p;
+  uint32_t *new_deg = c->upd_deg.p;
+  // The new CSRs go into the spare arrays an earlier update left, when they hold M2 edges; else into new arrays with
+  // room for growth (1/256 of the edges), which later updates reuse.
   uint32_t *n_in_off = nullptr, *n_in_src = nullptr, *n_out_off = nullptr, *n_out_dst = nullptr;
+  uint64_t n_cap = 0;
   auto alloc = [&](uint32_t **p, size_t n) { return dev_alloc(c, reinterpret_cast<void **>(p), std::max<size_t>(n, 4) * 4); };
-  int st = alloc(&n_in_off, N1);
-  if (st == PRH_SUCCESS) st = alloc(&n_in_src, M2);
-  if (st == PRH_SUCCESS) st = alloc(&n_out_off, N1);
-  if (st == PRH_SUCCESS) st = alloc(&n_out_dst, M2);
+  int st = PRH_SUCCESS;
+  if (g->spare_in_src && g->spare_cap >= M2) {
+    n_in_off = g->spare_in_off;
+    n_in_src = g->spare_in_src;
+    n_out_off = g->spare_out_off;
+    n_out_dst = g->spare_out_dst;
+    n_cap = g->spare_cap;
+    g->spare_in_off = g->spare_in_src = g->spare_out_off = g->spare_out_dst = nullptr;
+    g->spare_cap = 0;
+  } else {
+    dev_free(c, g->spare_in_off, N1 * 4);
+    dev_free(c, g->spare_in_src, std::max<size_t>(g->spare_cap, 4) * 4);
+    dev_free(c, g->spare_out_off, N1 * 4);
+    dev_free(c, g->spare_out_dst, std::max<size_t>(g->spare_cap, 4) * 4);
+    g->spare_in_off = g->spare_in_src = g->spare_out_off = g->spare_out_dst = nullptr;
+    g->spare_cap = 0;
+    n_cap = std::max<uint64_t>(M2 + M2 / 256, 4);
+    st = alloc(&n_in_off, N1);
+    if (st == PRH_SUCCESS) st = alloc(&n_in_src, n_cap);
+    if (st == PRH_SUCCESS) st = alloc(&n_out_off, N1);
+    if (st == PRH_SUCCESS) st = alloc(&n_out_dst, n_cap);
+  }
   auto apply_csr = [&](int kd, int ki, const uint32_t *off, const uint32_t *idx, uint32_t *noff, uint32_t *nidx) -> int {
     PRH_HIP(c, prh::launch_fill_i32(delta, 0, N, s));
     PRH_HIP(c, prh::launch_fill_u8(changed, 0, N, s));
@@ -320,31 +341,37 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   if (st != PRH_SUCCESS) {
     (void)hipStreamSynchronize(s);
     dev_free(c, n_in_off, N1 * 4);
-    dev_free(c, n_in_src, std::max<size_t>(M2, 4) * 4);
+    dev_free(c, n_in_src, std::max<size_t>(n_cap, 4) * 4);
     dev_free(c, n_out_off, N1 * 4);
-    dev_free(c, n_out_dst, std::max<size_t>(M2, 4) * 4);
+    dev_free(c, n_out_dst, std::max<size_t>(n_cap, 4) * 4);
     return st;
   }
   // Swap in the new CSRs. The blocked plan (blocks of at most 8192 vertices) is patched; its first update keeps the
   // in-CSR it was built from as its base. A plan with wider blocks has no room for the dead flag: it is rebuilt on
   // its next use.
+  // The old CSRs become the spare arrays of the next update, except an in-CSR that becomes the plan's base.
   BinnedCache &bc = g->bin;
   const bool patch = bc.valid && bc.B <= 8192 && bc.plan.slot;
   if (patch && !bc.base_off) {
     bc.base_off = g->l_in_off;
     bc.base_src = g->l_in_src;
     bc.base_M = M;
+    bc.base_cap = g->l_cap;
+    dev_free(c, g->l_out_off, N1 * 4);
+    dev_free(c, g->l_out_dst, std::max<size_t>(g->l_cap, 4) * 4);
   } else {
-    dev_free(c, g->l_in_off, N1 * 4);
-    dev_free(c, g->l_in_src, std::max<size_t>(M, 4) * 4);
+    g->spare_in_off = g->l_in_off;
+    g->spare_in_src = g->l_in_src;
+    g->spare_out_off = g->l_out_off;
+    g->spare_out_dst = g->l_out_dst;
+    g->spare_cap = g->l_cap;
   }
   if (!patch && bc.valid) free_binned(c, bc);
-  dev_free(c, g->l_out_off, N1 * 4);
-  dev_free(c, g->l_out_dst, std::max<size_t>(M, 4) * 4);
   g->l_in_off = n_in_off;
   g->l_in_src = n_in_src;
   g->l_out_off = n_out_off;
   g->l_out_dst = n_out_dst;
+  g->l_cap = n_cap;
   g->M = M2;
   PRH_HIP(c, prh::launch_degrees(g->l_out_off, N, g->l_outdeg, s));
   uint32_t *zeros = nullptr;
@@ -352,8 +379,9 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   PRH_HIP(c, prh::launch_fill_u32(zeros, 0, 1, s));
   count_zero_kernel<<<256, 256, 0, s>>>(g->l_outdeg, N, zeros);
   PRH_HIP(c, hipGetLastError());
-  PRH_HIP(c, hipMemcpyAsync(c->pinned, zeros, 4, hipMemcpyDeviceToHost, s));
+  // patch_plan reads its own words back through the pinned area: read the dead-end count after it
   if (patch) PRH_CHECK(patch_plan(c, g, compact[0], counts + 0, cnt[0], compact[1], counts + 1, cnt[1]));
+  PRH_HIP(c, hipMemcpyAsync(c->pinned, zeros, 4, hipMemcpyDeviceToHost, s));
   PRH_HIP(c, hipStreamSynchronize(s));
   g->dead_ends = *reinterpret_cast<const uint32_t *>(c->pinned);
   for (unsigned long long k : hd) --g->sorted_deg[uint32_t(k >> 32)];

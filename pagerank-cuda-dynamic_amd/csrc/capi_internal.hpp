@@ -73,6 +73,10 @@ struct prh_context {
   prhc::DevBuf<double> ext_stats;         // per rank workgroup: pull and push costs of the next extension
   prhc::DevBuf<uint32_t> ext_chunks;      // initial extension: chunk counts and their scan per batch source
   prhc::DevBuf<uint8_t> ext_tmp;          // its scan's temporary storage
+  // in-place graph updates: per-row degree change, changed-row flags, new degrees (kept between updates)
+  prhc::DevBuf<int> upd_delta;
+  prhc::DevBuf<uint8_t> upd_changed;
+  prhc::DevBuf<uint32_t> upd_deg;
   // batched personalized PageRank
   prhc::DevBuf<double> ppr_rank[2], ppr_contrib[2], ppr_partial, ppr_mass, ppr_out;
   prhc::DevBuf<int32_t> ppr_row;
@@ -107,7 +111,7 @@ struct BinnedCache {
   // In-place updates (prh_graph_update): the in-CSR the plan was built from (kept from the first update on), the
   // inverse of its slot map, and the overlay of inserted edges that are not slots of the plan (capacity ov_cap).
   uint32_t *base_off = nullptr, *base_src = nullptr, *slot_of = nullptr;
-  uint64_t base_M = 0;
+  uint64_t base_M = 0, base_cap = 0;  // base edges, and the capacity of base_src
   unsigned long long *ov_key = nullptr;
   uint32_t *ov_off = nullptr;
   double *ovX = nullptr;
@@ -134,6 +138,11 @@ struct prh_graph {
   // In-place updates (prh_graph_update): once one has run, the caller-order arrays above are freed (the batched and
   // weighted solvers then refuse the graph); the layout's out-rows are kept ascending for the batch tidy.
   bool updated = false, out_sorted = false;
+  // Arrays of an earlier CSR kept for the next update to write into (capacity in elements), so that a series of
+  // updates does not allocate per batch.
+  uint32_t *spare_in_off = nullptr, *spare_in_src = nullptr, *spare_out_off = nullptr, *spare_out_dst = nullptr;
+  uint64_t spare_cap = 0;
+  uint64_t l_cap = 0;  // capacity (elements) of l_in_src and l_out_dst
   prhc::Prep update_prep;  // update time not yet reported by a solve
   prhc::Prep layout_prep, plan_prep, sliced_prep, out_prep;
   // Rank-kernel plan for the current tuning: section table and the chunk tables of the heaviest rows.

@@ -223,11 +223,11 @@ void free_binned(prh_context *c, BinnedCache &bc) {
   bc.valid = false;
   const uint32_t N = bc.plan.N;
   dev_free(c, bc.base_off, bc.base_off ? (size_t(N) + 1) * 4 : 0);
-  dev_free(c, bc.base_src, bc.base_src ? std::max<size_t>(bc.base_M, 4) * 4 : 0);
+  dev_free(c, bc.base_src, bc.base_src ? std::max<size_t>(bc.base_cap, 4) * 4 : 0);
   dev_free(c, bc.slot_of, bc.slot_of ? std::max<size_t>(bc.base_M, 4) * 4 : 0);
   free_overlay(c, bc);
   bc.base_off = bc.base_src = bc.slot_of = nullptr;
-  bc.base_M = 0;
+  bc.base_M = bc.base_cap = 0;
 }
 
 void free_overlay(prh_context *c, BinnedCache &bc) {
@@ -588,10 +588,10 @@ int relayout(prh_context *c, prh_graph *g, int mode) {
   dev_free(c, g->perm, size_t(N) * 4);
   dev_free(c, g->inv, size_t(N) * 4);
   dev_free(c, g->l_in_off, N1 * 4);
-  dev_free(c, g->l_in_src, (M ? M : 4) * 4);
+  dev_free(c, g->l_in_src, std::max<size_t>(g->l_cap, 4) * 4);
   dev_free(c, g->l_outdeg, size_t(N) * 4);
   dev_free(c, g->l_out_off, N1 * 4);
-  dev_free(c, g->l_out_dst, (M ? M : 4) * 4);
+  dev_free(c, g->l_out_dst, std::max<size_t>(g->l_cap, 4) * 4);
   g->perm = g->inv = g->l_in_off = g->l_in_src = g->l_outdeg = g->l_out_off = g->l_out_dst = nullptr;
   g->out_sorted = false;
   std::vector<uint32_t> in_off(N1), outdeg(N);
@@ -600,10 +600,11 @@ int relayout(prh_context *c, prh_graph *g, int mode) {
   if (mode == 0) return build_layout(c, g, in_off.data(), outdeg.data(), with_out);
   PrepTimer timer(g->layout_prep);
   auto copy = [&](uint32_t **dst, const uint32_t *src, size_t n) -> int {
-    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(dst), std::max<size_t>(n, 1) * 4));
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(dst), std::max<size_t>(n, 4) * 4));
     if (n) PRH_HIP(c, hipMemcpyAsync(*dst, src, n * 4, hipMemcpyDeviceToDevice, s));
     return PRH_SUCCESS;
   };
+  g->l_cap = std::max<uint64_t>(M, 4);
   PRH_CHECK(copy(&g->l_in_off, g->in_off, N1));
   PRH_CHECK(copy(&g->l_in_src, g->in_src, M));
   PRH_CHECK(copy(&g->l_outdeg, g->outdeg, N));
@@ -737,20 +738,20 @@ int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uin
   PRH_CHECK(alloc(&g->perm, N));
   PRH_CHECK(alloc(&g->inv, N));
   PRH_CHECK(alloc(&g->l_in_off, size_t(N) + 1));
-  PRH_CHECK(alloc(&g->l_in_src, M));
+  g->l_cap = std::max<uint64_t>(M, 4);
+  PRH_CHECK(alloc(&g->l_in_src, g->l_cap));
   PRH_CHECK(alloc(&g->l_outdeg, N));
   if (with_out) {
     PRH_CHECK(alloc(&g->l_out_off, size_t(N) + 1));
-    PRH_CHECK(alloc(&g->l_out_dst, M));
+    PRH_CHECK(alloc(&g->l_out_dst, g->l_cap));
   }
   PRH_HIP(c, hipMemcpyAsync(g->perm, perm.data(), size_t(N) * 4, hipMemcpyHostToDevice, s));
   PRH_HIP(c, hipMemcpyAsync(g->l_in_off, loff.data(), (size_t(N) + 1) * 4, hipMemcpyHostToDevice, s));
   if (with_out) PRH_HIP(c, hipMemcpyAsync(g->l_out_off, oout.data(), (size_t(N) + 1) * 4, hipMemcpyHostToDevice, s));
   PRH_HIP(c, prh::launch_invert(g->perm, N, g->inv, s));
   PRH_HIP(c, prh::launch_gather_u32(g->outdeg, g->perm, N, g->l_outdeg, s));
-  if (with_out) PRH_HIP(c, prh::launch_relabel_rows(g->out_off, g->out_dst, g->perm, g->inv, g->l_out_off, N,
-                                                    g->l_out_dst, s));
-  // Relabelled in-rows into a scratch array, then sorted per row into place.
+  // Relabelled in-rows (and out-rows) into a scratch array, then sorted per row into place: in-rows by new source
+  // id for the rank kernels' gathers, out-rows by new target id for the batch tidy's binary searches.
   uint32_t *tmp = nullptr;
   void *sort_tmp = nullptr;
   size_t sort_bytes = 0;
@@ -763,6 +764,9 @@ int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uin
   if (st == PRH_SUCCESS) st = dev_alloc(c, &sort_tmp, sort_bytes);
   if (st == PRH_SUCCESS) {
     hipError_t e = prh::sort_rows(sort_tmp, sort_bytes, tmp, g->l_in_src, M, N, g->l_in_off, s);
+    if (e == hipSuccess && with_out)
+      e = prh::launch_relabel_rows(g->out_off, g->out_dst, g->perm, g->inv, g->l_out_off, N, tmp, s);
+    if (e == hipSuccess && with_out) e = prh::sort_rows(sort_tmp, sort_bytes, tmp, g->l_out_dst, M, N, g->l_out_off, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) st = fail(c, PRH_ERROR_DEVICE, "layout row sort: %s", hipGetErrorString(e));
   }
@@ -770,6 +774,7 @@ int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uin
   dev_free(c, sort_tmp, sort_bytes);
   dev_free(c, tmp, (M ? M : 4) * 4);
   PRH_CHECK(st);
+  g->out_sorted = with_out;
   return PRH_SUCCESS;
 }
 

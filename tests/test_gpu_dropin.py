@@ -115,7 +115,7 @@ def test_dropin_large_graph_takes_autotuned_plan(tmp_path, oracle_mod):
     for r, got in zip(rows, ranks):
         assert abs(int(r["iterations"]) - it) <= 1
         assert np.abs(got - ref).max() <= 1e-9
-        assert r["init"] >= r["time"] - r["comp"] - 1e-3  # time = init + comp (no marking for Static)
+        assert r["init"] >= r["time"] - r["comp"] - 0.05  # time = init + comp (+ an empty marking phase)
 
 
 def test_dropin_deterministic_keeps_row_plan(tmp_path, oracle_mod):
