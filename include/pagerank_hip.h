@@ -182,6 +182,17 @@ int prh_solve_device(prh_context *context, const prh_graph *graph, prh_method me
 int prh_partition(prh_context *context, uint32_t n, const uint32_t *offsets, uint32_t switch_degree,
                   uint32_t *order, uint32_t *partition_point);
 
+/* prh_partition with every array in device memory of the context's device and no device allocation: the partition's
+ * scratch (per-tile counts and offsets) is a fixed array of 1.5 MiB that the library keeps on each device until the
+ * process ends. No counterpart in the reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). order receives N vertex ids and
+ * *partition_point the size of the first part (one uint32 in device memory). The work is enqueued on the context's
+ * stream (prh_context_stream) and the call returns without waiting for it. At most 2^28 vertices
+ * (PRH_ERROR_INVALID_ARGUMENT above that). Calls on one device share the scratch: calls that overlap in time on one
+ * device (other streams, contexts or threads) race and must be ordered by the caller. */
+int prh_partition_noalloc(prh_context *context, uint32_t n, const uint32_t *offsets, uint32_t switch_degree,
+                          uint32_t *order, uint32_t *partition_point);
+
 /* Affected flags right after the marking phase of a dynamic method (host pointers; affected receives N bytes,
  * 0 or 1). PRH_DYNAMIC_TRAVERSAL gives the traversal closure (pagerankAffectedTraversalCuW,
  * pagerankCuda.hxx:438-456); PRH_DYNAMIC_FRONTIER and PRH_PRUNE_DYNAMIC_FRONTIER give the initial frontier

@@ -123,6 +123,10 @@ int prh_context_create(int device, prh_context **out) {
       prh_context_destroy(c);
       return PRH_ERROR_DEVICE;
     }
+  if (noalloc_setup(c) != PRH_SUCCESS) {
+    prh_context_destroy(c);
+    return PRH_ERROR_DEVICE;
+  }
   *out = c;
   return PRH_SUCCESS;
 }

@@ -96,6 +96,7 @@ struct prh_context {
   // one by one, for tools that trace kernels.
   bool capture = true;
   uint8_t *staging = nullptr;  // host-pinned staging area of large device-to-host copies (kStagingBytes), on demand
+  uint32_t *noalloc_scratch = nullptr;  // device address of noalloc.hip's fixed scratch on this device
   hipEvent_t ev[4] = {};
   bool ktiming = false;
   std::vector<hipEvent_t> kev;  // start/stop pairs per iteration
@@ -271,6 +272,9 @@ int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot = false);
 int build_binned_plan_t(prh_context *c, prh_graph *g);
 int build_rank_plan(prh_context *c, prh_graph *g);
 int out_schedule(prh_context *c, prh_graph *g, prh::ExtendArgs &E);
+
+// noalloc.hip
+int noalloc_setup(prh_context *c);
 
 // apply.hip
 int csr_to_caller(prh_context *c, const uint32_t *perm_d, const uint32_t *inv_d, uint32_t N, const uint32_t *loff_d,

@@ -71,6 +71,7 @@ ENGINE_SIGNATURES = {
     "prh_solve_device": (C.c_int, [vp, vp, C.c_int, C.c_int, C.POINTER(prh_options), vp, vp, C.c_size_t, vp,
                                    C.c_size_t, vp, vp, C.POINTER(prh_stats)]),
     "prh_partition": (C.c_int, [vp, C.c_uint32, u32p, C.c_uint32, u32p, u32p]),
+    "prh_partition_noalloc": (C.c_int, [vp, C.c_uint32, vp, C.c_uint32, vp, vp]),
     "prh_mark_affected": (C.c_int, [vp, vp, C.c_int, u32p, u32p, C.c_size_t, u32p, C.c_size_t, u8p]),
     "prh_ppr_batched": (C.c_int, [vp, vp, C.c_int, u32p, u32p, C.POINTER(prh_options), f64p, C.POINTER(C.c_int),
                                   C.POINTER(prh_stats)]),
