@@ -25,31 +25,13 @@
 namespace {
 
 using prhc::fail;
+using prhc::Temps;
 
 __global__ void count_zero_kernel(const uint32_t *deg, uint32_t N, uint32_t *out) {
   uint32_t z = 0;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) z += deg[v] == 0;
   if (z) atomicAdd(out, z);
 }
-
-// Device temporaries of one update, freed when it ends.
-struct Temps {
-  prh_context *c;
-  std::vector<std::pair<void *, size_t>> v;
-  explicit Temps(prh_context *cc) : c(cc) {}
-  ~Temps() {
-    for (auto &b : v) prhc::dev_free(c, b.first, b.second);
-  }
-  template <class T>
-  int get(T **p, size_t n) {
-    void *q = nullptr;
-    const size_t bytes = std::max<size_t>(n * sizeof(T), 16);
-    PRH_CHECK(prhc::dev_alloc(c, &q, bytes));
-    v.emplace_back(q, bytes);
-    *p = static_cast<T *>(q);
-    return PRH_SUCCESS;
-  }
-};
 
 // The tidied keys of one CSR side: keys sorted, first of each key whose edge exists (deletions) or is absent
 // (insertions), compacted into out with their count in *count.
@@ -286,9 +268,11 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   PRH_CHECK(ensure(c, c->upd_delta, N));
   PRH_CHECK(ensure(c, c->upd_changed, N));
   PRH_CHECK(ensure(c, c->upd_deg, N1));
+  PRH_CHECK(ensure(c, c->upd_ranges, N));
   int *delta = c->upd_delta.p;
   uint8_t *changed = c->upd_changed.p;
   uint32_t *new_deg = c->upd_deg.p;
+  uint4 *ranges = c->upd_ranges.p;
   // The new CSRs go into the spare arrays an earlier update left, when they hold M2 edges; else into new arrays with
   // room for growth (1/256 of the edges), which later updates reuse.
   uint32_t *n_in_off = nullptr, *n_in_src = nullptr, *n_out_off = nullptr, *n_out_dst = nullptr;
@@ -319,12 +303,12 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   auto apply_csr = [&](int kd, int ki, const uint32_t *off, const uint32_t *idx, uint32_t *noff, uint32_t *nidx) -> int {
     PRH_HIP(c, prh::launch_fill_i32(delta, 0, N, s));
     PRH_HIP(c, prh::launch_fill_u8(changed, 0, N, s));
-    PRH_HIP(c, prh::launch_row_delta(compact[kd], counts + kd, compact[ki], counts + ki, cd + ci, delta, changed, s));
+    PRH_HIP(c, prh::launch_row_ranges(compact[kd], counts + kd, compact[ki], counts + ki, cd + ci, delta, changed,
+                                      ranges, s));
     PRH_HIP(c, prh::launch_new_degrees(off, delta, N, new_deg, flags, 0, s));
     PRH_HIP(c, prh::scan_u32(scan_tmp, scan_bytes, new_deg, noff, N1, s));
-    PRH_HIP(c, prh::launch_rewrite(off, idx, noff, changed, compact[kd], counts + kd, compact[ki], counts + ki, N, M,
-                                   nidx, s));
-    PRH_HIP(c, prh::launch_insert(off, idx, noff, compact[kd], counts + kd, compact[ki], counts + ki, ci, nidx, s));
+    PRH_HIP(c, prh::launch_rewrite(off, idx, noff, changed, ranges, compact[kd], compact[ki], N, M, nidx, s));
+    PRH_HIP(c, prh::launch_insert(off, idx, noff, ranges, compact[kd], compact[ki], counts + ki, ci, nidx, s));
     return PRH_SUCCESS;
   };
   if (st == PRH_SUCCESS) st = apply_csr(0, 1, g->l_in_off, g->l_in_src, n_in_off, n_in_src);

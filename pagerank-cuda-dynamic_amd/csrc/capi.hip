@@ -49,6 +49,36 @@ void dev_free(prh_context *c, void *p, size_t bytes) {
   c->used -= std::min(c->used, bytes ? bytes : 16);
 }
 
+Temps::Temps(prh_context *cc) : c(cc), mark(cc->arena_top) {}
+
+Temps::~Temps() {
+  for (auto &b : own) dev_free(c, b.first, b.second);
+  c->arena_top = mark;
+  if (mark == 0 && c->arena_want > c->arena.cap) {
+    // the work that used the arena has to end before its memory is released
+    if (c->arena.p && hipStreamSynchronize(c->stream) != hipSuccess) (void)hipGetLastError();
+    release(c, c->arena);
+    const std::string keep = c->error;  // the arena is an optimisation: a failed regrow reports nothing
+    if (ensure(c, c->arena, std::min(c->arena_want, kArenaMax)) != PRH_SUCCESS) c->error = keep;
+  }
+}
+
+int Temps::get_bytes(void **p, size_t bytes) {
+  const size_t size = (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+  if (bytes <= kArenaItem) {
+    const size_t top = c->arena_top + size;
+    c->arena_want = std::max(c->arena_want, top);
+    if (c->arena.p && top <= c->arena.cap) {
+      *p = c->arena.p + c->arena_top;
+      c->arena_top = top;
+      return PRH_SUCCESS;
+    }
+  }
+  PRH_CHECK(dev_alloc(c, p, bytes));
+  own.emplace_back(*p, bytes);
+  return PRH_SUCCESS;
+}
+
 int copy_to_host(prh_context *c, void *dst, const void *src, size_t n) {
   hipStream_t s = c->stream;
   if (n <= kStagingBytes) {
@@ -151,6 +181,8 @@ void prh_context_destroy(prh_context *c) {
   release(c, c->upd_delta);
   release(c, c->upd_changed);
   release(c, c->upd_deg);
+  release(c, c->upd_ranges);
+  release(c, c->arena);
   release(c, c->batch);
   release(c, c->scratch_u32);
   release(c, c->loop);

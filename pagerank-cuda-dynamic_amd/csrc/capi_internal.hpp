@@ -77,6 +77,11 @@ struct prh_context {
   prhc::DevBuf<int> upd_delta;
   prhc::DevBuf<uint8_t> upd_changed;
   prhc::DevBuf<uint32_t> upd_deg;
+  prhc::DevBuf<uint4> upd_ranges;
+  // Arena of the updates' small temporaries (kept between calls; see prhc::Temps): in use up to arena_top, grown to
+  // arena_want when a call asked for more than it holds.
+  prhc::DevBuf<uint8_t> arena;
+  size_t arena_top = 0, arena_want = 0;
   // batched personalized PageRank
   prhc::DevBuf<double> ppr_rank[2], ppr_contrib[2], ppr_partial, ppr_mass, ppr_out;
   prhc::DevBuf<int32_t> ppr_row;
@@ -206,6 +211,30 @@ int ensure(prh_context *c, DevBuf<T> &b, size_t n) {
   b.cap = n;
   return PRH_SUCCESS;
 }
+
+// Device temporaries of one call. A request of at most kArenaItem bytes is carved from the context's arena when it
+// fits there; other requests are allocated here and freed when the Temps ends. Nested Temps end in reverse order of
+// creation. When the outermost ends, an arena smaller than the largest total of small requests a call made is
+// regrown (at most kArenaMax), so that repeated calls of one size allocate nothing.
+constexpr size_t kArenaItem = size_t(64) << 20, kArenaMax = size_t(512) << 20, kArenaAlign = 256;
+
+struct Temps {
+  prh_context *c;
+  size_t mark;
+  std::vector<std::pair<void *, size_t>> own;
+  explicit Temps(prh_context *cc);
+  ~Temps();
+  Temps(const Temps &) = delete;
+  Temps &operator=(const Temps &) = delete;
+  int get_bytes(void **p, size_t bytes);
+  template <class T>
+  int get(T **p, size_t n) {
+    void *q = nullptr;
+    PRH_CHECK(get_bytes(&q, std::max<size_t>(n * sizeof(T), 16)));
+    *p = static_cast<T *>(q);
+    return PRH_SUCCESS;
+  }
+};
 
 template <class T>
 void release(prh_context *c, DevBuf<T> &b) {

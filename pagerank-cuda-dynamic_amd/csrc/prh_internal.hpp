@@ -394,16 +394,17 @@ hipError_t launch_batch_valid(const unsigned long long *keys, uint32_t cap, cons
                               bool want_exists, uint32_t *valid, hipStream_t s);
 hipError_t launch_batch_compact(const unsigned long long *keys, const uint32_t *valid, const uint32_t *pos,
                                 uint32_t cap, unsigned long long *out, uint32_t *count, hipStream_t s);
-hipError_t launch_row_delta(const unsigned long long *dkeys, const uint32_t *nd, const unsigned long long *ikeys,
-                            const uint32_t *ni, uint32_t cap_total, int *delta, uint8_t *changed, hipStream_t s);
+hipError_t launch_row_ranges(const unsigned long long *dkeys, const uint32_t *nd, const unsigned long long *ikeys,
+                             const uint32_t *ni, uint32_t cap_total, int *delta, uint8_t *changed, uint4 *ranges,
+                             hipStream_t s);
 hipError_t launch_new_degrees(const uint32_t *off, const int *delta, uint32_t N, uint32_t *deg, int *flags,
                               int empty_bit, hipStream_t s);
 hipError_t launch_rewrite(const uint32_t *old_off, const uint32_t *old_idx, const uint32_t *new_off,
-                          const uint8_t *changed, const unsigned long long *dkeys, const uint32_t *nd,
-                          const unsigned long long *ikeys, const uint32_t *ni, uint32_t N, uint64_t max_edges,
-                          uint32_t *new_idx, hipStream_t s);
+                          const uint8_t *changed, const uint4 *ranges, const unsigned long long *dkeys,
+                          const unsigned long long *ikeys, uint32_t N, uint64_t max_edges, uint32_t *new_idx,
+                          hipStream_t s);
 hipError_t launch_insert(const uint32_t *old_off, const uint32_t *old_idx, const uint32_t *new_off,
-                         const unsigned long long *dkeys, const uint32_t *nd, const unsigned long long *ikeys,
+                         const uint4 *ranges, const unsigned long long *dkeys, const unsigned long long *ikeys,
                          const uint32_t *ni, uint32_t cap, uint32_t *new_idx, hipStream_t s);
 hipError_t launch_degrees(const uint32_t *off, uint32_t N, uint32_t *deg, hipStream_t s);
 hipError_t launch_refresh_chunks(const uint32_t *item_row, const uint32_t *row_first, uint32_t n_items,

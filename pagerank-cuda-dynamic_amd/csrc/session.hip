@@ -48,6 +48,7 @@ struct prh_stream {
   uint32_t sched_low = 0, sched_high = 0;
   int *delta = nullptr;
   uint8_t *changed = nullptr;
+  uint4 *ranges = nullptr;  // per changed row: its deletions and insertions in the compacted keys
   uint32_t *h_batch = nullptr;  // pinned: del_u, del_v, ins_u, ins_v [cap_batch each], nd, ni
   uint32_t *d_batch = nullptr;
   unsigned long long *keys[4] = {};     // raw keys: in-CSR del, in-CSR ins, out-CSR del, out-CSR ins
@@ -129,14 +130,14 @@ int stream_apply_csr(prh_stream *S, hipStream_t s, int kd, int ki, const uint32_
   }
   PRH_HIP(c, prh::launch_fill_i32(S->delta, 0, N, s));
   PRH_HIP(c, prh::launch_fill_u8(S->changed, 0, N, s));
-  PRH_HIP(c, prh::launch_row_delta(S->compact[kd], S->counts + kd, S->compact[ki], S->counts + ki, 2 * cap, S->delta,
-                                   S->changed, s));
+  PRH_HIP(c, prh::launch_row_ranges(S->compact[kd], S->counts + kd, S->compact[ki], S->counts + ki, 2 * cap,
+                                    S->delta, S->changed, S->ranges, s));
   PRH_HIP(c, prh::launch_new_degrees(off, S->delta, N, S->new_deg, S->ctl + 3, empty_bit, s));
   PRH_HIP(c, prh::scan_u32(S->scan_tmp, S->scan_bytes, S->new_deg, noff, size_t(N) + 1, s));
-  PRH_HIP(c, prh::launch_rewrite(off, idx, noff, S->changed, S->compact[kd], S->counts + kd, S->compact[ki],
-                                 S->counts + ki, N, S->cap_edges, nidx, s));
-  PRH_HIP(c, prh::launch_insert(off, idx, noff, S->compact[kd], S->counts + kd, S->compact[ki], S->counts + ki, cap,
-                                nidx, s));
+  PRH_HIP(c, prh::launch_rewrite(off, idx, noff, S->changed, S->ranges, S->compact[kd], S->compact[ki], N,
+                                 S->cap_edges, nidx, s));
+  PRH_HIP(c, prh::launch_insert(off, idx, noff, S->ranges, S->compact[kd], S->compact[ki], S->counts + ki, cap, nidx,
+                                s));
   return PRH_SUCCESS;
 }
 
@@ -318,6 +319,7 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
   step(stream_alloc(S, &S->sched_order, N));
   step(stream_alloc(S, &S->delta, N));
   step(stream_alloc(S, &S->changed, N));
+  step(stream_alloc(S, &S->ranges, N));
   step(stream_alloc(S, &S->d_batch, 4 * size_t(max_batch) + 2));
   for (int k = 0; k < 4; ++k) {
     step(stream_alloc(S, &S->keys[k], max_batch));

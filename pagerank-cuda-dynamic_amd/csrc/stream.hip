@@ -12,7 +12,7 @@
 //                  absent (insertions) in the current rows (binary search)
 //   (scan)         exclusive scan of the survivor flags
 //   batch_compact  survivors packed in key order, their count to a device word
-//   row_delta      per-row degree change and a "changed" flag
+//   row_ranges     per changed row: its deletions and insertions in the compacted keys, degree change, flag
 //   new_degrees    old degree + change (the out-CSR also flags rows left with no out-edge)
 //   (scan)         new offsets
 //   rewrite        every old edge to its new place: unchanged rows shift, changed rows skip deleted columns and
@@ -28,6 +28,7 @@ namespace prh {
 namespace {
 
 constexpr unsigned long long kNoKey = ~0ull;
+constexpr unsigned long long kRowMask = 0xFFFFFFFF00000000ull;
 
 inline uint32_t grid_for(size_t n, size_t per, uint32_t cap) {
   const size_t g = (n + per - 1) / per;
@@ -93,13 +94,29 @@ __global__ __launch_bounds__(kBlock) void batch_compact_kernel(const unsigned lo
   }
 }
 
-__global__ __launch_bounds__(kBlock) void row_delta_kernel(const unsigned long long *dkeys, const uint32_t *nd,
-                                                           const unsigned long long *ikeys, const uint32_t *ni,
-                                                           int *delta, uint8_t *changed) {
+// The first entry of a row on either side finds the row's deletions [x, y) and insertions [z, w) in the two
+// compacted key arrays and writes them with the row's degree change and its "changed" flag. A row with entries on
+// both sides is written twice with the same values; rows without entries are left alone (delta and changed are
+// cleared beforehand, the ranges are read only where changed is set).
+__global__ __launch_bounds__(kBlock) void row_ranges_kernel(const unsigned long long *dkeys, const uint32_t *nd,
+                                                            const unsigned long long *ikeys, const uint32_t *ni,
+                                                            int *delta, uint8_t *changed, uint4 *ranges) {
   const uint32_t a = *nd, b = *ni;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < a + b; i += gridDim.x * kBlock) {
-    const uint32_t row = uint32_t((i < a ? dkeys[i] : ikeys[i - a]) >> 32);
-    atomicAdd(&delta[row], i < a ? -1 : 1);
+    const bool del = i < a;
+    const unsigned long long *k = del ? dkeys : ikeys;
+    const uint32_t j = del ? i : i - a;
+    const unsigned long long rk = k[j] & kRowMask;
+    if (j > 0 && (k[j - 1] & kRowMask) == rk) continue;
+    const unsigned long long next = rk + (1ull << 32);  // rows are below 2^32 - 1
+    uint4 r;
+    r.x = lower_bound(dkeys, 0u, a, rk);
+    r.y = lower_bound(dkeys, r.x, a, next);
+    r.z = lower_bound(ikeys, 0u, b, rk);
+    r.w = lower_bound(ikeys, r.z, b, next);
+    const uint32_t row = uint32_t(rk >> 32);
+    ranges[row] = r;
+    delta[row] = int(r.w - r.z) - int(r.y - r.x);
     changed[row] = 1;
   }
 }
@@ -120,12 +137,12 @@ __global__ __launch_bounds__(kBlock) void new_degrees_kernel(const uint32_t *off
 constexpr uint32_t kRewriteTile = 4096;  // old edges per workgroup
 
 // Workgroup b moves old edges [b * tile, (b + 1) * tile): each edge finds its row by binary search in the offsets
-// of the rows that overlap the tile.
+// of the rows that overlap the tile; an edge of a changed row searches only its row's deletions and insertions.
 __global__ __launch_bounds__(kBlock) void rewrite_kernel(const uint32_t *old_off, const uint32_t *old_idx,
                                                          const uint32_t *new_off, const uint8_t *changed,
-                                                         const unsigned long long *dkeys, const uint32_t *nd,
-                                                         const unsigned long long *ikeys, const uint32_t *ni,
-                                                         uint32_t N, uint32_t *new_idx) {
+                                                         const uint4 *ranges, const unsigned long long *dkeys,
+                                                         const unsigned long long *ikeys, uint32_t N,
+                                                         uint32_t *new_idx) {
   const uint32_t M = old_off[N];
   const uint32_t e0 = blockIdx.x * kRewriteTile;
   if (e0 >= M) return;
@@ -133,7 +150,6 @@ __global__ __launch_bounds__(kBlock) void rewrite_kernel(const uint32_t *old_off
   // rows overlapping [e0, e1): from the row holding e0 to the row holding e1 - 1
   const uint32_t r_first = lower_bound(old_off, 0u, N + 1, e0 + 1) - 1;
   const uint32_t r_last = lower_bound(old_off, r_first, N + 1, e1) - 1;
-  const uint32_t D = *nd, I = *ni;
   for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
     const uint32_t row = lower_bound(old_off, r_first, r_last + 1, e + 1) - 1;
     const uint32_t col = old_idx[e];
@@ -142,27 +158,29 @@ __global__ __launch_bounds__(kBlock) void rewrite_kernel(const uint32_t *old_off
       new_idx[new_off[row] + k] = col;
       continue;
     }
-    const unsigned long long rk = (unsigned long long)row << 32, key = rk | col;
-    const uint32_t d_lo = lower_bound(dkeys, 0u, D, rk), d_at = lower_bound(dkeys, d_lo, D, key);
-    if (d_at < D && dkeys[d_at] == key) continue;  // deleted
-    const uint32_t i_lo = lower_bound(ikeys, 0u, I, rk), i_at = lower_bound(ikeys, i_lo, I, key);
-    new_idx[new_off[row] + k - (d_at - d_lo) + (i_at - i_lo)] = col;
+    const uint4 r = ranges[row];
+    const unsigned long long key = (unsigned long long)row << 32 | col;
+    const uint32_t d_at = lower_bound(dkeys, r.x, r.y, key);
+    if (d_at < r.y && dkeys[d_at] == key) continue;  // deleted
+    const uint32_t i_at = lower_bound(ikeys, r.z, r.w, key);
+    new_idx[new_off[row] + k - (d_at - r.x) + (i_at - r.z)] = col;
   }
 }
 
 __global__ __launch_bounds__(kBlock) void insert_kernel(const uint32_t *old_off, const uint32_t *old_idx,
-                                                        const uint32_t *new_off, const unsigned long long *dkeys,
-                                                        const uint32_t *nd, const unsigned long long *ikeys,
-                                                        const uint32_t *ni, uint32_t *new_idx) {
-  const uint32_t D = *nd, I = *ni;
+                                                        const uint32_t *new_off, const uint4 *ranges,
+                                                        const unsigned long long *dkeys,
+                                                        const unsigned long long *ikeys, const uint32_t *ni,
+                                                        uint32_t *new_idx) {
+  const uint32_t I = *ni;
   for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < I; j += gridDim.x * kBlock) {
-    const unsigned long long key = ikeys[j], rk = key & 0xFFFFFFFF00000000ull;
+    const unsigned long long key = ikeys[j];
     const uint32_t row = uint32_t(key >> 32), col = uint32_t(key);
-    const uint32_t i_lo = lower_bound(ikeys, 0u, I, rk);
+    const uint4 r = ranges[row];
     const uint32_t b = old_off[row], e = old_off[row + 1];
     const uint32_t o_at = lower_bound(old_idx, b, e, col) - b;
-    const uint32_t d_lo = lower_bound(dkeys, 0u, D, rk), d_at = lower_bound(dkeys, d_lo, D, key);
-    new_idx[new_off[row] + o_at - (d_at - d_lo) + (j - i_lo)] = col;
+    const uint32_t d_at = lower_bound(dkeys, r.x, r.y, key);
+    new_idx[new_off[row] + o_at - (d_at - r.x) + (j - r.z)] = col;
   }
 }
 
@@ -257,9 +275,10 @@ hipError_t launch_batch_compact(const unsigned long long *keys, const uint32_t *
   return hipGetLastError();
 }
 
-hipError_t launch_row_delta(const unsigned long long *dkeys, const uint32_t *nd, const unsigned long long *ikeys,
-                            const uint32_t *ni, uint32_t cap_total, int *delta, uint8_t *changed, hipStream_t s) {
-  row_delta_kernel<<<grid_for(cap_total, kBlock, 1024), kBlock, 0, s>>>(dkeys, nd, ikeys, ni, delta, changed);
+hipError_t launch_row_ranges(const unsigned long long *dkeys, const uint32_t *nd, const unsigned long long *ikeys,
+                             const uint32_t *ni, uint32_t cap_total, int *delta, uint8_t *changed, uint4 *ranges,
+                             hipStream_t s) {
+  row_ranges_kernel<<<grid_for(cap_total, kBlock, 1024), kBlock, 0, s>>>(dkeys, nd, ikeys, ni, delta, changed, ranges);
   return hipGetLastError();
 }
 
@@ -270,19 +289,19 @@ hipError_t launch_new_degrees(const uint32_t *off, const int *delta, uint32_t N,
 }
 
 hipError_t launch_rewrite(const uint32_t *old_off, const uint32_t *old_idx, const uint32_t *new_off,
-                          const uint8_t *changed, const unsigned long long *dkeys, const uint32_t *nd,
-                          const unsigned long long *ikeys, const uint32_t *ni, uint32_t N, uint64_t max_edges,
-                          uint32_t *new_idx, hipStream_t s) {
+                          const uint8_t *changed, const uint4 *ranges, const unsigned long long *dkeys,
+                          const unsigned long long *ikeys, uint32_t N, uint64_t max_edges, uint32_t *new_idx,
+                          hipStream_t s) {
   const uint64_t g = (max_edges + kRewriteTile - 1) / kRewriteTile;
-  rewrite_kernel<<<uint32_t(g ? g : 1), kBlock, 0, s>>>(old_off, old_idx, new_off, changed, dkeys, nd, ikeys, ni, N,
+  rewrite_kernel<<<uint32_t(g ? g : 1), kBlock, 0, s>>>(old_off, old_idx, new_off, changed, ranges, dkeys, ikeys, N,
                                                         new_idx);
   return hipGetLastError();
 }
 
 hipError_t launch_insert(const uint32_t *old_off, const uint32_t *old_idx, const uint32_t *new_off,
-                         const unsigned long long *dkeys, const uint32_t *nd, const unsigned long long *ikeys,
+                         const uint4 *ranges, const unsigned long long *dkeys, const unsigned long long *ikeys,
                          const uint32_t *ni, uint32_t cap, uint32_t *new_idx, hipStream_t s) {
-  insert_kernel<<<grid_for(cap, kBlock, 1024), kBlock, 0, s>>>(old_off, old_idx, new_off, dkeys, nd, ikeys, ni,
+  insert_kernel<<<grid_for(cap, kBlock, 1024), kBlock, 0, s>>>(old_off, old_idx, new_off, ranges, dkeys, ikeys, ni,
                                                                new_idx);
   return hipGetLastError();
 }

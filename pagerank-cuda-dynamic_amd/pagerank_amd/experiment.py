@@ -26,8 +26,9 @@ tidied against the current graph. --rows and --size are the reference's argument
 
 stream: 16 (--batches) consecutive DF-P batches of one size (--fractions, first value) through the streaming session
 (prh_stream: batch applied on the device, apply and iterations replayed from captured HIP graphs), against the same
-batches as separate calls (host-side apply, upload of the new graph, DF-P solve) and against the DF-P solve times
-alone. Every stream result is compared with the separate call's.
+batches as separate calls (host-side apply, upload of the new graph, DF-P solve), against the DF-P solve times
+alone, and against in-place updates of one device graph with a DF-P solve each (prh_graph_update, not captured).
+The final ranks of every path are compared with the separate calls'.
 
 ppr: one batched personalized solve of k teleport columns, each with a seed set of uniform random size 1..1000,
 against k single-column solves and k plain static solves on the same graph.
@@ -269,6 +270,20 @@ def run_stream(args, out: CsvLog) -> int:
         q = r.ranks
         results.append(r)
     sep_wall = (time.perf_counter() - t0) * 1e3
+    # in-place updates of one device graph (prh_graph_update) and a DF-P solve each, not captured
+    ug = ctx.upload(x)
+    solve(ug, engine.STATIC, PagerankOptions())  # builds the plan the updates patch (its time is not counted)
+    qu = q0
+    t0 = time.perf_counter()
+    upd_ms, upd_iters = 0.0, 0
+    for b, _ in seq:
+        upd_ms += ug.update(b)
+        r = solve(ug, engine.PRUNE_DYNAMIC_FRONTIER, PagerankOptions(), batch=b, q=qu)
+        upd_iters += r.iterations
+        qu = r.ranks
+    upd_wall = (time.perf_counter() - t0) * 1e3
+    ug.close()
+    worst_upd = float(np.abs(qu - q).max())
     worst = float(np.abs(final_stream - q).max())
     s_iters = sum(r.iterations for r in stream_res)
     s_apply = sum(r.initializationTime for r in stream_res)
@@ -278,8 +293,11 @@ def run_stream(args, out: CsvLog) -> int:
     log(f"separate calls: {sep_wall:.2f} ms wall (upload + layout + solve; host-side apply not counted), "
         f"DF-P solves alone {sep_solve_ms:.2f} ms, {sep_iters} iterations; stream speedup {sep_wall / stream_wall:.2f}x "
         f"(vs solves alone {sep_solve_ms / stream_wall:.2f}x); L-inf(final stream - final separate) {worst:.2e}")
+    log(f"in-place updates: {upd_wall:.2f} ms wall (update {upd_ms:.2f} ms + DF-P solve per batch), {upd_iters} "
+        f"iterations; stream speedup {upd_wall / stream_wall:.2f}x; L-inf(final update path - final separate) "
+        f"{worst_upd:.2e}")
     for tech, t, its in (("dfp_stream", stream_wall, s_iters), ("dfp_separate_calls", sep_wall, sep_iters),
-                         ("dfp_solves_only", sep_solve_ms, sep_iters)):
+                         ("dfp_solves_only", sep_solve_ms, sep_iters), ("dfp_inplace_updates", upd_wall, upd_iters)):
         out.row(mode="stream", graph=name, vertices=x.N, edges=x.M, batch_fraction=frac, technique=tech, time_ms=t,
                 iterations=its, extra=f"batches={args.batches};linf={worst:.3e}")
     return 0
