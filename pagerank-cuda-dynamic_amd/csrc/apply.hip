@@ -60,8 +60,9 @@ int patch_plan(prh_context *c, prh_graph *g, const unsigned long long *dkeys, co
   const uint32_t N = g->N;
   const uint32_t B = uint32_t(bc.B);
   const uint32_t nb = (N + B - 1) / B;
-  if (!bc.slot_of) {  // slot map inverse: base in-CSR slot -> plan slot
+  if (!bc.slot_of) {  // slot map inverse: base in-CSR slot -> plan slot (built with the plan when it has room)
     PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&bc.slot_of), std::max<size_t>(bc.base_M, 4) * 4));
+    bc.slot_of_n = bc.base_M;
     if (bc.base_M) PRH_HIP(c, prh::launch_invert(bc.plan.slot, uint32_t(bc.base_M), bc.slot_of, s));
   }
   Temps tmp(c);
@@ -237,6 +238,8 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   PRH_HIP(c, prh::scan_u32(nullptr, scan_bytes, valid, pos, cap, s));
   PRH_HIP(c, prh::scan_u32(nullptr, b1, valid, pos, N1, s));
   scan_bytes = std::max(scan_bytes, b1);
+  PRH_HIP(c, prh::scan_u8(nullptr, b1, nullptr, pos, N, s));
+  scan_bytes = std::max(scan_bytes, b1);
   uint8_t *sort_tmp = nullptr, *scan_tmp = nullptr;
   PRH_CHECK(tmp.get(&sort_tmp, sort_bytes));
   PRH_CHECK(tmp.get(&scan_tmp, scan_bytes));
@@ -273,6 +276,10 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   uint8_t *changed = c->upd_changed.p;
   uint32_t *new_deg = c->upd_deg.p;
   uint4 *ranges = c->upd_ranges.p;
+  // rewrite units: one per changed row and a closing one (a CSR has at most cd + ci changed rows)
+  const uint32_t max_units = cd + ci + 1;
+  uint32_t *units = nullptr;
+  PRH_CHECK(tmp.get(&units, 4 * size_t(max_units) + 1));
   // The new CSRs go into the spare arrays an earlier update left, when they hold M2 edges; else into new arrays with
   // room for growth (1/256 of the edges), which later updates reuse.
   uint32_t *n_in_off = nullptr, *n_in_src = nullptr, *n_out_off = nullptr, *n_out_dst = nullptr;
@@ -307,7 +314,10 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
                                       ranges, s));
     PRH_HIP(c, prh::launch_new_degrees(off, delta, N, new_deg, flags, 0, s));
     PRH_HIP(c, prh::scan_u32(scan_tmp, scan_bytes, new_deg, noff, N1, s));
-    PRH_HIP(c, prh::launch_rewrite(off, idx, noff, changed, ranges, compact[kd], compact[ki], N, M, nidx, s));
+    uint32_t *chpos = reinterpret_cast<uint32_t *>(delta);  // the degree changes are spent
+    PRH_HIP(c, prh::scan_u8(scan_tmp, scan_bytes, changed, chpos, N, s));
+    PRH_HIP(c, prh::launch_rewrite(off, idx, noff, changed, chpos, units, ranges, compact[kd], compact[ki], N,
+                                   max_units, M, nidx, s));
     PRH_HIP(c, prh::launch_insert(off, idx, noff, ranges, compact[kd], compact[ki], counts + ki, ci, nidx, s));
     return PRH_SUCCESS;
   };

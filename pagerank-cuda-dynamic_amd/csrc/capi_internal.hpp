@@ -118,6 +118,7 @@ struct BinnedCache {
   // inverse of its slot map, and the overlay of inserted edges that are not slots of the plan (capacity ov_cap).
   uint32_t *base_off = nullptr, *base_src = nullptr, *slot_of = nullptr;
   uint64_t base_M = 0, base_cap = 0;  // base edges, and the capacity of base_src
+  uint64_t slot_of_n = 0;             // entries of slot_of (the edges of the graph the plan was built from)
   unsigned long long *ov_key = nullptr;
   uint32_t *ov_off = nullptr;
   double *ovX = nullptr;

@@ -224,10 +224,10 @@ void free_binned(prh_context *c, BinnedCache &bc) {
   const uint32_t N = bc.plan.N;
   dev_free(c, bc.base_off, bc.base_off ? (size_t(N) + 1) * 4 : 0);
   dev_free(c, bc.base_src, bc.base_src ? std::max<size_t>(bc.base_cap, 4) * 4 : 0);
-  dev_free(c, bc.slot_of, bc.slot_of ? std::max<size_t>(bc.base_M, 4) * 4 : 0);
+  dev_free(c, bc.slot_of, bc.slot_of ? std::max<size_t>(bc.slot_of_n, 4) * 4 : 0);
   free_overlay(c, bc);
   bc.base_off = bc.base_src = bc.slot_of = nullptr;
-  bc.base_M = bc.base_cap = 0;
+  bc.base_M = bc.base_cap = bc.slot_of_n = 0;
 }
 
 void free_overlay(prh_context *c, BinnedCache &bc) {
@@ -442,9 +442,23 @@ int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, co
 
 // The plan of the layout in-CSR (rows in in-degree order).
 int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot) {
-  // A graph with its out-CSR can be updated in place, which patches the plan through its slot map: keep it.
-  return build_binned(c, g->N, g->M, g->l_in_off, g->l_in_src, g->sorted_deg, keep_slot || g->l_out_off != nullptr,
-                      g->bin);
+  // A graph with its out-CSR can be updated in place, which patches the plan through its slot map and the map's
+  // inverse (in-CSR edge -> plan slot). Both are built with the plan, as part of its one-time preparation.
+  const bool updatable = g->l_out_off != nullptr;
+  BinnedCache &bc = g->bin;
+  PRH_CHECK(build_binned(c, g->N, g->M, g->l_in_off, g->l_in_src, g->sorted_deg, keep_slot || updatable, bc));
+  if (!updatable || bc.B > 8192 || !bc.plan.slot || bc.slot_of || bc.base_off) return PRH_SUCCESS;
+  prhc::Prep inv;
+  {
+    PrepTimer timer(inv);
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&bc.slot_of), std::max<size_t>(g->M, 4) * 4));
+    bc.slot_of_n = g->M;
+    if (g->M) PRH_HIP(c, prh::launch_invert(bc.plan.slot, uint32_t(g->M), bc.slot_of, c->stream));
+    PRH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  bc.prep.ms = (bc.prep.reported ? 0.0f : bc.prep.ms) + inv.ms;
+  bc.prep.reported = false;
+  return PRH_SUCCESS;
 }
 
 // The plan of the layout out-CSR (the transpose: rows are sources), for the weighted adjoint.

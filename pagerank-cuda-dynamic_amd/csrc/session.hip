@@ -49,6 +49,7 @@ struct prh_stream {
   int *delta = nullptr;
   uint8_t *changed = nullptr;
   uint4 *ranges = nullptr;  // per changed row: its deletions and insertions in the compacted keys
+  uint32_t *units = nullptr;  // the rewrite's runs of rows: 4 (2 cap_batch + 1) + 1 words
   uint32_t *h_batch = nullptr;  // pinned: del_u, del_v, ins_u, ins_v [cap_batch each], nd, ni
   uint32_t *d_batch = nullptr;
   unsigned long long *keys[4] = {};     // raw keys: in-CSR del, in-CSR ins, out-CSR del, out-CSR ins
@@ -134,8 +135,10 @@ int stream_apply_csr(prh_stream *S, hipStream_t s, int kd, int ki, const uint32_
                                     S->delta, S->changed, S->ranges, s));
   PRH_HIP(c, prh::launch_new_degrees(off, S->delta, N, S->new_deg, S->ctl + 3, empty_bit, s));
   PRH_HIP(c, prh::scan_u32(S->scan_tmp, S->scan_bytes, S->new_deg, noff, size_t(N) + 1, s));
-  PRH_HIP(c, prh::launch_rewrite(off, idx, noff, S->changed, S->ranges, S->compact[kd], S->compact[ki], N,
-                                 S->cap_edges, nidx, s));
+  uint32_t *chpos = reinterpret_cast<uint32_t *>(S->delta);  // the degree changes are spent
+  PRH_HIP(c, prh::scan_u8(S->scan_tmp, S->scan_bytes, S->changed, chpos, N, s));
+  PRH_HIP(c, prh::launch_rewrite(off, idx, noff, S->changed, chpos, S->units, S->ranges, S->compact[kd],
+                                 S->compact[ki], N, 2 * cap + 1, S->cap_edges, nidx, s));
   PRH_HIP(c, prh::launch_insert(off, idx, noff, S->ranges, S->compact[kd], S->compact[ki], S->counts + ki, cap, nidx,
                                 s));
   return PRH_SUCCESS;
@@ -320,6 +323,7 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
   step(stream_alloc(S, &S->delta, N));
   step(stream_alloc(S, &S->changed, N));
   step(stream_alloc(S, &S->ranges, N));
+  step(stream_alloc(S, &S->units, 4 * (2 * size_t(max_batch) + 1) + 1));
   step(stream_alloc(S, &S->d_batch, 4 * size_t(max_batch) + 2));
   for (int k = 0; k < 4; ++k) {
     step(stream_alloc(S, &S->keys[k], max_batch));
@@ -346,6 +350,8 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
     if (e == hipSuccess) e = prh::scan_u32(nullptr, b1, S->valid, S->pos, max_batch, s);
     if (e == hipSuccess) e = prh::scan_u32(nullptr, b2, S->new_deg, S->in_off[0], N1, s);
     S->scan_bytes = std::max(b1, b2);
+    if (e == hipSuccess) e = prh::scan_u8(nullptr, b1, S->changed, S->pos, N, s);
+    S->scan_bytes = std::max(S->scan_bytes, b1);
     if (e != hipSuccess) st = fail(c, PRH_ERROR_DEVICE, "stream temporaries: %s", hipGetErrorString(e));
   }
   step(stream_alloc(S, &S->sort_tmp, S->sort_bytes));

@@ -15,11 +15,15 @@
 //   row_ranges     per changed row: its deletions and insertions in the compacted keys, degree change, flag
 //   new_degrees    old degree + change (the out-CSR also flags rows left with no out-edge)
 //   (scan)         new offsets
-//   rewrite        every old edge to its new place: unchanged rows shift, changed rows skip deleted columns and
-//                  make room for inserted ones (ranks found by binary search in the row's deletions/insertions)
+//   (scan)         positions of the changed rows among all changed rows
+//   units          per changed row its old and new offsets: the runs of unchanged rows between changed rows
+//   rewrite        every old edge to its new place: unchanged rows move by their run's shift, changed rows skip
+//                  deleted columns and make room for inserted ones (ranks found by binary search in the row's
+//                  deletions/insertions)
 //   insert         every inserted column to its place in its row
 // Sizes are read from device words, so one captured sequence serves every batch up to the capacity.
 #include <rocprim/device/device_merge_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "prh_device.hpp"
 #include "prh_internal.hpp"
@@ -136,34 +140,81 @@ __global__ __launch_bounds__(kBlock) void new_degrees_kernel(const uint32_t *off
 
 constexpr uint32_t kRewriteTile = 4096;  // old edges per workgroup
 
-// Workgroup b moves old edges [b * tile, (b + 1) * tile): each edge finds its row by binary search in the offsets
-// of the rows that overlap the tile; an edge of a changed row searches only its row's deletions and insertions.
+// Units of the rewrite: unit i (i < C) is the run of unchanged rows before the i-th changed row R_i, then R_i
+// itself; unit C runs from the last changed row's end to the last edge. Per unit: ue = old end of R_i (its edges are
+// [ub, ue)), ub = old start of R_i, unb = new start of R_i, urow = R_i (N for the closing unit, whose ub = ue = M).
+// Every unchanged row of unit i moves by unb - ub (rows between two changed rows keep their degrees).
+struct RewriteUnits {
+  uint32_t *ue, *ub, *unb, *urow;
+  uint32_t *count;  // C
+};
+
+// pos = exclusive scan of changed; the i-th changed row writes unit i, row N - 1 also the closing unit and C.
+__global__ __launch_bounds__(kBlock) void units_kernel(const uint32_t *old_off, const uint32_t *new_off,
+                                                       const uint8_t *changed, const uint32_t *pos, uint32_t N,
+                                                       RewriteUnits U) {
+  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < N; v += gridDim.x * kBlock) {
+    const bool ch = changed[v] != 0;
+    if (ch) {
+      const uint32_t i = pos[v];
+      U.ue[i] = old_off[v + 1];
+      U.ub[i] = old_off[v];
+      U.unb[i] = new_off[v];
+      U.urow[i] = v;
+    }
+    if (v == N - 1) {
+      const uint32_t C = pos[v] + (ch ? 1u : 0u);
+      U.ue[C] = U.ub[C] = old_off[N];
+      U.unb[C] = new_off[N];
+      U.urow[C] = N;
+      *U.count = C;
+    }
+  }
+}
+
+// First unit i in [lo, hi] whose old end ue[i] is above e (the closing unit's end is M, above every edge).
+__device__ __forceinline__ uint32_t unit_of(const uint32_t *ue, uint32_t lo, uint32_t hi, uint32_t e) {
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (ue[mid] <= e) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Workgroup b moves old edges [b * tile, (b + 1) * tile). The tile's units are found once; an edge of an unchanged
+// row moves by its unit's shift, an edge of a changed row searches only that row's deletions and insertions.
 __global__ __launch_bounds__(kBlock) void rewrite_kernel(const uint32_t *old_off, const uint32_t *old_idx,
-                                                         const uint32_t *new_off, const uint8_t *changed,
-                                                         const uint4 *ranges, const unsigned long long *dkeys,
+                                                         RewriteUnits U, const uint4 *ranges,
+                                                         const unsigned long long *dkeys,
                                                          const unsigned long long *ikeys, uint32_t N,
                                                          uint32_t *new_idx) {
   const uint32_t M = old_off[N];
   const uint32_t e0 = blockIdx.x * kRewriteTile;
   if (e0 >= M) return;
   const uint32_t e1 = min(M, e0 + kRewriteTile);
-  // rows overlapping [e0, e1): from the row holding e0 to the row holding e1 - 1
-  const uint32_t r_first = lower_bound(old_off, 0u, N + 1, e0 + 1) - 1;
-  const uint32_t r_last = lower_bound(old_off, r_first, N + 1, e1) - 1;
+  const uint32_t C = *U.count;
+  const uint32_t i0 = unit_of(U.ue, 0u, C, e0), i1 = unit_of(U.ue, i0, C, e1 - 1);
+  if (i0 == i1 && e1 <= U.ub[i0]) {  // the whole tile lies in unchanged rows of one unit
+    const int64_t shift = int64_t(U.unb[i0]) - int64_t(U.ub[i0]);
+    for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) new_idx[int64_t(e) + shift] = old_idx[e];
+    return;
+  }
   for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
-    const uint32_t row = lower_bound(old_off, r_first, r_last + 1, e + 1) - 1;
+    const uint32_t i = unit_of(U.ue, i0, i1, e);
     const uint32_t col = old_idx[e];
-    const uint32_t k = e - old_off[row];
-    if (!changed[row]) {
-      new_idx[new_off[row] + k] = col;
+    const uint32_t b = U.ub[i];
+    if (e < b) {
+      new_idx[int64_t(e) + int64_t(U.unb[i]) - int64_t(b)] = col;
       continue;
     }
+    const uint32_t row = U.urow[i];
     const uint4 r = ranges[row];
     const unsigned long long key = (unsigned long long)row << 32 | col;
     const uint32_t d_at = lower_bound(dkeys, r.x, r.y, key);
     if (d_at < r.y && dkeys[d_at] == key) continue;  // deleted
     const uint32_t i_at = lower_bound(ikeys, r.z, r.w, key);
-    new_idx[new_off[row] + k - (d_at - r.x) + (i_at - r.z)] = col;
+    new_idx[U.unb[i] + (e - b) - (d_at - r.x) + (i_at - r.z)] = col;
   }
 }
 
@@ -288,13 +339,20 @@ hipError_t launch_new_degrees(const uint32_t *off, const int *delta, uint32_t N,
   return hipGetLastError();
 }
 
+hipError_t scan_u8(void *tmp, size_t &tmp_bytes, const uint8_t *in, uint32_t *out, size_t n, hipStream_t s) {
+  return rocprim::exclusive_scan(tmp, tmp_bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), s);
+}
+
 hipError_t launch_rewrite(const uint32_t *old_off, const uint32_t *old_idx, const uint32_t *new_off,
-                          const uint8_t *changed, const uint4 *ranges, const unsigned long long *dkeys,
-                          const unsigned long long *ikeys, uint32_t N, uint64_t max_edges, uint32_t *new_idx,
-                          hipStream_t s) {
+                          const uint8_t *changed, const uint32_t *pos, uint32_t *units, const uint4 *ranges,
+                          const unsigned long long *dkeys, const unsigned long long *ikeys, uint32_t N,
+                          uint32_t max_units, uint64_t max_edges, uint32_t *new_idx, hipStream_t s) {
+  if (N == 0) return hipSuccess;
+  const RewriteUnits U{units, units + max_units, units + 2 * size_t(max_units), units + 3 * size_t(max_units),
+                       units + 4 * size_t(max_units)};
+  units_kernel<<<grid_for(N, kBlock, 16384), kBlock, 0, s>>>(old_off, new_off, changed, pos, N, U);
   const uint64_t g = (max_edges + kRewriteTile - 1) / kRewriteTile;
-  rewrite_kernel<<<uint32_t(g ? g : 1), kBlock, 0, s>>>(old_off, old_idx, new_off, changed, ranges, dkeys, ikeys, N,
-                                                        new_idx);
+  rewrite_kernel<<<uint32_t(g ? g : 1), kBlock, 0, s>>>(old_off, old_idx, U, ranges, dkeys, ikeys, N, new_idx);
   return hipGetLastError();
 }
 
