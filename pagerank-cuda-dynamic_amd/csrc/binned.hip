@@ -202,20 +202,25 @@ __device__ __forceinline__ uint32_t pair_entry(const BinnedPlan &P, uint4 t, uin
 // is set and its magnitude added.
 template <bool EXT>
 __device__ __forceinline__ void add_slots(double *sum, uint8_t *ext, uint32_t d, double x, bool ok) {
+  bool neg = false, any_neg = false;
   if constexpr (EXT) {
-    const bool neg = ok && x < 0.0;
-    if (__ballot(neg)) {
-      if (neg) ext[d] = 1;
-      x = fabs(x);
-    }
+    neg = ok && x < 0.0;
+    any_neg = __ballot(neg) != 0;
+    if (any_neg) x = fabs(x);
   }
   const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
   if (__ballot(d == d0 || !ok) == ~0ull) {
     const double t = group_sum<kWave>(ok ? x : 0.0);
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(&sum[d0], t);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      atomicAdd(&sum[d0], t);
+      if (EXT && any_neg) ext[d0] = 1;  // one flag store for the wave's one destination
+    }
     return;
   }
-  if (ok) atomicAdd(&sum[d], x);
+  if (ok) {
+    atomicAdd(&sum[d], x);
+    if (EXT && neg) ext[d] = 1;
+  }
 }
 
 // Frontier variants: a vertex with a frontier in-neighbour (flag set in ext) is marked affected before its update,
