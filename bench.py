@@ -29,9 +29,10 @@ compiled reference on the test graphs), run on this host's cores on the same gra
 iterations (about 10 s).
 
 --workload dfp: the dynamic side instead. Each step applies a fresh seeded batch of --batch-fraction |E| (80%
-insertions, 20% deletions) to the RMAT base graph and times a DF-P solve warm-started from the base graph's ranks
-(engine-measured solve time: initialisation, marking and iterations, as PagerankResult.time), beside a GPU static
-solve of the same updated graph. Its cpu_baseline is the oracle's port of the reference's asynchronous OpenMP DF-P
+insertions, 20% deletions) to the device graph in place (prh_graph_update: both CSRs rewritten, the blocked plan
+patched) and times a DF-P solve warm-started from the previous graph's static ranks (engine-measured time:
+the update, initialisation, marking and iterations, as PagerankResult.time), beside a GPU static solve of the same
+updated graph with the update added; config.dfp_ms_without_update gives the solve alone. Its cpu_baseline is the oracle's port of the reference's asynchronous OpenMP DF-P
 (pagerankPruneDynamicFrontierOmp<true>, the CPU solver the reference's experiment runs; bit-identical to the
 compiled reference on one thread) on the same batch, on this host's cores.
 """
@@ -155,7 +156,7 @@ def bench_c1(args) -> int:
     return 0
 
 
-def bench_dfp(args, ctx, g, desc) -> int:
+def bench_dfp(args, ctx, g, desc, overridden: bool = False) -> int:
     """GPU DF-P per update batch against GPU static on the same updated graph; CPU: the async DF-P port.
 
     Each step applies a fresh seeded batch to the device graph in place (prh_graph_update: both CSRs rewritten, the
@@ -165,7 +166,7 @@ def bench_dfp(args, ctx, g, desc) -> int:
     o = PagerankOptions()
     xg = ctx.upload(g)
     autotuned = False
-    if not args.no_autotune:
+    if not args.no_autotune and not overridden:
         t, ms, hit = ctx.autotune(xg, args.tuning_cache, iterations=10)
         autotuned = True
         log(f"autotune: {t} at {ms:.3f} ms/iteration")
@@ -314,8 +315,6 @@ def main() -> int:
     g, desc = make_graph(args)
     log(f"graph N={g.N} M={g.M} built in {time.time() - t0:.1f}s")
     ctx = Context(0)
-    if args.workload == "dfp":
-        return bench_dfp(args, ctx, g, desc)
     overridden = any(v is not None for v in (args.source_slices, args.huge_degree, args.blocking, args.layout))
     if overridden:
         t = ctx.get_tuning()
@@ -328,6 +327,8 @@ def main() -> int:
         if args.layout is not None:
             t.layout = args.layout
         ctx.set_tuning(t)
+    if args.workload == "dfp":
+        return bench_dfp(args, ctx, g, desc, overridden)
     yg = ctx.upload(g, with_out=False)
     autotuned = False
     if not args.no_autotune and not overridden:

@@ -213,17 +213,20 @@ int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int
     }
   for (prh_tuning &t : cand) t.layout = fixed_layout < 0 ? 0 : fixed_layout;
   if (max_deg <= 64 && fixed_layout < 0) {
-    for (uint32_t be : {2048u, 4096u}) {
-      prh_tuning t = base;
-      t.switch_degree = 64;
-      t.huge_degree = 16384;
-      t.low_width = 0;
-      t.source_slices = 0;
-      t.block_edges = be;
-      t.blocking = 0;
-      t.layout = 1;
-      cand.push_back(t);
-    }
+    // rows in the caller's order: lanes per row from the largest in-degree (0), or one or two lanes per row, which
+    // give a wave 64 or 32 neighbouring rows whose gathers land close together on a mesh-like graph
+    for (uint32_t lw : {0u, 1u, 2u})
+      for (uint32_t be : {2048u, 4096u}) {
+        prh_tuning t = base;
+        t.switch_degree = 64;
+        t.huge_degree = 16384;
+        t.low_width = lw;
+        t.source_slices = 0;
+        t.block_edges = be;
+        t.blocking = 0;
+        t.layout = 1;
+        cand.push_back(t);
+      }
     if (g->M >= (uint64_t(1) << 20))
       for (int bw : {2048, 4096, 8192, 16384}) {
         const uint64_t nb = (uint64_t(g->N) + bw - 1) / bw;

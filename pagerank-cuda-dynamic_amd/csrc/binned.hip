@@ -7,8 +7,10 @@
 // from one source into one bin share a single entry of X, the pair's FP64 contribution (on a power-law graph a
 // source's edges crowd into the bins of the high in-degree vertices: on RMAT-24 with B = 8192 there are 0.38
 // pairs per edge). Each iteration makes two streaming passes through X:
-//   expand      one workgroup per item of a source block: the block's B contributions are loaded into LDS, and
-//               every pair leaving the block copies its source's contribution into its entry of X;
+//   expand      one workgroup per item of a source block: the block's B contributions r/deg are computed into LDS
+//               from the ranks and out-degrees, and every pair leaving the block copies its source's contribution
+//               into its entry of X (the plan keeps no contribution array: 8 bytes per vertex and iteration less
+//               than writing one in finish and reading it back);
 //   accumulate  one workgroup per item of a destination bin: the bin's slots (one per edge) are read in order, each
 //               takes its pair's entry of X and adds it into one of B LDS sums (ds_add_f64); a bin covered by one
 //               item is finished right there, a split bin leaves one row of B partial sums per item;
@@ -59,14 +61,16 @@ __device__ __forceinline__ double bin_finish(const RankArgs &A, uint32_t v, doub
   }
   if constexpr (Traits<VARIANT>::frontier)
     if (extend) A.vaff[v] = 1;
+  // The unweighted variants keep no contributions here (expand computes r/deg): only DF-P's closed form needs the
+  // out-degree, and its own old contribution r/d.
   const double rv = A.r[v];
-  const double crv = Traits<VARIANT>::dynamic ? A.cr[v] : 0.0;
   RowFlags f;
   double e = 0.0;
-  if (affected<VARIANT>(A, v)) {  // the out-degree is read only here: unaffected rows are copied
-    e = finish_vertex<VARIANT>(A, v, cv, A.deg[v], rv, crv, f);
+  if (affected<VARIANT>(A, v)) {
+    const uint32_t d = Traits<VARIANT>::prune ? A.deg[v] : 1u;
+    e = finish_vertex<VARIANT, false>(A, v, cv, d, rv, Traits<VARIANT>::prune ? rv / d : 0.0, f);
   } else {
-    copy_vertex<VARIANT>(A, v, rv, crv);
+    copy_vertex<VARIANT, false>(A, v, rv, 0.0);
     f.affected = false;
   }
   if constexpr (Traits<VARIANT>::frontier)
@@ -113,9 +117,28 @@ __device__ __forceinline__ double bin_block_max(double x, double *lds) {
 
 // naff (frontier variants after their first iteration): the previous iteration's frontier flags, carried as the
 // sign of the contribution (contributions are positive), so that accumulate can extend the frontier.
-template <int B>
-__global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const double *cr, const uint8_t *naff,
-                                                           LoopCtl ctl) {
+// Contribution of vertex v: r[v]/deg[v] (FROM_RANKS: the unweighted variants, whose finish keeps no
+// contributions) or the kept value cr[v] (the weighted variants: a/W forward, a adjoint).
+template <bool FROM_RANKS>
+__device__ __forceinline__ d2 contrib2(const double *cr, const double *r, const uint32_t *deg, uint32_t v) {
+  if constexpr (FROM_RANKS) {
+    const d2 x = *reinterpret_cast<const d2 *>(r + v);
+    const uint2 d = *reinterpret_cast<const uint2 *>(deg + v);
+    return d2{x.x / d.x, x.y / d.y};
+  } else {
+    return *reinterpret_cast<const d2 *>(cr + v);
+  }
+}
+
+template <bool FROM_RANKS>
+__device__ __forceinline__ double contrib1(const double *cr, const double *r, const uint32_t *deg, uint32_t v) {
+  if constexpr (FROM_RANKS) return r[v] / deg[v];
+  else return cr[v];
+}
+
+template <int B, bool FROM_RANKS>
+__global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const double *cr, const double *r,
+                                                           const uint32_t *deg, const uint8_t *naff, LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;
   if (iter_index(ctl) == 0) naff = nullptr;  // the first iteration extends nothing (the marking did)
   __shared__ double c[B];
@@ -123,23 +146,25 @@ __global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const d
   const uint32_t base = it.x * uint32_t(B);
   const uint32_t n = min(uint32_t(B), P.N - base);
   // Pairs of contributions a lane (base is a multiple of B, so the pairs are 16-byte aligned); an odd n leaves one.
-  const d2 *cr2 = reinterpret_cast<const d2 *>(cr + base);
   if (naff) {
     const uint16_t *f2 = reinterpret_cast<const uint16_t *>(naff + base);
     for (uint32_t i = threadIdx.x; 2 * i + 1 < n; i += kBinBlock) {
-      const d2 v = cr2[i];
+      const d2 v = contrib2<FROM_RANKS>(cr, r, deg, base + 2 * i);
       const uint32_t f = f2[i];
       c[2 * i] = (f & 0xFFu) ? -v.x : v.x;
       c[2 * i + 1] = (f >> 8) ? -v.y : v.y;
     }
   } else {
     for (uint32_t i = threadIdx.x; 2 * i + 1 < n; i += kBinBlock) {
-      const d2 v = cr2[i];
+      const d2 v = contrib2<FROM_RANKS>(cr, r, deg, base + 2 * i);
       c[2 * i] = v.x;
       c[2 * i + 1] = v.y;
     }
   }
-  if ((n & 1u) && threadIdx.x == 0) c[n - 1] = naff && naff[base + n - 1] ? -cr[base + n - 1] : cr[base + n - 1];
+  if ((n & 1u) && threadIdx.x == 0) {
+    const double x = contrib1<FROM_RANKS>(cr, r, deg, base + n - 1);
+    c[n - 1] = naff && naff[base + n - 1] ? -x : x;
+  }
   __syncthreads();
   // Pairs of entries: thread t takes pairs p = t + m * kBinBlock of the even-aligned range (16-byte stores, with the
   // non-temporal hint: X is read back only by the next kernel, and keeping it out of L2 helps both); a lone first
@@ -387,14 +412,16 @@ __global__ __launch_bounds__(kBinBlock) void finish_bins_kernel(RankArgs A, Binn
 // The overlay's values for this iteration: entry i takes its source's contribution, negated when the source was in
 // the last frontier (naff, frontier variants after their first iteration), as expand does for X. It runs between
 // expand and accumulate: accumulate overwrites naff with the new frontier. The entry count is ov_off[bins].
-__global__ __launch_bounds__(kBlock) void overlay_expand_kernel(BinnedPlan P, const double *cr, const uint8_t *naff,
+template <bool FROM_RANKS>
+__global__ __launch_bounds__(kBlock) void overlay_expand_kernel(BinnedPlan P, const double *cr, const double *r,
+                                                                const uint32_t *deg, const uint8_t *naff,
                                                                 LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;
   if (iter_index(ctl) == 0) naff = nullptr;
   const uint32_t n = P.ov_off[(P.N + P.B - 1) / P.B];
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const uint32_t u = uint32_t(P.ov_key[i]);
-    const double c = cr[u];
+    const double c = contrib1<FROM_RANKS>(cr, r, deg, u);
     P.ovX[i] = naff && naff[u] ? -c : c;
   }
 }
@@ -403,10 +430,11 @@ template <int B, int VARIANT>
 hipError_t binned_update(const RankArgs &A, const BinnedPlan &P, const LoopCtl &ctl, bool extend, hipStream_t s,
                          hipEvent_t mark) {
   const uint8_t *naff = Traits<VARIANT>::frontier && extend ? A.naff : nullptr;
-  if (P.n_ex) expand_kernel<B><<<P.n_ex, kBinBlock, 0, s>>>(P, A.cr, naff, ctl);
+  constexpr bool kFromRanks = !Traits<VARIANT>::weighted;
+  if (P.n_ex) expand_kernel<B, kFromRanks><<<P.n_ex, kBinBlock, 0, s>>>(P, A.cr, A.r, A.deg, naff, ctl);
   if (P.ov_n) {
     const uint32_t g = (P.ov_n + kBlock - 1) / kBlock;
-    overlay_expand_kernel<<<g < 4096 ? g : 4096, kBlock, 0, s>>>(P, A.cr, naff, ctl);
+    overlay_expand_kernel<kFromRanks><<<g < 4096 ? g : 4096, kBlock, 0, s>>>(P, A.cr, A.r, A.deg, naff, ctl);
   }
   accumulate_kernel<B, VARIANT><<<P.n_ac, kBinBlock, 0, s>>>(A, P, ctl);
   if (P.n_multi) finish_bins_kernel<VARIANT><<<P.n_multi, kBinBlock, 0, s>>>(A, P, ctl);

@@ -30,7 +30,9 @@ struct RowFlags {
   bool affected = true, frontier = false;
 };
 
-template <int VARIANT>
+// CONTRIB: the plan keeps contributions (ca = a/d for the next gathers); the propagation-blocked plan computes them
+// from the ranks where it reads them, and passes crv = r/d itself (the same value, bit for bit).
+template <int VARIANT, bool CONTRIB = true>
 __device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv, uint32_t d, double rv,
                                                 double crv, RowFlags &f) {
   using T = Traits<VARIANT>;
@@ -38,7 +40,7 @@ __device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, d
   if constexpr (T::prune) av = fma(A.P, cv - crv, A.C0) * (1.0 / (1.0 - A.P / d));
   else av = fma(A.P, cv, A.C0);
   A.a[v] = av;
-  A.ca[v] = av / d;
+  if constexpr (CONTRIB) A.ca[v] = av / d;
   const double ev = fabs(av - rv);
   if constexpr (T::frontier) {
     const double rel = ev / fmax(rv, av);
@@ -73,11 +75,11 @@ __device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, d
   return finish_vertex<VARIANT>(A, v, cv, A.deg[v], A.r[v], Traits<VARIANT>::prune ? A.cr[v] : 0.0);
 }
 
-// Unaffected row of a dynamic variant: carried over unchanged (rank and contribution).
-template <int VARIANT>
+// Unaffected row of a dynamic variant: carried over unchanged (rank and, where the plan keeps them, contribution).
+template <int VARIANT, bool CONTRIB = true>
 __device__ __forceinline__ void copy_vertex(const RankArgs &A, uint32_t v, double rv, double crv) {
   A.a[v] = rv;
-  A.ca[v] = crv;
+  if constexpr (CONTRIB) A.ca[v] = crv;
   if constexpr (Traits<VARIANT>::frontier) A.naff[v] = 0;
 }
 
