@@ -12,8 +12,8 @@ and the ranks' return to the caller's vertex order). config.gedges_per_s_iter_ho
 host buffers (ranks copied over PCIe), reported beside it.
 
 roofline: the dominant work is the rank update, once per iteration: one rank_update_kernel launch with the row
-plan, or expand_kernel + accumulate_kernel (+ finish_bins_kernel) with the propagation-blocked plan
-(tuning.blocking), timed together. Its algorithmic bytes per iteration are 12|E| + 24|V| (per edge a 4-byte source
+plan, one window_update_kernel launch with the banded plan (tuning.window), or expand_kernel + accumulate_kernel
+(+ finish_bins_kernel) with the propagation-blocked plan (tuning.blocking), timed together. Its algorithmic bytes per iteration are 12|E| + 24|V| (per edge a 4-byte source
 id and an 8-byte FP64 contribution; per vertex a 4-byte offset, 4-byte out-degree, 8-byte old-rank read and 8-byte
 new-rank write; the L-inf reduction is fused), whatever the plan moves in addition. Its duration is measured with HIP
 events recorded on the engine's stream around every iteration's update that did work inside the timed region.
@@ -260,6 +260,8 @@ def measured_traffic(desc: str, plan: str):
 def plan_name(t) -> str:
     """The rank-update plan a static solve runs with tuning t, and the kernels of one iteration."""
     order = ", rows in the caller's vertex order" if t.layout == 1 else ""
+    if t.blocking == 0 and t.window == 1 and t.layout == 1:
+        return "banded plan (rows in the caller's vertex order, a window of contributions in LDS): window_update_kernel"
     if t.blocking > 0:
         return (f"propagation blocking {t.blocking}{order}: expand_kernel<{t.blocking}> + "
                 f"accumulate_kernel<{t.blocking}> + finish_bins_kernel")
@@ -294,6 +296,8 @@ def main() -> int:
     ap.add_argument("--huge-degree", type=int, default=None, help="override the tuning's huge_degree")
     ap.add_argument("--blocking", type=int, default=None, help="override the tuning's blocking (propagation blocking)")
     ap.add_argument("--layout", type=int, default=None, help="override the tuning's layout (1: the caller's order)")
+    ap.add_argument("--window", type=int, default=None, help="override the tuning's window (1: the banded plan, "
+                                                            "with --layout 1 --blocking 0)")
     ap.add_argument("--copy-calibration", type=float, metavar="GIB", default=0.0,
                     help="only run a plain device copy of GIB GiB (for scaling the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -315,7 +319,8 @@ def main() -> int:
     g, desc = make_graph(args)
     log(f"graph N={g.N} M={g.M} built in {time.time() - t0:.1f}s")
     ctx = Context(0)
-    overridden = any(v is not None for v in (args.source_slices, args.huge_degree, args.blocking, args.layout))
+    overridden = any(v is not None for v in (args.source_slices, args.huge_degree, args.blocking, args.layout,
+                                              args.window))
     if overridden:
         t = ctx.get_tuning()
         if args.source_slices is not None:
@@ -326,6 +331,10 @@ def main() -> int:
             t.blocking = args.blocking
         if args.layout is not None:
             t.layout = args.layout
+        if args.window is not None:
+            t.window = args.window
+        elif t.layout != 1:
+            t.window = 0
         ctx.set_tuning(t)
     if args.workload == "dfp":
         return bench_dfp(args, ctx, g, desc, overridden)
@@ -385,7 +394,8 @@ def main() -> int:
         "config": {"workload": desc, "vertices": g.N, "edges": g.M, "tolerance": 1e-10, "damping": 0.85,
                    "tuning": {"switch_degree": tuning.switch_degree, "huge_degree": tuning.huge_degree,
                               "source_slices": tuning.source_slices, "block_edges": tuning.block_edges,
-                              "blocking": tuning.blocking, "layout": tuning.layout, "autotuned": autotuned},
+                              "blocking": tuning.blocking, "layout": tuning.layout, "window": tuning.window,
+                              "autotuned": autotuned},
                    "iterations_per_solve": iters / args.steps,
                    "gedges_per_s_iter_host_buffers": round(host_rate, 4),
                    "gedges_per_s_iter_compute_only": round(g.M * iters / (tcomp / 1e3) / 1e9, 4)},

@@ -104,6 +104,12 @@ typedef struct prh_tuning {
                               row plan then runs one lane width for all rows and the sliced plan is not used. A graph
                               changes layout at its next solve (counted as preparation); one updated in place keeps
                               its layout [0] */
+  int window;              /* Static / Naive-dynamic / DT / DF / DF-P with layout = 1 and blocking = 0: 1 = the banded
+                              plan, in which each workgroup sweeps a contiguous range of rows chunk by chunk and gathers
+                              from a window of the contributions around the chunk held in LDS (for graphs whose edges
+                              join vertices close together in the caller's order: road networks, meshes, grids; sources
+                              outside the window are gathered directly, so every graph gets the same results); 0 = the
+                              row plan. Needs layout = 1; the weighted solves and the streaming session ignore it [0] */
 } prh_tuning;
 
 typedef struct prh_context prh_context;

@@ -49,10 +49,10 @@ std::string tuning_key(prh_context *c, const prh_graph *g) {
   return buf;
 }
 
-constexpr int kTuningCount = 9;
+constexpr int kTuningCount = 10;
 const char *const kTuningFields[kTuningCount] = {"switch_degree", "huge_degree", "low_width", "check_interval",
                                                  "source_slices", "block_edges", "extension_direction", "blocking",
-                                                 "layout"};
+                                                 "layout", "window"};
 
 void tuning_to_values(const prh_tuning &t, long long v[kTuningCount]) {
   v[0] = t.switch_degree;
@@ -64,6 +64,7 @@ void tuning_to_values(const prh_tuning &t, long long v[kTuningCount]) {
   v[6] = t.extension_direction;
   v[7] = t.blocking;
   v[8] = t.layout;
+  v[9] = t.window;
 }
 
 void values_to_tuning(const long long v[kTuningCount], prh_tuning &t) {
@@ -76,6 +77,7 @@ void values_to_tuning(const long long v[kTuningCount], prh_tuning &t) {
   t.extension_direction = int(v[6]);
   t.blocking = int(v[7]);
   t.layout = int(v[8]);
+  t.window = int(v[9]);
 }
 
 // The cache is a JSON object with one entry per line:
@@ -168,10 +170,10 @@ int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int
   // Candidates on the degree-ordered layout: the row plan over switch degree x chunk threshold x block size, the
   // sliced plan over slice count x block size, and the propagation-blocked plan over its block width (graphs of 2^20
   // edges and more, where its streaming passes pay). On graphs whose in-degrees all stay at or below 64 (a road
-  // network, a grid), also the row plan and the blocked plan on the caller's vertex order, which keeps their
-  // neighbourhoods together (tried last: each layout change rebuilds the layout). A graph updated in place keeps its
-  // layout, so its candidates all take that one. Each is timed on `iterations` static iterations (tolerance 0) after
-  // one warm-up solve.
+  // network, a grid), also the row plan, the banded plan and the blocked plan on the caller's vertex order, which
+  // keeps their neighbourhoods together (tried last: each layout change rebuilds the layout). A graph updated in
+  // place keeps its layout, so its candidates all take that one. Each is timed on `iterations` static iterations
+  // (tolerance 0) after one warm-up solve.
   const int fixed_layout = g->updated ? g->layout : -1;
   const uint32_t max_deg = *std::max_element(g->sorted_deg.begin(), g->sorted_deg.end());
   std::vector<prh_tuning> cand;
@@ -211,8 +213,31 @@ int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int
       t.blocking = bw;
       cand.push_back(t);
     }
-  for (prh_tuning &t : cand) t.layout = fixed_layout < 0 ? 0 : fixed_layout;
+  for (prh_tuning &t : cand) {
+    t.layout = fixed_layout < 0 ? 0 : fixed_layout;
+    t.window = 0;
+  }
+  if (max_deg <= 64 && fixed_layout == 1) {  // an updated graph in the caller's order: the banded plan too
+    prh_tuning t = base;
+    t.blocking = 0;
+    t.source_slices = 0;
+    t.layout = 1;
+    t.window = 1;
+    cand.push_back(t);
+  }
   if (max_deg <= 64 && fixed_layout < 0) {
+    {
+      prh_tuning t = base;
+      t.switch_degree = 64;
+      t.huge_degree = 16384;
+      t.low_width = 0;
+      t.source_slices = 0;
+      t.block_edges = 2048;
+      t.blocking = 0;
+      t.layout = 1;
+      t.window = 1;
+      cand.push_back(t);
+    }
     // rows in the caller's order: lanes per row from the largest in-degree (0), or one or two lanes per row, which
     // give a wave 64 or 32 neighbouring rows whose gathers land close together on a mesh-like graph
     for (uint32_t lw : {0u, 1u, 2u})
@@ -225,6 +250,7 @@ int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int
         t.block_edges = be;
         t.blocking = 0;
         t.layout = 1;
+        t.window = 0;
         cand.push_back(t);
       }
     if (g->M >= (uint64_t(1) << 20))
@@ -239,6 +265,7 @@ int prh_autotune(prh_context *c, const prh_graph *g, const char *cache_path, int
         t.block_edges = 2048;
         t.blocking = bw;
         t.layout = 1;
+        t.window = 0;
         cand.push_back(t);
       }
   }

@@ -78,43 +78,6 @@ __device__ __forceinline__ double bin_finish(const RankArgs &A, uint32_t v, doub
   return e;
 }
 
-// Workgroup totals of the extension costs to A.stats[2 slot], [2 slot + 1] (frontier variants). Every thread must
-// call it.
-template <int VARIANT>
-__device__ __forceinline__ void bin_store_costs(const RankArgs &A, const CostAcc &c, uint32_t slot, double *lds) {
-  if constexpr (Traits<VARIANT>::frontier) {
-    if (!A.stats) return;
-    const int w = threadIdx.x / kWave;
-    const double pull = group_sum<kWave>(c.pull), push = group_sum<kWave>(c.push);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-      lds[w] = pull;
-      lds[kBinWaves + w] = push;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double a = 0.0, b = 0.0;
-      for (int i = 0; i < kBinWaves; ++i) {
-        a += lds[i];
-        b += lds[kBinWaves + i];
-      }
-      A.stats[2 * size_t(slot)] = a;
-      A.stats[2 * size_t(slot) + 1] = b;
-    }
-  }
-}
-
-// Workgroup-wide max over kBinBlock threads, valid in thread 0. Every thread must call it.
-__device__ __forceinline__ double bin_block_max(double x, double *lds) {
-  x = group_max<kWave>(x);
-  const int w = threadIdx.x / kWave;
-  if ((threadIdx.x & (kWave - 1)) == 0) lds[w] = x;
-  __syncthreads();
-  double m = 0.0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < kBinWaves; ++i) m = fmax(m, lds[i]);
-  return m;
-}
-
 // naff (frontier variants after their first iteration): the previous iteration's frontier flags, carried as the
 // sign of the contribution (contributions are positive), so that accumulate can extend the frontier.
 // Contribution of vertex v: r[v]/deg[v] (FROM_RANKS: the unweighted variants, whose finish keeps no
@@ -338,8 +301,8 @@ __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, Binne
     double *p = P.partial + size_t(it.w) * B;
     for (uint32_t i = threadIdx.x; i < nb; i += kBinBlock) p[i] = EXT && ext[EXT ? i : 0] ? -sum[i] : sum[i];
   }
-  bin_store_costs<VARIANT>(A, cost, blockIdx.x, lds_cost);
-  emax = bin_block_max(emax, lds_max);
+  wg_store_costs<VARIANT, kBinWaves>(A, cost, blockIdx.x, lds_cost);
+  emax = wg_max<kBinWaves>(emax, lds_max);
   if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
 }
 
@@ -404,8 +367,8 @@ __global__ __launch_bounds__(kBinBlock) void finish_bins_kernel(RankArgs A, Binn
       emax = bin_finish<VARIANT>(A, v, t, cost, flag);
     }
   }
-  bin_store_costs<VARIANT>(A, cost, P.n_ac + blockIdx.x, lds_cost);
-  emax = bin_block_max(emax, lds_max);
+  wg_store_costs<VARIANT, kBinWaves>(A, cost, P.n_ac + blockIdx.x, lds_cost);
+  emax = wg_max<kBinWaves>(emax, lds_max);
   if (threadIdx.x == 0) atomic_max_nonneg(&ctl.err[iter_index(ctl)], emax);
 }
 

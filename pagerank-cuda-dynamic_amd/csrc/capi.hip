@@ -128,6 +128,7 @@ void prh_default_tuning(prh_tuning *t) {
   t->extension_direction = 1;
   t->blocking = 0;
   t->layout = 0;
+  t->window = 0;
 }
 
 int prh_context_create(int device, prh_context **out) {
@@ -240,8 +241,11 @@ int prh_context_set_tuning(prh_context *c, const prh_tuning *t) {
       t->block_edges < 256 || t->extension_direction < 0 || t->extension_direction > 2 ||
       !(t->blocking == 0 || t->blocking == 1024 || t->blocking == 2048 || t->blocking == 4096 ||
         t->blocking == 8192 || t->blocking == 16384) ||
-      !(t->layout == 0 || t->layout == 1))
+      !(t->layout == 0 || t->layout == 1) || !(t->window == 0 || t->window == 1))
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad tuning");
+  if (t->window && t->layout != 1)
+    return fail(c, PRH_ERROR_INVALID_ARGUMENT, "bad tuning: the banded plan (window = 1) runs over rows in the "
+                                                "caller's vertex order (layout = 1)");
   c->tuning = *t;
   return PRH_SUCCESS;
 }

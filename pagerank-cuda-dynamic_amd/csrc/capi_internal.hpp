@@ -176,6 +176,11 @@ struct prh_graph {
   // Propagation-blocked plans, built on first use for the current block width: of the layout in-CSR (the rank
   // update, the weighted forward) and of its out-CSR (the weighted adjoint).
   BinnedCache bin, bin_t;
+  // Banded plan (rows in the caller's order), built on first use from the layout in-CSR's spread; an in-place update
+  // keeps it (edges beyond its window are gathered directly).
+  bool win_valid = false;
+  prh::WindowPlan win{};
+  prhc::Prep win_prep;
 };
 
 namespace prhc {
@@ -301,6 +306,7 @@ void free_overlay(prh_context *c, BinnedCache &bc);
 int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot = false);
 int build_binned_plan_t(prh_context *c, prh_graph *g);
 int build_rank_plan(prh_context *c, prh_graph *g);
+int build_window_plan(prh_context *c, prh_graph *g);
 int out_schedule(prh_context *c, prh_graph *g, prh::ExtendArgs &E);
 
 // noalloc.hip

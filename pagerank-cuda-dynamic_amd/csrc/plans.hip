@@ -370,15 +370,17 @@ int build_binned(prh_context *c, uint32_t N, uint64_t M, const uint32_t *off, co
     PRH_CHECK(keep(&src16, 2));
     PRH_CHECK(keep(&X, 16));
   }
-  // Items: expand items of at most 32 per CU over all pairs, accumulate items of at most 2 per CU over all slots
-  // (fewer split bins, hence fewer partial rows); both at least 16K entries. Measured on RMAT-24 (B = 8192) per
-  // update: expand spans of 6.4K, 12.9K, 16K, 32K and 64K entries 763, 712, 709, 724 and 744 us; accumulate
-  // spans of M/(k CUs) for k = 8, 4, 3, 2, 3/2, 1 and 1/2 733, 709, 711, 696, 710, 807 and 1195 us (k = 2: as many
-  // items as the two resident workgroups of every CU).
+  // Items: expand items of 16K pairs whatever the graph's size, accumulate items of at most 2 per CU over all slots
+  // (fewer split bins, hence fewer partial rows), at least 16K slots. Measured per update (B = 8192): on RMAT-24,
+  // expand spans of 6.4K, 8K, 12K, 12.9K, 16K, 32K and 64K entries 763, 697, 684, 712, 675-709, 724 and 744 us;
+  // on RMAT-26 with edge factor 30 (0.78G pairs), 8K, 12K, 16K, 24K, 32K, 64K and 95K (Q / (32 CUs)) 4418-4450,
+  // 4348-4350, 4257-4366, 4414, 4315, 4364 and 4513-4574 us. Accumulate spans of M/(k CUs) for k = 8, 4, 3, 2, 3/2,
+  // 1 and 1/2 on RMAT-24: 733, 709, 700-711, 675-696, 710, 807 and 1195 us (k = 2: as many items as the two resident
+  // workgroups of every CU); on RMAT-26, k = 1, 2, 3 and 4: 5580, 4257-4366, 4246 and 4316 us.
   hipDeviceProp_t prop{};
   PRH_HIP(c, hipGetDeviceProperties(&prop, c->device));
   const uint64_t cus = uint64_t(std::max(prop.multiProcessorCount, 1));
-  const uint64_t span_ex = std::max<uint64_t>(Q / (cus * 32), 16384);
+  const uint64_t span_ex = 16384;
   const uint64_t span = std::max<uint64_t>(M / (cus * 2), 16384);
   std::vector<uint4> ex, ac, multi;
   for (uint64_t b = 0; b < nb; ++b)
@@ -576,6 +578,39 @@ int build_rank_plan(prh_context *c, prh_graph *g) {
 
 // The layout in the caller's order (mode 1: row i is vertex i; the layout arrays are copies of the uploaded ones) or
 // by in-degree (mode 0, from the uploaded arrays read back to the host). Plans built on the old layout are dropped.
+// Banded plan: the window reaches H = the largest |source - destination| of the layout in-CSR (rows in the caller's
+// order) on each side of a chunk, capped at what the larger ring holds (edges beyond it are gathered directly). The
+// smaller ring (two workgroups per CU) serves H <= 4544; each workgroup sweeps about N / wgs rows.
+int build_window_plan(prh_context *c, prh_graph *g) {
+  if (g->win_valid) return PRH_SUCCESS;
+  if (g->layout != 1) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "the banded plan needs rows in the caller's order");
+  if (g->N > (1u << 31)) return fail(c, PRH_ERROR_UNSUPPORTED, "the banded plan takes at most 2^31 vertices");
+  PrepTimer timer(g->win_prep);
+  hipStream_t s = c->stream;
+  uint32_t *spread = nullptr;
+  PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&spread), 4));
+  uint32_t H = 0;
+  hipError_t e = prh::launch_max_spread(g->l_in_off, g->l_in_src, g->N, spread, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&H, spread, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dev_free(c, spread, 4);
+  if (e != hipSuccess) return fail(c, PRH_ERROR_DEVICE, "banded plan: %s", hipGetErrorString(e));
+  const uint32_t R = uint32_t(prh::kBinBlock);
+  const uint32_t h_small = (prh::kWindowRingSmall - R) / 2, h_large = (prh::kWindowRingLarge - R) / 2;
+  hipDeviceProp_t prop{};
+  PRH_HIP(c, hipGetDeviceProperties(&prop, c->device));
+  const uint32_t cus = uint32_t(std::max(prop.multiProcessorCount, 1));
+  prh::WindowPlan W{};
+  W.N = g->N;
+  W.S = H <= h_small ? prh::kWindowRingSmall : prh::kWindowRingLarge;
+  W.H = std::min(H, W.S == prh::kWindowRingSmall ? h_small : h_large);
+  const uint32_t chunks = (g->N + R - 1) / R;
+  W.wgs = std::max(1u, std::min((W.S == prh::kWindowRingSmall ? 2u : 1u) * cus, chunks));
+  g->win = W;
+  g->win_valid = true;
+  return PRH_SUCCESS;
+}
+
 int relayout(prh_context *c, prh_graph *g, int mode) {
   if (g->layout == mode) return PRH_SUCCESS;
   if (g->updated) return fail(c, PRH_ERROR_UNSUPPORTED, "a graph updated in place keeps its layout");
@@ -587,6 +622,7 @@ int relayout(prh_context *c, prh_graph *g, int mode) {
   const bool with_out = g->out_off != nullptr;
   // Everything built on the old layout.
   g->plan_valid = false;
+  g->win_valid = false;
   free_sliced(c, g);
   free_binned(c, g->bin);
   free_binned(c, g->bin_t);

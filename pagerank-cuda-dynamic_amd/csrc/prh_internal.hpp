@@ -185,6 +185,18 @@ struct BinnedPlan {
   double *ovX;
 };
 
+// Banded plan (window.hip): rows in the caller's vertex order, workgroup b sweeping rows [b N / wgs, (b + 1) N / wgs)
+// in chunks of kBinBlock rows through a ring of S contributions in LDS that holds the window [c0 - H, c0 + R + H) of
+// chunk c0 (S >= kBinBlock + 2H); sources outside it are gathered from the ranks.
+constexpr uint32_t kWindowRingSmall = 10112;  // 79 KiB of LDS: two workgroups per CU, H <= 4544
+constexpr uint32_t kWindowRingLarge = 20224;  // 158 KiB: one workgroup per CU, H <= 9600
+struct WindowPlan {
+  uint32_t N;
+  uint32_t H;    // rows the window reaches on each side of a chunk
+  uint32_t S;    // ring entries: kWindowRingSmall or kWindowRingLarge
+  uint32_t wgs;  // workgroups
+};
+
 // ---- partition.hip ----
 // Stable 3-way partition of [0,N) by degree d = off[v+1]-off[v]: d <= t1, t1 < d <= t2, d > t2.
 // tiles: scratch of 3 * partition_tiles(N) uint32; counts_dev: 3 uint32 outputs (device).
@@ -230,6 +242,11 @@ hipError_t launch_gather_ranges(const uint32_t *off, const uint32_t *ids, size_t
 // recorded after the last kernel.
 hipError_t launch_binned_update(int variant, const RankArgs &A, const BinnedPlan &P, unsigned long long *err,
                                 int *done, int l, double tol, bool extend, hipStream_t s, hipEvent_t mark = nullptr);
+// Rank update of iteration l through the banded plan (window.hip); unweighted variants only.
+hipError_t launch_window_update(int variant, const RankArgs &A, const WindowPlan &W, unsigned long long *err, int *done,
+                                int l, double tol, hipStream_t s);
+// *out = the largest |source - destination| over the in-CSR's edges.
+hipError_t launch_max_spread(const uint32_t *off, const uint32_t *src, uint32_t N, uint32_t *out, hipStream_t s);
 // In-place update of a plan (see binned.hip): dead flags and fresh overlay entries from a batch's tidied in-CSR keys
 // (the current overlay: *ov_count sorted keys), the next overlay's candidates (ov_cap + fresh_cap of them, the live
 // count added to *kept), and the overlay's per-bin offsets over the first min(*count, cap) sorted keys (a count above

@@ -368,11 +368,13 @@ __global__ __launch_bounds__(kBlock) void init_ranks_kernel(double *r, double *a
                                                             const uint32_t *deg, uint32_t N, int from_r) {
   for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < N; v += gridDim.x * kBlock) {
     const double rv = from_r ? r[v] : 1.0 / N;
-    const double c = rv / deg[v];
     if (!from_r) r[v] = rv;
     a[v] = rv;
-    cr[v] = c;
-    ca[v] = c;
+    if (cr) {  // null: the plan computes contributions from the ranks (propagation blocking) and keeps none
+      const double c = rv / deg[v];
+      cr[v] = c;
+      ca[v] = c;
+    }
   }
 }
 

@@ -153,8 +153,10 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   prh_graph *gm = const_cast<prh_graph *>(g);
   if (c->tuning.layout != g->layout && !g->updated) PRH_CHECK(relayout(c, gm, c->tuning.layout));
   const bool binned = c->tuning.blocking > 0;
-  const bool sliced = !binned && variant == prh::kPlain && c->tuning.source_slices > 0 && g->layout == 0;
+  const bool window = !binned && c->tuning.window != 0 && g->layout == 1;
+  const bool sliced = !binned && !window && variant == prh::kPlain && c->tuning.source_slices > 0 && g->layout == 0;
   if (binned) PRH_CHECK(build_binned_plan(c, gm));
+  else if (window) PRH_CHECK(build_window_plan(c, gm));
   else if (sliced) PRH_CHECK(build_sliced_plan(c, gm));
   else PRH_CHECK(build_rank_plan(c, gm));
   // One-time preparation of this graph not yet reported by a solve: its layout (built at upload) and the plan this
@@ -162,7 +164,10 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   float prep = 0.0f;
   if (account_prep)
     prep = gm->layout_prep.take() + gm->update_prep.take() +
-           (binned ? gm->bin.prep.take() : sliced ? gm->sliced_prep.take() : gm->plan_prep.take());
+           (binned   ? gm->bin.prep.take()
+            : window ? gm->win_prep.take()
+            : sliced ? gm->sliced_prep.take()
+                     : gm->plan_prep.take());
   DevBatch b;
   if (dynamic) PRH_CHECK(stage_batch(c, g, du, dv, nd, iu, ni, host, b));
   if (q && !host) {  // device initial ranks: reject NaN/Inf (the reference would propagate them)
@@ -191,8 +196,9 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   const int dir_mode = frontier ? c->tuning.extension_direction : 0;
   const bool dir_auto = dir_mode == 1 && !binned;
   // Per-workgroup extension costs: one slot per workgroup of the launches that finish vertices.
-  const uint32_t stat_slots = binned ? g->bin.plan.n_ac + g->bin.plan.n_multi
-                                     : plan.blocks + (plan.n_chunked ? prh::finish_chunked_grid(plan.n_chunked) : 0);
+  const uint32_t stat_slots = binned   ? g->bin.plan.n_ac + g->bin.plan.n_multi
+                              : window ? g->win.wgs
+                                       : plan.blocks + (plan.n_chunked ? prh::finish_chunked_grid(plan.n_chunked) : 0);
   if (dir_auto) PRH_CHECK(ensure(c, c->ext_stats, 2 * size_t(std::max<uint32_t>(stat_slots, 1))));
   int *dir = frontier ? c->flags.p + 2 : nullptr;
   const prh::PullArgs pull{c->vaff.p, c->naff.p, g->l_in_off, g->l_in_src, layout_rows_above(g, 64),
@@ -209,7 +215,10 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
       PRH_CHECK(out_schedule(c, gm, xo));
       if (account_prep) prep += gm->out_prep.take();
     }
-    PRH_HIP(c, prh::launch_init_ranks(c->rank[0].p, c->rank[1].p, c->contrib[0].p, c->contrib[1].p, g->l_outdeg, N,
+    // The blocked and banded plans compute contributions from the ranks where they read them and keep none.
+    const bool keeps = !binned && !window;
+    PRH_HIP(c, prh::launch_init_ranks(c->rank[0].p, c->rank[1].p, keeps ? c->contrib[0].p : nullptr,
+                                      keeps ? c->contrib[1].p : nullptr, g->l_outdeg, N,
                                       q != nullptr, s));
     PRH_HIP(c, hipEventRecord(c->ev[1], s));
     if (dynamic) PRH_CHECK(mark_affected(c, g, method, b, xo));
@@ -229,7 +238,10 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
       if (binned)
         PRH_HIP(c, prh::launch_binned_update(variant, A, g->bin.plan, err, done, l, o->tolerance, frontier && l > 0, s,
                                              mark));
-      else if (sliced) PRH_HIP(c, prh::launch_sliced_update(A, g->sliced, err, done, l, o->tolerance, s, mark));
+      else if (window) {
+        PRH_HIP(c, prh::launch_window_update(variant, A, g->win, err, done, l, o->tolerance, s));
+        if (mark) PRH_HIP(c, hipEventRecord(mark, s));
+      } else if (sliced) PRH_HIP(c, prh::launch_sliced_update(A, g->sliced, err, done, l, o->tolerance, s, mark));
       else PRH_HIP(c, prh::launch_rank_update(variant, A, plan, err, done, l, o->tolerance, s, mark));
       if (frontier && !binned) {  // the blocked plan extends inside the next iteration's update
         if (dir_auto) PRH_HIP(c, prh::launch_direction(c->ext_stats.p, stat_slots, dir, err, done, l, o->tolerance, s));

@@ -70,6 +70,44 @@ struct CostAcc {
   }
 };
 
+// Workgroup totals of the extension costs to A.stats[2 slot], [2 slot + 1] (frontier variants with A.stats), for a
+// workgroup of WAVES waves. Every thread must call it.
+template <int VARIANT, int WAVES>
+__device__ __forceinline__ void wg_store_costs(const RankArgs &A, const CostAcc &c, uint32_t slot, double *lds) {
+  if constexpr (Traits<VARIANT>::frontier) {
+    if (!A.stats) return;
+    const int w = threadIdx.x / kWave;
+    const double pull = group_sum<kWave>(c.pull), push = group_sum<kWave>(c.push);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      lds[w] = pull;
+      lds[WAVES + w] = push;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double a = 0.0, b = 0.0;
+      for (int i = 0; i < WAVES; ++i) {
+        a += lds[i];
+        b += lds[WAVES + i];
+      }
+      A.stats[2 * size_t(slot)] = a;
+      A.stats[2 * size_t(slot) + 1] = b;
+    }
+  }
+}
+
+// Workgroup-wide max over WAVES waves, valid in thread 0. Every thread must call it.
+template <int WAVES>
+__device__ __forceinline__ double wg_max(double x, double *lds) {
+  x = group_max<kWave>(x);
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) lds[w] = x;
+  __syncthreads();
+  double m = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < WAVES; ++i) m = fmax(m, lds[i]);
+  return m;
+}
+
 template <int VARIANT>
 __device__ __forceinline__ double finish_vertex(const RankArgs &A, uint32_t v, double cv) {
   return finish_vertex<VARIANT>(A, v, cv, A.deg[v], A.r[v], Traits<VARIANT>::prune ? A.cr[v] : 0.0);

@@ -45,7 +45,8 @@ class prh_stats(C.Structure):
 class prh_tuning(C.Structure):
     _fields_ = [("switch_degree", C.c_uint32), ("huge_degree", C.c_uint32), ("low_width", C.c_int),
                 ("check_interval", C.c_int), ("source_slices", C.c_int), ("block_edges", C.c_uint32),
-                ("extension_direction", C.c_int), ("blocking", C.c_int), ("layout", C.c_int)]
+                ("extension_direction", C.c_int), ("blocking", C.c_int), ("layout", C.c_int),
+                ("window", C.c_int)]
 
 
 # Every symbol include/pagerank_hip.h declares, with its ctypes signature (restype, argtypes).

@@ -74,11 +74,12 @@ class Tuning:
     extension_direction: int = 1
     blocking: int = 0
     layout: int = 0
+    window: int = 0
 
 
 def _tuning_of(n) -> Tuning:
     return Tuning(n.switch_degree, n.huge_degree, n.low_width, n.check_interval, n.source_slices, n.block_edges,
-                  n.extension_direction, n.blocking, n.layout)
+                  n.extension_direction, n.blocking, n.layout, n.window)
 
 
 class Context:
@@ -130,7 +131,7 @@ class Context:
         self.check(self._lib.prh_context_set_tuning(
             self._h, C.byref(prh_tuning(t.switch_degree, t.huge_degree, t.low_width, t.check_interval,
                                         t.source_slices, t.block_edges, t.extension_direction, t.blocking,
-                                        t.layout))))
+                                        t.layout, t.window))))
 
     def get_tuning(self) -> Tuning:
         n = prh_tuning()
