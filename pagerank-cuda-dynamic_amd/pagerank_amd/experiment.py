@@ -257,6 +257,14 @@ def run_stream(args, out: CsvLog) -> int:
     stream_wall = (time.perf_counter() - t0) * 1e3
     final_stream = st.ranks()
     st.close()
+    if args.stream_only:  # the session alone (for a kernel trace of it)
+        s_iters = sum(r.iterations for r in stream_res)
+        log(f"stream: {stream_wall:.2f} ms wall for {args.batches} batches ({stream_wall / args.batches:.3f} ms/batch), "
+            f"device apply+mark {sum(r.initializationTime for r in stream_res):.2f} ms, iterations "
+            f"{sum(r.computationTime for r in stream_res):.2f} ms, {s_iters} iterations")
+        out.row(mode="stream", graph=name, vertices=x.N, edges=x.M, batch_fraction=frac, technique="dfp_stream",
+                time_ms=stream_wall, iterations=s_iters, extra=f"batches={args.batches}")
+        return 0
     # separate calls: host apply (prepared above, not timed), upload, DF-P solve
     q = q0
     t0 = time.perf_counter()
@@ -416,6 +424,8 @@ def main(argv=None) -> int:
                     help="dynamic, temporal: build each updated graph on the host and upload it (a new layout and plan "
                          "per batch) instead of updating the device graph in place")
     ap.add_argument("--columns", type=int, default=64)
+    ap.add_argument("--stream-only", action="store_true",
+                    help="stream mode: run the streaming session alone (no separate-call or in-place comparison)")
     ap.add_argument("--csv", default=None)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch plan (dynamic, temporal, ppr, adjoint)")
     ap.add_argument("--tuning-cache", default=None, help="the autotuner's JSON cache")
