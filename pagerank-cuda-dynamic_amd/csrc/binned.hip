@@ -145,14 +145,15 @@ __global__ __launch_bounds__(kBinBlock) void expand_kernel(BinnedPlan P, const d
   d2 *x2 = reinterpret_cast<d2 *>(P.X);
   const uint32_t p0 = k0 / 2, p1 = k1 / 2;
   uint32_t p = p0 + threadIdx.x;
-  for (; p + 3 * kBinBlock < p1; p += 4 * kBinBlock) {
-    const uint32_t a0 = __builtin_nontemporal_load(s32 + p), a1 = __builtin_nontemporal_load(s32 + p + kBinBlock);
-    const uint32_t a2 = __builtin_nontemporal_load(s32 + p + 2 * kBinBlock);
-    const uint32_t a3 = __builtin_nontemporal_load(s32 + p + 3 * kBinBlock);
-    __builtin_nontemporal_store(d2{c[a0 & 0xFFFFu], c[a0 >> 16]}, x2 + p);
-    __builtin_nontemporal_store(d2{c[a1 & 0xFFFFu], c[a1 >> 16]}, x2 + p + kBinBlock);
-    __builtin_nontemporal_store(d2{c[a2 & 0xFFFFu], c[a2 >> 16]}, x2 + p + 2 * kBinBlock);
-    __builtin_nontemporal_store(d2{c[a3 & 0xFFFFu], c[a3 >> 16]}, x2 + p + 3 * kBinBlock);
+  // Eight source-id loads in flight per lane before their stores (four: 4351 against 4229 us per update at 2.04B
+  // edges).
+  for (; p + 7 * kBinBlock < p1; p += 8 * kBinBlock) {
+    uint32_t a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = __builtin_nontemporal_load(s32 + p + u * kBinBlock);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      __builtin_nontemporal_store(d2{c[a[u] & 0xFFFFu], c[a[u] >> 16]}, x2 + p + u * kBinBlock);
   }
   for (; p < p1; p += kBinBlock) {
     const uint32_t a = __builtin_nontemporal_load(s32 + p);

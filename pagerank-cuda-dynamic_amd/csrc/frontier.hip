@@ -154,6 +154,63 @@ __global__ __launch_bounds__(kBlock) void extend_batch_kernel(const uint32_t *de
   }
 }
 
+// The same two steps over a tidied batch held as out-CSR keys (source << 32 | target), as the streaming session keeps
+// it on the device: deletions dk[0, *nd) in entries [0, cap) and insertions ik[0, *ni) in entries [cap, 2 cap); an
+// entry past its side's count has no source and owns no chunk. Sizes come from the capacity, so the launches can be
+// captured once and replayed for every batch.
+__device__ __forceinline__ bool key_source(const unsigned long long *dk, const uint32_t *nd,
+                                           const unsigned long long *ik, const uint32_t *ni, uint32_t cap, uint32_t i,
+                                           uint32_t &u) {
+  if (i < cap) {
+    if (i >= *nd) return false;
+    u = uint32_t(dk[i] >> 32);
+    return true;
+  }
+  i -= cap;
+  if (i >= *ni) return false;
+  u = uint32_t(ik[i] >> 32);
+  return true;
+}
+
+__global__ __launch_bounds__(kBlock) void batch_chunks_keys_kernel(const unsigned long long *dk, const uint32_t *nd,
+                                                                   const unsigned long long *ik, const uint32_t *ni,
+                                                                   uint32_t cap, const uint32_t *off, uint8_t *naff,
+                                                                   uint32_t *cnt) {
+  const uint32_t n = 2 * cap;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i <= n; i += gridDim.x * kBlock) {
+    uint32_t u = 0;
+    cnt[i] = i < n && key_source(dk, nd, ik, ni, cap, i, u) && claim_source(naff, u)
+                 ? (off[u + 1] - off[u] + kBatchChunk - 1) / kBatchChunk
+                 : 0u;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void extend_batch_keys_kernel(const unsigned long long *dk, const uint32_t *nd,
+                                                                   const unsigned long long *ik, const uint32_t *ni,
+                                                                   uint32_t cap, const uint32_t *off,
+                                                                   const uint32_t *dst, const uint32_t *start,
+                                                                   uint8_t *vaff) {
+  const uint32_t n = 2 * cap;
+  const uint32_t total = __builtin_amdgcn_readfirstlane(start[n]);
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t nw = gridDim.x * kWavesPerBlock;
+  for (uint32_t t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave); t < total;
+       t += nw) {
+    uint32_t lo = 0, hi = n;  // start[n] = total > t, so the owner is below n
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (__builtin_amdgcn_readfirstlane(start[mid]) <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    uint32_t u = 0;
+    key_source(dk, nd, ik, ni, cap, lo, u);  // entry lo owns chunk t, so it has a source
+    u = __builtin_amdgcn_readfirstlane(u);
+    const uint32_t e0 = off[u] + (t - start[lo]) * kBatchChunk;
+    const uint32_t e1 = min(off[u + 1], e0 + kBatchChunk);
+    for (uint32_t e = e0 + lane; e < e1; e += kWave) vaff[dst[e]] = 1;
+  }
+}
+
 constexpr int kDirThreads = 1024;
 
 // One workgroup: totals of the per-workgroup costs, then the choice (pull only when strictly cheaper).
@@ -270,6 +327,19 @@ hipError_t launch_extend_batch(uint8_t *vaff, uint8_t *naff, const uint32_t *del
   hipError_t e = scan_u32(tmp, tmp_bytes, cnt, start, n + 1, s);
   if (e != hipSuccess) return e;
   extend_batch_kernel<<<1024, kBlock, 0, s>>>(delu, ND, insu, NI, off, dst, start, vaff);
+  return hipGetLastError();
+}
+
+hipError_t launch_extend_batch_keys(uint8_t *vaff, uint8_t *naff, const unsigned long long *dk, const uint32_t *nd,
+                                    const unsigned long long *ik, const uint32_t *ni, uint32_t cap,
+                                    const uint32_t *off, const uint32_t *dst, uint32_t *cnt, uint32_t *start,
+                                    void *tmp, size_t &tmp_bytes, hipStream_t s) {
+  const size_t n = 2 * size_t(cap);
+  if (!tmp) return scan_u32(nullptr, tmp_bytes, cnt, start, n + 1, s);
+  batch_chunks_keys_kernel<<<clamp_blocks(n + 1, kBlock, 1024), kBlock, 0, s>>>(dk, nd, ik, ni, cap, off, naff, cnt);
+  hipError_t e = scan_u32(tmp, tmp_bytes, cnt, start, n + 1, s);
+  if (e != hipSuccess) return e;
+  extend_batch_keys_kernel<<<1024, kBlock, 0, s>>>(dk, nd, ik, ni, cap, off, dst, start, vaff);
   return hipGetLastError();
 }
 

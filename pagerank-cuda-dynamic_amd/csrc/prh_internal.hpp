@@ -451,6 +451,12 @@ hipError_t launch_mark_partial(uint8_t *naff, uint8_t *vaff, const uint32_t *del
 hipError_t launch_extend_batch(uint8_t *vaff, uint8_t *naff, const uint32_t *delu, size_t ND, const uint32_t *insu,
                                size_t NI, const uint32_t *off, const uint32_t *dst, uint32_t *cnt, uint32_t *start,
                                void *tmp, size_t &tmp_bytes, hipStream_t s);
+// The same from a tidied batch of out-CSR keys (source << 32 | target) with device counts: deletions dk[0, *nd),
+// insertions ik[0, *ni), at most cap each; cnt and start hold 2 cap + 1 entries (the streaming session's marking).
+hipError_t launch_extend_batch_keys(uint8_t *vaff, uint8_t *naff, const unsigned long long *dk, const uint32_t *nd,
+                                    const unsigned long long *ik, const uint32_t *ni, uint32_t cap,
+                                    const uint32_t *off, const uint32_t *dst, uint32_t *cnt, uint32_t *start,
+                                    void *tmp, size_t &tmp_bytes, hipStream_t s);
 hipError_t launch_extend(const ExtendArgs &A, unsigned long long *err, int *done, int l, double tol,
                          hipStream_t s);
 // Sums the per-workgroup costs the rank launches left in stats[0 .. 2 n) and sets *direction to 1 (pull) when
