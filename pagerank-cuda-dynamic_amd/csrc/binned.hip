@@ -165,7 +165,13 @@ constexpr uint32_t kPairStart = 0x8000u, kRunStart = 0x4000u;
 // A slot whose edge an in-place update deleted (bins of at most 8192 vertices, whose local ids take 13 bits): it
 // keeps its place, and its pair and run flags, but adds nothing.
 constexpr uint32_t kDeadSlot = 0x2000u;
-constexpr int kAccGroups = 2;  // groups of 64 slots a wave of accumulate has in flight (2 measured faster than 4 or 8)
+// Groups of 64 slots a wave of accumulate has in flight: 3 for Static / Naive-dynamic (the update at 2.04B edges:
+// 4189 us; with 2 groups 4231 us, and 4 were slower than 2), 2 for the others (DF-P's accumulate took 4400 us per
+// iteration at 2.04B edges with 3 groups, 3187 us with 2).
+template <int VARIANT>
+constexpr int acc_groups() {
+  return VARIANT == kPlain ? 3 : 2;
+}
 
 // X entry of the pair of slot g + lane, for a group of 64 slots starting at a multiple of 64, from t (the group
 // table: run r0 of slot g, run_off of runs r0 and r0 + 1, pair q0 of slot g) and d (the slot's 16-bit local
@@ -222,6 +228,7 @@ __device__ __forceinline__ void add_slots(double *sum, uint8_t *ext, uint32_t d,
 template <int B, int VARIANT>
 __global__ __launch_bounds__(kBinBlock) void accumulate_kernel(RankArgs A, BinnedPlan P, LoopCtl ctl) {
   if (!iteration_begin(ctl)) return;
+  constexpr int kAccGroups = acc_groups<VARIANT>();
   constexpr bool EXT = Traits<VARIANT>::frontier;
   constexpr bool WEIGHTED = Traits<VARIANT>::weighted;
   constexpr uint32_t kMask = uint32_t(B) - 1;  // local destination id
