@@ -61,3 +61,21 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_native, "ENGINE_LIB", tmp_path / "absent.so")
     with pytest.raises(RuntimeError, match="not built"):
         _native.engine()
+
+
+def test_default_tuning_matches_python_mirror():
+    """prh_default_tuning fills exactly the fields the ctypes mirror declares (the reference's fixed launch constants
+    where it has them, the banded plan off) and writes nothing past them."""
+    from pagerank_amd import Tuning, _native
+
+    class Guarded(C.Structure):
+        _fields_ = [("t", _native.prh_tuning), ("guard", C.c_uint32 * 8)]
+
+    g = Guarded()
+    for i in range(8):
+        g.guard[i] = 0xA5A5A5A5
+    _native.engine().prh_default_tuning(C.byref(g.t))
+    assert list(g.guard) == [0xA5A5A5A5] * 8
+    got = {name: getattr(g.t, name) for name, _ in _native.prh_tuning._fields_}
+    assert got == Tuning().__dict__
+    assert (got["switch_degree"], got["blocking"], got["layout"], got["window"]) == (64, 0, 0, 0)
