@@ -27,10 +27,13 @@ namespace {
 using prhc::fail;
 using prhc::Temps;
 
-__global__ void count_zero_kernel(const uint32_t *deg, uint32_t N, uint32_t *out) {
+// Vertices with no out-edge. A wave counts its lanes' zeros with a ballot per step; lane 0 takes part in every step
+// any lane of its wave takes part in (its vertex is the wave's lowest), so its count is the wave's.
+__global__ __launch_bounds__(256) void count_zero_kernel(const uint32_t *deg, uint32_t N, uint32_t *out) {
   uint32_t z = 0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) z += deg[v] == 0;
-  if (z) atomicAdd(out, z);
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+    z += uint32_t(__popcll(__ballot(deg[v] == 0)));
+  if (z && (threadIdx.x & 63) == 0) atomicAdd(out, z);
 }
 
 // The tidied keys of one CSR side: keys sorted, first of each key whose edge exists (deletions) or is absent
@@ -317,7 +320,7 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
     uint32_t *chpos = reinterpret_cast<uint32_t *>(delta);  // the degree changes are spent
     PRH_HIP(c, prh::scan_u8(scan_tmp, scan_bytes, changed, chpos, N, s));
     PRH_HIP(c, prh::launch_rewrite(off, idx, noff, changed, chpos, units, ranges, compact[kd], compact[ki], N,
-                                   max_units, M, nidx, s));
+                                   max_units, M, n_cap, nidx, s));
     PRH_HIP(c, prh::launch_insert(off, idx, noff, ranges, compact[kd], compact[ki], counts + ki, ci, nidx, s));
     return PRH_SUCCESS;
   };
@@ -371,7 +374,7 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   uint32_t *zeros = nullptr;
   PRH_CHECK(tmp.get(&zeros, 1));
   PRH_HIP(c, prh::launch_fill_u32(zeros, 0, 1, s));
-  count_zero_kernel<<<256, 256, 0, s>>>(g->l_outdeg, N, zeros);
+  count_zero_kernel<<<std::min<uint32_t>(8192, (N + 255) / 256), 256, 0, s>>>(g->l_outdeg, N, zeros);
   PRH_HIP(c, hipGetLastError());
   // patch_plan reads its own words back through the pinned area: read the dead-end count after it
   if (patch) PRH_CHECK(patch_plan(c, g, compact[0], counts + 0, cnt[0], compact[1], counts + 1, cnt[1]));

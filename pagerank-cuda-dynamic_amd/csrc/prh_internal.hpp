@@ -417,12 +417,14 @@ hipError_t launch_row_ranges(const unsigned long long *dkeys, const uint32_t *nd
 hipError_t launch_new_degrees(const uint32_t *off, const int *delta, uint32_t N, uint32_t *deg, int *flags,
                               int empty_bit, hipStream_t s);
 // units: 4 * max_units + 1 words (max_units >= changed rows + 1); pos: N words, the exclusive scan of changed
-// (scan_u8).
+// (scan_u8); max_edges: the old edge count or a bound on it (sizes the grid); new_cap: entries of new_idx (no store
+// goes past it).
 hipError_t scan_u8(void *tmp, size_t &tmp_bytes, const uint8_t *in, uint32_t *out, size_t n, hipStream_t s);
 hipError_t launch_rewrite(const uint32_t *old_off, const uint32_t *old_idx, const uint32_t *new_off,
                           const uint8_t *changed, const uint32_t *pos, uint32_t *units, const uint4 *ranges,
                           const unsigned long long *dkeys, const unsigned long long *ikeys, uint32_t N,
-                          uint32_t max_units, uint64_t max_edges, uint32_t *new_idx, hipStream_t s);
+                          uint32_t max_units, uint64_t max_edges, uint64_t new_cap, uint32_t *new_idx,
+                          hipStream_t s);
 hipError_t launch_insert(const uint32_t *old_off, const uint32_t *old_idx, const uint32_t *new_off,
                          const uint4 *ranges, const unsigned long long *dkeys, const unsigned long long *ikeys,
                          const uint32_t *ni, uint32_t cap, uint32_t *new_idx, hipStream_t s);

@@ -139,7 +139,7 @@ int stream_apply_csr(prh_stream *S, hipStream_t s, int kd, int ki, const uint32_
   uint32_t *chpos = reinterpret_cast<uint32_t *>(S->delta);  // the degree changes are spent
   PRH_HIP(c, prh::scan_u8(S->scan_tmp, S->scan_bytes, S->changed, chpos, N, s));
   PRH_HIP(c, prh::launch_rewrite(off, idx, noff, S->changed, chpos, S->units, S->ranges, S->compact[kd],
-                                 S->compact[ki], N, 2 * cap + 1, S->cap_edges, nidx, s));
+                                 S->compact[ki], N, 2 * cap + 1, S->cap_edges, S->cap_edges, nidx, s));
   PRH_HIP(c, prh::launch_insert(off, idx, noff, S->ranges, S->compact[kd], S->compact[ki], S->counts + ki, cap, nidx,
                                 s));
   return PRH_SUCCESS;

@@ -138,7 +138,10 @@ __global__ __launch_bounds__(kBlock) void new_degrees_kernel(const uint32_t *off
   }
 }
 
-constexpr uint32_t kRewriteTile = 4096;  // old edges per workgroup
+constexpr uint32_t kRewriteTile = 16384;  // old edges per workgroup
+constexpr int kRewriteLoads = 16;         // loads a thread keeps in flight
+constexpr uint32_t kTileUnits = 8;        // a tile over more units than this moves edge by edge
+constexpr uint32_t kRowKeys = kBlock;     // a changed row's deletions and insertions staged in LDS, up to this many
 
 // Units of the rewrite: unit i (i < C) is the run of unchanged rows before the i-th changed row R_i, then R_i
 // itself; unit C runs from the last changed row's end to the last edge. Per unit: ue = old end of R_i (its edges are
@@ -182,39 +185,143 @@ __device__ __forceinline__ uint32_t unit_of(const uint32_t *ue, uint32_t lo, uin
   return lo;
 }
 
-// Workgroup b moves old edges [b * tile, (b + 1) * tile). The tile's units are found once; an edge of an unchanged
-// row moves by its unit's shift, an edge of a changed row searches only that row's deletions and insertions.
-__global__ __launch_bounds__(kBlock) void rewrite_kernel(const uint32_t *old_off, const uint32_t *old_idx,
-                                                         RewriteUnits U, const uint4 *ranges,
-                                                         const unsigned long long *dkeys,
-                                                         const unsigned long long *ikeys, uint32_t N,
-                                                         uint32_t *new_idx) {
+// The same search by the whole workgroup (every thread calls it with the same arguments): each round tests kBlock
+// evenly spaced units at once and keeps the interval after the last one whose end is at or below e, so a search over
+// C units takes about log_kBlock(C) rounds of one load each. ue is non-decreasing (empty changed rows repeat an end)
+// and the answer lies in [lo, hi].
+__device__ uint32_t block_unit_of(const uint32_t *ue, uint32_t lo, uint32_t hi, uint32_t e) {
+  while (lo < hi) {
+    const uint32_t step = (hi - lo + kBlock - 1) / kBlock;
+    const uint32_t p = lo + threadIdx.x * step;
+    const uint32_t below = uint32_t(__syncthreads_count(p < hi && ue[p] <= e));
+    if (below == 0) return lo;
+    const uint32_t last = lo + (below - 1) * step;  // ue[last] <= e; unit last + step, if below hi, ends above e
+    lo = last + 1;
+    hi = min(hi, last + step);
+  }
+  return lo;
+}
+
+// A store that stays inside the new index array whatever position it is given.
+__device__ __forceinline__ void store_in(uint32_t *__restrict__ dst, int64_t p, uint64_t cap, uint32_t v) {
+  if (uint64_t(p) < cap) dst[p] = v;
+}
+
+// Old edges [a, b) of unchanged rows to their new places, shifted by `shift`; each thread keeps kRewriteLoads loads
+// in flight before it stores.
+__device__ __forceinline__ void move_run(const uint32_t *__restrict__ src, uint64_t a, uint64_t b, int64_t shift,
+                                         uint32_t *__restrict__ dst, uint64_t cap) {
+  for (uint64_t e = a + threadIdx.x; e < b; e += uint64_t(kRewriteLoads) * kBlock) {
+    uint32_t v[kRewriteLoads];
+#pragma unroll
+    for (int k = 0; k < kRewriteLoads; ++k)
+      if (e + uint64_t(k) * kBlock < b) v[k] = src[e + uint64_t(k) * kBlock];
+#pragma unroll
+    for (int k = 0; k < kRewriteLoads; ++k)
+      if (e + uint64_t(k) * kBlock < b) store_in(dst, int64_t(e + uint64_t(k) * kBlock) + shift, cap, v[k]);
+  }
+}
+
+// Old edges [a, b) of changed row `row` (old start ub, new start nb; its deletions dkeys[r.x, r.y) and insertions
+// ikeys[r.z, r.w), keys row << 32 | column, sorted): deleted columns are dropped, the others move past the row's
+// earlier deletions and insertions. The row's columns of both kinds are staged in LDS when they fit; all threads call
+// this (it synchronises).
+__device__ void move_changed_row(const uint32_t *__restrict__ src, uint64_t a, uint64_t b, uint32_t row, uint32_t ub,
+                                 uint32_t nb, uint4 r, const unsigned long long *__restrict__ dkeys,
+                                 const unsigned long long *__restrict__ ikeys, uint32_t *sdel, uint32_t *sins,
+                                 uint32_t *__restrict__ dst, uint64_t cap) {
+  const uint32_t nd = r.y - r.x, ni = r.w - r.z;
+  const bool staged = nd <= kRowKeys && ni <= kRowKeys;
+  if (staged) {
+    if (threadIdx.x < nd) sdel[threadIdx.x] = uint32_t(dkeys[r.x + threadIdx.x]);
+    if (threadIdx.x < ni) sins[threadIdx.x] = uint32_t(ikeys[r.z + threadIdx.x]);
+    __syncthreads();
+  }
+  for (uint64_t e = a + threadIdx.x; e < b; e += uint64_t(kRewriteLoads) * kBlock) {
+    uint32_t v[kRewriteLoads];
+#pragma unroll
+    for (int k = 0; k < kRewriteLoads; ++k)
+      if (e + uint64_t(k) * kBlock < b) v[k] = src[e + uint64_t(k) * kBlock];
+#pragma unroll
+    for (int k = 0; k < kRewriteLoads; ++k) {
+      const uint64_t ek = e + uint64_t(k) * kBlock;
+      if (ek >= b) continue;
+      const uint32_t col = v[k];
+      uint32_t d_at, i_at;
+      if (staged) {
+        d_at = lower_bound(sdel, 0u, nd, col);
+        if (d_at < nd && sdel[d_at] == col) continue;  // deleted
+        i_at = lower_bound(sins, 0u, ni, col);
+      } else {
+        const unsigned long long key = (unsigned long long)row << 32 | col;
+        d_at = lower_bound(dkeys, r.x, r.y, key);
+        if (d_at < r.y && dkeys[d_at] == key) continue;  // deleted
+        d_at -= r.x;
+        i_at = lower_bound(ikeys, r.z, r.w, key) - r.z;
+      }
+      store_in(dst, int64_t(nb) + int64_t(ek - ub) - int64_t(d_at) + int64_t(i_at), cap, col);
+    }
+  }
+  __syncthreads();  // the staged columns are read before the next row overwrites them
+}
+
+// Workgroup b moves old edges [b * tile, (b + 1) * tile) into the new index array (new_cap entries). The tile's
+// first and last units are found by the whole workgroup. A tile over a few units moves unit by unit: the unchanged
+// rows as one shifted run, the changed row against its staged deletions and insertions. A tile over many units (many
+// short changed rows) moves edge by edge, each finding its unit by binary search.
+__global__ __launch_bounds__(kBlock) void rewrite_kernel(const uint32_t *__restrict__ old_off,
+                                                         const uint32_t *__restrict__ old_idx, RewriteUnits U,
+                                                         const uint4 *__restrict__ ranges,
+                                                         const unsigned long long *__restrict__ dkeys,
+                                                         const unsigned long long *__restrict__ ikeys, uint32_t N,
+                                                         uint64_t new_cap, uint32_t *__restrict__ new_idx) {
+  __shared__ uint32_t sdel[kRowKeys], sins[kRowKeys];
   const uint32_t M = old_off[N];
-  const uint32_t e0 = blockIdx.x * kRewriteTile;
+  const uint64_t e0 = uint64_t(blockIdx.x) * kRewriteTile;
   if (e0 >= M) return;
-  const uint32_t e1 = min(M, e0 + kRewriteTile);
+  const uint64_t e1 = min(uint64_t(M), e0 + kRewriteTile);
   const uint32_t C = *U.count;
-  const uint32_t i0 = unit_of(U.ue, 0u, C, e0), i1 = unit_of(U.ue, i0, C, e1 - 1);
-  if (i0 == i1 && e1 <= U.ub[i0]) {  // the whole tile lies in unchanged rows of one unit
-    const int64_t shift = int64_t(U.unb[i0]) - int64_t(U.ub[i0]);
-    for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) new_idx[int64_t(e) + shift] = old_idx[e];
+  const uint32_t i0 = block_unit_of(U.ue, 0u, C, uint32_t(e0));
+  const uint32_t i1 = block_unit_of(U.ue, i0, C, uint32_t(e1 - 1));
+  if (i1 - i0 < kTileUnits) {
+    for (uint32_t i = i0; i <= i1; ++i) {
+      const uint32_t ub = U.ub[i], ue = U.ue[i], nb = U.unb[i];
+      const uint64_t lo = i == i0 ? e0 : uint64_t(U.ue[i - 1]);  // unit i starts where unit i - 1 ends
+      const uint64_t hi = min(e1, uint64_t(ue));
+      const uint64_t mid = max(lo, min(hi, uint64_t(ub)));
+      if (lo < mid) move_run(old_idx, lo, mid, int64_t(nb) - int64_t(ub), new_idx, new_cap);
+      if (mid < hi) {
+        const uint32_t row = U.urow[i];
+        move_changed_row(old_idx, mid, hi, row, ub, nb, ranges[row], dkeys, ikeys, sdel, sins, new_idx, new_cap);
+      }
+    }
     return;
   }
-  for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
-    const uint32_t i = unit_of(U.ue, i0, i1, e);
-    const uint32_t col = old_idx[e];
-    const uint32_t b = U.ub[i];
-    if (e < b) {
-      new_idx[int64_t(e) + int64_t(U.unb[i]) - int64_t(b)] = col;
-      continue;
+  for (uint64_t e = e0 + threadIdx.x; e < e1; e += uint64_t(kRewriteLoads) * kBlock) {
+    uint32_t v[kRewriteLoads];
+#pragma unroll
+    for (int k = 0; k < kRewriteLoads; ++k)
+      if (e + uint64_t(k) * kBlock < e1) v[k] = old_idx[e + uint64_t(k) * kBlock];
+#pragma unroll
+    for (int k = 0; k < kRewriteLoads; ++k) {
+      const uint64_t ek = e + uint64_t(k) * kBlock;
+      if (ek >= e1) continue;
+      const uint32_t col = v[k];
+      const uint32_t i = unit_of(U.ue, i0, i1, uint32_t(ek));
+      const uint32_t b = U.ub[i];
+      if (ek < b) {
+        store_in(new_idx, int64_t(ek) + int64_t(U.unb[i]) - int64_t(b), new_cap, col);
+        continue;
+      }
+      const uint32_t row = U.urow[i];
+      const uint4 r = ranges[row];
+      const unsigned long long key = (unsigned long long)row << 32 | col;
+      const uint32_t d_at = lower_bound(dkeys, r.x, r.y, key);
+      if (d_at < r.y && dkeys[d_at] == key) continue;  // deleted
+      const uint32_t i_at = lower_bound(ikeys, r.z, r.w, key);
+      store_in(new_idx, int64_t(U.unb[i]) + int64_t(ek - b) - int64_t(d_at - r.x) + int64_t(i_at - r.z), new_cap,
+               col);
     }
-    const uint32_t row = U.urow[i];
-    const uint4 r = ranges[row];
-    const unsigned long long key = (unsigned long long)row << 32 | col;
-    const uint32_t d_at = lower_bound(dkeys, r.x, r.y, key);
-    if (d_at < r.y && dkeys[d_at] == key) continue;  // deleted
-    const uint32_t i_at = lower_bound(ikeys, r.z, r.w, key);
-    new_idx[U.unb[i] + (e - b) - (d_at - r.x) + (i_at - r.z)] = col;
   }
 }
 
@@ -346,13 +453,15 @@ hipError_t scan_u8(void *tmp, size_t &tmp_bytes, const uint8_t *in, uint32_t *ou
 hipError_t launch_rewrite(const uint32_t *old_off, const uint32_t *old_idx, const uint32_t *new_off,
                           const uint8_t *changed, const uint32_t *pos, uint32_t *units, const uint4 *ranges,
                           const unsigned long long *dkeys, const unsigned long long *ikeys, uint32_t N,
-                          uint32_t max_units, uint64_t max_edges, uint32_t *new_idx, hipStream_t s) {
+                          uint32_t max_units, uint64_t max_edges, uint64_t new_cap, uint32_t *new_idx,
+                          hipStream_t s) {
   if (N == 0) return hipSuccess;
   const RewriteUnits U{units, units + max_units, units + 2 * size_t(max_units), units + 3 * size_t(max_units),
                        units + 4 * size_t(max_units)};
   units_kernel<<<grid_for(N, kBlock, 16384), kBlock, 0, s>>>(old_off, new_off, changed, pos, N, U);
   const uint64_t g = (max_edges + kRewriteTile - 1) / kRewriteTile;
-  rewrite_kernel<<<uint32_t(g ? g : 1), kBlock, 0, s>>>(old_off, old_idx, U, ranges, dkeys, ikeys, N, new_idx);
+  rewrite_kernel<<<uint32_t(g ? g : 1), kBlock, 0, s>>>(old_off, old_idx, U, ranges, dkeys, ikeys, N, new_cap,
+                                                        new_idx);
   return hipGetLastError();
 }
 
