@@ -17,7 +17,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
+#include <iterator>
 #include <vector>
 
 #include "capi_internal.hpp"
@@ -148,6 +150,142 @@ int csr_to_caller(prh_context *c, const uint32_t *perm_d, const uint32_t *inv_d,
   return PRH_SUCCESS;
 }
 
+// One CSR side of an update: the rows of (off, idx), M edges, less the tidied deletions dk plus the tidied insertions
+// ik (sorted keys row << 32 | column; their counts on the device at dcnt / icnt, at most nd / ni), written to (noff,
+// nidx) of capacity ncap. Rows keep their columns ascending.
+int apply_side(prh_context *c, Temps &tmp, uint32_t N, uint64_t M, const uint32_t *off, const uint32_t *idx,
+               const unsigned long long *dk, const uint32_t *dcnt, uint32_t nd, const unsigned long long *ik,
+               const uint32_t *icnt, uint32_t ni, uint32_t *noff, uint32_t *nidx, uint64_t ncap) {
+  hipStream_t s = c->stream;
+  const size_t N1 = size_t(N) + 1;
+  PRH_CHECK(ensure(c, c->upd_delta, N));
+  PRH_CHECK(ensure(c, c->upd_changed, N));
+  PRH_CHECK(ensure(c, c->upd_deg, N1));
+  PRH_CHECK(ensure(c, c->upd_ranges, N));
+  int *delta = c->upd_delta.p;
+  uint8_t *changed = c->upd_changed.p;
+  uint32_t *new_deg = c->upd_deg.p;
+  uint4 *ranges = c->upd_ranges.p;
+  uint32_t *chpos = reinterpret_cast<uint32_t *>(delta);  // positions of the changed rows, once the degrees are spent
+  size_t scan_bytes = 0, b1 = 0;
+  PRH_HIP(c, prh::scan_u32(nullptr, scan_bytes, new_deg, noff, N1, s));
+  PRH_HIP(c, prh::scan_u8(nullptr, b1, changed, chpos, N, s));
+  scan_bytes = std::max(scan_bytes, b1);
+  // rewrite units: one per changed row and a closing one (a side has at most nd + ni changed rows)
+  const uint32_t max_units = nd + ni + 1;
+  uint8_t *scan_tmp = nullptr;
+  uint32_t *units = nullptr;
+  int *flags = nullptr;
+  PRH_CHECK(tmp.get(&scan_tmp, scan_bytes));
+  PRH_CHECK(tmp.get(&units, 4 * size_t(max_units) + 1));
+  PRH_CHECK(tmp.get(&flags, 1));
+  PRH_HIP(c, prh::launch_fill_i32(delta, 0, N, s));
+  PRH_HIP(c, prh::launch_fill_u8(changed, 0, N, s));
+  PRH_HIP(c, prh::launch_row_ranges(dk, dcnt, ik, icnt, nd + ni, delta, changed, ranges, s));
+  PRH_HIP(c, prh::launch_new_degrees(off, delta, N, new_deg, flags, 0, s));
+  PRH_HIP(c, prh::scan_u32(scan_tmp, scan_bytes, new_deg, noff, N1, s));
+  PRH_HIP(c, prh::scan_u8(scan_tmp, scan_bytes, changed, chpos, N, s));
+  PRH_HIP(c, prh::launch_rewrite(off, idx, noff, changed, chpos, units, ranges, dk, ik, N, max_units, M, ncap, nidx,
+                                 s));
+  PRH_HIP(c, prh::launch_insert(off, idx, noff, ranges, dk, ik, icnt, ni, nidx, s));
+  return PRH_SUCCESS;
+}
+
+// The net change of the in-CSR since the plan's base (D: base edges now absent, I: edges now present that the base
+// lacks) after one more tidied batch (d: edges it deletes, all present before it; i: edges it inserts, all absent
+// before it; every vector ascending). A deleted edge leaves I if an earlier batch inserted it, else joins D; an
+// inserted edge leaves D if an earlier batch deleted it, else joins I.
+void compose_net(std::vector<unsigned long long> &D, std::vector<unsigned long long> &I,
+                 const std::vector<unsigned long long> &d, const std::vector<unsigned long long> &i) {
+  std::vector<unsigned long long> d_in_I, d_rest, i_in_D, i_rest, D2, I2;
+  std::set_intersection(d.begin(), d.end(), I.begin(), I.end(), std::back_inserter(d_in_I));
+  std::set_difference(d.begin(), d.end(), I.begin(), I.end(), std::back_inserter(d_rest));
+  std::set_intersection(i.begin(), i.end(), D.begin(), D.end(), std::back_inserter(i_in_D));
+  std::set_difference(i.begin(), i.end(), D.begin(), D.end(), std::back_inserter(i_rest));
+  std::set_difference(D.begin(), D.end(), i_in_D.begin(), i_in_D.end(), std::back_inserter(D2));
+  std::set_difference(I.begin(), I.end(), d_in_I.begin(), d_in_I.end(), std::back_inserter(I2));
+  D.clear();
+  I.clear();
+  std::merge(D2.begin(), D2.end(), d_rest.begin(), d_rest.end(), std::back_inserter(D));
+  std::merge(I2.begin(), I2.end(), i_rest.begin(), i_rest.end(), std::back_inserter(I));
+}
+
+// New arrays for a CSR side of M2 edges: the side's spare arrays when they hold M2 edges, else new ones with room for
+// growth (1/256 of the edges), which later updates reuse.
+int side_arrays(prh_context *c, uint32_t N, uint64_t M2, uint32_t *&spare_off, uint32_t *&spare_idx,
+                uint64_t &spare_cap, uint32_t **noff, uint32_t **nidx, uint64_t *ncap) {
+  const size_t N1 = size_t(N) + 1;
+  if (spare_idx && spare_cap >= M2) {
+    *noff = spare_off;
+    *nidx = spare_idx;
+    *ncap = spare_cap;
+    spare_off = spare_idx = nullptr;
+    spare_cap = 0;
+    return PRH_SUCCESS;
+  }
+  dev_free(c, spare_off, N1 * 4);
+  dev_free(c, spare_idx, std::max<size_t>(spare_cap, 4) * 4);
+  spare_off = spare_idx = nullptr;
+  spare_cap = 0;
+  *ncap = std::max<uint64_t>(M2 + M2 / 256, 4);
+  PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(noff), N1 * 4));
+  return dev_alloc(c, reinterpret_cast<void **>(nidx), *ncap * 4);
+}
+
+void free_side(prh_context *c, uint32_t N, uint32_t *off, uint32_t *idx, uint64_t cap) {
+  dev_free(c, off, (size_t(N) + 1) * 4);
+  dev_free(c, idx, std::max<size_t>(cap, 4) * 4);
+}
+
+int ensure_in_csr(prh_context *c, prh_graph *g) {
+  if (!g->in_stale) return PRH_SUCCESS;
+  const BinnedCache &bc = g->bin;
+  if (!bc.base_off) return fail(c, PRH_ERROR_DEVICE, "internal: an unbuilt in-CSR without its base");
+  PRH_HIP(c, hipSetDevice(c->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = c->stream;
+  const uint32_t N = g->N;
+  const uint32_t nd = uint32_t(g->in_net_del.size()), ni = uint32_t(g->in_net_ins.size());
+  uint32_t *noff = nullptr, *nidx = nullptr;
+  uint64_t ncap = 0;
+  int st = side_arrays(c, N, g->M, g->spare_in_off, g->spare_in_src, g->spare_in_cap, &noff, &nidx, &ncap);
+  {
+    Temps tmp(c);
+    unsigned long long *dk = nullptr, *ik = nullptr;
+    uint32_t *cnt = nullptr;
+    if (st == PRH_SUCCESS) st = tmp.get(&dk, std::max<uint32_t>(nd, 1));
+    if (st == PRH_SUCCESS) st = tmp.get(&ik, std::max<uint32_t>(ni, 1));
+    if (st == PRH_SUCCESS) st = tmp.get(&cnt, 2);
+    const uint32_t counts[2] = {nd, ni};
+    if (st == PRH_SUCCESS &&
+        ((nd && hipMemcpyAsync(dk, g->in_net_del.data(), size_t(nd) * 8, hipMemcpyHostToDevice, s) != hipSuccess) ||
+         (ni && hipMemcpyAsync(ik, g->in_net_ins.data(), size_t(ni) * 8, hipMemcpyHostToDevice, s) != hipSuccess) ||
+         hipMemcpyAsync(cnt, counts, 8, hipMemcpyHostToDevice, s) != hipSuccess))
+      st = fail(c, PRH_ERROR_DEVICE, "in-CSR build: upload");
+    if (st == PRH_SUCCESS)
+      st = apply_side(c, tmp, N, bc.base_M, bc.base_off, bc.base_src, dk, cnt, nd, ik, cnt + 1, ni, noff, nidx, ncap);
+    if (hipStreamSynchronize(s) != hipSuccess && st == PRH_SUCCESS) st = fail(c, PRH_ERROR_DEVICE, "in-CSR build: sync");
+    uint32_t edges = 0;
+    if (st == PRH_SUCCESS && hipMemcpy(&edges, noff + N, 4, hipMemcpyDeviceToHost) != hipSuccess)
+      st = fail(c, PRH_ERROR_DEVICE, "in-CSR build: readback");
+    if (st == PRH_SUCCESS && edges != g->M)
+      st = fail(c, PRH_ERROR_DEVICE, "internal: the in-CSR build gave %u edges, %llu expected", edges,
+                (unsigned long long)g->M);
+  }
+  if (st != PRH_SUCCESS) {
+    free_side(c, N, noff, nidx, ncap);
+    return st;
+  }
+  g->l_in_off = noff;
+  g->l_in_src = nidx;
+  g->l_in_cap = ncap;
+  g->in_stale = false;
+  const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  g->update_prep.ms = (g->update_prep.reported ? 0.0f : g->update_prep.ms) + ms;
+  g->update_prep.reported = false;
+  return PRH_SUCCESS;
+}
+
 }  // namespace prhc
 
 using namespace prhc;
@@ -164,6 +302,7 @@ int prh_graph_csr(prh_context *c, const prh_graph *g, int which, uint32_t *offse
   }
   if (which == 1 && !g->l_out_off) return fail(c, PRH_ERROR_UNSUPPORTED, "the graph was uploaded without its out-CSR");
   PRH_HIP(c, hipSetDevice(c->device));
+  if (which == 0) PRH_CHECK(ensure_in_csr(c, const_cast<prh_graph *>(g)));
   PRH_HIP(c, hipStreamSynchronize(c->stream));
   return csr_to_caller(c, g->perm, g->inv, g->N, which ? g->l_out_off : g->l_in_off,
                        which ? g->l_out_dst : g->l_in_src, g->M, offsets, ids);
@@ -184,6 +323,12 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   const auto t0 = std::chrono::steady_clock::now();
   const uint64_t M = g->M;
   const size_t N1 = size_t(N) + 1;
+  // The blocked plan (blocks of at most 8192 vertices) is patched. Its iterations read no in-CSR, so the update then
+  // leaves the in-CSR unbuilt: it stays the plan's base with the net change since, built when something reads it
+  // (ensure_in_csr). A plan with wider blocks has no room for the dead flag: it is rebuilt on its next use.
+  BinnedCache &bc = g->bin;
+  const bool patch = bc.valid && bc.B <= 8192 && bc.plan.slot;
+  if (!patch) PRH_CHECK(ensure_in_csr(c, g));
   Temps tmp(c);
   // The layout's out-rows ascending (the tidy searches them), once per graph.
   if (!g->out_sorted) {
@@ -236,13 +381,9 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   PRH_CHECK(tmp.get(&flags, 1));
   PRH_HIP(c, prh::launch_fill_u32(counts, 0, 4, s));
   PRH_HIP(c, prh::launch_fill_i32(flags, 0, 1, s));
-  size_t sort_bytes = 0, scan_bytes = 0, b1 = 0;
+  size_t sort_bytes = 0, scan_bytes = 0;
   PRH_HIP(c, prh::sort_keys_u64(nullptr, sort_bytes, keys[0], sorted, cap, s));
   PRH_HIP(c, prh::scan_u32(nullptr, scan_bytes, valid, pos, cap, s));
-  PRH_HIP(c, prh::scan_u32(nullptr, b1, valid, pos, N1, s));
-  scan_bytes = std::max(scan_bytes, b1);
-  PRH_HIP(c, prh::scan_u8(nullptr, b1, nullptr, pos, N, s));
-  scan_bytes = std::max(scan_bytes, b1);
   uint8_t *sort_tmp = nullptr, *scan_tmp = nullptr;
   PRH_CHECK(tmp.get(&sort_tmp, sort_bytes));
   PRH_CHECK(tmp.get(&scan_tmp, scan_bytes));
@@ -250,14 +391,25 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   if (ci)
     PRH_HIP(c, prh::launch_batch_keys(d_batch + 2 * nd, d_batch + 2 * nd + ni, sizes + 1, ci, g->inv, N, keys[1],
                                       keys[3], flags, 1, s));
-  PRH_CHECK(tidy(c, keys[0], cd, g->l_in_off, g->l_in_src, true, sorted, valid, pos, sort_tmp, sort_bytes, scan_tmp,
-                 scan_bytes, compact[0], counts + 0));
-  PRH_CHECK(tidy(c, keys[1], ci, g->l_in_off, g->l_in_src, false, sorted, valid, pos, sort_tmp, sort_bytes, scan_tmp,
-                 scan_bytes, compact[1], counts + 1));
   PRH_CHECK(tidy(c, keys[2], cd, g->l_out_off, g->l_out_dst, true, sorted, valid, pos, sort_tmp, sort_bytes,
                  scan_tmp, scan_bytes, compact[2], counts + 2));
   PRH_CHECK(tidy(c, keys[3], ci, g->l_out_off, g->l_out_dst, false, sorted, valid, pos, sort_tmp, sort_bytes,
                  scan_tmp, scan_bytes, compact[3], counts + 3));
+  if (patch) {
+    // The in-CSR keys of the tidied batch: its out-CSR keys transposed and sorted.
+    for (int k = 0; k < 2; ++k) {
+      const uint32_t n = k ? ci : cd;
+      if (!n) continue;
+      PRH_HIP(c, prh::launch_transpose_keys(compact[2 + k], counts + 2 + k, n, keys[k], s));
+      PRH_HIP(c, prh::sort_keys_u64(sort_tmp, sort_bytes, keys[k], compact[k], n, s));
+      PRH_HIP(c, hipMemcpyAsync(counts + k, counts + 2 + k, 4, hipMemcpyDeviceToDevice, s));
+    }
+  } else {
+    PRH_CHECK(tidy(c, keys[0], cd, g->l_in_off, g->l_in_src, true, sorted, valid, pos, sort_tmp, sort_bytes,
+                   scan_tmp, scan_bytes, compact[0], counts + 0));
+    PRH_CHECK(tidy(c, keys[1], ci, g->l_in_off, g->l_in_src, false, sorted, valid, pos, sort_tmp, sort_bytes,
+                   scan_tmp, scan_bytes, compact[1], counts + 1));
+  }
   PRH_HIP(c, hipMemcpyAsync(c->pinned, counts, 16, hipMemcpyDeviceToHost, s));
   PRH_HIP(c, hipMemcpyAsync(c->pinned + 2, flags, 4, hipMemcpyDeviceToHost, s));
   PRH_HIP(c, hipStreamSynchronize(s));
@@ -270,63 +422,22 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
                                             "insertions); the graph is unchanged", cnt[0], cnt[2], cnt[1], cnt[3]);
   const uint64_t M2 = M - cnt[0] + cnt[1];
   if (M2 >= (uint64_t(1) << 32)) return fail(c, PRH_ERROR_UNSUPPORTED, "the update would exceed 2^32 - 1 edges");
-  // New CSRs. Nothing of the graph changes before this point, so an error above leaves it as it was.
-  PRH_CHECK(ensure(c, c->upd_delta, N));
-  PRH_CHECK(ensure(c, c->upd_changed, N));
-  PRH_CHECK(ensure(c, c->upd_deg, N1));
-  PRH_CHECK(ensure(c, c->upd_ranges, N));
-  int *delta = c->upd_delta.p;
-  uint8_t *changed = c->upd_changed.p;
-  uint32_t *new_deg = c->upd_deg.p;
-  uint4 *ranges = c->upd_ranges.p;
-  // rewrite units: one per changed row and a closing one (a CSR has at most cd + ci changed rows)
-  const uint32_t max_units = cd + ci + 1;
-  uint32_t *units = nullptr;
-  PRH_CHECK(tmp.get(&units, 4 * size_t(max_units) + 1));
-  // The new CSRs go into the spare arrays an earlier update left, when they hold M2 edges; else into new arrays with
-  // room for growth (1/256 of the edges), which later updates reuse.
+  // New CSRs, in the spare arrays an earlier update left or in new ones. Nothing of the graph changes before this
+  // point, so an error above leaves it as it was.
   uint32_t *n_in_off = nullptr, *n_in_src = nullptr, *n_out_off = nullptr, *n_out_dst = nullptr;
-  uint64_t n_cap = 0;
-  auto alloc = [&](uint32_t **p, size_t n) { return dev_alloc(c, reinterpret_cast<void **>(p), std::max<size_t>(n, 4) * 4); };
-  int st = PRH_SUCCESS;
-  if (g->spare_in_src && g->spare_cap >= M2) {
-    n_in_off = g->spare_in_off;
-    n_in_src = g->spare_in_src;
-    n_out_off = g->spare_out_off;
-    n_out_dst = g->spare_out_dst;
-    n_cap = g->spare_cap;
-    g->spare_in_off = g->spare_in_src = g->spare_out_off = g->spare_out_dst = nullptr;
-    g->spare_cap = 0;
-  } else {
-    dev_free(c, g->spare_in_off, N1 * 4);
-    dev_free(c, g->spare_in_src, std::max<size_t>(g->spare_cap, 4) * 4);
-    dev_free(c, g->spare_out_off, N1 * 4);
-    dev_free(c, g->spare_out_dst, std::max<size_t>(g->spare_cap, 4) * 4);
-    g->spare_in_off = g->spare_in_src = g->spare_out_off = g->spare_out_dst = nullptr;
-    g->spare_cap = 0;
-    n_cap = std::max<uint64_t>(M2 + M2 / 256, 4);
-    st = alloc(&n_in_off, N1);
-    if (st == PRH_SUCCESS) st = alloc(&n_in_src, n_cap);
-    if (st == PRH_SUCCESS) st = alloc(&n_out_off, N1);
-    if (st == PRH_SUCCESS) st = alloc(&n_out_dst, n_cap);
-  }
-  auto apply_csr = [&](int kd, int ki, const uint32_t *off, const uint32_t *idx, uint32_t *noff, uint32_t *nidx) -> int {
-    PRH_HIP(c, prh::launch_fill_i32(delta, 0, N, s));
-    PRH_HIP(c, prh::launch_fill_u8(changed, 0, N, s));
-    PRH_HIP(c, prh::launch_row_ranges(compact[kd], counts + kd, compact[ki], counts + ki, cd + ci, delta, changed,
-                                      ranges, s));
-    PRH_HIP(c, prh::launch_new_degrees(off, delta, N, new_deg, flags, 0, s));
-    PRH_HIP(c, prh::scan_u32(scan_tmp, scan_bytes, new_deg, noff, N1, s));
-    uint32_t *chpos = reinterpret_cast<uint32_t *>(delta);  // the degree changes are spent
-    PRH_HIP(c, prh::scan_u8(scan_tmp, scan_bytes, changed, chpos, N, s));
-    PRH_HIP(c, prh::launch_rewrite(off, idx, noff, changed, chpos, units, ranges, compact[kd], compact[ki], N,
-                                   max_units, M, n_cap, nidx, s));
-    PRH_HIP(c, prh::launch_insert(off, idx, noff, ranges, compact[kd], compact[ki], counts + ki, ci, nidx, s));
-    return PRH_SUCCESS;
-  };
-  if (st == PRH_SUCCESS) st = apply_csr(0, 1, g->l_in_off, g->l_in_src, n_in_off, n_in_src);
-  if (st == PRH_SUCCESS) st = apply_csr(2, 3, g->l_out_off, g->l_out_dst, n_out_off, n_out_dst);
-  // Host copies of the tidied in-CSR keys: the layout's in-degrees kept on the host follow them.
+  uint64_t n_in_cap = 0, n_out_cap = 0;
+  int st = side_arrays(c, N, M2, g->spare_out_off, g->spare_out_dst, g->spare_out_cap, &n_out_off, &n_out_dst,
+                       &n_out_cap);
+  if (st == PRH_SUCCESS && !patch)
+    st = side_arrays(c, N, M2, g->spare_in_off, g->spare_in_src, g->spare_in_cap, &n_in_off, &n_in_src, &n_in_cap);
+  if (st == PRH_SUCCESS && !patch)
+    st = apply_side(c, tmp, N, M, g->l_in_off, g->l_in_src, compact[0], counts + 0, cd, compact[1], counts + 1, ci,
+                    n_in_off, n_in_src, n_in_cap);
+  if (st == PRH_SUCCESS)
+    st = apply_side(c, tmp, N, M, g->l_out_off, g->l_out_dst, compact[2], counts + 2, cd, compact[3], counts + 3, ci,
+                    n_out_off, n_out_dst, n_out_cap);
+  // Host copies of the tidied in-CSR keys: the layout's in-degrees kept on the host and the in-CSR's net change
+  // follow them.
   std::vector<unsigned long long> hd(cnt[0]), hi(cnt[1]);
   if (st == PRH_SUCCESS && cnt[0] &&
       hipMemcpyAsync(hd.data(), compact[0], size_t(cnt[0]) * 8, hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -337,38 +448,46 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   if (st == PRH_SUCCESS && hipStreamSynchronize(s) != hipSuccess) st = fail(c, PRH_ERROR_DEVICE, "update: sync");
   if (st != PRH_SUCCESS) {
     (void)hipStreamSynchronize(s);
-    dev_free(c, n_in_off, N1 * 4);
-    dev_free(c, n_in_src, std::max<size_t>(n_cap, 4) * 4);
-    dev_free(c, n_out_off, N1 * 4);
-    dev_free(c, n_out_dst, std::max<size_t>(n_cap, 4) * 4);
+    free_side(c, N, n_in_off, n_in_src, n_in_cap);
+    free_side(c, N, n_out_off, n_out_dst, n_out_cap);
     return st;
   }
-  // Swap in the new CSRs. The blocked plan (blocks of at most 8192 vertices) is patched; its first update keeps the
-  // in-CSR it was built from as its base. A plan with wider blocks has no room for the dead flag: it is rebuilt on
-  // its next use.
-  // The old CSRs become the spare arrays of the next update, except an in-CSR that becomes the plan's base.
-  BinnedCache &bc = g->bin;
-  const bool patch = bc.valid && bc.B <= 8192 && bc.plan.slot;
-  if (patch && !bc.base_off) {
-    bc.base_off = g->l_in_off;
-    bc.base_src = g->l_in_src;
-    bc.base_M = M;
-    bc.base_cap = g->l_cap;
-    dev_free(c, g->l_out_off, N1 * 4);
-    dev_free(c, g->l_out_dst, std::max<size_t>(g->l_cap, 4) * 4);
+  // Swap in the new CSRs; the old ones become the spare arrays of the next update. With the patched plan, the
+  // in-CSR the plan was built from becomes its base on the first update (ownership moves from the graph), and an
+  // in-CSR built since the last update becomes the spare of the next build.
+  if (patch) {
+    if (!bc.base_off) {
+      bc.base_off = g->l_in_off;
+      bc.base_src = g->l_in_src;
+      bc.base_M = M;
+      bc.base_cap = g->l_in_cap;
+      g->in_net_del.clear();
+      g->in_net_ins.clear();
+    } else if (g->l_in_off) {
+      free_side(c, N, g->spare_in_off, g->spare_in_src, g->spare_in_cap);
+      g->spare_in_off = g->l_in_off;
+      g->spare_in_src = g->l_in_src;
+      g->spare_in_cap = g->l_in_cap;
+    }
+    g->l_in_off = g->l_in_src = nullptr;
+    g->l_in_cap = 0;
+    g->in_stale = true;
+    compose_net(g->in_net_del, g->in_net_ins, hd, hi);
   } else {
     g->spare_in_off = g->l_in_off;
     g->spare_in_src = g->l_in_src;
-    g->spare_out_off = g->l_out_off;
-    g->spare_out_dst = g->l_out_dst;
-    g->spare_cap = g->l_cap;
+    g->spare_in_cap = g->l_in_cap;
+    g->l_in_off = n_in_off;
+    g->l_in_src = n_in_src;
+    g->l_in_cap = n_in_cap;
+    if (bc.valid) free_binned(c, bc);
   }
-  if (!patch && bc.valid) free_binned(c, bc);
-  g->l_in_off = n_in_off;
-  g->l_in_src = n_in_src;
+  g->spare_out_off = g->l_out_off;
+  g->spare_out_dst = g->l_out_dst;
+  g->spare_out_cap = g->l_cap;
   g->l_out_off = n_out_off;
   g->l_out_dst = n_out_dst;
-  g->l_cap = n_cap;
+  g->l_cap = n_out_cap;
   g->M = M2;
   PRH_HIP(c, prh::launch_degrees(g->l_out_off, N, g->l_outdeg, s));
   uint32_t *zeros = nullptr;

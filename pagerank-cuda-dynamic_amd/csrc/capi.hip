@@ -329,14 +329,14 @@ void prh_graph_destroy(prh_graph *g) {
   dev_free(c, g->perm, N * 4);
   dev_free(c, g->inv, N * 4);
   dev_free(c, g->l_in_off, (N + 1) * 4);
-  dev_free(c, g->l_in_src, std::max<size_t>(g->l_cap, 4) * 4);
+  dev_free(c, g->l_in_src, std::max<size_t>(g->l_in_cap, 4) * 4);
   dev_free(c, g->l_outdeg, N * 4);
   dev_free(c, g->l_out_off, (N + 1) * 4);
   dev_free(c, g->l_out_dst, std::max<size_t>(g->l_cap, 4) * 4);
   dev_free(c, g->spare_in_off, (N + 1) * 4);
-  dev_free(c, g->spare_in_src, std::max<size_t>(g->spare_cap, 4) * 4);
+  dev_free(c, g->spare_in_src, std::max<size_t>(g->spare_in_cap, 4) * 4);
   dev_free(c, g->spare_out_off, (N + 1) * 4);
-  dev_free(c, g->spare_out_dst, std::max<size_t>(g->spare_cap, 4) * 4);
+  dev_free(c, g->spare_out_dst, std::max<size_t>(g->spare_out_cap, 4) * 4);
   dev_free(c, g->chunk_tables, g->chunk_tables_bytes);
   dev_free(c, g->chunk_partial, g->chunk_partial_bytes);
   dev_free(c, g->out_order, std::max<size_t>(N, 1) * 4);

@@ -148,8 +148,15 @@ struct prh_graph {
   // Arrays of an earlier CSR kept for the next update to write into (capacity in elements), so that a series of
   // updates does not allocate per batch.
   uint32_t *spare_in_off = nullptr, *spare_in_src = nullptr, *spare_out_off = nullptr, *spare_out_dst = nullptr;
-  uint64_t spare_cap = 0;
-  uint64_t l_cap = 0;  // capacity (elements) of l_in_src and l_out_dst
+  uint64_t spare_in_cap = 0, spare_out_cap = 0;
+  uint64_t l_cap = 0;     // capacity (elements) of l_out_dst
+  uint64_t l_in_cap = 0;  // capacity (elements) of l_in_src
+  // An update that patches the blocked plan leaves the in-CSR unbuilt (the plan's iterations do not read it): while
+  // in_stale, l_in_off and l_in_src are null and the in-CSR is the plan's base (bin.base_off, bin.base_src) less the
+  // edges of in_net_del plus those of in_net_ins (keys row << 32 | column in layout ids, ascending, relative to the
+  // base as long as it exists). ensure_in_csr builds it for whatever reads it.
+  bool in_stale = false;
+  std::vector<unsigned long long> in_net_del, in_net_ins;
   prhc::Prep update_prep;  // update time not yet reported by a solve
   prhc::Prep layout_prep, plan_prep, sliced_prep, out_prep;
   // Rank-kernel plan for the current tuning: section table and the chunk tables of the heaviest rows.
@@ -295,6 +302,8 @@ int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uin
 // Rebuilds the graph's layout in `mode` (0 degree-ordered, 1 the caller's order) from its uploaded arrays and drops
 // every plan built on the old one.
 int relayout(prh_context *c, prh_graph *g, int mode);
+// Builds the layout in-CSR of a graph whose last updates left it unbuilt (in_stale); no-op otherwise.
+int ensure_in_csr(prh_context *c, prh_graph *g);
 void free_sliced(prh_context *c, prh_graph *g);
 int build_sliced_plan(prh_context *c, prh_graph *g);
 // The propagation-blocked plan of a pull CSR (off, idx; N rows of degrees rowdeg, M slots) into bc, for the

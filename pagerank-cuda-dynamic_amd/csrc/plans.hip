@@ -31,6 +31,7 @@ void free_sliced(prh_context *c, prh_graph *g) {
 //   5. per slice, a section table: its chunks, then bands of items by row in-degree d: a wave per item for
 //      d > 64 S, W lanes for d in (W S, 2 W S], W = 32 .. 1 (items hold about d / S sources).
 int build_sliced_plan(prh_context *c, prh_graph *g) {
+  PRH_CHECK(ensure_in_csr(c, g));
   const prh_tuning &t = c->tuning;
   const int S = t.source_slices;
   if (g->sliced_valid && g->sliced_slices == S && g->sliced_huge == t.huge_degree &&
@@ -448,6 +449,12 @@ int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot) {
   // inverse (in-CSR edge -> plan slot). Both are built with the plan, as part of its one-time preparation.
   const bool updatable = g->l_out_off != nullptr;
   BinnedCache &bc = g->bin;
+  if (!(bc.valid && bc.B == c->tuning.blocking && (bc.slot || !(keep_slot || updatable)))) {
+    // A new plan is built from the current in-CSR; the old plan's base, and the net change kept against it, go.
+    PRH_CHECK(ensure_in_csr(c, g));
+    g->in_net_del.clear();
+    g->in_net_ins.clear();
+  }
   PRH_CHECK(build_binned(c, g->N, g->M, g->l_in_off, g->l_in_src, g->sorted_deg, keep_slot || updatable, bc));
   if (!updatable || bc.B > 8192 || !bc.plan.slot || bc.slot_of || bc.base_off) return PRH_SUCCESS;
   prhc::Prep inv;
@@ -477,6 +484,7 @@ int build_binned_plan_t(prh_context *c, prh_graph *g) {
 //   in-degree <= switch_degree: bands of equal lanes-per-row W, W = pow2ceil(d)/2 in 1..32 (or the fixed
 //   low_width), each band a contiguous row range.
 int build_rank_plan(prh_context *c, prh_graph *g) {
+  PRH_CHECK(ensure_in_csr(c, g));
   const prh_tuning &t = c->tuning;
   if (g->plan_valid && std::memcmp(&g->plan_tuning, &t, sizeof(t)) == 0) return PRH_SUCCESS;
   PrepTimer timer(g->plan_prep);
@@ -582,6 +590,7 @@ int build_rank_plan(prh_context *c, prh_graph *g) {
 // order) on each side of a chunk, capped at what the larger ring holds (edges beyond it are gathered directly). The
 // smaller ring (two workgroups per CU) serves H <= 4544; each workgroup sweeps about N / wgs rows.
 int build_window_plan(prh_context *c, prh_graph *g) {
+  PRH_CHECK(ensure_in_csr(c, g));
   if (g->win_valid) return PRH_SUCCESS;
   if (g->layout != 1) return fail(c, PRH_ERROR_INVALID_ARGUMENT, "the banded plan needs rows in the caller's order");
   if (g->N > (1u << 31)) return fail(c, PRH_ERROR_UNSUPPORTED, "the banded plan takes at most 2^31 vertices");
@@ -638,7 +647,7 @@ int relayout(prh_context *c, prh_graph *g, int mode) {
   dev_free(c, g->perm, size_t(N) * 4);
   dev_free(c, g->inv, size_t(N) * 4);
   dev_free(c, g->l_in_off, N1 * 4);
-  dev_free(c, g->l_in_src, std::max<size_t>(g->l_cap, 4) * 4);
+  dev_free(c, g->l_in_src, std::max<size_t>(g->l_in_cap, 4) * 4);
   dev_free(c, g->l_outdeg, size_t(N) * 4);
   dev_free(c, g->l_out_off, N1 * 4);
   dev_free(c, g->l_out_dst, std::max<size_t>(g->l_cap, 4) * 4);
@@ -654,7 +663,7 @@ int relayout(prh_context *c, prh_graph *g, int mode) {
     if (n) PRH_HIP(c, hipMemcpyAsync(*dst, src, n * 4, hipMemcpyDeviceToDevice, s));
     return PRH_SUCCESS;
   };
-  g->l_cap = std::max<uint64_t>(M, 4);
+  g->l_cap = g->l_in_cap = std::max<uint64_t>(M, 4);
   PRH_CHECK(copy(&g->l_in_off, g->in_off, N1));
   PRH_CHECK(copy(&g->l_in_src, g->in_src, M));
   PRH_CHECK(copy(&g->l_outdeg, g->outdeg, N));
@@ -788,8 +797,8 @@ int build_layout(prh_context *c, prh_graph *g, const uint32_t *in_off, const uin
   PRH_CHECK(alloc(&g->perm, N));
   PRH_CHECK(alloc(&g->inv, N));
   PRH_CHECK(alloc(&g->l_in_off, size_t(N) + 1));
-  g->l_cap = std::max<uint64_t>(M, 4);
-  PRH_CHECK(alloc(&g->l_in_src, g->l_cap));
+  g->l_cap = g->l_in_cap = std::max<uint64_t>(M, 4);
+  PRH_CHECK(alloc(&g->l_in_src, g->l_in_cap));
   PRH_CHECK(alloc(&g->l_outdeg, N));
   if (with_out) {
     PRH_CHECK(alloc(&g->l_out_off, size_t(N) + 1));

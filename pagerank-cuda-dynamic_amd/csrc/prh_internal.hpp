@@ -407,6 +407,9 @@ hipError_t launch_batch_keys(const uint32_t *bu, const uint32_t *bv, const uint3
                              unsigned long long *out_keys, int *flags, int bad_bit, hipStream_t s);
 hipError_t sort_keys_u64(void *tmp, size_t &tmp_bytes, const unsigned long long *in, unsigned long long *out,
                          uint32_t n, hipStream_t s);
+// in-CSR keys (target << 32 | source) of the first *count out-CSR keys, kNoKey for the rest of the cap entries
+hipError_t launch_transpose_keys(const unsigned long long *in, const uint32_t *count, uint32_t cap,
+                                unsigned long long *out, hipStream_t s);
 hipError_t launch_batch_valid(const unsigned long long *keys, uint32_t cap, const uint32_t *off, const uint32_t *idx,
                               bool want_exists, uint32_t *valid, hipStream_t s);
 hipError_t launch_batch_compact(const unsigned long long *keys, const uint32_t *valid, const uint32_t *pos,

@@ -283,7 +283,7 @@ int prh_stream_create(prh_context *c, const prh_graph *g, prh_method method, con
     return fail(c, PRH_ERROR_INVALID_ARGUMENT, "edge capacity must hold the graph and stay below 2^32");
   PRH_HIP(c, hipSetDevice(c->device));
   if (c->tuning.layout != g->layout && !g->updated) PRH_CHECK(relayout(c, const_cast<prh_graph *>(g), c->tuning.layout));
-  PRH_CHECK(build_rank_plan(c, const_cast<prh_graph *>(g)));
+  PRH_CHECK(build_rank_plan(c, const_cast<prh_graph *>(g)));  // builds the in-CSR an update may have left unbuilt
   auto *S = new prh_stream;
   S->ctx = c;
   S->g = g;

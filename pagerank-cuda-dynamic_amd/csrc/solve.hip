@@ -155,6 +155,7 @@ int solve_impl(prh_context *c, const prh_graph *g, int method, const prh_options
   const bool binned = c->tuning.blocking > 0;
   const bool window = !binned && c->tuning.window != 0 && g->layout == 1;
   const bool sliced = !binned && !window && variant == prh::kPlain && c->tuning.source_slices > 0 && g->layout == 0;
+  if (!binned) PRH_CHECK(ensure_in_csr(c, gm));  // the other plans read the in-CSR in every iteration
   if (binned) PRH_CHECK(build_binned_plan(c, gm));
   else if (window) PRH_CHECK(build_window_plan(c, gm));
   else if (sliced) PRH_CHECK(build_sliced_plan(c, gm));

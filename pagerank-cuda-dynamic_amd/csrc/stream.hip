@@ -72,6 +72,21 @@ __global__ __launch_bounds__(kBlock) void batch_keys_kernel(const uint32_t *bu, 
   }
 }
 
+// In-CSR keys of a tidied batch from its out-CSR keys (source << 32 | target becomes target << 32 | source); the
+// entries past the count become kNoKey, so that a sort of all cap entries puts the count keys first.
+__global__ __launch_bounds__(kBlock) void transpose_keys_kernel(const unsigned long long *in, const uint32_t *count,
+                                                                uint32_t cap, unsigned long long *out) {
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cap; i += gridDim.x * kBlock) {
+    unsigned long long k = kNoKey;
+    if (i < n) {
+      const unsigned long long o = in[i];
+      k = o << 32 | o >> 32;
+    }
+    out[i] = k;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void batch_valid_kernel(const unsigned long long *keys, uint32_t cap,
                                                              const uint32_t *off, const uint32_t *idx, int want_exists,
                                                              uint32_t *valid) {
@@ -419,6 +434,12 @@ hipError_t launch_batch_keys(const uint32_t *bu, const uint32_t *bv, const uint3
 hipError_t sort_keys_u64(void *tmp, size_t &tmp_bytes, const unsigned long long *in, unsigned long long *out,
                          uint32_t n, hipStream_t s) {
   return rocprim::merge_sort(tmp, tmp_bytes, in, out, n, rocprim::less<unsigned long long>(), s);
+}
+
+hipError_t launch_transpose_keys(const unsigned long long *in, const uint32_t *count, uint32_t cap,
+                                unsigned long long *out, hipStream_t s) {
+  transpose_keys_kernel<<<grid_for(cap, kBlock, 1024), kBlock, 0, s>>>(in, count, cap, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_batch_valid(const unsigned long long *keys, uint32_t cap, const uint32_t *off, const uint32_t *idx,

@@ -107,3 +107,78 @@ def test_update_errors_leave_graph_unchanged():
         assert e.value.status == 5
         xg.close()
         static_only.close()
+
+
+def test_deferred_in_csr(oracle_mod):
+    """With the blocked plan patched, an update leaves the in-CSR unbuilt (the plan's base plus the net change since)
+    and whatever reads it builds it: the CSR read-back after three updates with no reader between them (the net
+    change composed over batches that re-insert deleted edges and delete inserted ones), the row plan, a rebuild of
+    the blocked plan at another block size (the old base goes; the next update starts from the new plan), and a
+    streaming session created from the updated graph. Blocked solves between the updates run on the unbuilt in-CSR.
+    Bars as above: CSRs bit for bit, ranks within 1e-9 of the oracle, iterations within 1."""
+    from pagerank_amd import Batch, Context, HostGraph, Stream, Tuning, apply_batch, solve
+    x = HostGraph.rmat(13, 16, seed=12)
+    seq = _series(x, 6, max(1, int(2e-3 * x.M)), 80)
+    seq = seq[:2] + [seq[-1]] + seq[2:-1]  # the undo batch third, before any read-back
+
+    def check(xg, y, tag):
+        for which, off, idx in (("out", y.out_off, y.out_edg), ("in", y.in_off, y.in_edg)):
+            go, gi = xg.csr(which)
+            assert np.array_equal(go, off) and np.array_equal(gi, idx), (tag, which)
+
+    def ranks_ok(xg, y, m, b, q, tag):
+        res = solve(xg, m, batch=b, q=None if m == 0 else q)
+        ref, it, _ = oracle_mod.pagerank_cuda(m, y, b, None if m == 0 else q)
+        assert abs(res.iterations - it) <= 1, (tag, m, res.iterations, it)
+        assert np.abs(res.ranks - ref).max() <= 1e-9, (tag, m)
+
+    with Context(0) as c:
+        base = c.get_tuning().__dict__
+        c.set_tuning(Tuning(**{**base, "blocking": 1024}))
+        xg = c.upload(x)
+        q, _, _ = oracle_mod.pagerank_cuda(0, x)
+        solve(xg, 0)  # builds the plan the updates patch
+        # the undo batch undoes edges of the first two, so the first three are tidied against the graph they meet
+        y = x
+        for i in range(3):
+            b = Batch.tidy(y, list(zip(seq[i][0].del_u.tolist(), seq[i][0].del_v.tolist())),
+                           list(zip(seq[i][0].ins_u.tolist(), seq[i][0].ins_v.tolist())))
+            print(f"[deferred] update {i}: -{b.deletions}/+{b.insertions}", flush=True)
+            xg.update(b)
+            y = apply_batch(y, b)
+            ranks_ok(xg, y, 4, b, q, ("blocked", i))
+            q, _, _ = oracle_mod.pagerank_cuda(0, y)
+        check(xg, y, "three updates")
+        for i in (3, 4):
+            b = Batch.random(y, max(1, int(2e-3 * y.M)), 0.8, seed=90 + i)
+            xg.update(b)
+            y = apply_batch(y, b)
+        print("[deferred] row plan", flush=True)
+        c.set_tuning(Tuning(**{**base, "blocking": 0}))
+        ranks_ok(xg, y, 0, None, None, "rows")
+        ranks_ok(xg, y, 3, b, q, "rows")
+        print("[deferred] blocked plan rebuilt at 2048", flush=True)
+        c.set_tuning(Tuning(**{**base, "blocking": 2048}))
+        ranks_ok(xg, y, 0, None, None, "rebuilt")
+        q, _, _ = oracle_mod.pagerank_cuda(0, y)
+        for i in (5, 6):
+            b = Batch.random(y, max(1, int(2e-3 * y.M)), 0.8, seed=90 + i)
+            xg.update(b)
+            y = apply_batch(y, b)
+            ranks_ok(xg, y, 4, b, q, ("rebuilt", i))
+            q, _, _ = oracle_mod.pagerank_cuda(0, y)
+        check(xg, y, "after the rebuild")
+        print("[deferred] session from the updated graph", flush=True)
+        b = Batch.random(y, 40, 0.8, seed=99)
+        xg.update(b)
+        y = apply_batch(y, b)
+        st = Stream(xg, 1, max_batch=256)
+        b2 = Batch.random(y, 40, 0.8, seed=100)
+        st.update(b2, want_ranks=False)
+        z = apply_batch(y, b2)
+        for which, off, idx in (("out", z.out_off, z.out_edg), ("in", z.in_off, z.in_edg)):
+            go, gi = st.csr(which)
+            assert np.array_equal(go, off) and np.array_equal(gi, idx), ("session", which)
+        st.close()
+        c.set_tuning(Tuning(**base))
+        xg.close()
