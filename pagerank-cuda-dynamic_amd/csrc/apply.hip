@@ -7,7 +7,11 @@
 // and a new blocked plan per batch: three radix sorts over the edges, about as long as a static solve. Instead:
 //   tidy        the batch's keys sorted per CSR and kept as tidyBatchUpdateU keeps them (deletions of existing edges,
 //               insertions of absent ones, each once), with the session's kernels (stream.hip);
-//   apply       both CSRs of the layout rewritten into new arrays (rows stay ascending), the out-degrees recounted;
+//   apply       the layout's CSRs rewritten into new arrays (rows stay ascending), the out-degrees recounted. With
+//               the blocked plan patched (below) the in-CSR waits: its iterations read none, so the in-CSR is kept
+//               as the plan's base and the net change since (deletions and insertions, composed per batch on the
+//               host) and built by ensure_in_csr when a reader needs it (the row, banded and sliced plans, a new
+//               blocked plan, a read-back, a session);
 //   patch       the blocked plan keeps its slots: a deleted edge that is a slot gets the dead flag, an inserted one
 //               that is not becomes an entry of the plan's overlay (sorted by destination), which accumulate adds
 //               per bin. The vertex order (the layout's permutation) stays that of the upload.
@@ -19,7 +23,6 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
-#include <iterator>
 #include <vector>
 
 #include "capi_internal.hpp"
@@ -191,23 +194,70 @@ int apply_side(prh_context *c, Temps &tmp, uint32_t N, uint64_t M, const uint32_
   return PRH_SUCCESS;
 }
 
-// The net change of the in-CSR since the plan's base (D: base edges now absent, I: edges now present that the base
-// lacks) after one more tidied batch (d: edges it deletes, all present before it; i: edges it inserts, all absent
-// before it; every vector ascending). A deleted edge leaves I if an earlier batch inserted it, else joins D; an
-// inserted edge leaves D if an earlier batch deleted it, else joins I.
-void compose_net(std::vector<unsigned long long> &D, std::vector<unsigned long long> &I,
-                 const std::vector<unsigned long long> &d, const std::vector<unsigned long long> &i) {
-  std::vector<unsigned long long> d_in_I, d_rest, i_in_D, i_rest, D2, I2;
-  std::set_intersection(d.begin(), d.end(), I.begin(), I.end(), std::back_inserter(d_in_I));
-  std::set_difference(d.begin(), d.end(), I.begin(), I.end(), std::back_inserter(d_rest));
-  std::set_intersection(i.begin(), i.end(), D.begin(), D.end(), std::back_inserter(i_in_D));
-  std::set_difference(i.begin(), i.end(), D.begin(), D.end(), std::back_inserter(i_rest));
-  std::set_difference(D.begin(), D.end(), i_in_D.begin(), i_in_D.end(), std::back_inserter(D2));
-  std::set_difference(I.begin(), I.end(), d_in_I.begin(), d_in_I.end(), std::back_inserter(I2));
-  D.clear();
-  I.clear();
-  std::merge(D2.begin(), D2.end(), d_rest.begin(), d_rest.end(), std::back_inserter(D));
-  std::merge(I2.begin(), I2.end(), i_rest.begin(), i_rest.end(), std::back_inserter(I));
+// The net change of the in-CSR since the plan's base after one more tidied batch (d: nd edges it deletes, all
+// present before it; i: ni edges it inserts, all absent before it; ascending keys on the device). The net change D
+// (base edges now absent) and d are disjoint, as are I (edges now present that the base lacks) and i; a key in both
+// D + d and I + i is an edge the batch restores (in D and i) or removes again (in I and d), and it drops out. So
+// D' = (D + d) - (I + i) and I' = (I + i) - (D + d): two merges, a search of each side in the other and two
+// compactions, on the device; the new counts land in g->net_counts (read back by the caller).
+int compose_net(prh_context *c, prh_graph *g, Temps &tmp, const unsigned long long *d, uint32_t nd,
+                const unsigned long long *i, uint32_t ni) {
+  hipStream_t s = c->stream;
+  const uint32_t nx = uint32_t(g->net_del_n) + nd, ny = uint32_t(g->net_ins_n) + ni;
+  unsigned long long *x = nullptr, *y = nullptr;
+  uint32_t *vx = nullptr, *vy = nullptr, *px = nullptr, *py = nullptr;
+  PRH_CHECK(tmp.get(&x, std::max<uint32_t>(nx, 1)));
+  PRH_CHECK(tmp.get(&y, std::max<uint32_t>(ny, 1)));
+  PRH_CHECK(tmp.get(&vx, std::max<uint32_t>(nx, 1)));
+  PRH_CHECK(tmp.get(&vy, std::max<uint32_t>(ny, 1)));
+  PRH_CHECK(tmp.get(&px, std::max<uint32_t>(nx, 1)));
+  PRH_CHECK(tmp.get(&py, std::max<uint32_t>(ny, 1)));
+  auto merged = [&](const unsigned long long *a, uint32_t na, const unsigned long long *b, uint32_t nb,
+                    unsigned long long *out) -> int {
+    if (na && nb) {
+      size_t bytes = 0;
+      PRH_HIP(c, prh::merge_keys_u64(nullptr, bytes, a, na, b, nb, out, s));
+      uint8_t *work = nullptr;
+      PRH_CHECK(tmp.get(&work, bytes));
+      PRH_HIP(c, prh::merge_keys_u64(work, bytes, a, na, b, nb, out, s));
+    } else if (na || nb) {
+      PRH_HIP(c, hipMemcpyAsync(out, na ? a : b, size_t(na ? na : nb) * 8, hipMemcpyDeviceToDevice, s));
+    }
+    return PRH_SUCCESS;
+  };
+  PRH_CHECK(merged(g->net_del, uint32_t(g->net_del_n), d, nd, x));
+  PRH_CHECK(merged(g->net_ins, uint32_t(g->net_ins_n), i, ni, y));
+  // Room for the new net change (the old arrays are read by the merges above: wait for them before a free).
+  auto room = [&](unsigned long long *&arr, uint64_t &cap, uint32_t n) -> int {
+    if (cap >= n) return PRH_SUCCESS;
+    const uint64_t want = std::max<uint64_t>(uint64_t(n) + n / 2, 4096);
+    unsigned long long *fresh = nullptr;
+    PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&fresh), want * 8));
+    PRH_HIP(c, hipStreamSynchronize(s));
+    dev_free(c, arr, cap * 8);
+    arr = fresh;
+    cap = want;
+    return PRH_SUCCESS;
+  };
+  PRH_CHECK(room(g->net_del, g->net_del_cap, nx));
+  PRH_CHECK(room(g->net_ins, g->net_ins_cap, ny));
+  if (!g->net_counts) PRH_CHECK(dev_alloc(c, reinterpret_cast<void **>(&g->net_counts), 8));
+  PRH_HIP(c, prh::launch_fill_u32(g->net_counts, 0, 2, s));
+  size_t scan_bytes = 0;
+  PRH_HIP(c, prh::scan_u32(nullptr, scan_bytes, vx, px, std::max(nx, ny), s));
+  uint8_t *scan_tmp = nullptr;
+  PRH_CHECK(tmp.get(&scan_tmp, scan_bytes));
+  if (nx) {
+    PRH_HIP(c, prh::launch_absent_from(x, nx, y, ny, vx, s));
+    PRH_HIP(c, prh::scan_u32(scan_tmp, scan_bytes, vx, px, nx, s));
+    PRH_HIP(c, prh::launch_batch_compact(x, vx, px, nx, g->net_del, g->net_counts + 0, s));
+  }
+  if (ny) {
+    PRH_HIP(c, prh::launch_absent_from(y, ny, x, nx, vy, s));
+    PRH_HIP(c, prh::scan_u32(scan_tmp, scan_bytes, vy, py, ny, s));
+    PRH_HIP(c, prh::launch_batch_compact(y, vy, py, ny, g->net_ins, g->net_counts + 1, s));
+  }
+  return PRH_SUCCESS;
 }
 
 // New arrays for a CSR side of M2 edges: the side's spare arrays when they hold M2 edges, else new ones with room for
@@ -245,23 +295,20 @@ int ensure_in_csr(prh_context *c, prh_graph *g) {
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = c->stream;
   const uint32_t N = g->N;
-  const uint32_t nd = uint32_t(g->in_net_del.size()), ni = uint32_t(g->in_net_ins.size());
+  const uint32_t nd = uint32_t(g->net_del_n), ni = uint32_t(g->net_ins_n);
   uint32_t *noff = nullptr, *nidx = nullptr;
   uint64_t ncap = 0;
   int st = side_arrays(c, N, g->M, g->spare_in_off, g->spare_in_src, g->spare_in_cap, &noff, &nidx, &ncap);
   {
     Temps tmp(c);
-    unsigned long long *dk = nullptr, *ik = nullptr;
+    unsigned long long *none = nullptr;
     uint32_t *cnt = nullptr;
-    if (st == PRH_SUCCESS) st = tmp.get(&dk, std::max<uint32_t>(nd, 1));
-    if (st == PRH_SUCCESS) st = tmp.get(&ik, std::max<uint32_t>(ni, 1));
+    if (st == PRH_SUCCESS) st = tmp.get(&none, 1);
     if (st == PRH_SUCCESS) st = tmp.get(&cnt, 2);
     const uint32_t counts[2] = {nd, ni};
-    if (st == PRH_SUCCESS &&
-        ((nd && hipMemcpyAsync(dk, g->in_net_del.data(), size_t(nd) * 8, hipMemcpyHostToDevice, s) != hipSuccess) ||
-         (ni && hipMemcpyAsync(ik, g->in_net_ins.data(), size_t(ni) * 8, hipMemcpyHostToDevice, s) != hipSuccess) ||
-         hipMemcpyAsync(cnt, counts, 8, hipMemcpyHostToDevice, s) != hipSuccess))
+    if (st == PRH_SUCCESS && hipMemcpyAsync(cnt, counts, 8, hipMemcpyHostToDevice, s) != hipSuccess)
       st = fail(c, PRH_ERROR_DEVICE, "in-CSR build: upload");
+    const unsigned long long *dk = nd ? g->net_del : none, *ik = ni ? g->net_ins : none;
     if (st == PRH_SUCCESS)
       st = apply_side(c, tmp, N, bc.base_M, bc.base_off, bc.base_src, dk, cnt, nd, ik, cnt + 1, ni, noff, nidx, ncap);
     if (hipStreamSynchronize(s) != hipSuccess && st == PRH_SUCCESS) st = fail(c, PRH_ERROR_DEVICE, "in-CSR build: sync");
@@ -461,8 +508,7 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
       bc.base_src = g->l_in_src;
       bc.base_M = M;
       bc.base_cap = g->l_in_cap;
-      g->in_net_del.clear();
-      g->in_net_ins.clear();
+      g->net_del_n = g->net_ins_n = 0;
     } else if (g->l_in_off) {
       free_side(c, N, g->spare_in_off, g->spare_in_src, g->spare_in_cap);
       g->spare_in_off = g->l_in_off;
@@ -472,7 +518,6 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
     g->l_in_off = g->l_in_src = nullptr;
     g->l_in_cap = 0;
     g->in_stale = true;
-    compose_net(g->in_net_del, g->in_net_ins, hd, hi);
   } else {
     g->spare_in_off = g->l_in_off;
     g->spare_in_src = g->l_in_src;
@@ -496,10 +541,20 @@ int prh_graph_update(prh_context *c, prh_graph *g, const uint32_t *del_u, const 
   count_zero_kernel<<<std::min<uint32_t>(8192, (N + 255) / 256), 256, 0, s>>>(g->l_outdeg, N, zeros);
   PRH_HIP(c, hipGetLastError());
   // patch_plan reads its own words back through the pinned area: read the dead-end count after it
-  if (patch) PRH_CHECK(patch_plan(c, g, compact[0], counts + 0, cnt[0], compact[1], counts + 1, cnt[1]));
+  if (patch) {
+    PRH_CHECK(patch_plan(c, g, compact[0], counts + 0, cnt[0], compact[1], counts + 1, cnt[1]));
+    PRH_CHECK(compose_net(c, g, tmp, compact[0], cnt[0], compact[1], cnt[1]));
+    PRH_HIP(c, hipMemcpyAsync(c->pinned + 4, g->net_counts, 8, hipMemcpyDeviceToHost, s));
+  }
   PRH_HIP(c, hipMemcpyAsync(c->pinned, zeros, 4, hipMemcpyDeviceToHost, s));
   PRH_HIP(c, hipStreamSynchronize(s));
   g->dead_ends = *reinterpret_cast<const uint32_t *>(c->pinned);
+  if (patch) {
+    uint32_t net[2];
+    std::memcpy(net, c->pinned + 4, 8);
+    g->net_del_n = net[0];
+    g->net_ins_n = net[1];
+  }
   for (unsigned long long k : hd) --g->sorted_deg[uint32_t(k >> 32)];
   for (unsigned long long k : hi) ++g->sorted_deg[uint32_t(k >> 32)];
   // What depends on the edges is rebuilt on its next use; the caller-order arrays are dropped.

@@ -337,6 +337,9 @@ void prh_graph_destroy(prh_graph *g) {
   dev_free(c, g->spare_in_src, std::max<size_t>(g->spare_in_cap, 4) * 4);
   dev_free(c, g->spare_out_off, (N + 1) * 4);
   dev_free(c, g->spare_out_dst, std::max<size_t>(g->spare_out_cap, 4) * 4);
+  dev_free(c, g->net_del, g->net_del_cap * 8);
+  dev_free(c, g->net_ins, g->net_ins_cap * 8);
+  dev_free(c, g->net_counts, 8);
   dev_free(c, g->chunk_tables, g->chunk_tables_bytes);
   dev_free(c, g->chunk_partial, g->chunk_partial_bytes);
   dev_free(c, g->out_order, std::max<size_t>(N, 1) * 4);

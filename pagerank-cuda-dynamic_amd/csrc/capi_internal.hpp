@@ -153,10 +153,12 @@ struct prh_graph {
   uint64_t l_in_cap = 0;  // capacity (elements) of l_in_src
   // An update that patches the blocked plan leaves the in-CSR unbuilt (the plan's iterations do not read it): while
   // in_stale, l_in_off and l_in_src are null and the in-CSR is the plan's base (bin.base_off, bin.base_src) less the
-  // edges of in_net_del plus those of in_net_ins (keys row << 32 | column in layout ids, ascending, relative to the
-  // base as long as it exists). ensure_in_csr builds it for whatever reads it.
+  // net_del_n edges of net_del plus the net_ins_n edges of net_ins (device arrays of keys row << 32 | column in
+  // layout ids, ascending, relative to the base as long as it exists). ensure_in_csr builds it for whatever reads it.
   bool in_stale = false;
-  std::vector<unsigned long long> in_net_del, in_net_ins;
+  unsigned long long *net_del = nullptr, *net_ins = nullptr;
+  uint64_t net_del_n = 0, net_ins_n = 0, net_del_cap = 0, net_ins_cap = 0;
+  uint32_t *net_counts = nullptr;  // device: the two counts written by the last composition
   prhc::Prep update_prep;  // update time not yet reported by a solve
   prhc::Prep layout_prep, plan_prep, sliced_prep, out_prep;
   // Rank-kernel plan for the current tuning: section table and the chunk tables of the heaviest rows.

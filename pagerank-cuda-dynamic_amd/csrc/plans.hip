@@ -452,8 +452,7 @@ int build_binned_plan(prh_context *c, prh_graph *g, bool keep_slot) {
   if (!(bc.valid && bc.B == c->tuning.blocking && (bc.slot || !(keep_slot || updatable)))) {
     // A new plan is built from the current in-CSR; the old plan's base, and the net change kept against it, go.
     PRH_CHECK(ensure_in_csr(c, g));
-    g->in_net_del.clear();
-    g->in_net_ins.clear();
+    g->net_del_n = g->net_ins_n = 0;
   }
   PRH_CHECK(build_binned(c, g->N, g->M, g->l_in_off, g->l_in_src, g->sorted_deg, keep_slot || updatable, bc));
   if (!updatable || bc.B > 8192 || !bc.plan.slot || bc.slot_of || bc.base_off) return PRH_SUCCESS;

@@ -407,6 +407,12 @@ hipError_t launch_batch_keys(const uint32_t *bu, const uint32_t *bv, const uint3
                              unsigned long long *out_keys, int *flags, int bad_bit, hipStream_t s);
 hipError_t sort_keys_u64(void *tmp, size_t &tmp_bytes, const unsigned long long *in, unsigned long long *out,
                          uint32_t n, hipStream_t s);
+// out = the merge of ascending a[0, na) and b[0, nb) (rocPRIM)
+hipError_t merge_keys_u64(void *tmp, size_t &tmp_bytes, const unsigned long long *a, uint32_t na,
+                          const unsigned long long *b, uint32_t nb, unsigned long long *out, hipStream_t s);
+// valid[j] = 1 when a[j] is not in b (both ascending)
+hipError_t launch_absent_from(const unsigned long long *a, uint32_t na, const unsigned long long *b, uint32_t nb,
+                              uint32_t *valid, hipStream_t s);
 // in-CSR keys (target << 32 | source) of the first *count out-CSR keys, kNoKey for the rest of the cap entries
 hipError_t launch_transpose_keys(const unsigned long long *in, const uint32_t *count, uint32_t cap,
                                 unsigned long long *out, hipStream_t s);

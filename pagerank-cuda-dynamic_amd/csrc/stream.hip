@@ -22,6 +22,7 @@
 //                  deletions/insertions)
 //   insert         every inserted column to its place in its row
 // Sizes are read from device words, so one captured sequence serves every batch up to the capacity.
+#include <rocprim/device/device_merge.hpp>
 #include <rocprim/device/device_merge_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -69,6 +70,17 @@ __global__ __launch_bounds__(kBlock) void batch_keys_kernel(const uint32_t *bu, 
     }
     in_keys[i] = ki;
     out_keys[i] = ko;
+  }
+}
+
+// valid[j] = 1 when a[j] does not occur in b (both ascending).
+__global__ __launch_bounds__(kBlock) void absent_from_kernel(const unsigned long long *a, uint32_t na,
+                                                             const unsigned long long *b, uint32_t nb,
+                                                             uint32_t *valid) {
+  for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < na; j += gridDim.x * kBlock) {
+    const unsigned long long k = a[j];
+    const uint32_t at = lower_bound(b, 0u, nb, k);
+    valid[j] = at < nb && b[at] == k ? 0u : 1u;
   }
 }
 
@@ -434,6 +446,17 @@ hipError_t launch_batch_keys(const uint32_t *bu, const uint32_t *bv, const uint3
 hipError_t sort_keys_u64(void *tmp, size_t &tmp_bytes, const unsigned long long *in, unsigned long long *out,
                          uint32_t n, hipStream_t s) {
   return rocprim::merge_sort(tmp, tmp_bytes, in, out, n, rocprim::less<unsigned long long>(), s);
+}
+
+hipError_t merge_keys_u64(void *tmp, size_t &tmp_bytes, const unsigned long long *a, uint32_t na,
+                          const unsigned long long *b, uint32_t nb, unsigned long long *out, hipStream_t s) {
+  return rocprim::merge(tmp, tmp_bytes, a, b, out, na, nb, rocprim::less<unsigned long long>(), s);
+}
+
+hipError_t launch_absent_from(const unsigned long long *a, uint32_t na, const unsigned long long *b, uint32_t nb,
+                              uint32_t *valid, hipStream_t s) {
+  absent_from_kernel<<<grid_for(na, kBlock, 4096), kBlock, 0, s>>>(a, na, b, nb, valid);
+  return hipGetLastError();
 }
 
 hipError_t launch_transpose_keys(const unsigned long long *in, const uint32_t *count, uint32_t cap,
