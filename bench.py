@@ -29,8 +29,8 @@ compiled reference on the test graphs), run on this host's cores on the same gra
 iterations (about 10 s).
 
 --workload dfp: the dynamic side instead. Each step applies a fresh seeded batch of --batch-fraction |E| (80%
-insertions, 20% deletions) to the device graph in place (prh_graph_update: both CSRs rewritten, the blocked plan
-patched) and times a DF-P solve warm-started from the previous graph's static ranks (engine-measured time:
+insertions, 20% deletions) to the device graph in place (prh_graph_update: the out-CSR rewritten, the blocked plan
+patched, the in-CSR left to be built when read) and times a DF-P solve warm-started from the previous graph's static ranks (engine-measured time:
 the update, initialisation, marking and iterations, as PagerankResult.time), beside a GPU static solve of the same
 updated graph with the update added; config.dfp_ms_without_update gives the solve alone. Its cpu_baseline is the oracle's port of the reference's asynchronous OpenMP DF-P
 (pagerankPruneDynamicFrontierOmp<true>, the CPU solver the reference's experiment runs; bit-identical to the
@@ -159,8 +159,8 @@ def bench_c1(args) -> int:
 def bench_dfp(args, ctx, g, desc, overridden: bool = False) -> int:
     """GPU DF-P per update batch against GPU static on the same updated graph; CPU: the async DF-P port.
 
-    Each step applies a fresh seeded batch to the device graph in place (prh_graph_update: both CSRs rewritten, the
-    propagation-blocked plan patched) and solves DF-P from the previous graph's static ranks, then Static on the same
+    Each step applies a fresh seeded batch to the device graph in place (prh_graph_update: the out-CSR rewritten,
+    the propagation-blocked plan patched, the in-CSR left to be built when read) and solves DF-P from the previous graph's static ranks, then Static on the same
     graph. Both times include the update (DF-P's solve reports it as preparation; it is added to Static's)."""
     from pagerank_amd import PRUNE_DYNAMIC_FRONTIER, STATIC, Batch, PagerankOptions, apply_batch, solve
     o = PagerankOptions()

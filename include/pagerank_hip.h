@@ -152,7 +152,8 @@ uint64_t prh_graph_size(const prh_graph *graph);
  * uploaded with its out-CSR and afterwards is y for every solver (prh_solve, prh_mark_affected, prh_stream_create);
  * its propagation-blocked plan (blocks of at most 8192 vertices) is patched in place rather than rebuilt, and the
  * update's time (also returned in *update_ms, may be NULL) is added to the initialization and preparation times of
- * the next solve. The vertex set is fixed: an id >= N fails and leaves the graph unchanged. After an update the
+ * the next solve. While that plan is patched the in-CSR is built only when something reads it (a solve with another
+ * plan, prh_graph_csr, prh_stream_create), whose first such call then also reports the build as preparation. The vertex set is fixed: an id >= N fails and leaves the graph unchanged. After an update the
  * batched and weighted solvers refuse the graph (PRH_ERROR_UNSUPPORTED): upload y for them. */
 int prh_graph_update(prh_context *context, prh_graph *graph, const uint32_t *deletion_sources,
                      const uint32_t *deletion_targets, size_t n_deletions, const uint32_t *insertion_sources,
