@@ -583,8 +583,6 @@ int build_rank_plan(prh_context *c, prh_graph *g) {
   return PRH_SUCCESS;
 }
 
-// The layout in the caller's order (mode 1: row i is vertex i; the layout arrays are copies of the uploaded ones) or
-// by in-degree (mode 0, from the uploaded arrays read back to the host). Plans built on the old layout are dropped.
 // Banded plan: the window reaches H = the largest |source - destination| of the layout in-CSR (rows in the caller's
 // order) on each side of a chunk, capped at what the larger ring holds (edges beyond it are gathered directly). The
 // smaller ring (two workgroups per CU) serves H <= 4544; each workgroup sweeps about N / wgs rows.
@@ -619,6 +617,8 @@ int build_window_plan(prh_context *c, prh_graph *g) {
   return PRH_SUCCESS;
 }
 
+// The layout in the caller's order (mode 1: row i is vertex i; the layout arrays are copies of the uploaded ones) or
+// by in-degree (mode 0, from the uploaded arrays read back to the host). Plans built on the old layout are dropped.
 int relayout(prh_context *c, prh_graph *g, int mode) {
   if (g->layout == mode) return PRH_SUCCESS;
   if (g->updated) return fail(c, PRH_ERROR_UNSUPPORTED, "a graph updated in place keeps its layout");
