@@ -169,6 +169,7 @@ constexpr uint32_t kRewriteTile = 16384;  // old edges per workgroup
 constexpr int kRewriteLoads = 16;         // loads a thread keeps in flight
 constexpr uint32_t kTileUnits = 8;        // a tile over more units than this moves edge by edge
 constexpr uint32_t kRowKeys = kBlock;     // a changed row's deletions and insertions staged in LDS, up to this many
+constexpr uint32_t kStageUnits = kBlock;  // a tile's units staged in LDS for the edge-by-edge move, up to this many
 
 // Units of the rewrite: unit i (i < C) is the run of unchanged rows before the i-th changed row R_i, then R_i
 // itself; unit C runs from the last changed row's end to the last edge. Per unit: ue = old end of R_i (its edges are
@@ -295,7 +296,8 @@ __device__ void move_changed_row(const uint32_t *__restrict__ src, uint64_t a, u
 // Workgroup b moves old edges [b * tile, (b + 1) * tile) into the new index array (new_cap entries). The tile's
 // first and last units are found by the whole workgroup. A tile over a few units moves unit by unit: the unchanged
 // rows as one shifted run, the changed row against its staged deletions and insertions. A tile over many units (many
-// short changed rows) moves edge by edge, each finding its unit by binary search.
+// short changed rows) moves edge by edge, each finding its unit by binary search, in LDS when the tile's units (at
+// most kStageUnits) and their rows' ranges are staged there.
 __global__ __launch_bounds__(kBlock) void rewrite_kernel(const uint32_t *__restrict__ old_off,
                                                          const uint32_t *__restrict__ old_idx, RewriteUnits U,
                                                          const uint4 *__restrict__ ranges,
@@ -303,6 +305,8 @@ __global__ __launch_bounds__(kBlock) void rewrite_kernel(const uint32_t *__restr
                                                          const unsigned long long *__restrict__ ikeys, uint32_t N,
                                                          uint64_t new_cap, uint32_t *__restrict__ new_idx) {
   __shared__ uint32_t sdel[kRowKeys], sins[kRowKeys];
+  __shared__ uint32_t s_ue[kStageUnits], s_ub[kStageUnits], s_unb[kStageUnits], s_row[kStageUnits];
+  __shared__ uint4 s_rng[kStageUnits];
   const uint32_t M = old_off[N];
   const uint64_t e0 = uint64_t(blockIdx.x) * kRewriteTile;
   if (e0 >= M) return;
@@ -324,6 +328,21 @@ __global__ __launch_bounds__(kBlock) void rewrite_kernel(const uint32_t *__restr
     }
     return;
   }
+  // Units i0..i1 staged: s_*[j] is unit i0 + j; the closing unit (i = C, row N) has no changed edges and no range.
+  const uint32_t nu = i1 - i0 + 1;
+  const bool staged = nu <= kStageUnits;
+  if (staged) {
+    if (threadIdx.x < nu) {
+      const uint32_t i = i0 + threadIdx.x;
+      s_ue[threadIdx.x] = U.ue[i];
+      s_ub[threadIdx.x] = U.ub[i];
+      s_unb[threadIdx.x] = U.unb[i];
+      const uint32_t row = U.urow[i];
+      s_row[threadIdx.x] = row;
+      if (i < C) s_rng[threadIdx.x] = ranges[row];
+    }
+    __syncthreads();
+  }
   for (uint64_t e = e0 + threadIdx.x; e < e1; e += uint64_t(kRewriteLoads) * kBlock) {
     uint32_t v[kRewriteLoads];
 #pragma unroll
@@ -334,20 +353,34 @@ __global__ __launch_bounds__(kBlock) void rewrite_kernel(const uint32_t *__restr
       const uint64_t ek = e + uint64_t(k) * kBlock;
       if (ek >= e1) continue;
       const uint32_t col = v[k];
-      const uint32_t i = unit_of(U.ue, i0, i1, uint32_t(ek));
-      const uint32_t b = U.ub[i];
-      if (ek < b) {
-        store_in(new_idx, int64_t(ek) + int64_t(U.unb[i]) - int64_t(b), new_cap, col);
-        continue;
+      uint32_t b, nb, row;
+      uint4 r;
+      if (staged) {
+        const uint32_t j = unit_of(s_ue, 0u, nu - 1, uint32_t(ek));
+        b = s_ub[j];
+        nb = s_unb[j];
+        if (ek < b) {
+          store_in(new_idx, int64_t(ek) + int64_t(nb) - int64_t(b), new_cap, col);
+          continue;
+        }
+        row = s_row[j];
+        r = s_rng[j];
+      } else {
+        const uint32_t i = unit_of(U.ue, i0, i1, uint32_t(ek));
+        b = U.ub[i];
+        nb = U.unb[i];
+        if (ek < b) {
+          store_in(new_idx, int64_t(ek) + int64_t(nb) - int64_t(b), new_cap, col);
+          continue;
+        }
+        row = U.urow[i];
+        r = ranges[row];
       }
-      const uint32_t row = U.urow[i];
-      const uint4 r = ranges[row];
       const unsigned long long key = (unsigned long long)row << 32 | col;
       const uint32_t d_at = lower_bound(dkeys, r.x, r.y, key);
       if (d_at < r.y && dkeys[d_at] == key) continue;  // deleted
       const uint32_t i_at = lower_bound(ikeys, r.z, r.w, key);
-      store_in(new_idx, int64_t(U.unb[i]) + int64_t(ek - b) - int64_t(d_at - r.x) + int64_t(i_at - r.z), new_cap,
-               col);
+      store_in(new_idx, int64_t(nb) + int64_t(ek - b) - int64_t(d_at - r.x) + int64_t(i_at - r.z), new_cap, col);
     }
   }
 }

@@ -7,6 +7,8 @@ each) or, past that, searched in the batch's keys. A tile over many units moves 
   tile boundaries;
 - a hub whose in-row and out-row each span three tiles, with 50 (staged) or 300 (searched) deletions and as many
   insertions in each;
+- a ring of 30,000 vertices (three edges a row) with batches of 500 and 5,000 random edges: tiles over tens of
+  units (their units and row ranges staged in LDS) and over more than 256 (searched in memory);
 - an RMAT graph of 2^13 vertices with a batch of 0.2% of its edges (tens of units per tile), covered by
   test_gpu_update.py.
 
@@ -65,6 +67,16 @@ def _batches(kind):
             y = apply_batch(y, b)
             seq.append((b, y))
         return x, seq
+    if kind == "ring":
+        n = 30_000
+        v = np.arange(n, dtype=np.uint32)
+        x = HostGraph.from_edges(n, np.concatenate([v, v]), np.concatenate([(v + 1) % n, (v + 7) % n]))
+        seq, y = [], x
+        for i, size in enumerate((500, 5000)):
+            b = Batch.random(y, size, 0.8, seed=60 + i)
+            y = apply_batch(y, b)
+            seq.append((b, y))
+        return x, seq
     k = 50 if kind == "hub-staged" else 300
     x = _hub_graph(100_000)  # the hub's rows: 50,000 edges each, over three or four tiles
     seq, y = [], x
@@ -76,7 +88,7 @@ def _batches(kind):
     return x, seq
 
 
-KINDS = ["rmat16", "hub-staged", "hub-searched"]
+KINDS = ["rmat16", "hub-staged", "hub-searched", "ring"]
 
 
 @pytest.mark.parametrize("kind", KINDS)
@@ -98,7 +110,7 @@ def test_stream_rewrite(ctx, kind):
     from pagerank_amd import Stream
     x, seq = _batches(kind)
     xg = ctx.upload(x)
-    st = Stream(xg, 1, max_batch=1024)
+    st = Stream(xg, 1, max_batch=8192)
     for i, (b, y) in enumerate(seq):
         print(f"[rewrite] stream {kind} batch={i} -{b.deletions}/+{b.insertions} M={y.M}", flush=True)
         st.update(b, want_ranks=False)
